@@ -1,0 +1,191 @@
+"""CPU ORACLE — Python handle on the compiled oracle (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  It loads oracle/_build/liboracle_<system>.so
+(built from oracle/oracle_capi.cpp + oracle/zamr_oracle.hpp, a restatement of
+the reference's Source/make_rhs.H and the single-level Source/ZoomyAmr.cpp
+stage/step logic) and drives it with the same case dictionary the product's
+Python layer accepts.
+
+Case keys (deck names of the reference, build.py:55-120 / ZoomyAmr.cpp:29-83):
+  nx, ny, prob_lo, prob_hi, max_grid_size, spatial_order, implicit_source,
+  well_balanced, clamp_positivity, positivity ("none"|"mood"), face_based_rhs,
+  cfl, dtmin, dtmax, time_end, max_step, bc (x_lo, x_hi, y_lo, y_hi tag names),
+  params ({name: value} overrides), cfl_dimension, cfl_degree_factor.
+"""
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+BUILD = HERE / "_build"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+DEFAULTS = dict(
+    prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=64, spatial_order=1,
+    implicit_source=True, well_balanced=False, clamp_positivity=True,
+    positivity="none", face_based_rhs=1, cfl=0.5, dtmin=1.0e-7, dtmax=1.0e-2,
+    time_end=1.0, max_step=-1, bc=("", "", "", ""), params=None,
+    cfl_dimension=None, cfl_degree_factor=None)
+
+_libs = {}
+
+
+def load(system):
+    if system in _libs:
+        return _libs[system]
+    path = BUILD / f"liboracle_{system}.so"
+    if not path.exists():
+        raise FileNotFoundError(f"oracle library {path} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(path))
+    for name in ("orc_nq", "orc_na", "orc_np", "orc_dim", "orc_ntags"):
+        getattr(lib, name).restype = ctypes.c_int
+    lib.orc_tag_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.orc_default_params.argtypes = [_D]
+    lib.orc_create.restype = ctypes.c_void_p
+    lib.orc_create.argtypes = [_I, _D, _D]
+    for name in ("orc_destroy", "orc_init_solution"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+    lib.orc_nboxes.argtypes = [ctypes.c_void_p]
+    for name in ("orc_set_state", "orc_get_state", "orc_set_aux", "orc_get_aux"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, _D]
+    for name in ("orc_max_inv", "orc_compute_dt", "orc_total_mass"):
+        getattr(lib, name).restype = ctypes.c_double
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+    lib.orc_update_state.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    lib.orc_advance.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    lib.orc_mood_redos.restype = ctypes.c_long
+    lib.orc_mood_redos.argtypes = [ctypes.c_void_p]
+    lib.orc_evolve.restype = ctypes.c_int
+    lib.orc_evolve.argtypes = [ctypes.c_void_p, _D, _D, _D, ctypes.c_int, ctypes.c_int]
+    lib.orc_ge_solve.restype = ctypes.c_int
+    lib.orc_ge_solve.argtypes = [ctypes.c_int, _D, _D, _D]
+    _libs[system] = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class Oracle:
+    """One single-level CPU run of `system` ("swe1d", "swe2d", "sme2")."""
+
+    def __init__(self, system, case, threads=1):
+        self.lib = lib = load(system)
+        c = dict(DEFAULTS)
+        c.update(case)
+        self.case = c
+        self.nq, self.na, self.np_ = lib.orc_nq(), lib.orc_na(), lib.orc_np()
+        self.dim = lib.orc_dim()
+        self.tags = []
+        buf = ctypes.create_string_buffer(64)
+        for k in range(lib.orc_ntags()):
+            lib.orc_tag_name(k, buf, 64)
+            self.tags.append(buf.value.decode())
+        p = np.zeros(self.np_)
+        lib.orc_default_params(_dp(p))
+        self.param_names = None
+        for name, v in (c["params"] or {}).items():
+            p[self.param_index(name)] = v
+        self.params = p
+        bcs = []
+        for name in c["bc"]:
+            if not name:
+                bcs.append(0)
+            elif name in self.tags:
+                bcs.append(self.tags.index(name))
+            else:
+                raise ValueError(f"boundary tag {name!r} is not declared by this model: {self.tags}")
+        order = int(c["spatial_order"])
+        cfl_dim = c["cfl_dimension"] if c["cfl_dimension"] is not None else (2 if c["ny"] > 1 or self.dim == 2 else 1)
+        cfl_deg = c["cfl_degree_factor"] if c["cfl_degree_factor"] is not None else 2 * (order - 1) + 1
+        iset = np.array([c["nx"], c["ny"], c["max_grid_size"], order, int(bool(c["implicit_source"])),
+                         int(bool(c["well_balanced"])), int(bool(c["clamp_positivity"])),
+                         int(c["positivity"] == "mood"), int(c["face_based_rhs"]), int(c["max_step"]),
+                         *bcs, cfl_dim, cfl_deg, threads], dtype=np.int32)
+        dset = np.array([*c["prob_lo"], *c["prob_hi"], c["cfl"], c["dtmin"], c["dtmax"], c["time_end"]],
+                        dtype=np.float64)
+        self.h = lib.orc_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p))
+        self.nx, self.ny = int(c["nx"]), int(c["ny"])
+
+    _NAMES = ("g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s")
+
+    def param_index(self, name):
+        names = self.case.get("param_names") or self._NAMES
+        return list(names).index(name)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def nboxes(self):
+        return self.lib.orc_nboxes(self.h)
+
+    def init_solution(self):
+        self.lib.orc_init_solution(self.h)
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(self.nq, self.ny, self.nx)
+        self.lib.orc_set_state(self.h, _dp(Q))
+
+    def get_state(self):
+        Q = np.empty((self.nq, self.ny, self.nx))
+        self.lib.orc_get_state(self.h, _dp(Q))
+        return Q
+
+    def set_aux(self, A):
+        A = np.ascontiguousarray(A, dtype=np.float64).reshape(self.na, self.ny, self.nx)
+        self.lib.orc_set_aux(self.h, _dp(A))
+
+    def get_aux(self):
+        A = np.empty((self.na, self.ny, self.nx))
+        self.lib.orc_get_aux(self.h, _dp(A))
+        return A
+
+    def max_inv(self):
+        return self.lib.orc_max_inv(self.h)
+
+    def compute_dt(self):
+        return self.lib.orc_compute_dt(self.h)
+
+    def update_state(self, time=0.0):
+        self.lib.orc_update_state(self.h, time)
+
+    def stage(self, order, time, dt):
+        self.lib.orc_stage(self.h, order, time, dt)
+
+    def advance(self, time, dt):
+        self.lib.orc_advance(self.h, time, dt)
+
+    def total_mass(self):
+        return self.lib.orc_total_mass(self.h)
+
+    def mood_redos(self):
+        return self.lib.orc_mood_redos(self.h)
+
+    def evolve(self, step_limit=-1, cap=100000):
+        t = np.zeros(1)
+        dts = np.zeros(cap)
+        ms = np.zeros(cap)
+        n = self.lib.orc_evolve(self.h, _dp(t), _dp(dts), _dp(ms), cap, step_limit)
+        return n, float(t[0]), dts[:min(n, cap)].copy(), ms[:min(n, cap)].copy()
+
+
+def ge_solve(A, rhs):
+    """make_rhs.H's elimination on a dense n x n system (n in 3, 4, 8)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros(len(rhs))
+    lib = load("swe2d")
+    ok = lib.orc_ge_solve(len(rhs), _dp(A), _dp(rhs), _dp(x))
+    return ok == 1, x

@@ -1,0 +1,672 @@
+#pragma once
+// CPU ORACLE — test infrastructure only, never part of the product path.
+//
+// A restatement, on AMReX-style SoA boxes, of the reference's per-box RHS and
+// time advance: Source/make_rhs.H (all of it) and the single-level parts of
+// Source/ZoomyAmr.cpp (UpdateState :293-334, FillPhysicalBC_mf :342-443,
+// ComputeDt :448-515, ComputeTotalMass :517-536, Advance :538-731,
+// TimeStep :733-762, Evolve :767-816) plus Source/init_solution.cpp:78-115.
+// It consumes the same Model.H / Numerics.H device-function headers the HIP
+// kernels consume, exactly as the reference compiles them into its driver.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it.  Build with FMA contraction OFF (-ffp-contract=off), mirroring the
+// reference's -fmad=false (build.py:33-41).
+//
+// Pinning (what checks each part; none of them re-types these formulas):
+//   * the reference's own golden fixtures tests/refs/{stoker_wet,ritter_dry,
+//     wb_lake,swe_2d,parallel_2rank,parity_numpy_n100,parity_numpy_n200}.npz
+//     (order 1, single level, explicit source, 1-D and 2-D, extrapolation and
+//     wall, with and without well-balancing), compared under the reference's
+//     own np.allclose rule (tests/refs.py:33) and found equal to ~1e-15;
+//   * closed forms and invariants: lake at rest to round-off at order 2,
+//     constant state => RHS == 0, wall box conserves mass to round-off,
+//     order-2 convergence rate on a smooth wave, box-decomposition invariance
+//     (bitwise), wave-free domain => dtmax, uniform state => closed-form dt;
+//   * the point-implicit solve against numpy.linalg.solve on random systems
+//     and an exact backward-Euler step for a linear source, plus the explicit
+//     fallback on a singular system.
+// Parity unpinned: the SME level-2 system's order-2 behaviour has no
+// reference fixture; it is checked only by the invariants above.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace zorc {
+
+using Real = double;
+constexpr int idx_b = 0;  // Source/constants.H:6-7
+constexpr int idx_h = 1;
+
+struct Box {
+    int lo[2], hi[2];
+    int len(int d) const { return hi[d] - lo[d] + 1; }
+};
+
+// One FArrayBox: a box grown by ng, ncomp components, i fastest, then j, then comp.
+struct Fab {
+    Box vb;  // valid box
+    int ng = 0, nc = 0;
+    int gl[2], gh[2];  // grown bounds
+    long sx = 1, sy = 0, sc = 0;
+    std::vector<Real> d;
+    void define(const Box& b, int ncomp, int nghost, int ngy) {
+        vb = b; ng = nghost; nc = ncomp;
+        gl[0] = b.lo[0] - ng; gh[0] = b.hi[0] + ng;
+        gl[1] = b.lo[1] - ngy; gh[1] = b.hi[1] + ngy;
+        sy = gh[0] - gl[0] + 1;
+        sc = sy * (gh[1] - gl[1] + 1);
+        d.assign((size_t)sc * nc, 0.0);
+    }
+    Real& operator()(int i, int j, int n) { return d[(i - gl[0]) + (j - gl[1]) * sy + n * sc]; }
+    Real operator()(int i, int j, int n) const { return d[(i - gl[0]) + (j - gl[1]) * sy + n * sc]; }
+};
+
+struct MultiFab {
+    std::vector<Fab> fabs;
+};
+
+struct Settings {
+    int nx = 0, ny = 0;
+    Real plo[2] = {0, 0}, phi[2] = {1, 1};
+    int max_grid_size = 64;
+    int spatial_order = 1;
+    bool implicit_source = true;
+    bool well_balanced = false;
+    bool clamp_positivity = true;
+    bool mood = false;  // solver.positivity == "mood"
+    bool face_based_rhs = true;
+    Real cfl = 0.5, dtmin = 1.0e-7, dtmax = 1.0e-2, time_end = 1.0;
+    int max_step = -1;
+    int bc_idx[4] = {0, 0, 0, 0};  // x_lo, x_hi, y_lo, y_hi tag indices
+    int cfl_dimension = 1, cfl_degree_factor = 1;  // MarchConstants
+};
+
+// minmod: make_rhs.H:9-13 (ties return the second argument).
+inline Real minmod(Real a, Real b) {
+    if (a * b <= 0.0) return 0.0;
+    return (std::abs(a) < std::abs(b)) ? a : b;
+}
+
+// AMReX BoxList::maxSize: numblk = ceil(L/m) chunks per axis, the first
+// (L % numblk) one cell longer.
+inline std::vector<Box> chop_domain(int nx, int ny, int mgs) {
+    auto split = [&](int L) {
+        std::vector<std::pair<int, int>> r;
+        int nb = (L + mgs - 1) / mgs;
+        int sz = L / nb, extra = L % nb, s = 0;
+        for (int k = 0; k < nb; ++k) {
+            int l = sz + (k < extra ? 1 : 0);
+            r.push_back({s, s + l - 1});
+            s += l;
+        }
+        return r;
+    };
+    std::vector<Box> out;
+    auto xs = split(nx), ys = split(ny);
+    for (auto& y : ys)
+        for (auto& x : xs) out.push_back(Box{{x.first, y.first}, {x.second, y.second}});
+    return out;
+}
+
+// Gaussian elimination with partial pivoting, make_rhs.H:467-499: strict '>'
+// pivot search, failure on !(best > 0) or a non-finite pivot, back
+// substitution failing on any non-finite component.  A and rhs are consumed.
+template <int N>
+inline bool ge_solve(amrex::SmallMatrix<Real, N, N>& A, amrex::SmallMatrix<Real, N, 1>& rhs,
+                     amrex::SmallMatrix<Real, N, 1>& dq) {
+    bool ok = true;
+    for (int k = 0; k < N && ok; ++k) {
+        int piv = k;
+        Real best = std::abs(A(k, k));
+        for (int r = k + 1; r < N; ++r)
+            if (std::abs(A(r, k)) > best) { best = std::abs(A(r, k)); piv = r; }
+        if (!(best > 0.0) || !std::isfinite(best)) { ok = false; break; }
+        if (piv != k) {
+            for (int c = k; c < N; ++c) std::swap(A(k, c), A(piv, c));
+            std::swap(rhs(k, 0), rhs(piv, 0));
+        }
+        for (int r = k + 1; r < N; ++r) {
+            Real f = A(r, k) / A(k, k);
+            for (int c = k; c < N; ++c) A(r, c) -= f * A(k, c);
+            rhs(r, 0) -= f * rhs(k, 0);
+        }
+    }
+    if (!ok) return false;
+    for (int r = N - 1; r >= 0; --r) {
+        Real sum = 0.0;
+        for (int c = r + 1; c < N; ++c) sum += A(r, c) * dq(c, 0);
+        dq(r, 0) = (rhs(r, 0) - sum) / A(r, r);
+        if (!std::isfinite(dq(r, 0))) return false;
+    }
+    return true;
+}
+
+template <class Model, class Numerics>
+struct Solver {
+    static constexpr int nq = Model::n_dof_q;
+    static constexpr int na = Model::n_dof_qaux;
+    static constexpr int np = Model::n_parameters;
+    static constexpr int dim = Model::dimension;
+    using VQ = amrex::SmallMatrix<Real, nq, 1>;
+    using VA = amrex::SmallMatrix<Real, na, 1>;
+    using VP = amrex::SmallMatrix<Real, np, 1>;
+    using VN = amrex::SmallMatrix<Real, dim, 1>;
+    using VX = amrex::SmallMatrix<Real, 3, 1>;
+    static constexpr int n_face_comp = 3 * nq + 2;  // make_rhs.H:315
+
+    Settings s;
+    VP p;
+    Real dx[2];
+    std::vector<Box> boxes;
+    MultiFab Q, Qtmp, Qold, Qaux, Fx, Fy;
+    int ng = 1;
+    int nthreads = 1;
+    long mood_redos = 0;
+
+    explicit Solver(const Settings& st, const Real* params) : s(st) {
+        for (int i = 0; i < np; ++i) p(i, 0) = params[i];
+        dx[0] = (s.phi[0] - s.plo[0]) / s.nx;
+        dx[1] = (s.phi[1] - s.plo[1]) / s.ny;
+        ng = (s.spatial_order == 2) ? 2 : 1;  // ZoomyAmr.H:152
+        boxes = chop_domain(s.nx, s.ny, s.max_grid_size);
+        // A 2-D AMReX build grows boxes in y too, even for a 1-D model.
+        for (auto* mf : {&Q, &Qtmp, &Qaux}) mf->fabs.resize(boxes.size());
+        Qold.fabs.resize(boxes.size());
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            Q.fabs[b].define(boxes[b], nq, ng, ng);
+            Qtmp.fabs[b].define(boxes[b], nq, ng, ng);
+            Qaux.fabs[b].define(boxes[b], std::max(na, 1), ng, ng);
+            Qold.fabs[b].define(boxes[b], nq, 0, 0);
+        }
+        if (s.face_based_rhs) {
+            Fx.fabs.resize(boxes.size());
+            Fy.fabs.resize(boxes.size());
+            for (size_t b = 0; b < boxes.size(); ++b) {
+                Box bx = boxes[b];
+                bx.hi[0] += 1;  // nodal in x
+                Fx.fabs[b].define(bx, n_face_comp, 0, 0);
+                Box by = boxes[b];
+                by.hi[1] += 1;  // nodal in y
+                Fy.fabs[b].define(by, n_face_comp, 0, 0);
+            }
+        }
+    }
+
+    VX cell_center(int i, int j) const {  // make_rhs.H:17-26
+        VX X{};
+        X(0, 0) = s.plo[0] + (i + 0.5) * dx[0];
+        X(1, 0) = (dim == 2) ? s.plo[1] + (j + 0.5) * dx[1] : Real(0.0);
+        X(2, 0) = Real(0.0);
+        return X;
+    }
+
+    // init_solution.cpp:78-115 on every fab with >= 2 comps (Q AND Qaux, :174-175).
+    void init_solution(MultiFab& mf) {
+        const Real x_mid = 0.5 * (s.plo[0] + s.phi[0]);
+        for (auto& f : mf.fabs) {
+            const bool is_state = f.nc >= 2;
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i) {
+                    for (int n = 0; n < f.nc; ++n) f(i, j, n) = 0.0;
+                    if (is_state) {
+                        const Real x = s.plo[0] + (i + 0.5) * dx[0];
+                        f(i, j, idx_b) = 0.0;
+                        f(i, j, idx_h) = (x < x_mid) ? 2.0 : 1.0;
+                    }
+                }
+        }
+    }
+
+    // Global (nc, ny, nx) array <-> valid cells.
+    void scatter(MultiFab& mf, const Real* src, int nc) {
+        for (auto& f : mf.fabs)
+            for (int n = 0; n < nc; ++n)
+                for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                    for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i)
+                        f(i, j, n) = src[((size_t)n * s.ny + j) * s.nx + i];
+    }
+    void gather(const MultiFab& mf, Real* dst, int nc) const {
+        for (auto& f : mf.fabs)
+            for (int n = 0; n < nc; ++n)
+                for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                    for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i)
+                        dst[((size_t)n * s.ny + j) * s.nx + i] = f(i, j, n);
+    }
+
+    // Same-level ghost exchange (FillPatchSingleLevel / FillBoundary): every
+    // ghost cell inside the domain takes the owning box's valid value.
+    void fill_boundary(MultiFab& mf) {
+        if (boxes.size() < 2) return;
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+        for (int b = 0; b < nb; ++b) {
+            Fab& f = mf.fabs[b];
+            for (int o = 0; o < nb; ++o) {
+                if (o == b) continue;
+                const Fab& g = mf.fabs[o];
+                int ilo = std::max(f.gl[0], g.vb.lo[0]), ihi = std::min(f.gh[0], g.vb.hi[0]);
+                int jlo = std::max(f.gl[1], g.vb.lo[1]), jhi = std::min(f.gh[1], g.vb.hi[1]);
+                for (int n = 0; n < f.nc; ++n)
+                    for (int j = jlo; j <= jhi; ++j)
+                        for (int i = ilo; i <= ihi; ++i) f(i, j, n) = g(i, j, n);
+            }
+        }
+    }
+
+    // ZoomyAmr::UpdateState (ZoomyAmr.cpp:293-334), over the grown box.
+    void update_state(Real time) {
+        const bool clamp = s.clamp_positivity && !s.mood;
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+        for (int b = 0; b < nb; ++b) {
+            Fab& fq = Q.fabs[b];
+            Fab& fa = Qaux.fabs[b];
+            for (int j = fq.gl[1]; j <= fq.gh[1]; ++j)
+                for (int i = fq.gl[0]; i <= fq.gh[0]; ++i) {
+                    VQ q;
+                    VA a;
+                    for (int n = 0; n < nq; ++n) q(n, 0) = fq(i, j, n);
+                    for (int n = 0; n < na; ++n) a(n, 0) = fa(i, j, n);
+                    VX X{};
+                    X(0, 0) = s.plo[0] + (i + 0.5) * dx[0];
+                    X(1, 0) = (dim == 2) ? s.plo[1] + (j + 0.5) * dx[1] : Real(0.0);
+                    X(2, 0) = Real(0.0);
+                    q = Model::update_variables(q, a, p);
+                    if (clamp && q(idx_h, 0) < 0.0) q(idx_h, 0) = 0.0;
+                    a = Model::update_aux_variables(q, a, p, time, X);
+                    for (int n = 0; n < nq; ++n) fq(i, j, n) = q(n, 0);
+                    for (int n = 0; n < na; ++n) fa(i, j, n) = a(n, 0);
+                }
+        }
+    }
+
+    // ZoomyAmr::FillPhysicalBC_mf (ZoomyAmr.cpp:342-443): mirror-index ghosts,
+    // corner ghosts take the x-side tag.
+    void fill_physical_bc(Real time) {
+        const int dlx = 0, dhx = s.nx - 1, dly = 0, dhy = s.ny - 1;
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+        for (int b = 0; b < nb; ++b) {
+            Fab& fq = Q.fabs[b];
+            Fab& fa = Qaux.fabs[b];
+            for (int j = fq.gl[1]; j <= fq.gh[1]; ++j)
+                for (int i = fq.gl[0]; i <= fq.gh[0]; ++i) {
+                    const bool out_x = (i < dlx || i > dhx);
+                    const bool out_y = (dim == 2) && (j < dly || j > dhy);
+                    if (!out_x && !out_y) continue;
+                    int ii = i;
+                    if (i < dlx) ii = 2 * dlx - 1 - i;
+                    else if (i > dhx) ii = 2 * dhx + 1 - i;
+                    int jj = j;
+                    if (dim == 2) {
+                        if (j < dly) jj = 2 * dly - 1 - j;
+                        else if (j > dhy) jj = 2 * dhy + 1 - j;
+                    }
+                    int bc = -1;
+                    VN nh{};
+                    if (i < dlx) { bc = s.bc_idx[0]; nh(0, 0) = -1.0; }
+                    else if (i > dhx) { bc = s.bc_idx[1]; nh(0, 0) = 1.0; }
+                    else if (dim == 2) {
+                        if (j < dly) { bc = s.bc_idx[2]; nh(dim - 1, 0) = -1.0; }
+                        else if (j > dhy) { bc = s.bc_idx[3]; nh(dim - 1, 0) = 1.0; }
+                    }
+                    if (bc == -1) continue;
+                    VQ qi;
+                    VA ai;
+                    for (int n = 0; n < nq; ++n) qi(n, 0) = fq(ii, jj, n);
+                    for (int n = 0; n < na; ++n) ai(n, 0) = fa(ii, jj, n);
+                    VX X;
+                    X(0, 0) = s.plo[0] + (i + 0.5) * dx[0];
+                    X(1, 0) = (dim == 2) ? s.plo[1] + (j + 0.5) * dx[1] : 0.0;
+                    X(2, 0) = 0.0;
+                    VQ qg = Model::boundary_conditions(bc, qi, ai, nh, X, time, dx[0]);
+                    VA ag = Model::aux_boundary_conditions(bc, qi, ai, nh, X, time, dx[0]);
+                    for (int n = 0; n < nq; ++n) fq(i, j, n) = qg(n, 0);
+                    for (int n = 0; n < na; ++n) fa(i, j, n) = ag(n, 0);
+                }
+        }
+    }
+
+    // ZoomyAmr::FillPatch (lev 0) + the FillBoundary pair at :571-572.
+    void fill_patch(Real time) {
+        fill_boundary(Q);
+        fill_physical_bc(time);
+        fill_boundary(Q);
+        fill_boundary(Qaux);
+    }
+
+    // make_rhs.H:31-131.
+    void eval_face(const Fab& fq, const Fab& fa, int iL, int jL, int iR, int jR, int dir, int order,
+                   VQ& flux, amrex::SmallMatrix<Real, 2 * nq, 1>& fluct, Real time, bool wb,
+                   Real* hLstar, Real* hRstar) const {
+        VQ qL, qR;
+        VA aL, aR;
+        VN n_hat{};
+        for (int d = 0; d < dim; ++d) n_hat(d, 0) = 0.0;
+        n_hat(dir, 0) = 1.0;
+        for (int n = 0; n < nq; ++n) {
+            if (order == 1) {
+                qL(n, 0) = fq(iL, jL, n);
+                qR(n, 0) = fq(iR, jR, n);
+            } else if (dir == 0) {
+                Real gL = minmod(fq(iL, jL, n) - fq(iL - 1, jL, n), fq(iL + 1, jL, n) - fq(iL, jL, n));
+                Real gR = minmod(fq(iR, jR, n) - fq(iR - 1, jR, n), fq(iR + 1, jR, n) - fq(iR, jR, n));
+                qL(n, 0) = fq(iL, jL, n) + 0.5 * gL;
+                qR(n, 0) = fq(iR, jR, n) - 0.5 * gR;
+            } else {
+                Real gL = minmod(fq(iL, jL, n) - fq(iL, jL - 1, n), fq(iL, jL + 1, n) - fq(iL, jL, n));
+                Real gR = minmod(fq(iR, jR, n) - fq(iR, jR - 1, n), fq(iR, jR + 1, n) - fq(iR, jR, n));
+                qL(n, 0) = fq(iL, jL, n) + 0.5 * gL;
+                qR(n, 0) = fq(iR, jR, n) - 0.5 * gR;
+            }
+        }
+        Real hLs = qL(idx_h, 0), hRs = qR(idx_h, 0);
+        if (wb) {  // Audusse hydrostatic reconstruction, make_rhs.H:84-98
+            Real bL = qL(idx_b, 0), bR = qR(idx_b, 0);
+            Real bf = amrex::max(bL, bR);
+            hLs = amrex::max(Real(0.0), qL(idx_h, 0) + bL - bf);
+            hRs = amrex::max(Real(0.0), qR(idx_h, 0) + bR - bf);
+            Real sL = (qL(idx_h, 0) > Real(0)) ? hLs / qL(idx_h, 0) : Real(0.0);
+            Real sR = (qR(idx_h, 0) > Real(0)) ? hRs / qR(idx_h, 0) : Real(0.0);
+            for (int d = 0; d < dim; ++d) {
+                qL(idx_h + 1 + d, 0) *= sL;
+                qR(idx_h + 1 + d, 0) *= sR;
+            }
+            qL(idx_h, 0) = hLs;
+            qR(idx_h, 0) = hRs;
+            qL(idx_b, 0) = bf;
+            qR(idx_b, 0) = bf;
+        }
+        if (hLstar) *hLstar = hLs;
+        if (hRstar) *hRstar = hRs;
+        for (int n = 0; n < na; ++n) {
+            aL(n, 0) = fa(iL, jL, n);
+            aR(n, 0) = fa(iR, jR, n);
+        }
+        aL = Model::update_aux_variables(qL, aL, p, time, cell_center(iL, jL));
+        aR = Model::update_aux_variables(qR, aR, p, time, cell_center(iR, jR));
+        flux = Numerics::numerical_flux(qL, qR, aL, aR, p, n_hat);
+        fluct = Numerics::numerical_fluctuations(qL, qR, aL, aR, p, n_hat);
+    }
+
+    static Real face_divergence(Real f_r, Real f_l, Real dm_r, Real dp_l) {  // make_rhs.H:142-147
+        return -(f_r - f_l) - dm_r - dp_l;
+    }
+
+    void add_wb_bed_source(Fab& R, int i, int j, Real hL_xr, Real hR_xl, Real hL_yr, Real hR_yl) const {
+        R(i, j, idx_h + 1) += 0.5 * p(0, 0) * (hL_xr * hL_xr - hR_xl * hR_xl) / dx[0];  // make_rhs.H:160-162
+        if constexpr (dim == 2) R(i, j, idx_h + 2) += 0.5 * p(0, 0) * (hL_yr * hL_yr - hR_yl * hR_yl) / dx[1];
+    }
+
+    // make_rhs.H:234-290 (cell-centric fused kernel).
+    void compute_cell_rhs(int b, int i, int j, int order, Real time) {
+        const Fab& fq = Q.fabs[b];
+        const Fab& fa = Qaux.fabs[b];
+        Fab& R = Qtmp.fabs[b];
+        VQ fxr, fxl, fyr{}, fyl{};
+        amrex::SmallMatrix<Real, 2 * nq, 1> dxr, dxl, dyr{}, dyl{};
+        Real hL_xr = 0.0, hR_xl = 0.0, hL_yr = 0.0, hR_yl = 0.0, dum;
+        eval_face(fq, fa, i, j, i + 1, j, 0, order, fxr, dxr, time, s.well_balanced, &hL_xr, &dum);
+        eval_face(fq, fa, i - 1, j, i, j, 0, order, fxl, dxl, time, s.well_balanced, &dum, &hR_xl);
+        if constexpr (dim == 2) {
+            eval_face(fq, fa, i, j, i, j + 1, 1, order, fyr, dyr, time, s.well_balanced, &hL_yr, &dum);
+            eval_face(fq, fa, i, j - 1, i, j, 1, order, fyl, dyl, time, s.well_balanced, &dum, &hR_yl);
+        }
+        VQ qc, src;
+        VA ac;
+        for (int n = 0; n < nq; ++n) qc(n, 0) = fq(i, j, n);
+        for (int n = 0; n < na; ++n) ac(n, 0) = fa(i, j, n);
+        for (int n = 0; n < nq; ++n) src(n, 0) = 0.0;
+        if (!s.implicit_source) src = Model::source(qc, ac, p, time, cell_center(i, j));
+        for (int n = 0; n < nq; ++n) {
+            Real F_x = face_divergence(fxr(n, 0), fxl(n, 0), dxr(nq + n, 0), dxl(n, 0));
+            Real F_y = 0.0;
+            if constexpr (dim == 2) F_y = face_divergence(fyr(n, 0), fyl(n, 0), dyr(nq + n, 0), dyl(n, 0));
+            R(i, j, n) = F_x / dx[0] + F_y / dx[1];
+            if (!s.implicit_source) R(i, j, n) += src(n, 0);
+        }
+        if (s.well_balanced) add_wb_bed_source(R, i, j, hL_xr, hR_xl, hL_yr, hR_yl);
+    }
+
+    // make_rhs.H:320-342 (face pass) and :347-399 (cell pass).
+    void compute_face_data(int b, int i, int j, int dir, int order, Real time) {
+        const Fab& fq = Q.fabs[b];
+        const Fab& fa = Qaux.fabs[b];
+        Fab& F = (dir == 0) ? Fx.fabs[b] : Fy.fabs[b];
+        VQ flux;
+        amrex::SmallMatrix<Real, 2 * nq, 1> fl;
+        Real hLs = 0.0, hRs = 0.0;
+        const int iL = (dir == 0) ? i - 1 : i;
+        const int jL = (dir == 1) ? j - 1 : j;
+        eval_face(fq, fa, iL, jL, i, j, dir, order, flux, fl, time, s.well_balanced, &hLs, &hRs);
+        for (int n = 0; n < nq; ++n) F(i, j, n) = flux(n, 0);
+        for (int n = 0; n < 2 * nq; ++n) F(i, j, nq + n) = fl(n, 0);
+        F(i, j, 3 * nq) = hLs;
+        F(i, j, 3 * nq + 1) = hRs;
+    }
+
+    void assemble_cell_rhs(int b, int i, int j, Real time) {
+        const Fab& fq = Q.fabs[b];
+        const Fab& fa = Qaux.fabs[b];
+        const Fab& FX = Fx.fabs[b];
+        const Fab& FY = Fy.fabs[b];
+        Fab& R = Qtmp.fabs[b];
+        VQ qc, src;
+        VA ac;
+        for (int n = 0; n < nq; ++n) qc(n, 0) = fq(i, j, n);
+        for (int n = 0; n < na; ++n) ac(n, 0) = fa(i, j, n);
+        for (int n = 0; n < nq; ++n) src(n, 0) = 0.0;
+        if (!s.implicit_source) src = Model::source(qc, ac, p, time, cell_center(i, j));
+        for (int n = 0; n < nq; ++n) {
+            Real F_x = face_divergence(FX(i + 1, j, n), FX(i, j, n), FX(i + 1, j, 2 * nq + n), FX(i, j, nq + n));
+            Real F_y = 0.0;
+            if constexpr (dim == 2)
+                F_y = face_divergence(FY(i, j + 1, n), FY(i, j, n), FY(i, j + 1, 2 * nq + n), FY(i, j, nq + n));
+            R(i, j, n) = F_x / dx[0] + F_y / dx[1];
+            if (!s.implicit_source) R(i, j, n) += src(n, 0);
+        }
+        if (s.well_balanced) {
+            const Real hL_xr = FX(i + 1, j, 3 * nq);
+            const Real hR_xl = FX(i, j, 3 * nq + 1);
+            Real hL_yr = 0.0, hR_yl = 0.0;
+            if constexpr (dim == 2) {
+                hL_yr = FY(i, j + 1, 3 * nq);
+                hR_yl = FY(i, j, 3 * nq + 1);
+            }
+            add_wb_bed_source(R, i, j, hL_xr, hR_xl, hL_yr, hR_yl);
+        }
+    }
+
+    // make_rhs.H:429-509: ONE point-implicit step with Gaussian elimination.
+    void apply_implicit_source(Fab& fq, const Fab& fa, int i, int j, Real dt, Real time) const {
+        VQ q_old;
+        VA a_in;
+        for (int n = 0; n < nq; ++n) q_old(n, 0) = fq(i, j, n);
+        for (int n = 0; n < na; ++n) a_in(n, 0) = fa(i, j, n);
+        const VX X = cell_center(i, j);
+        auto S = Model::source(q_old, a_in, p, time, X);
+        auto dS_dQ = Model::source_jacobian_wrt_variables(q_old, a_in, p);
+        auto dS_dA = Model::source_jacobian_wrt_aux_variables(q_old, a_in, p);
+        auto dA_dQ = Model::update_aux_variables_jacobian_wrt_variables(q_old, a_in, p, time, X);
+        amrex::SmallMatrix<Real, nq, nq> A;
+        VQ rhs;
+        for (int r = 0; r < nq; ++r) {
+            rhs(r, 0) = dt * S(r, 0);
+            for (int c = 0; c < nq; ++c) {
+                Real term = dS_dQ(r * nq + c, 0);
+                for (int k = 0; k < na; ++k) term += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
+                A(r, c) = (r == c ? 1.0 : 0.0) - dt * term;
+            }
+        }
+        VQ dq;
+        if (ge_solve<nq>(A, rhs, dq)) {
+            for (int n = 0; n < nq; ++n) fq(i, j, n) = q_old(n, 0) + dq(n, 0);
+            return;
+        }
+        for (int n = 0; n < nq; ++n) {
+            Real v = q_old(n, 0) + dt * S(n, 0);
+            fq(i, j, n) = std::isfinite(v) ? v : q_old(n, 0);
+        }
+    }
+
+    // ZoomyAmr::ComputeDt (ZoomyAmr.cpp:448-515), returning max_inv and dt.
+    Real compute_max_inv() {
+        Real gmax = 0.0;
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(max : gmax)
+        for (int b = 0; b < nb; ++b) {
+            const Fab& fq = Q.fabs[b];
+            const Fab& fa = Qaux.fabs[b];
+            for (int j = fq.vb.lo[1]; j <= fq.vb.hi[1]; ++j)
+                for (int i = fq.vb.lo[0]; i <= fq.vb.hi[0]; ++i) {
+                    VQ q;
+                    VA a;
+                    for (int n = 0; n < nq; ++n) q(n, 0) = fq(i, j, n);
+                    for (int n = 0; n < na; ++n) a(n, 0) = fa(i, j, n);
+                    Real mi = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        VN nh{};
+                        nh(d, 0) = 1.0;
+                        auto ev = Numerics::local_max_abs_eigenvalue(q, a, p, nh);
+                        Real inv = ev(0, 0) / dx[d];
+                        if (inv > mi) mi = inv;
+                    }
+                    gmax = std::max(gmax, mi);
+                }
+        }
+        return gmax;
+    }
+    Real compute_dt() {
+        Real gmax = compute_max_inv();
+        if (gmax < 1e-14) return s.dtmax;
+        const Real div = Real(s.cfl_dimension) * Real(s.cfl_degree_factor);
+        Real dt = s.cfl / (div * gmax);
+        return amrex::max(amrex::min(dt, s.dtmax), s.dtmin);
+    }
+
+    Real total_mass() const {  // ComputeTotalMass, single level, serial sum in box order
+        Real tot = 0.0;
+        for (auto& f : Q.fabs)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i) tot += f(i, j, idx_h);
+        return tot * (dx[0] * dx[1]);
+    }
+
+    // do_stage (ZoomyAmr.cpp:561-662).
+    void do_stage(int ord, Real time, Real dt) {
+        update_state(time);
+        fill_patch(time);
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+        for (int b = 0; b < nb; ++b) {
+            const Box& vb = boxes[b];
+            if (!s.face_based_rhs) {
+                for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)
+                    for (int i = vb.lo[0]; i <= vb.hi[0]; ++i) compute_cell_rhs(b, i, j, ord, time);
+            } else {
+                for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)
+                    for (int i = vb.lo[0]; i <= vb.hi[0] + 1; ++i) compute_face_data(b, i, j, 0, ord, time);
+                if constexpr (dim == 2)
+                    for (int j = vb.lo[1]; j <= vb.hi[1] + 1; ++j)
+                        for (int i = vb.lo[0]; i <= vb.hi[0]; ++i) compute_face_data(b, i, j, 1, ord, time);
+                for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)
+                    for (int i = vb.lo[0]; i <= vb.hi[0]; ++i) assemble_cell_rhs(b, i, j, time);
+            }
+            Fab& fq = Q.fabs[b];
+            const Fab& R = Qtmp.fabs[b];
+            for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)  // ConsUpdate
+                for (int i = vb.lo[0]; i <= vb.hi[0]; ++i)
+                    for (int n = 0; n < nq; ++n) fq(i, j, n) += dt * R(i, j, n);
+            if (s.implicit_source) {
+                const Fab& fa = Qaux.fabs[b];
+                for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)
+                    for (int i = vb.lo[0]; i <= vb.hi[0]; ++i) apply_implicit_source(fq, fa, i, j, dt, time);
+            }
+        }
+    }
+
+    void heun() {  // ZoomyAmr.cpp:665-675
+        const int nb = (int)boxes.size();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+        for (int b = 0; b < nb; ++b) {
+            Fab& fq = Q.fabs[b];
+            const Fab& fo = Qold.fabs[b];
+            for (int j = fq.vb.lo[1]; j <= fq.vb.hi[1]; ++j)
+                for (int i = fq.vb.lo[0]; i <= fq.vb.hi[0]; ++i)
+                    for (int n = 0; n < nq; ++n) fq(i, j, n) = 0.5 * fo(i, j, n) + 0.5 * fq(i, j, n);
+        }
+    }
+
+    void copy_valid(MultiFab& dst, const MultiFab& src) {
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            Fab& d = dst.fabs[b];
+            const Fab& sfab = src.fabs[b];
+            for (int n = 0; n < nq; ++n)
+                for (int j = d.vb.lo[1]; j <= d.vb.hi[1]; ++j)
+                    for (int i = d.vb.lo[0]; i <= d.vb.hi[0]; ++i) d(i, j, n) = sfab(i, j, n);
+        }
+    }
+
+    long troubled_cells() const {  // MOOD mask, ZoomyAmr.cpp:700-711
+        long c = 0;
+        for (auto& f : Q.fabs)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i) {
+                    Real h = f(i, j, idx_h);
+                    if (!(h >= 0.0) || !(h < 1.0e300)) ++c;
+                }
+        return c;
+    }
+
+    // ZoomyAmr::Advance (ZoomyAmr.cpp:538-731).
+    void advance(Real time, Real dt) {
+        if (s.spatial_order == 1) {
+            do_stage(1, time, dt);
+        } else {
+            copy_valid(Qold, Q);
+            do_stage(s.spatial_order, time, dt);
+            do_stage(s.spatial_order, time, dt);
+            heun();
+            if (s.mood) {
+                long nt = troubled_cells();
+                if (nt > 0) {
+                    ++mood_redos;
+                    copy_valid(Q, Qold);
+                    do_stage(1, time, dt);
+                    do_stage(1, time, dt);
+                    heun();
+                }
+            }
+        }
+        update_state(time);
+    }
+
+    // ZoomyAmr::TimeStep lev 0 + the Evolve loop (ZoomyAmr.cpp:733-816),
+    // without plotfiles.  Returns the number of steps taken.
+    int evolve(Real* time_out, std::vector<Real>* dts = nullptr, std::vector<Real>* masses = nullptr,
+               int step_limit = -1) {
+        Real time = 0.0;
+        int step = 0;
+        while (time < s.time_end && (s.max_step < 0 || step < s.max_step)) {
+            if (step_limit >= 0 && step >= step_limit) break;
+            Real dt = compute_dt();
+            const Real remaining = s.time_end - time;
+            if (remaining > 0.0 && dt > remaining) dt = remaining;
+            advance(time, dt);
+            time += dt;
+            ++step;
+            if (dts) dts->push_back(dt);
+            if (masses) masses->push_back(total_mass());
+        }
+        if (time_out) *time_out = time;
+        return step;
+    }
+};
+
+}  // namespace zorc

@@ -1,0 +1,277 @@
+#pragma once
+// Shallow-water system, one horizontal direction: SME(level=0, dimension=2)
+// with Newtonian, Manning and stress-free closures, in the interface
+// zoomy-amrex's AmrexSystemModelPrinter emits
+// (zoomy_amrex/transformation/to_amrex.py:326-516).  This is the model behind
+// the reference's 1-D golden fixtures (tests/refs/{stoker_wet,ritter_dry,
+// wb_lake,parallel_2rank,parity_numpy_n*}.npz), which it reproduces to round-off.
+//
+// State  Q    = [b, h, hu]
+// Aux    Qaux = [a0, a1, a2 (identity rows), hinv]
+// Params p    = [g, ex, ey, ez, rho, wet_dry_eps, nu, n_m, lambda_s]
+// Formulas as in the two-direction system: KP-desingularised hinv, Rusanov
+// flux [0, hu, hu u + g h^2/2], |lam| = |u| + sqrt(g max(h,0)), Manning friction.
+#include <AMReX_Array4.H>
+#include <AMReX_Vector.H>
+#include <AMReX_SmallMatrix.H>
+#include <vector>
+#include <string>
+#include <algorithm>
+#include <cstring>
+
+#if defined(__CUDACC__) || defined(__HIP__)
+#define PORTABLE_FN __host__ __device__
+#else
+#define PORTABLE_FN
+#endif
+
+#include "UserFunctions.H"
+
+struct Model {
+    using T = amrex::Real;
+    static constexpr int n_dof_q    = 3;
+    static constexpr int n_dof_eq   = 3;
+    static constexpr int n_dof_qaux = 4;
+    static constexpr int n_parameters = 9;
+    static constexpr int dimension  = 1;
+    static constexpr int n_dof_gradQ = 3;
+    static constexpr bool has_diffusion = false;
+    static constexpr int n_boundary_tags = 3;
+    static constexpr bool zamr_aux_is_local = true;
+    static constexpr int zamr_idx_hinv = 3;
+    static const std::vector<std::string> get_boundary_tags() { return { "East", "West", "wall" }; }
+    static const std::vector<std::string> parameter_names() { return { "g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s" }; }
+    static const std::vector<T> default_parameters() { return { 9.81, 0.0, 0.0, 1.0, 1.0, 0.0001, 0.0, 0.0, 0.0 }; }
+
+    using VQ = amrex::SmallMatrix<amrex::Real, 3, 1>;
+    using VA = amrex::SmallMatrix<amrex::Real, 4, 1>;
+    using VP = amrex::SmallMatrix<amrex::Real, 9, 1>;
+    using VN = amrex::SmallMatrix<amrex::Real, 1, 1>;
+    using VX = amrex::SmallMatrix<amrex::Real, 3, 1>;
+    // x^(-1/3) for x > 0 from IEEE + - * only: exponent-halving seed, then
+    // Newton on y^-3 = x.  Deterministic, so host and device agree bitwise.
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_rcbrt(amrex::Real x)
+    {
+        unsigned long long bits;
+        __builtin_memcpy(&bits, &x, sizeof(bits));
+        bits = 0x553ef0ff289dd796ULL - bits / 3ULL;
+        amrex::Real y;
+        __builtin_memcpy(&y, &bits, sizeof(y));
+        const amrex::Real third = 1.0 / 3.0;
+        for (int it = 0; it < 4; ++it) {
+            const amrex::Real y3 = y * y * y;
+            y = y * (4.0 - x * y3) * third;
+        }
+        return y;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_hinv(amrex::Real h, amrex::Real eps)
+    {
+        const amrex::Real h4 = amrex::Math::powi<4>(h);
+        const amrex::Real e4 = amrex::Math::powi<4>(eps);
+        return 1.4142135623730951 * h / std::sqrt(h4 + amrex::max(h4, e4));
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ flux(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = VQ{};
+        const amrex::Real u = Q(2, 0) * Qaux(3, 0);
+        res(0, 0) = 0.0;
+        res(1, 0) = Q(2, 0);
+        res(2, 0) = Q(2, 0) * u + 0.5 * p(0, 0) * Q(1, 0) * Q(1, 0);
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ dflux(VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ hydrostatic_pressure(VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> nonconservative_matrix(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = amrex::SmallMatrix<amrex::Real, 9, 1>{};
+        res(2 * 3 + 0, 0) = p(0, 0) * Q(1, 0);
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> quasilinear_matrix(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = amrex::SmallMatrix<amrex::Real, 9, 1>{};
+        const amrex::Real u = Q(2, 0) * Qaux(3, 0);
+        res(1 * 3 + 2, 0) = 1.0;
+        res(2 * 3 + 0, 0) = p(0, 0) * Q(1, 0);
+        res(2 * 3 + 1, 0) = p(0, 0) * Q(1, 0) - u * u;
+        res(2 * 3 + 2, 0) = 2.0 * u;
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ diffusive_flux(
+        VQ const& Q, VA const& Qaux, amrex::SmallMatrix<amrex::Real, 3, 1> const& gradQ, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ eigenvalues(
+        VQ const& Q, VA const& Qaux, VP const& p, VN const& n)
+    {
+        auto res = VQ{};
+        const amrex::Real un = Q(2, 0) * n(0, 0) * Qaux(3, 0);
+        const amrex::Real c = std::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
+        res(0, 0) = 0.0;
+        res(1, 0) = un - c;
+        res(2, 0) = un + c;
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> left_eigenvectors(
+        VQ const& Q, VA const& Qaux, VP const& p, VN const& n)
+    {
+        return amrex::SmallMatrix<amrex::Real, 9, 1>{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> right_eigenvectors(
+        VQ const& Q, VA const& Qaux, VP const& p, VN const& n)
+    {
+        return amrex::SmallMatrix<amrex::Real, 9, 1>{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ source(
+        VQ const& Q, VA const& Qaux, VP const& p, amrex::Real const time, VX const& X)
+    {
+        auto res = VQ{};
+        const amrex::Real u = Q(2, 0) * Qaux(3, 0);
+        const amrex::Real speed = std::sqrt(u * u);
+        const amrex::Real r = zamr_rcbrt(amrex::max(Q(1, 0), p(5, 0)));
+        const amrex::Real k = p(0, 0) * p(7, 0) * p(7, 0) * speed * amrex::Math::powi<4>(r);
+        res(0, 0) = 0.0;
+        res(1, 0) = 0.0;
+        res(2, 0) = -k * Q(2, 0);
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> source_jacobian_wrt_variables(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = amrex::SmallMatrix<amrex::Real, 9, 1>{};
+        const amrex::Real hinv = Qaux(3, 0);
+        const amrex::Real u = Q(2, 0) * hinv;
+        const amrex::Real speed = std::sqrt(u * u);
+        const amrex::Real hf = amrex::max(Q(1, 0), p(5, 0));
+        const amrex::Real r = zamr_rcbrt(hf);
+        const amrex::Real c = p(0, 0) * p(7, 0) * p(7, 0) * amrex::Math::powi<4>(r);
+        const amrex::Real k = c * speed;
+        const amrex::Real dh = Q(1, 0) - p(5, 0);
+        const amrex::Real heav = (dh > 0.0) ? amrex::Real(1.0) : ((dh < 0.0) ? amrex::Real(0.0) : amrex::Real(0.5));
+        const amrex::Real dk_dh = -(4.0 / 3.0) * k / hf * heav;
+        const amrex::Real ws = (speed > 0.0) ? hinv / speed : amrex::Real(0.0);
+        const amrex::Real dk_dhu = c * u * ws;
+        res(2 * 3 + 1, 0) = -Q(2, 0) * dk_dh;
+        res(2 * 3 + 2, 0) = -k - Q(2, 0) * dk_dhu;
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 12, 1> source_jacobian_wrt_aux_variables(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = amrex::SmallMatrix<amrex::Real, 12, 1>{};
+        const amrex::Real m = std::sqrt(Q(2, 0) * Q(2, 0));
+        const amrex::Real r = zamr_rcbrt(amrex::max(Q(1, 0), p(5, 0)));
+        const amrex::Real c = p(0, 0) * p(7, 0) * p(7, 0) * amrex::Math::powi<4>(r);
+        res(2 * 4 + 3, 0) = -c * m * Q(2, 0);
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 6, 1> project_2d_to_3d(
+        VX const& X, VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return amrex::SmallMatrix<amrex::Real, 6, 1>{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ project_3d_to_2d(VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ residual(
+        amrex::Real const time, VX const& X, amrex::Real const dX, VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ interpolate(VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return Q;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ initial_condition(VX const& X, VP const& p)
+    {
+        return VQ{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VA initial_aux_condition(VX const& X, VP const& p)
+    {
+        return VA{};
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ update_variables(VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        return Q;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VA update_aux_variables(
+        VQ const& Q, VA const& Qaux, VP const& p, amrex::Real const time, VX const& X)
+    {
+        VA res = Qaux;
+        res(3, 0) = zamr_hinv(Q(1, 0), p(5, 0));
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 9, 1> update_variables_jacobian_wrt_variables(
+        VQ const& Q, VA const& Qaux, VP const& p)
+    {
+        auto res = amrex::SmallMatrix<amrex::Real, 9, 1>{};
+        for (int r = 0; r < 3; ++r) res(r * 3 + r, 0) = 1.0;
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 12, 1> update_aux_variables_jacobian_wrt_variables(
+        VQ const& Q, VA const& Qaux, VP const& p, amrex::Real const time, VX const& X)
+    {
+        return amrex::SmallMatrix<amrex::Real, 12, 1>{};
+    }
+
+    // Tags sorted: East, West (extrapolation) and wall (reflective).
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VQ boundary_conditions(
+        const int bc_idx, VQ const& Q, VA const& Qaux, VN const& n, VX const& X,
+        amrex::Real const time, amrex::Real const dX)
+    {
+        VQ res = Q;
+        if (bc_idx == 2) {
+            const amrex::Real qn = Q(2, 0) * n(0, 0);
+            res(2, 0) = Q(2, 0) - 2.0 * qn * n(0, 0);
+        }
+        return res;
+    }
+
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE VA aux_boundary_conditions(
+        const int bc_idx, VQ const& Q, VA const& Qaux, VN const& n, VX const& X,
+        amrex::Real const time, amrex::Real const dX)
+    {
+        return Qaux;
+    }
+
+    static constexpr int n_e2s = 3;
+    static constexpr int e2s[3] = { 0, 1, 2 };
+    static constexpr int n_deriv_aux = 0;
+    static constexpr int deriv_aux[1][4] = {{0,0,0,0}};
+    static constexpr int n_input_aux = 0;
+    static constexpr int input_aux[1][4] = {{0,0,0,0}};
+};

@@ -1,0 +1,142 @@
+"""Benchmark: cell-updates per second per RK stage (FP64) of the single-level
+2-D shallow-water advance on one MI355X, with the HBM-roofline fraction of
+the dominant kernel and the CPU oracle timed beside it.
+
+Workload: 4096^2 cells on [0,1]^2, bathymetry = 32 seeded Gaussian bumps,
+eta = 1 + 1e-2 random-phase perturbation, dry where b > eta, Manning
+n = 0.03 through the point-implicit source, well-balanced order-2 MUSCL /
+Rusanov, SSP-RK2.  A "step" is one SSP-RK2 step (two RK stages).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import pathlib
+import re
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "zoomy-amrex_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+N_Q = 4            # state rows b, h, hu, hv
+BYTES_STAGE1 = 16 * N_Q  # read U^n, write U^(1)
+BYTES_STAGE2 = 24 * N_Q  # read U^(1) and U^n, write U^(n+1)
+
+
+def parse_traffic(n_cells):
+    """HBM bytes per stage-2 launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE in KiB, corrected by the copy-kernel factor the
+    summary states), or None when no summary is present."""
+    f = ROOT / "profiles" / "stage2_swe4096_pmc.txt"
+    if not f.exists():
+        return None
+    txt = f.read_text()
+    m = re.search(r"stage2_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(case, Q0, threads):
+    """The oracle (C++ restatement of the reference, OpenMP over boxes) on one
+    SSP-RK2 step of the same 4096^2 workload, boxes of 256^2."""
+    from oracle.pyoracle import Oracle
+    c = dict(case, max_grid_size=256, max_step=1)
+    o = Oracle("swe2d", c, threads=threads)
+    o.init_solution()
+    o.set_state(Q0)
+    t0 = time.perf_counter()
+    n, _, _, _ = o.evolve()
+    el = time.perf_counter() - t0
+    cells = case["nx"] * case["ny"]
+    return dict(value=2 * n * cells / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"1 SSP-RK2 step (2 stages) of the same {case['nx']}^2 workload, {el:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("this benchmark runs on one GPU only; the multi-GPU decomposition is a design in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from zamr import cases
+    from zamr.deck import deck_from_case
+    from zamr.lib import ZoomyAmr
+
+    case, Q0 = cases.swe_bumps(n=args.n, seed=0, steps=args.warmup + args.steps)
+    z = ZoomyAmr(deck_from_case(case), system="swe2d", dim=2, order=2)
+    z.init_data()
+    z.set_state(Q0)
+    if args.rows:
+        z.set_tuning(rows=args.rows)
+    if args.warmup:
+        z.run_steps(args.warmup)
+    torch.cuda.synchronize()
+
+    z.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, _ = z.run_steps(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ktimes, nk = z.kernel_times()
+    z.set_profiling(False)
+    assert done == args.steps, (done, args.steps)
+    Q = z.get_state()
+    assert np.isfinite(Q).all()
+
+    cells = args.n * args.n
+    cu = 2 * args.steps * cells
+    value = cu / el
+    t2 = ktimes[1] * 1e-3
+    t1 = ktimes[0] * 1e-3
+    achieved = BYTES_STAGE2 * cells / t2 if t2 > 0 else 0.0
+    traffic = parse_traffic(cells)
+    out = {
+        "metric": "cell-updates/s per RK stage (FP64) + % HBM roofline; adjoint/forward time ratio",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2-D SWE {args.n}x{args.n}, 32 seeded Gaussian bumps, wet/dry, Manning 0.03 "
+                               "point-implicit, well-balanced order-2 MUSCL/Rusanov, SSP-RK2",
+                   "nx": args.n, "ny": args.n, "steps": args.steps, "seed": 0},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "stage_kernel<swe2d, order 2, stage 2>",
+                     "algorithmic_bytes_per_launch": BYTES_STAGE2 * cells,
+                     "avg_launch_ms": ktimes[1], "stage1_avg_launch_ms": ktimes[0],
+                     "stage1_frac": (BYTES_STAGE1 * cells / t1) / HBM_PEAK if t1 > 0 else 0.0,
+                     "step_frac": ((BYTES_STAGE1 + BYTES_STAGE2) * cells / (t1 + t2)) / HBM_PEAK if t1 + t2 > 0
+                     else 0.0,
+                     "timed_launches": nk},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        del z
+        out["cpu_baseline"] = cpu_baseline(case, Q0, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

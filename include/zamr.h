@@ -1,0 +1,139 @@
+/*
+ * zamr — C-ABI of the MI355X per-box RHS / time-advance library.
+ *
+ * One shared library is built per generated system (Model.H + Numerics.H +
+ * MarchConstants.H, exactly what zoomy_amrex's printers emit), the way the
+ * reference compiles one executable per system (zoomy_amrex/run_case.py:
+ * 210-289, 390-398).  Every entry point returns 0 on success and a non-zero
+ * ZAMR_E* code on failure; zamr_last_error() gives the message.  No exception
+ * crosses the ABI.  A context is driven by one host thread; device work is
+ * asynchronous on the context's stream and entry points that return host
+ * data synchronise it.
+ *
+ * Array convention: dense float64 arrays of shape (ncomp, ny, nx), i fastest
+ * — the raster order of Source/init_solution.cpp:63 (idx = i + nx*j).
+ *
+ * The reference has no C-ABI; its in-process analogue is
+ *   ZoomyAmr solver; solver.InitData(); solver.Evolve();   (Source/main.cpp:23-39)
+ * and each entry below names the member function it replaces.
+ */
+#ifndef ZAMR_H
+#define ZAMR_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zamr_ctx zamr_ctx;
+
+enum {
+    ZAMR_OK = 0,
+    ZAMR_EINVAL = 1,    /* bad argument / deck value */
+    ZAMR_ETAG = 2,      /* boundary tag not declared by the model (ZoomyAmr.cpp:361-372) */
+    ZAMR_ERASTER = 3,   /* raster missing or too small (init_solution.cpp:20-40) */
+    ZAMR_EHIP = 4,      /* HIP runtime failure */
+    ZAMR_EUNSUP = 5,    /* a deck option outside this build (e.g. amr.max_level > 0) */
+    ZAMR_EIO = 6        /* plotfile / file write failure */
+};
+
+/* One box of an independent box list (batched mode).  Each box is its own
+ * physical domain: lo/hi are its valid cells, dx/prob_lo its geometry, group
+ * the time-step group it shares dt with (an AMR level or an ensemble member),
+ * bc the boundary tag names of its x_lo, x_hi, y_lo, y_hi sides ("" = tag 0). */
+typedef struct zamr_box {
+    int lo[2], hi[2];
+    double dx[2], prob_lo[2];
+    int group;
+    const char* bc[4];
+} zamr_box;
+
+/* Model introspection: Model::n_dof_q, n_dof_qaux, n_parameters, dimension,
+ * n_boundary_tags (to_amrex.py:348-360). */
+int zamr_model_info(int* n_dof_q, int* n_dof_qaux, int* n_parameters, int* dimension, int* n_tags);
+/* Model::get_boundary_tags()[k] / parameter_names()[k] / default_parameters()[k]. */
+int zamr_tag_name(int k, char* buf, int cap);
+int zamr_param_name(int k, char* buf, int cap);
+int zamr_default_param(int k, double* value);
+/* ZoomyConstants::c_cfl_dimension and c_cfl_degree_factor compiled in (_constants.py:59-88). */
+int zamr_march_constants(int* c_cfl_dimension, int* c_cfl_degree_factor);
+
+/* ZoomyAmr::ZoomyAmr() (ZoomyAmr.cpp:27-107): reads the ParmParse deck text
+ * written by build.py:write_inputs (build.py:55-120).  Keys outside this build
+ * (amr.max_level > 0, solver.implicit_global = true) fail with ZAMR_EUNSUP. */
+int zamr_create(const char* inputs_text, zamr_ctx** out);
+/* Batched mode: nbox independent boxes sharing one launch per RK stage.
+ * inputs_text supplies the solver.* / params.* keys (geometry keys ignored). */
+int zamr_create_boxlist(const zamr_box* boxes, int nbox, const char* inputs_text, zamr_ctx** out);
+int zamr_destroy(zamr_ctx* ctx);
+const char* zamr_last_error(const zamr_ctx* ctx);
+
+/* ZoomyAmr::InitData (ZoomyAmr.cpp:112-156): init_solution on Q and Qaux
+ * (init_solution.cpp:78-115), then the init.* rasters. */
+int zamr_init_data(zamr_ctx* ctx);
+
+/* Geometry of level 0 (or of box `box` in batched mode). */
+int zamr_level_shape(const zamr_ctx* ctx, int lev, int* nx, int* ny, int* nboxes);
+
+/* Whole-level state copy in/out (field 0 = Q, 1 = Qaux), dense (ncomp, ny, nx).
+ * In batched mode lev is the box index. n = number of doubles in host. */
+int zamr_set_state(zamr_ctx* ctx, int lev, int field, const double* host, size_t n);
+int zamr_get_state(zamr_ctx* ctx, int lev, int field, double* host, size_t n);
+
+/* ZoomyAmr::ComputeDt (ZoomyAmr.cpp:448-515) over the state and aux as stored. */
+int zamr_compute_dt(zamr_ctx* ctx, int lev, double* dt);
+/* ZoomyAmr::UpdateState (ZoomyAmr.cpp:293-334). */
+int zamr_update_state(zamr_ctx* ctx, int lev, double time);
+/* ZoomyAmr::FillPatch (ZoomyAmr.cpp:258-277, single level): same-level ghost
+ * exchange plus physical boundary conditions, written into the ghost cells. */
+int zamr_fill_patch(zamr_ctx* ctx, int lev, double time);
+/* One RK stage, do_stage(order) of ZoomyAmr::Advance (ZoomyAmr.cpp:561-662),
+ * applied to the current state in place. */
+int zamr_stage(zamr_ctx* ctx, int lev, int order, double time, double dt);
+/* ZoomyAmr::Advance (ZoomyAmr.cpp:538-731) with a caller-given dt. */
+int zamr_advance(zamr_ctx* ctx, int lev, double time, double dt);
+/* ZoomyAmr::TimeStep (ZoomyAmr.cpp:733-762): ComputeDt, time_end cap, Advance.
+ * On return *dt_taken holds the step size. */
+int zamr_time_step(zamr_ctx* ctx, int lev, double time, int iteration, double* dt_taken);
+/* Up to nsteps iterations of the Evolve loop body (ZoomyAmr.cpp:782-810) with
+ * dt computed and applied on the device; steps are captured in a hipGraph.
+ * Stops early when time_end or max_step is reached. */
+int zamr_run_steps(zamr_ctx* ctx, int nsteps, int* steps_done, double* time);
+/* ZoomyAmr::Evolve (ZoomyAmr.cpp:767-816) including plotfiles and the
+ * "Initial total mass" / "Step" / "Total evolution time" lines on stdout. */
+int zamr_evolve(zamr_ctx* ctx);
+/* ZoomyAmr::ComputeTotalMass (ZoomyAmr.cpp:517-536). */
+int zamr_total_mass(zamr_ctx* ctx, double* mass);
+/* Time, step count and last dt of level 0 (of group `lev` in batched mode). */
+int zamr_get_time(const zamr_ctx* ctx, int lev, double* time, int* step, double* dt);
+/* Reset time/step of every group to zero (a fresh Evolve on the current state). */
+int zamr_reset_clock(zamr_ctx* ctx);
+
+/* ZoomyAmr::WritePlotFile (ZoomyAmr.cpp:821-882): AMReX plotfile
+ * plt_<identifier><step,5 digits> under dir (NULL = current directory). */
+int zamr_write_plotfile(zamr_ctx* ctx, const char* dir, int step, double time);
+
+/* Raw device pointer of a field of box `box` of level `lev` (field 0 = Q,
+ * 1 = Qaux), the grown-box corner of an AMReX Array4 layout, with its strides
+ * in elements, and the context's HIP stream (for the PyTorch wrapper). */
+int zamr_device_ptr(zamr_ctx* ctx, int lev, int box, int field, void** ptr, long long* row_stride,
+                    long long* comp_stride, int* ngrow, void** stream);
+/* Run on a caller-owned HIP stream (hipStream_t) instead of the context's own. */
+int zamr_set_stream(zamr_ctx* ctx, void* stream);
+
+/* Tuning: rows per work item for the stage kernels (0 = autotune per box size
+ * class on first use), per-box launches instead of one batched launch
+ * (baseline for the batching speed-up), and XCD-aware item order. */
+int zamr_set_tuning(zamr_ctx* ctx, int rows_per_item, int per_box_launch, int xcd_remap);
+/* Average device time (ms) per launch of each stage kernel over the last
+ * zamr_run_steps call with profiling on: out[0] stage 1, out[1] stage 2,
+ * out[2] per-step bookkeeping; returns the number of timed steps in *nsteps. */
+int zamr_set_profiling(zamr_ctx* ctx, int on);
+int zamr_kernel_times(zamr_ctx* ctx, double* out_ms, int* nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZAMR_H */

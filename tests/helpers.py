@@ -1,0 +1,91 @@
+"""Shared test helpers: reference fixture cases, comparisons, deck builders."""
+import pathlib
+
+import numpy as np
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden" / "refs"
+
+
+def rel_linf(a, b):
+    """max over rows of max|a - b| / max|b| (the per-row scale floored at 1e-300)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    worst = 0.0
+    for r in range(b.shape[0]):
+        scale = max(np.abs(b[r]).max(), 1e-300)
+        worst = max(worst, float(np.abs(a[r] - b[r]).max() / scale))
+    return worst
+
+
+def stoker_ic(x):  # reference tests/cases.py:195-204
+    h = np.where(x < 5.0, 0.005, 0.001)
+    return np.array([0 * x, h, 0 * x])
+
+
+def ritter_ic(x):  # reference tests/cases.py:207-210
+    h = np.where(x < 5.0, 0.005, 0.0)
+    return np.array([0 * x, h, 0 * x])
+
+
+def lake_ic(x):  # reference tests/cases.py:213-221
+    b = 0.2 * np.exp(-((x - 5.0) ** 2))
+    return np.array([b, 1.0 - b, 0 * b])
+
+
+# name -> (ic, nx, t_end, well_balanced, wall)   (reference tests/test_*.py)
+FIXTURES_1D = {
+    "stoker_wet": (stoker_ic, 100, 1.0, False, False),
+    "ritter_dry": (ritter_ic, 100, 1.0, False, False),
+    "wb_lake": (lake_ic, 100, 1.0, True, True),
+    "parity_numpy_n100": (stoker_ic, 100, 0.5, False, False),
+    "parity_numpy_n200": (stoker_ic, 200, 0.5, False, False),
+    "parallel_2rank": (stoker_ic, 128, 0.5, False, False),
+}
+
+
+def case_1d(nx, t_end, wb, wall):
+    """The deck run_case writes for a 1-D test (run_case.py:75-84, 235-288):
+    nx x 1 cells on [0,10] x [0,dx], order 1, CFL 0.9 with d = 1, dtmax = t_end,
+    clamp off, explicit source.  The reference's wall model names its sides
+    West/East with a wall definition; this system's reflective tag is 'wall'."""
+    L = 10.0
+    dx = L / nx
+    side = ("wall", "wall") if wall else ("West", "East")
+    return dict(nx=nx, ny=1, prob_lo=(0.0, 0.0), prob_hi=(L, dx), max_grid_size=64, spatial_order=1,
+                implicit_source=False, well_balanced=wb, clamp_positivity=False, cfl=0.9, dtmin=1e-7,
+                dtmax=t_end, time_end=t_end, max_step=-1, bc=(side[0], side[1], "", ""),
+                cfl_dimension=1, cfl_degree_factor=1)
+
+
+def case_2d_pulse():
+    """reference tests/test_swe_2d.py: 32^2 on [-1,1]^2, Gaussian pulse, t=0.1, CFL 0.9, order 1."""
+    n = 32
+    c = dict(nx=n, ny=n, prob_lo=(-1.0, -1.0), prob_hi=(1.0, 1.0), max_grid_size=64, spatial_order=1,
+             implicit_source=False, well_balanced=False, clamp_positivity=False, cfl=0.9, dtmin=1e-7,
+             dtmax=0.1, time_end=0.1, max_step=-1, bc=("West", "East", "South", "North"),
+             cfl_dimension=2, cfl_degree_factor=1)
+    dx = 2.0 / n
+    xc = -1 + (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc, indexing="xy")
+    h = 1.0 + 0.1 * np.exp(-(X ** 2 + Y ** 2) / 0.05)
+    z = 0 * h
+    return c, np.array([z, h, z, z])
+
+
+def fixture(name):
+    d = np.load(GOLDEN / f"{name}.npz")
+    return d["Q"], d["Qaux"]
+
+
+def smooth_state(nx, ny, seed=1, bed=True, amp=0.05):
+    """A smooth wet state with momentum and (optionally) a bed, for stage parity."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    b = 0.3 * np.exp(-((X - 0.4) ** 2 + (Y - 0.6) ** 2) / 0.02) if bed else 0 * X
+    eta = 1.0 + amp * np.sin(2 * np.pi * X + rng.uniform(0, 6)) * np.cos(2 * np.pi * Y + rng.uniform(0, 6))
+    h = np.maximum(eta - b, 0.0)
+    hu = h * 0.3 * np.sin(2 * np.pi * Y)
+    hv = h * 0.2 * np.cos(2 * np.pi * X)
+    return np.array([b, h, hu, hv])

@@ -1,0 +1,864 @@
+#pragma once
+// Host driver: the single-level stage/step logic of ZoomyAmr
+// (Source/ZoomyAmr.cpp) over device-resident SoA boxes, with dt, time and the
+// loop condition kept on the device so a step is four kernel launches that
+// can be captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "zamr_deck.hpp"
+#include "zamr_kernels.hpp"
+
+namespace zamr {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define ZAMR_HIP(call)                                                                                  \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::zamr::Error(4, std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+struct HostBox {
+    int lo[2], hi[2];
+    int ng;
+    int dlo[2], dhi[2];
+    double dx[2], plo[2];
+    int group;
+    int bc[4];
+    size_t off_q[2], off_aux;  // element offsets into the pool
+    int gx, gy;                // grown extents
+};
+
+// AMReX BoxList::maxSize chopping (max_grid_size): ceil(L/m) chunks per axis,
+// the first L % chunks of them one cell longer.
+inline std::vector<std::pair<int, int>> chop_axis(int L, int m) {
+    std::vector<std::pair<int, int>> r;
+    const int nb = (L + m - 1) / m;
+    const int sz = L / nb, extra = L % nb;
+    int s = 0;
+    for (int k = 0; k < nb; ++k) {
+        const int l = sz + (k < extra ? 1 : 0);
+        r.push_back({s, s + l - 1});
+        s += l;
+    }
+    return r;
+}
+
+template <class M, class N>
+class Solver {
+   public:
+    using S = Sys<M>;
+    static constexpr int nq = M::n_dof_q;
+    static constexpr int na = M::n_dof_qaux;
+    static constexpr int np = M::n_parameters;
+    static constexpr int dim = M::dimension;
+
+    // --- ZoomyAmr members (ZoomyAmr.H:93-159) ---
+    int nx = 0, ny = 0, max_grid_size = 64, max_level = 0;
+    double plo[2] = {0, 0}, phi[2] = {1, 1};
+    double time_end = 1.0;
+    int max_step = -1;
+    double cfl = 0.5, dtmin = 1.0e-7, dtmax = 1.0e-2;
+    int spatial_order = 1;
+    bool implicit_source = true, implicit_global = false, well_balanced = false, clamp_positivity = true;
+    std::string positivity = "none";
+    int face_based_rhs = 1;
+    double plot_dt_interval = 0.1;
+    int identifier = 0;
+    std::string bc_name[4];
+    std::string dem_file, release_file, friction_file;
+    std::vector<std::string> state_rasters;
+    Params prm{};
+    int c_cfl_dimension = 1, c_cfl_degree_factor = 1;
+
+    bool batched = false;
+    std::vector<HostBox> hb;
+    int ngroups = 1;
+
+    // --- device state ---
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* d_pool = nullptr;
+    size_t pool_elems = 0;
+    BoxDesc* d_boxes = nullptr;
+    GroupState* d_groups = nullptr;
+    std::vector<GroupState> h_groups;
+    CopyRect* d_rects = nullptr;
+    int nrects = 0;
+    double* d_hist = nullptr;
+    int hist_cap = 1 << 16;
+    double* d_partial = nullptr;
+    int partial_cap = 0;
+    int cur = 0;                    // buffer holding the current state
+    bool maxinv_fresh = false;      // group maxinv_bits already hold this state's reduction
+    bool aux_stale = false;         // stored Qaux lags the last UpdateState (steps ran since)
+    int rows_per_item = 0;          // 0 = autotune
+    int per_box_launch = 0;
+    int xcd_remap = 1;
+    int chosen_rows = 32;
+
+    struct Items {
+        int hx = -1, rows = -1;
+        WorkItem* d = nullptr;
+        int n = 0;
+        std::vector<int> box_begin;  // item range per box (per-box launch mode)
+    };
+    Items items_[3];
+
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key = -1;
+
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    double prof_ms[3] = {0, 0, 0};
+    int prof_steps = 0;
+
+    int plot_step = 0;
+    double next_plot_time = 0.0;
+
+    ~Solver() { release(); }
+
+    void release() {
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        for (auto& it : items_)
+            if (it.d) hipFree(it.d), it.d = nullptr;
+        if (d_pool) hipFree(d_pool);
+        if (d_boxes) hipFree(d_boxes);
+        if (d_groups) hipFree(d_groups);
+        if (d_rects) hipFree(d_rects);
+        if (d_hist) hipFree(d_hist);
+        if (d_partial) hipFree(d_partial);
+        d_pool = nullptr;
+        d_boxes = nullptr;
+        d_groups = nullptr;
+        d_rects = nullptr;
+        d_hist = nullptr;
+        d_partial = nullptr;
+        for (auto e : ev) hipEventDestroy(e);
+        ev.clear();
+        if (own_stream && stream) hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // ---------------- configuration ----------------
+    static int tag_index(const std::string& name) {  // ZoomyAmr.cpp:361-372
+        if (name.empty()) return 0;
+        auto tags = M::get_boundary_tags();
+        for (int t = 0; t < (int)tags.size(); ++t)
+            if (tags[t] == name) return t;
+        std::string known;
+        for (auto const& t : tags) known += (known.empty() ? "" : ", ") + t;
+        throw Error(2, "boundary tag '" + name + "' is not declared by this model. Known tags: [" + known +
+                           "]. Check the bc.* inputs keys against the model's boundary_conditions tags.");
+    }
+
+    void read_solver_keys(const Deck& pp) {
+        pp.query("output.identifier", identifier);
+        pp.query("output.plot_dt_interval", plot_dt_interval);
+        pp.query("solver.time_end", time_end);
+        pp.query("solver.max_step", max_step);
+        pp.query("solver.cfl", cfl);
+        pp.query("solver.dtmin", dtmin);
+        pp.query("solver.dtmax", dtmax);
+        pp.query("solver.spatial_order", spatial_order);
+        pp.query("solver.implicit_source", implicit_source);
+        pp.query("solver.implicit_global", implicit_global);
+        pp.query("solver.well_balanced", well_balanced);
+        pp.query("solver.clamp_positivity", clamp_positivity);
+        pp.query("solver.positivity", positivity);
+        pp.query("solver.face_based_rhs", face_based_rhs);
+        pp.query("solver.profile_split", face_based_rhs);
+        if (spatial_order != 1 && spatial_order != 2) throw Error(1, "solver.spatial_order must be 1 or 2");
+        if (implicit_global)
+            throw Error(5, "solver.implicit_global = true (matrix-free JFNK source solve) is not part of this build");
+        if (positivity == "mood")
+            throw Error(5, "solver.positivity = mood (a-posteriori order-1 redo) is not part of this build yet");
+        if (positivity != "none") throw Error(1, "solver.positivity must be none or mood");
+        auto defs = M::default_parameters();
+        for (int i = 0; i < np; ++i) prm.v[i] = defs[i];
+        auto names = M::parameter_names();
+        for (int i = 0; i < np; ++i) {
+            double v;
+            if (pp.query("params." + names[i], v)) prm.v[i] = v;
+        }
+    }
+
+    void configure_deck(const std::string& text) {
+        Deck pp(text);
+        read_solver_keys(pp);
+        pp.query("amr.max_level", max_level);
+        if (max_level != 0)
+            throw Error(5, "amr.max_level > 0: the AMR hierarchy (regrid, two-level FillPatch, average_down) is "
+                           "not part of this build; use a single level or the batched box list");
+        std::vector<double> nc;
+        if (!pp.queryarr("amr.n_cell", nc) || nc.size() < 2) throw Error(1, "amr.n_cell needs at least 2 values");
+        nx = (int)nc[0];
+        ny = (int)nc[1];
+        pp.query("amr.max_grid_size", max_grid_size);
+        std::vector<double> lo, hi;
+        if (pp.queryarr("geometry.prob_lo", lo) && lo.size() >= 2) plo[0] = lo[0], plo[1] = lo[1];
+        if (pp.queryarr("geometry.prob_hi", hi) && hi.size() >= 2) phi[0] = hi[0], phi[1] = hi[1];
+        pp.query("bc.x_lo", bc_name[0]);
+        pp.query("bc.x_hi", bc_name[1]);
+        pp.query("bc.y_lo", bc_name[2]);
+        pp.query("bc.y_hi", bc_name[3]);
+        pp.query("init.dem_file", dem_file);
+        pp.query("init.release_file", release_file);
+        pp.query("init.friction_file", friction_file);
+        pp.queryarr("init.state_rasters", state_rasters);
+        if (nx <= 0 || ny <= 0 || max_grid_size <= 0) throw Error(1, "bad amr.n_cell / amr.max_grid_size");
+        int bcidx[4];
+        for (int k = 0; k < 4; ++k) bcidx[k] = tag_index(bc_name[k]);
+        const double dx[2] = {(phi[0] - plo[0]) / nx, (phi[1] - plo[1]) / ny};
+        const int ng = (spatial_order == 2) ? 2 : 1;
+        for (auto& y : chop_axis(ny, max_grid_size))
+            for (auto& x : chop_axis(nx, max_grid_size)) {
+                HostBox b{};
+                b.lo[0] = x.first;
+                b.hi[0] = x.second;
+                b.lo[1] = y.first;
+                b.hi[1] = y.second;
+                b.ng = 2;  // storage always carries 2 ghosts so either order can run
+                b.dlo[0] = 0;
+                b.dhi[0] = nx - 1;
+                b.dlo[1] = 0;
+                b.dhi[1] = ny - 1;
+                b.dx[0] = dx[0];
+                b.dx[1] = dx[1];
+                b.plo[0] = plo[0];
+                b.plo[1] = plo[1];
+                b.group = 0;
+                for (int k = 0; k < 4; ++k) b.bc[k] = bcidx[k];
+                if (b.hi[0] - b.lo[0] + 1 < ng || (dim == 2 && b.hi[1] - b.lo[1] + 1 < ng))
+                    throw Error(1, "boxes narrower than the ghost width are not supported");
+                hb.push_back(b);
+            }
+        ngroups = 1;
+        allocate();
+    }
+
+    void configure_boxlist(const zamr_box* boxes, int nbox, const std::string& text) {
+        Deck pp(text);
+        read_solver_keys(pp);
+        batched = true;
+        if (nbox <= 0) throw Error(1, "empty box list");
+        int gmax = 0;
+        for (int k = 0; k < nbox; ++k) {
+            const zamr_box& z = boxes[k];
+            HostBox b{};
+            for (int d = 0; d < 2; ++d) {
+                b.lo[d] = z.lo[d];
+                b.hi[d] = z.hi[d];
+                b.dlo[d] = z.lo[d];
+                b.dhi[d] = z.hi[d];
+                b.dx[d] = z.dx[d];
+                b.plo[d] = z.prob_lo[d] - z.lo[d] * z.dx[d];  // cell lo sits at prob_lo
+            }
+            b.ng = 2;
+            b.group = z.group;
+            if (z.group < 0) throw Error(1, "negative group id");
+            gmax = std::max(gmax, z.group);
+            for (int s = 0; s < 4; ++s) b.bc[s] = tag_index(z.bc[s] ? std::string(z.bc[s]) : std::string());
+            const int ng = (spatial_order == 2) ? 2 : 1;
+            if (b.hi[0] - b.lo[0] + 1 < ng || (dim == 2 && b.hi[1] - b.lo[1] + 1 < ng))
+                throw Error(1, "boxes narrower than the ghost width are not supported");
+            hb.push_back(b);
+        }
+        ngroups = gmax + 1;
+        nx = hb[0].hi[0] - hb[0].lo[0] + 1;
+        ny = hb[0].hi[1] - hb[0].lo[1] + 1;
+        allocate();
+    }
+
+    void allocate() {
+        if (!stream) {
+            ZAMR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        size_t off = 0;
+        auto align = [](size_t x) { return (x + 31) & ~size_t(31); };  // 256-byte alignment
+        for (auto& b : hb) {
+            b.gx = b.hi[0] - b.lo[0] + 1 + 2 * b.ng;
+            b.gy = (dim == 2 || true) ? b.hi[1] - b.lo[1] + 1 + 2 * b.ng : 1;
+            const size_t cells = (size_t)b.gx * b.gy;
+            b.off_q[0] = off;
+            off = align(off + cells * nq);
+            b.off_q[1] = off;
+            off = align(off + cells * nq);
+            b.off_aux = off;
+            off = align(off + cells * std::max(na, 1));
+        }
+        pool_elems = off;
+        ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
+        ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));
+        std::vector<BoxDesc> bd(hb.size());
+        for (size_t k = 0; k < hb.size(); ++k) {
+            const HostBox& b = hb[k];
+            BoxDesc& d = bd[k];
+            d.q[0] = d_pool + b.off_q[0];
+            d.q[1] = d_pool + b.off_q[1];
+            d.aux = d_pool + b.off_aux;
+            for (int i = 0; i < 2; ++i) {
+                d.lo[i] = b.lo[i];
+                d.hi[i] = b.hi[i];
+                d.gl[i] = b.lo[i] - b.ng;
+                d.dlo[i] = b.dlo[i];
+                d.dhi[i] = b.dhi[i];
+                d.dx[i] = b.dx[i];
+                d.plo[i] = b.plo[i];
+            }
+            d.sy = b.gx;
+            d.sc = (long long)b.gx * b.gy;
+            d.group = b.group;
+            for (int s = 0; s < 4; ++s) d.bc[s] = b.bc[s];
+        }
+        ZAMR_HIP(hipMalloc(&d_boxes, bd.size() * sizeof(BoxDesc)));
+        ZAMR_HIP(hipMemcpyAsync(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, stream));
+        h_groups.assign(ngroups, GroupState{});
+        for (auto& g : h_groups) {
+            g.dt = 0.0;
+            g.time = 0.0;
+            g.time_end = time_end;
+            g.cfl = cfl;
+            g.dtmin = dtmin;
+            g.dtmax = dtmax;
+            g.divisor = double(c_cfl_dimension) * double(c_cfl_degree_factor);
+            g.maxinv_bits = 0ull;
+            g.step = 0;
+            g.max_step = max_step;
+            g.active = 1;
+        }
+        ZAMR_HIP(hipMalloc(&d_groups, ngroups * sizeof(GroupState)));
+        upload_groups();
+        ZAMR_HIP(hipMalloc(&d_hist, 2 * (size_t)hist_cap * sizeof(double)));
+        build_rects();
+        ZAMR_HIP(hipStreamSynchronize(stream));
+    }
+
+    void upload_groups() {
+        ZAMR_HIP(hipMemcpyAsync(d_groups, h_groups.data(), ngroups * sizeof(GroupState), hipMemcpyHostToDevice,
+                                stream));
+    }
+    void download_groups() {
+        ZAMR_HIP(hipMemcpyAsync(h_groups.data(), d_groups, ngroups * sizeof(GroupState), hipMemcpyDeviceToHost,
+                                stream));
+        ZAMR_HIP(hipStreamSynchronize(stream));
+    }
+
+    // Ghost cells of box b that lie inside the domain and inside another box
+    // (same-level FillBoundary).  Batched boxes are independent domains.
+    void build_rects() {
+        if (batched || hb.size() < 2) return;
+        std::vector<CopyRect> rr;
+        for (size_t b = 0; b < hb.size(); ++b)
+            for (size_t o = 0; o < hb.size(); ++o) {
+                if (o == b) continue;
+                const HostBox& B = hb[b];
+                const HostBox& O = hb[o];
+                const int gy = (dim == 2) ? B.ng : 0;
+                const int ilo = std::max(B.lo[0] - B.ng, O.lo[0]), ihi = std::min(B.hi[0] + B.ng, O.hi[0]);
+                const int jlo = std::max(B.lo[1] - gy, O.lo[1]), jhi = std::min(B.hi[1] + gy, O.hi[1]);
+                if (ilo > ihi || jlo > jhi) continue;
+                rr.push_back(CopyRect{(int)o, (int)b, ilo, jlo, ihi - ilo + 1, jhi - jlo + 1});
+            }
+        nrects = (int)rr.size();
+        if (nrects) {
+            ZAMR_HIP(hipMalloc(&d_rects, rr.size() * sizeof(CopyRect)));
+            ZAMR_HIP(hipMemcpyAsync(d_rects, rr.data(), rr.size() * sizeof(CopyRect), hipMemcpyHostToDevice, stream));
+        }
+    }
+
+    int rows_for(int h) const {
+        if (rows_per_item > 0) return rows_per_item;
+        (void)h;
+        return chosen_rows;
+    }
+
+    // Strips of NT-2*hx columns by segments of `rows` rows, box by box.
+    Items& items(int hx) {
+        Items& it = items_[hx];
+        const int rows = (dim == 2) ? rows_for(hx) : 1;
+        if (it.d && it.hx == hx && it.rows == rows) return it;
+        if (it.d) hipFree(it.d), it.d = nullptr;
+        std::vector<WorkItem> v;
+        it.box_begin.clear();
+        const int txc = NT - 2 * hx;
+        for (size_t b = 0; b < hb.size(); ++b) {
+            it.box_begin.push_back((int)v.size());
+            const HostBox& B = hb[b];
+            for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += rows)
+                for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += txc)
+                    v.push_back(WorkItem{(int)b, i0, j0, std::min(rows, B.hi[1] - j0 + 1)});
+        }
+        it.box_begin.push_back((int)v.size());
+        it.hx = hx;
+        it.rows = rows;
+        it.n = (int)v.size();
+        ZAMR_HIP(hipMalloc(&it.d, v.size() * sizeof(WorkItem)));
+        ZAMR_HIP(hipMemcpyAsync(it.d, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        if (partial_cap < it.n) {
+            if (d_partial) hipFree(d_partial);
+            partial_cap = it.n;
+            ZAMR_HIP(hipMalloc(&d_partial, (size_t)partial_cap * sizeof(double)));
+        }
+        invalidate_graph();
+        return it;
+    }
+
+    void invalidate_graph() {
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        graph_key = -1;
+    }
+
+    // ---------------- state I/O ----------------
+    HostBox& level_box_check(int lev) {
+        if (batched) {
+            if (lev < 0 || lev >= (int)hb.size()) throw Error(1, "box index out of range");
+            return hb[lev];
+        }
+        if (lev != 0) throw Error(1, "only level 0 exists in this build");
+        return hb[0];
+    }
+
+    // Dense (ncomp, ny, nx) array of level 0 (or of box `lev` in batched mode).
+    void copy_level(int lev, int field, double* host, size_t n, bool to_device) {
+        level_box_check(lev);
+        const int nc = (field == 0) ? nq : na;
+        if (field != 0 && field != 1) throw Error(1, "field must be 0 (Q) or 1 (Qaux)");
+        int gx0 = 0, gy0 = 0, NX = nx, NY = ny;
+        std::vector<int> which;
+        if (batched) {
+            which.push_back(lev);
+            gx0 = hb[lev].lo[0];
+            gy0 = hb[lev].lo[1];
+            NX = hb[lev].hi[0] - hb[lev].lo[0] + 1;
+            NY = hb[lev].hi[1] - hb[lev].lo[1] + 1;
+        } else {
+            for (size_t b = 0; b < hb.size(); ++b) which.push_back((int)b);
+        }
+        if (n != (size_t)nc * NX * NY) throw Error(1, "array size mismatch: expected ncomp*ny*nx doubles");
+        // Qaux is brought up to date before it is read, and before Q is
+        // replaced (it then keeps what the last UpdateState left, as in the reference).
+        if (field == 1 || to_device) ensure_aux();
+        for (int b : which) {
+            const HostBox& B = hb[b];
+            const size_t cells = (size_t)B.gx * B.gy;
+            std::vector<double> tmp(cells * nc, 0.0);
+            double* dptr = d_pool + ((field == 0) ? B.off_q[cur] : B.off_aux);
+            ZAMR_HIP(hipMemcpyAsync(tmp.data(), dptr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            ZAMR_HIP(hipStreamSynchronize(stream));
+            for (int c = 0; c < nc; ++c)
+                for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                    for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                        const size_t li = (size_t)(i - B.lo[0] + B.ng) + (size_t)(j - B.lo[1] + B.ng) * B.gx +
+                                          (size_t)c * cells;
+                        const size_t gi = ((size_t)c * NY + (j - gy0)) * NX + (i - gx0);
+                        if (to_device) tmp[li] = host[gi];
+                        else host[gi] = tmp[li];
+                    }
+            if (to_device) {
+                ZAMR_HIP(hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+                ZAMR_HIP(hipStreamSynchronize(stream));
+            }
+        }
+        if (to_device) maxinv_fresh = false;
+    }
+
+    // init_solution (init_solution.cpp:78-115) on Q and Qaux: zero every
+    // component, then b = 0 and h = 2 left of the domain's x midpoint, 1 right.
+    void init_solution() {
+        aux_stale = false;
+        for (size_t b = 0; b < hb.size(); ++b) {
+            const HostBox& B = hb[b];
+            const double xmid = batched ? B.plo[0] + 0.5 * (B.dlo[0] + B.dhi[0] + 1) * B.dx[0]
+                                        : 0.5 * (plo[0] + phi[0]);
+            const size_t cells = (size_t)B.gx * B.gy;
+            for (int field = 0; field < 2; ++field) {
+                const int nc = field == 0 ? nq : std::max(na, 1);
+                std::vector<double> tmp(cells * nc, 0.0);
+                if (nc >= 2)
+                    for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                        for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                            const double x = B.plo[0] + (i + 0.5) * B.dx[0];
+                            const size_t li = (size_t)(i - B.lo[0] + B.ng) + (size_t)(j - B.lo[1] + B.ng) * B.gx;
+                            tmp[li] = 0.0;
+                            tmp[li + cells] = (x < xmid) ? 2.0 : 1.0;
+                        }
+                double* dptr = d_pool + (field == 0 ? B.off_q[cur] : B.off_aux);
+                ZAMR_HIP(hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+                ZAMR_HIP(hipStreamSynchronize(stream));
+            }
+        }
+        maxinv_fresh = false;
+    }
+
+    // readRasterIntoComponent (init_solution.cpp:7-75): raw float64, idx = i + nx*j.
+    void read_raster(const std::string& file, int field, int comp) {
+        std::ifstream ifs(file, std::ios::binary);
+        if (!ifs.good()) throw Error(3, "### Cannot open raw file: " + file);
+        ifs.seekg(0, std::ios::end);
+        const size_t actual = (size_t)ifs.tellg();
+        ifs.seekg(0);
+        const size_t need = (size_t)nx * ny * sizeof(double);
+        if (actual < need)
+            throw Error(3, "Raw file size mismatch: " + file + " has " + std::to_string(actual) + " < expected " +
+                               std::to_string(need) + " bytes");
+        std::vector<double> r((size_t)nx * ny);
+        ifs.read(reinterpret_cast<char*>(r.data()), need);
+        const int nc = field == 0 ? nq : na;
+        std::vector<double> all((size_t)nc * nx * ny);
+        copy_level(0, field, all.data(), all.size(), false);
+        std::copy(r.begin(), r.end(), all.begin() + (size_t)comp * nx * ny);
+        copy_level(0, field, all.data(), all.size(), true);
+    }
+
+    static bool exists(const std::string& f) {
+        std::ifstream i(f);
+        return i.good();
+    }
+
+    void init_data() {  // ZoomyAmr::InitData (ZoomyAmr.cpp:112-156), single level
+        init_solution();
+        if (!batched) {
+            for (int c = 0; c < (int)state_rasters.size() && c < nq; ++c)
+                if (exists(state_rasters[c])) read_raster(state_rasters[c], 0, c);
+            if (!dem_file.empty() && exists(dem_file)) read_raster(dem_file, 0, 0);
+            if (!release_file.empty() && exists(release_file)) read_raster(release_file, 0, 1);
+            if (!friction_file.empty() && exists(friction_file)) read_raster(friction_file, 1, 1);
+        }
+        reset_clock();
+    }
+
+    void reset_clock() {
+        download_groups();
+        for (auto& g : h_groups) {
+            g.time = 0.0;
+            g.step = 0;
+            g.dt = 0.0;
+            g.active = 1;
+            g.maxinv_bits = 0ull;
+        }
+        upload_groups();
+        maxinv_fresh = false;
+        plot_step = 0;
+        next_plot_time = 0.0;
+    }
+
+    // ---------------- kernels ----------------
+    int hx_of(int order) const { return order == 2 ? 2 : 1; }
+
+    StageArgs stage_args(int src, int dst, int old) const {
+        StageArgs a;
+        a.p = prm;
+        a.src = src;
+        a.dst = dst;
+        a.old = old;
+        a.wb = well_balanced ? 1 : 0;
+        a.impl = implicit_source ? 1 : 0;
+        a.clamp = (clamp_positivity && positivity != "mood") ? 1 : 0;
+        return a;
+    }
+
+    void fill_boundary(int buf) {
+        if (nrects == 0) return;
+        hipLaunchKernelGGL(fill_boundary_kernel, dim3(nrects), dim3(256), 0, stream, d_boxes, d_rects, buf, nq);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    template <int ORDER, int EPI>
+    void launch_stage(const StageArgs& a) {
+        Items& it = items(hx_of(ORDER));
+        auto kern = stage_kernel<M, N, ORDER, EPI>;
+        if (per_box_launch) {
+            for (size_t b = 0; b + 1 < it.box_begin.size(); ++b) {
+                const int n0 = it.box_begin[b], n = it.box_begin[b + 1] - n0;
+                if (n <= 0) continue;
+                hipLaunchKernelGGL(kern, dim3(n), dim3(NT), 0, stream, d_boxes, it.d + n0, n, a, d_groups, 0);
+            }
+        } else {
+            const int grid = ((it.n + 7) / 8) * 8;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, d_boxes, it.d, it.n, a, d_groups, xcd_remap);
+        }
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    void launch_stage_dyn(int order, int epi, const StageArgs& a) {
+        if (order == 2) {
+            if (epi == 0) launch_stage<2, 0>(a);
+            else if (epi == 1) launch_stage<2, 1>(a);
+            else launch_stage<2, 2>(a);
+        } else {
+            if (epi == 0) launch_stage<1, 0>(a);
+            else if (epi == 1) launch_stage<1, 1>(a);
+            else launch_stage<1, 2>(a);
+        }
+    }
+
+    void begin_step() {
+        hipLaunchKernelGGL(begin_step_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, d_groups, ngroups);
+        ZAMR_HIP(hipGetLastError());
+    }
+    void end_step() {
+        hipLaunchKernelGGL(end_step_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, d_groups, ngroups, d_hist,
+                           hist_cap);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    // Group maxima from the state and aux as stored (ComputeDt's reduction).
+    void reduce_stored() {
+        ensure_aux();
+        download_groups();
+        for (auto& g : h_groups) g.maxinv_bits = 0ull;
+        upload_groups();
+        Items& it = items(hx_of(spatial_order));
+        hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur, prm,
+                           d_groups, it.hx);
+        ZAMR_HIP(hipGetLastError());
+        maxinv_fresh = true;
+    }
+
+    void ensure_aux() {
+        if (aux_stale) materialize_aux();
+    }
+
+    void materialize_aux() {
+        Items& it = items(hx_of(spatial_order));
+        download_groups();
+        hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur,
+                           prm, h_groups[0].time, it.hx);
+        ZAMR_HIP(hipGetLastError());
+        ZAMR_HIP(hipStreamSynchronize(stream));
+        aux_stale = false;
+    }
+
+    // The stage kernels of Advance for an order-`order` step on the device
+    // clock (dt and time read from the group state).
+    void advance_kernels() {
+        aux_stale = true;
+        if (spatial_order == 1) {
+            fill_boundary(cur);
+            launch_stage_dyn(1, 2, stage_args(cur, 1 - cur, cur));
+            cur = 1 - cur;
+        } else {
+            fill_boundary(cur);
+            record(0);
+            launch_stage_dyn(2, 0, stage_args(cur, 1 - cur, cur));
+            record(1);
+            fill_boundary(1 - cur);
+            launch_stage_dyn(2, 1, stage_args(1 - cur, cur, cur));
+            record(2);
+        }
+    }
+
+    // Profiling: four events per step (before begin_step, stage 1, stage 2,
+    // after stage 2) recorded on the stream without any synchronisation; the
+    // elapsed times are read after the run.
+    int prof_slot = -1;
+    void record(int k) {
+        if (!profiling || prof_slot < 0) return;
+        const size_t idx = (size_t)prof_slot * 4 + k;
+        while (ev.size() <= idx) {
+            hipEvent_t e;
+            ZAMR_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        ZAMR_HIP(hipEventRecord(ev[idx], stream));
+    }
+
+    bool graph_ok() const { return spatial_order == 2 && !profiling; }
+
+    void run_steps(int nsteps, int* done, double* time) {
+        download_groups();
+        const int step0 = h_groups[0].step;
+        if (!maxinv_fresh) reduce_stored();
+        prof_ms[0] = prof_ms[1] = prof_ms[2] = 0.0;
+        prof_steps = 0;
+        int k = 0;
+        const int chunk = profiling ? std::max(nsteps, 1) : 16;
+        while (k < nsteps) {
+            const int n = std::min(chunk, nsteps - k);
+            if (graph_ok() && n == chunk) {
+                if (!graph_exec || graph_key != chunk) {
+                    invalidate_graph();
+                    items(hx_of(spatial_order));
+                    hipGraph_t g;
+                    ZAMR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                    for (int s = 0; s < chunk; ++s) {
+                        begin_step();
+                        advance_kernels();
+                        end_step();
+                    }
+                    ZAMR_HIP(hipStreamEndCapture(stream, &g));
+                    ZAMR_HIP(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+                    hipGraphDestroy(g);
+                    graph_key = chunk;
+                }
+                ZAMR_HIP(hipGraphLaunch(graph_exec, stream));
+            } else {
+                for (int s = 0; s < n; ++s) {
+                    if (profiling) prof_slot = k + s;
+                    record(3);
+                    begin_step();
+                    advance_kernels();
+                    end_step();
+                }
+                prof_slot = -1;
+            }
+            k += n;
+            download_groups();
+            bool any = false;
+            for (auto& g : h_groups) any = any || ((g.time < g.time_end) && (g.max_step < 0 || g.step < g.max_step));
+            if (!any) break;
+        }
+        maxinv_fresh = true;
+        download_groups();
+        if (profiling && spatial_order == 2) {
+            const int n = std::min(k, nsteps);
+            for (int i = 0; i < n; ++i) {
+                float a = 0, b = 0, c = 0;
+                ZAMR_HIP(hipEventElapsedTime(&a, ev[4 * i + 0], ev[4 * i + 1]));
+                ZAMR_HIP(hipEventElapsedTime(&b, ev[4 * i + 1], ev[4 * i + 2]));
+                ZAMR_HIP(hipEventElapsedTime(&c, ev[4 * i + 3], ev[4 * i + 0]));
+                prof_ms[0] += a;
+                prof_ms[1] += b;
+                prof_ms[2] += c;
+            }
+            prof_steps = n;
+        }
+        if (done) *done = h_groups[0].step - step0;
+        if (time) *time = h_groups[0].time;
+    }
+
+    // ---------------- reference-shaped entry points ----------------
+    double compute_dt_host() {  // ZoomyAmr::ComputeDt
+        reduce_stored();
+        download_groups();
+        const double m = [&] {
+            double mm = 0.0;
+            for (auto& g : h_groups) {
+                double v;
+                std::memcpy(&v, &g.maxinv_bits, sizeof(v));
+                mm = std::max(mm, v);
+            }
+            return mm;
+        }();
+        maxinv_fresh = false;
+        if (m < 1e-14) return dtmax;
+        const double div = double(c_cfl_dimension) * double(c_cfl_degree_factor);
+        const double dt = cfl / (div * m);
+        return std::max(std::min(dt, dtmax), dtmin);
+    }
+
+    void set_clock(double dt, double time) {
+        hipLaunchKernelGGL(set_dt_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, d_groups, ngroups, dt, time);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    void advance_host(double time, double dt) {  // ZoomyAmr::Advance with the caller's dt
+        set_clock(dt, time);
+        advance_kernels();
+        materialize_aux();
+        maxinv_fresh = false;
+        download_groups();
+    }
+
+    void stage_host(int order, double time, double dt) {  // do_stage(order) in place
+        aux_stale = true;
+        materialize_aux();  // do_stage's UpdateState leaves Qaux = update_aux(stage-start state)
+        set_clock(dt, time);
+        fill_boundary(cur);
+        launch_stage_dyn(order, 0, stage_args(cur, 1 - cur, cur));
+        cur = 1 - cur;
+        ZAMR_HIP(hipStreamSynchronize(stream));
+        maxinv_fresh = false;
+    }
+
+    void update_state_host(double time) {
+        Items& it = items(hx_of(spatial_order));
+        hipLaunchKernelGGL((update_state_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur, prm,
+                           time, (clamp_positivity && positivity != "mood") ? 1 : 0, it.hx);
+        ZAMR_HIP(hipGetLastError());
+        ZAMR_HIP(hipStreamSynchronize(stream));
+        maxinv_fresh = false;
+    }
+
+    double total_mass() {  // ComputeTotalMass, level 0 (deterministic item-order sum)
+        Items& it = items(hx_of(spatial_order));
+        hipLaunchKernelGGL((mass_partial_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur,
+                           d_partial, it.hx);
+        ZAMR_HIP(hipGetLastError());
+        std::vector<double> part(it.n);
+        ZAMR_HIP(hipMemcpyAsync(part.data(), d_partial, it.n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        ZAMR_HIP(hipStreamSynchronize(stream));
+        double s = 0.0;
+        for (double v : part) s += v;
+        return s;
+    }
+
+    double time_step_host(double time) {  // ZoomyAmr::TimeStep, level 0
+        double dt = compute_dt_host();
+        const double remaining = time_end - time;
+        if (remaining > 0.0 && dt > remaining) dt = remaining;
+        advance_host(time, dt);
+        return dt;
+    }
+
+    void write_plotfile(const std::string& dir, int step, double time);
+
+    void evolve() {  // ZoomyAmr::Evolve
+        reset_clock();
+        double time = 0.0;
+        int step = 0;
+        next_plot_time = 0.0;
+        const auto t0 = std::chrono::steady_clock::now();
+        const double mass0 = total_mass();
+        std::cout << "Initial total mass = " << std::setprecision(12) << mass0 << "\n";
+        while (time < time_end && (max_step < 0 || step < max_step)) {
+            if (time >= next_plot_time) {
+                write_plotfile("", plot_step, time);
+                next_plot_time += plot_dt_interval;
+                plot_step++;
+            }
+            int done = 0;
+            double tnew = time;
+            run_steps(1, &done, &tnew);
+            if (done == 0) break;
+            const double dt0 = h_groups[0].dt;
+            time = tnew;
+            step++;
+            const double mass = total_mass();
+            const double drift = (mass0 != 0.0) ? (mass - mass0) / mass0 : (mass - mass0);
+            std::cout << "Step " << step << " dt: " << std::setprecision(12) << dt0 << " time: "
+                      << std::setprecision(12) << time << "s"
+                      << " levels: " << 1 << " mass: " << std::setprecision(14) << mass
+                      << " drift: " << std::setprecision(4) << drift << "\n";
+        }
+        materialize_aux();
+        write_plotfile("", plot_step, time);
+        const double evo = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << "Total evolution time = " << evo << " seconds\n";
+        std::cout.flush();
+    }
+};
+
+}  // namespace zamr

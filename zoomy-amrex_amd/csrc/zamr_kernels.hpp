@@ -1,0 +1,756 @@
+#pragma once
+// gfx950 device kernels for the per-box RHS and time advance.
+//
+// One fused kernel per RK stage does what the reference spreads over seven
+// per-box launches (Source/ZoomyAmr.cpp:561-675): UpdateState, physical-BC
+// ghost fill, MUSCL/minmod reconstruction, Rusanov / path-conservative face
+// fluxes, flux divergence + well-balanced bed source, ConsUpdate, the
+// point-implicit source solve and (stage 2) the Heun combine, the end-of-step
+// UpdateState and the CFL max-reduction for the next step.
+//
+// Work decomposition: a work item is a strip of NT-2*HX columns of one box and
+// a segment of up to TY rows.  A 256-thread workgroup owns NT = 256 columns
+// (the strip plus an HX-column halo on each side) and marches up the segment
+// row by row.  Every thread keeps a (2*HX+1)-row register window of its own
+// column, so each y-face is evaluated once, by the thread that owns the
+// column, and carried to the next row; each x-face is evaluated once, by the
+// thread right of it, and handed to the left neighbour through LDS.  State is
+// read row by row with consecutive lanes on consecutive cells (coalesced SoA),
+// and every cell is loaded once per workgroup.
+//
+// The Model / Numerics headers are the reference's generated device-function
+// API (to_amrex.py:326-516), used unchanged.  Models that declare
+// `zamr_aux_is_local` have an aux vector that is a per-cell function of the
+// state, so aux is evaluated in registers rather than streamed from HBM.
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zamr {
+
+constexpr int kMaxParams = 32;
+constexpr int NT = 256;  // threads (columns) per workgroup
+
+struct Params {
+    double v[kMaxParams];
+};
+
+// One box (patch) on the device: AMReX Array4 layout, i fastest, then j, then
+// component; base pointers address the grown-box corner (lo - ng).
+struct BoxDesc {
+    double* q[2];    // two state buffers
+    double* aux;     // stored Qaux (na comps, same layout)
+    int lo[2], hi[2];
+    int gl[2];       // grown lower corner
+    int sy;          // row stride (elements)
+    long long sc;    // component stride (elements)
+    int dlo[2], dhi[2];  // physical domain of the box's level / member
+    double dx[2], plo[2];
+    int group;       // time-step group (level or ensemble member)
+    int bc[4];       // tag index x_lo, x_hi, y_lo, y_hi
+};
+
+struct WorkItem {
+    int box, i0, j0, nj;
+};
+
+// The descriptor as kernels hold it: scalars only, so no member is ever
+// indexed at run time (which would demote the whole struct to scratch).
+struct BoxView {
+    double *q0, *q1, *aux;
+    int lo0, lo1, hi0, hi1, gl0, gl1, sy;
+    long long sc;
+    int dlo0, dlo1, dhi0, dhi1;
+    double dx0, dx1, plo0, plo1;
+    int group, bc0, bc1, bc2, bc3;
+};
+
+__device__ __forceinline__ BoxView view(const BoxDesc* __restrict__ boxes, int b) {
+    const BoxDesc& d = boxes[b];
+    BoxView v;
+    v.q0 = d.q[0];
+    v.q1 = d.q[1];
+    v.aux = d.aux;
+    v.lo0 = d.lo[0];
+    v.lo1 = d.lo[1];
+    v.hi0 = d.hi[0];
+    v.hi1 = d.hi[1];
+    v.gl0 = d.gl[0];
+    v.gl1 = d.gl[1];
+    v.sy = d.sy;
+    v.sc = d.sc;
+    v.dlo0 = d.dlo[0];
+    v.dlo1 = d.dlo[1];
+    v.dhi0 = d.dhi[0];
+    v.dhi1 = d.dhi[1];
+    v.dx0 = d.dx[0];
+    v.dx1 = d.dx[1];
+    v.plo0 = d.plo[0];
+    v.plo1 = d.plo[1];
+    v.group = d.group;
+    v.bc0 = d.bc[0];
+    v.bc1 = d.bc[1];
+    v.bc2 = d.bc[2];
+    v.bc3 = d.bc[3];
+    return v;
+}
+
+// Per-group stepping state, all on the device so that steps never round-trip
+// dt through the host (ZoomyAmr.cpp:448-515 and :733-816 moved on-device).
+struct GroupState {
+    double dt, time, time_end, cfl, dtmin, dtmax, divisor;
+    unsigned long long maxinv_bits;  // max over cells of |lambda_d| / dx_d (bit pattern, >= 0)
+    int step, max_step, active, pad;
+};
+
+struct StageArgs {
+    Params p;
+    int src, dst, old;           // buffer indices: read, write, Heun's Q^n
+    int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
+};
+
+template <class T>
+__device__ __forceinline__ T dev_max(T a, T b) { return (a < b) ? b : a; }
+
+// make_rhs.H:9-13
+__device__ __forceinline__ double minmod(double a, double b) {
+    if (a * b <= 0.0) return 0.0;
+    return (fabs(a) < fabs(b)) ? a : b;
+}
+
+template <class M>
+struct Sys {
+    static constexpr int nq = M::n_dof_q;
+    static constexpr int na = M::n_dof_qaux;
+    static constexpr int np = M::n_parameters;
+    static constexpr int dim = M::dimension;
+    using VQ = amrex::SmallMatrix<double, nq, 1>;
+    using VA = amrex::SmallMatrix<double, na, 1>;
+    using VP = amrex::SmallMatrix<double, np, 1>;
+    using VN = amrex::SmallMatrix<double, dim, 1>;
+    using VX = amrex::SmallMatrix<double, 3, 1>;
+    using VF = amrex::SmallMatrix<double, 2 * nq, 1>;
+    static_assert(M::zamr_aux_is_local,
+                  "this build evaluates Qaux in registers; the model must declare zamr_aux_is_local");
+    static_assert(np <= kMaxParams, "too many model parameters");
+
+    __device__ static __forceinline__ VX center(const BoxView& B, int i, int j) {
+        VX X{};
+        X(0, 0) = B.plo0 + (i + 0.5) * B.dx0;
+        X(1, 0) = (dim == 2) ? B.plo1 + (j + 0.5) * B.dx1 : 0.0;
+        X(2, 0) = 0.0;
+        return X;
+    }
+
+    __device__ static __forceinline__ VA aux_of(const VQ& q, const VP& p, double time, const VX& X) {
+        VA a0{};
+        return M::update_aux_variables(q, a0, p, time, X);
+    }
+
+    // UpdateState (ZoomyAmr.cpp:326-327) on one cell.
+    __device__ static __forceinline__ VQ update_state(VQ q, const VP& p, int clamp) {
+        VA a0{};
+        q = M::update_variables(q, a0, p);
+        if (clamp && q(1, 0) < 0.0) q(1, 0) = 0.0;
+        return q;
+    }
+
+    // Value of cell (c, j) as the face pass sees it: interior/ghost storage
+    // after UpdateState, or, outside the physical domain, the mirror-index
+    // interior cell after UpdateState pushed through Model::boundary_conditions
+    // (ZoomyAmr.cpp:385-441; corner ghosts take the x-side tag).
+    __device__ static __forceinline__ VQ load(const BoxView& B, const double* __restrict__ q, int c, int j,
+                                              const VP& p, int clamp, double time) {
+        const bool ox = (c < B.dlo0) || (c > B.dhi0);
+        const bool oy = (dim == 2) && ((j < B.dlo1) || (j > B.dhi1));
+        int ci = c, cj = j;
+        if (c < B.dlo0) ci = 2 * B.dlo0 - 1 - c;
+        else if (c > B.dhi0) ci = 2 * B.dhi0 + 1 - c;
+        if (dim == 2) {
+            if (j < B.dlo1) cj = 2 * B.dlo1 - 1 - j;
+            else if (j > B.dhi1) cj = 2 * B.dhi1 + 1 - j;
+        }
+        const long long idx = (long long)(ci - B.gl0) + (long long)(cj - B.gl1) * B.sy;
+        VQ v;
+#pragma unroll
+        for (int n = 0; n < nq; ++n) v(n, 0) = q[idx + n * B.sc];
+        v = update_state(v, p, clamp);
+        if (!ox && !oy) return v;
+        // Side index -> tag by arithmetic on all four tags: a select between
+        // two field loads would be folded into a load through a selected
+        // address, which pins the descriptor in scratch memory.
+        const int side = ox ? ((c < B.dlo0) ? 0 : 1) : ((j < B.dlo1) ? 2 : 3);
+        const int bc = (side == 0) * B.bc0 + (side == 1) * B.bc1 + (side == 2) * B.bc2 + (side == 3) * B.bc3;
+        const double sgn = (side == 0 || side == 2) ? -1.0 : 1.0;
+        VN nh{};
+        if constexpr (dim == 2) {
+            nh(0, 0) = ox ? sgn : 0.0;
+            nh(1, 0) = ox ? 0.0 : sgn;
+        } else {
+            nh(0, 0) = sgn;
+        }
+        const VA ai = aux_of(v, p, time, center(B, ci, cj));
+        return M::boundary_conditions(bc, v, ai, nh, center(B, c, j), time, B.dx0);
+    }
+
+    // eval_face (make_rhs.H:31-131) from the four cells straddling the face.
+    template <int ORDER, int DIR, class N>
+    __device__ static __forceinline__ void face(const VQ& Lm, const VQ& L, const VQ& R, const VQ& Rp,
+                                                const VX& XL, const VX& XR, const VP& p, double time, int wb,
+                                                VQ& flux, VF& fluct, double& hLs, double& hRs) {
+        VQ qL, qR;
+#pragma unroll
+        for (int n = 0; n < nq; ++n) {
+            if constexpr (ORDER == 1) {
+                qL(n, 0) = L(n, 0);
+                qR(n, 0) = R(n, 0);
+            } else {
+                const double gL = minmod(L(n, 0) - Lm(n, 0), R(n, 0) - L(n, 0));
+                const double gR = minmod(R(n, 0) - L(n, 0), Rp(n, 0) - R(n, 0));
+                qL(n, 0) = L(n, 0) + 0.5 * gL;
+                qR(n, 0) = R(n, 0) - 0.5 * gR;
+            }
+        }
+        hLs = qL(1, 0);
+        hRs = qR(1, 0);
+        if (wb) {  // Audusse hydrostatic reconstruction, make_rhs.H:84-98
+            const double bL = qL(0, 0), bR = qR(0, 0);
+            const double bf = dev_max(bL, bR);
+            hLs = dev_max(0.0, qL(1, 0) + bL - bf);
+            hRs = dev_max(0.0, qR(1, 0) + bR - bf);
+            const double sL = (qL(1, 0) > 0.0) ? hLs / qL(1, 0) : 0.0;
+            const double sR = (qR(1, 0) > 0.0) ? hRs / qR(1, 0) : 0.0;
+#pragma unroll
+            for (int d = 0; d < dim; ++d) {
+                qL(2 + d, 0) *= sL;
+                qR(2 + d, 0) *= sR;
+            }
+            qL(1, 0) = hLs;
+            qR(1, 0) = hRs;
+            qL(0, 0) = bf;
+            qR(0, 0) = bf;
+        }
+        const VA aL = aux_of(qL, p, time, XL);
+        const VA aR = aux_of(qR, p, time, XR);
+        VN nh{};
+        nh(DIR, 0) = 1.0;
+        flux = N::numerical_flux(qL, qR, aL, aR, p, nh);
+        fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
+    }
+
+    // apply_implicit_source (make_rhs.H:429-509): one linearised backward-Euler
+    // step with Gaussian elimination and partial pivoting; explicit fallback.
+    __device__ static __forceinline__ VQ implicit_source(const VQ& q_old, const VA& a_in, double dt, const VP& p,
+                                                         double time, const VX& X) {
+        const VQ S = M::source(q_old, a_in, p, time, X);
+        const auto dS_dQ = M::source_jacobian_wrt_variables(q_old, a_in, p);
+        const auto dS_dA = M::source_jacobian_wrt_aux_variables(q_old, a_in, p);
+        const auto dA_dQ = M::update_aux_variables_jacobian_wrt_variables(q_old, a_in, p, time, X);
+        double A[nq][nq];
+        double rhs[nq];
+#pragma unroll
+        for (int r = 0; r < nq; ++r) {
+            rhs[r] = dt * S(r, 0);
+#pragma unroll
+            for (int c = 0; c < nq; ++c) {
+                double term = dS_dQ(r * nq + c, 0);
+#pragma unroll
+                for (int k = 0; k < na; ++k) term += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
+                A[r][c] = (r == c ? 1.0 : 0.0) - dt * term;
+            }
+        }
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < nq; ++k) {
+            if (ok) {
+                int piv = k;
+                double best = fabs(A[k][k]);
+#pragma unroll
+                for (int r = k + 1; r < nq; ++r)
+                    if (fabs(A[r][k]) > best) { best = fabs(A[r][k]); piv = r; }
+                if (!(best > 0.0) || !isfinite(best)) {
+                    ok = false;
+                } else {
+                    // Row swap by predication: registers cannot be indexed.
+#pragma unroll
+                    for (int r = k + 1; r < nq; ++r) {
+                        if (r == piv) {
+#pragma unroll
+                            for (int c = k; c < nq; ++c) {
+                                const double t = A[k][c];
+                                A[k][c] = A[r][c];
+                                A[r][c] = t;
+                            }
+                            const double t = rhs[k];
+                            rhs[k] = rhs[r];
+                            rhs[r] = t;
+                        }
+                    }
+#pragma unroll
+                    for (int r = k + 1; r < nq; ++r) {
+                        const double f = A[r][k] / A[k][k];
+#pragma unroll
+                        for (int c = k; c < nq; ++c) A[r][c] -= f * A[k][c];
+                        rhs[r] -= f * rhs[k];
+                    }
+                }
+            }
+        }
+        VQ out;
+        if (ok) {
+            double dq[nq];
+#pragma unroll
+            for (int r = nq - 1; r >= 0; --r) {
+                double sum = 0.0;
+#pragma unroll
+                for (int c = r + 1; c < nq; ++c) sum += A[r][c] * dq[c];
+                dq[r] = (rhs[r] - sum) / A[r][r];
+                if (!isfinite(dq[r])) ok = false;
+            }
+            if (ok) {
+#pragma unroll
+                for (int n = 0; n < nq; ++n) out(n, 0) = q_old(n, 0) + dq[n];
+                return out;
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < nq; ++n) {
+            const double v = q_old(n, 0) + dt * S(n, 0);
+            out(n, 0) = isfinite(v) ? v : q_old(n, 0);
+        }
+        return out;
+    }
+};
+
+__device__ __forceinline__ double* qsel(const BoxView& B, int k) { return k ? B.q1 : B.q0; }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dev_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// XCD-aware item order: the hardware deals consecutive workgroups round-robin
+// over the 8 XCDs; give each XCD a contiguous run of items so that row-segment
+// neighbours (which share halo rows) hit the same L2.
+__device__ __forceinline__ int remap_item(int b, int nitems) {
+    const int per = (nitems + 7) / 8;
+    const int x = b & 7, k = b >> 3;
+    const int it = x * per + k;
+    return (it < nitems) ? it : -1;
+}
+
+// The fused stage.  EPI: 0 = plain stage (writes dst), 1 = stage + Heun with
+// buffer `old` + end-of-step UpdateState + CFL max (order 2, second stage),
+// 2 = stage + end-of-step UpdateState + CFL max (order 1 step).
+template <class M, class N, int ORDER, int EPI>
+__global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ boxes,
+                                                   const WorkItem* __restrict__ items, int nitems,
+                                                   StageArgs a, GroupState* __restrict__ gs, int xcd_remap) {
+    using S = Sys<M>;
+    using VQ = typename S::VQ;
+    using VA = typename S::VA;
+    using VP = typename S::VP;
+    using VF = typename S::VF;
+    using VN = typename S::VN;
+    constexpr int nq = S::nq;
+    constexpr int dim = S::dim;
+    constexpr int HX = (ORDER == 2) ? 2 : 1;
+    constexpr int TXC = NT - 2 * HX;
+    constexpr int WR = (dim == 2) ? 2 * HX + 1 : 1;  // window rows
+    constexpr int FW = 2 * nq + 1;                    // face words handed left
+
+    __shared__ double s_row[nq][NT];
+    __shared__ double s_face[FW][NT];
+    __shared__ double s_red[NT / 64];
+
+    VP p;
+#pragma unroll
+    for (int k = 0; k < S::np; ++k) p(k, 0) = a.p.v[k];
+
+    const int t = threadIdx.x;
+    for (int blk = blockIdx.x; blk < ((nitems + 7) / 8) * 8; blk += gridDim.x) {
+        const int it = xcd_remap ? remap_item(blk, nitems) : blk;
+        if (it < 0 || it >= nitems) continue;
+        const WorkItem w = items[it];
+        const BoxView B = view(boxes, w.box);
+        GroupState* G = gs + B.group;
+        if (!G->active) continue;
+        const double dt = G->dt;
+        const double time = G->time;
+        // select, never index: a runtime index would push the descriptor to scratch
+        const double* __restrict__ qs = a.src ? B.q1 : B.q0;
+        double* __restrict__ qd = a.dst ? B.q1 : B.q0;
+        const double* __restrict__ qo = a.old ? B.q1 : B.q0;
+
+        const int c = w.i0 - HX + t;                       // this thread's column
+        const bool col_in = c <= B.hi0 + HX;               // inside the grown box
+        const bool own = (t >= HX) && (t < HX + TXC) && (c <= B.hi0);  // assembles cells
+        const bool xf = (t >= HX) && (t <= HX + TXC) && (c <= B.hi0 + 1);  // computes face c-1/2
+
+        VQ win[WR];
+        const int jstart = w.j0;
+        const int jend = w.j0 + w.nj;
+        // prologue: rows jstart-HX .. jstart+HX-1
+#pragma unroll
+        for (int r = 0; r < WR - 1; ++r) {
+            const int j = jstart - (WR == 1 ? 0 : HX) + r;
+            if (col_in) win[r] = S::load(B, qs, c, j, p, a.clamp, time);
+            else
+#pragma unroll
+                for (int n = 0; n < nq; ++n) win[r](n, 0) = 0.0;
+        }
+        // bottom y-face of the first row (face jstart - 1/2)
+        VQ fyB;
+        VF dyB;
+        double hyB = 0.0;
+        if constexpr (dim == 2) {
+            if (own) {
+                double hl;
+                const VQ& Lm = win[(HX == 2) ? 0 : 0];
+                const VQ& L = win[HX - 1];
+                const VQ& R = win[HX];
+                const VQ& Rp = win[(HX == 2) ? HX + 1 : HX];
+                S::template face<ORDER, 1, N>(Lm, L, R, Rp, S::center(B, c, jstart - 1), S::center(B, c, jstart), p,
+                                              time, a.wb, fyB, dyB, hl, hyB);
+            }
+        }
+        double lam = 0.0;
+        for (int j = jstart; j < jend; ++j) {
+            // stream in the leading row
+            if constexpr (WR > 1) {
+                if (col_in) win[WR - 1] = S::load(B, qs, c, j + HX, p, a.clamp, time);
+                else
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) win[WR - 1](n, 0) = 0.0;
+            } else {
+                if (col_in) win[0] = S::load(B, qs, c, j, p, a.clamp, time);
+                else
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) win[0](n, 0) = 0.0;
+            }
+            const VQ& qc = win[WR / 2];
+            // share row j across the workgroup
+#pragma unroll
+            for (int n = 0; n < nq; ++n) s_row[n][t] = qc(n, 0);
+            __syncthreads();
+            // x-face c-1/2 (left face of this column)
+            VQ fxL;
+            VF dxL;
+            double hxL = 0.0;
+            if (xf) {
+                VQ Lm, L, Rp;
+#pragma unroll
+                for (int n = 0; n < nq; ++n) {
+                    L(n, 0) = s_row[n][t - 1];
+                    if constexpr (ORDER == 2) {
+                        Lm(n, 0) = s_row[n][t - 2];
+                        Rp(n, 0) = s_row[n][t + 1];
+                    } else {
+                        Lm(n, 0) = 0.0;
+                        Rp(n, 0) = 0.0;
+                    }
+                }
+                double hl;
+                S::template face<ORDER, 0, N>(Lm, L, qc, Rp, S::center(B, c - 1, j), S::center(B, c, j), p, time,
+                                              a.wb, fxL, dxL, hl, hxL);
+                // the left cell needs flux, D- and its own reconstructed depth
+#pragma unroll
+                for (int n = 0; n < nq; ++n) {
+                    s_face[n][t] = fxL(n, 0);
+                    s_face[nq + n][t] = dxL(nq + n, 0);
+                }
+                s_face[2 * nq][t] = hl;
+            }
+            // top y-face j+1/2 (needs rows j-1 .. j+2)
+            VQ fyT;
+            VF dyT;
+            double hyT = 0.0, hyTnext = 0.0;
+            if constexpr (dim == 2) {
+                if (own) {
+                    const VQ& Lm = win[(HX == 2) ? 1 : 1];
+                    const VQ& L = win[HX];
+                    const VQ& R = win[HX + 1];
+                    const VQ& Rp = win[(HX == 2) ? HX + 2 : HX + 1];
+                    S::template face<ORDER, 1, N>(Lm, L, R, Rp, S::center(B, c, j), S::center(B, c, j + 1), p, time,
+                                                  a.wb, fyT, dyT, hyT, hyTnext);
+                }
+            }
+            __syncthreads();
+            if (own) {
+                double fR[nq], dmR[nq];
+#pragma unroll
+                for (int n = 0; n < nq; ++n) {
+                    fR[n] = s_face[n][t + 1];
+                    dmR[n] = s_face[nq + n][t + 1];
+                }
+                const double hL_xr = s_face[2 * nq][t + 1];
+                const typename S::VX X = S::center(B, c, j);
+                const VA ac = S::aux_of(qc, p, time, X);
+                VQ src{};
+                if (!a.impl) src = M::source(qc, ac, p, time, X);
+                VQ qn;
+#pragma unroll
+                for (int n = 0; n < nq; ++n) {
+                    const double Fx = -(fR[n] - fxL(n, 0)) - dmR[n] - dxL(n, 0);
+                    double Fy = 0.0;
+                    if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
+                    double rhs = Fx / B.dx0 + Fy / B.dx1;
+                    if (!a.impl) rhs += src(n, 0);
+                    if (a.wb) {
+                        if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
+                        if constexpr (dim == 2)
+                            if (n == 3) rhs += 0.5 * p(0, 0) * (hyT * hyT - hyB * hyB) / B.dx1;
+                    }
+                    qn(n, 0) = qc(n, 0) + dt * rhs;
+                }
+                if (a.impl) qn = S::implicit_source(qn, ac, dt, p, time, X);
+                const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+                if constexpr (EPI == 1) {
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qo[idx + n * B.sc] + 0.5 * qn(n, 0);
+                }
+                if constexpr (EPI != 0) {
+                    qn = S::update_state(qn, p, a.clamp);
+                    const VA af = S::aux_of(qn, p, time, X);
+#pragma unroll
+                    for (int d = 0; d < dim; ++d) {
+                        VN nh{};
+                        nh(d, 0) = 1.0;
+                        const double inv = N::local_max_abs_eigenvalue(qn, af, p, nh)(0, 0) / (d == 0 ? B.dx0 : B.dx1);
+                        if (inv > lam) lam = inv;
+                    }
+                }
+#pragma unroll
+                for (int n = 0; n < nq; ++n) __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);
+            }
+            // carry the top face down and shift the window
+            if constexpr (dim == 2) {
+                fyB = fyT;
+                dyB = dyT;
+                hyB = hyTnext;
+#pragma unroll
+                for (int r = 0; r < WR - 1; ++r) win[r] = win[r + 1];
+            }
+        }
+        if constexpr (EPI != 0) {
+            lam = wave_max(lam);
+            if ((t & 63) == 0) s_red[t >> 6] = lam;
+            __syncthreads();
+            if (t == 0) {
+                double m = s_red[0];
+#pragma unroll
+                for (int k = 1; k < NT / 64; ++k) m = dev_max(m, s_red[k]);
+                if (m > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(m));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ComputeDt's reduction (ZoomyAmr.cpp:465-489) over the state and aux AS
+// STORED — used for a step whose dt must see the stored aux (the first step
+// after initialisation or after the caller replaced the state).
+template <class M, class N>
+__global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__ boxes,
+                                                     const WorkItem* __restrict__ items, int nitems, int src,
+                                                     Params pp, GroupState* __restrict__ gs, int halo) {
+    using S = Sys<M>;
+    constexpr int nq = S::nq, na = S::na, dim = S::dim;
+    __shared__ double s_red[NT / 64];
+    typename S::VP p;
+    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
+    const int t = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const BoxView B = view(boxes, w.box);
+        GroupState* G = gs + B.group;
+        const int c = w.i0 - halo + t;
+        const bool own = (t >= halo) && (t < NT - halo) && (c <= B.hi0);
+        double lam = 0.0;
+        if (own) {
+            for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+                const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+                typename S::VQ q;
+                typename S::VA av;
+                for (int n = 0; n < nq; ++n) q(n, 0) = qsel(B, src)[idx + n * B.sc];
+                for (int n = 0; n < na; ++n) av(n, 0) = B.aux[idx + n * B.sc];
+                double mi = 0.0;
+                for (int d = 0; d < dim; ++d) {
+                    typename S::VN nh{};
+                    nh(d, 0) = 1.0;
+                    const double inv = N::local_max_abs_eigenvalue(q, av, p, nh)(0, 0) / (d == 0 ? B.dx0 : B.dx1);
+                    if (inv > mi) mi = inv;
+                }
+                lam = dev_max(lam, mi);
+            }
+        }
+        lam = wave_max(lam);
+        if ((t & 63) == 0) s_red[t >> 6] = lam;
+        __syncthreads();
+        if (t == 0) {
+            double m = s_red[0];
+            for (int k = 1; k < NT / 64; ++k) m = dev_max(m, s_red[k]);
+            if (m > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(m));
+        }
+        __syncthreads();
+    }
+}
+
+// Start of TimeStep for every group: dt from the reduced max (ZoomyAmr.cpp:
+// 494-514), capped to time_end (:746-749); the loop condition of Evolve
+// (:782) decides whether the group is still active.
+__global__ void begin_step_kernel(GroupState* gs, int ngroups) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    GroupState& G = gs[g];
+    G.active = (G.time < G.time_end) && (G.max_step < 0 || G.step < G.max_step);
+    const double m = __longlong_as_double((long long)G.maxinv_bits);
+    G.maxinv_bits = 0ull;
+    if (!G.active) {
+        G.dt = 0.0;
+        return;
+    }
+    double dt;
+    if (m < 1e-14) {
+        dt = G.dtmax;
+    } else {
+        dt = G.cfl / (G.divisor * m);
+        dt = dev_max((dt < G.dtmax) ? dt : G.dtmax, G.dtmin);
+    }
+    const double remaining = G.time_end - G.time;
+    if (remaining > 0.0 && dt > remaining) dt = remaining;
+    G.dt = dt;
+}
+
+// End of the step: time += dt, step += 1 (ZoomyAmr.cpp:791-793); the per-step
+// (dt, time) pair goes to a history ring the host prints the Step lines from.
+__global__ void end_step_kernel(GroupState* gs, int ngroups, double* hist, int hist_cap) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    GroupState& G = gs[g];
+    if (!G.active) return;
+    G.time += G.dt;
+    const int s = G.step;
+    G.step = s + 1;
+    if (hist && g == 0 && s < hist_cap) {
+        hist[2 * s] = G.dt;
+        hist[2 * s + 1] = G.time;
+    }
+}
+
+__global__ void set_dt_kernel(GroupState* gs, int ngroups, double dt, double time) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    gs[g].dt = dt;
+    gs[g].time = time;
+    gs[g].active = 1;
+    gs[g].maxinv_bits = 0ull;
+}
+
+// Same-level ghost exchange (FillPatchSingleLevel / FillBoundary): copy the
+// owning box's valid cells into another box's ghost cells.
+struct CopyRect {
+    int src_box, dst_box, ilo, jlo, ni, nj;
+};
+__global__ void fill_boundary_kernel(const BoxDesc* __restrict__ boxes, const CopyRect* __restrict__ rects, int buf,
+                                     int nq) {
+    const CopyRect r = rects[blockIdx.x];
+    const BoxView S = view(boxes, r.src_box);
+    const BoxView D = view(boxes, r.dst_box);
+    const int ncell = r.ni * r.nj;
+    for (int k = threadIdx.x; k < ncell * nq; k += blockDim.x) {
+        const int n = k / ncell, cell = k % ncell;
+        const int i = r.ilo + cell % r.ni, j = r.jlo + cell / r.ni;
+        const long long si = (long long)(i - S.gl0) + (long long)(j - S.gl1) * S.sy + n * S.sc;
+        const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy + n * D.sc;
+        qsel(D, buf)[di] = qsel(S, buf)[si];
+    }
+}
+
+// Qaux as the reference holds it after UpdateState (ZoomyAmr.cpp:328):
+// update_aux_variables(Q, Qaux_stored) over the valid cells.
+template <class M>
+__global__ void materialize_aux_kernel(const BoxDesc* __restrict__ boxes, const WorkItem* __restrict__ items,
+                                       int nitems, int src, Params pp, double time, int halo) {
+    using S = Sys<M>;
+    typename S::VP p;
+    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
+    const int t = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const BoxView B = view(boxes, w.box);
+        const int c = w.i0 - halo + t;
+        if (!((t >= halo) && (t < NT - halo) && (c <= B.hi0))) continue;
+        for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+            const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+            typename S::VQ q;
+            typename S::VA av;
+            for (int n = 0; n < S::nq; ++n) q(n, 0) = qsel(B, src)[idx + n * B.sc];
+            for (int n = 0; n < S::na; ++n) av(n, 0) = B.aux[idx + n * B.sc];
+            av = M::update_aux_variables(q, av, p, time, S::center(B, c, j));
+            for (int n = 0; n < S::na; ++n) B.aux[idx + n * B.sc] = av(n, 0);
+        }
+    }
+}
+
+// UpdateState over the valid cells of buffer src (in place) plus the aux.
+template <class M>
+__global__ void update_state_kernel(const BoxDesc* __restrict__ boxes, const WorkItem* __restrict__ items,
+                                    int nitems, int src, Params pp, double time, int clamp, int halo) {
+    using S = Sys<M>;
+    typename S::VP p;
+    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
+    const int t = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const BoxView B = view(boxes, w.box);
+        const int c = w.i0 - halo + t;
+        if (!((t >= halo) && (t < NT - halo) && (c <= B.hi0))) continue;
+        for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+            const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+            typename S::VQ q;
+            typename S::VA av;
+            for (int n = 0; n < S::nq; ++n) q(n, 0) = qsel(B, src)[idx + n * B.sc];
+            for (int n = 0; n < S::na; ++n) av(n, 0) = B.aux[idx + n * B.sc];
+            q = M::update_variables(q, av, p);
+            if (clamp && q(1, 0) < 0.0) q(1, 0) = 0.0;
+            av = M::update_aux_variables(q, av, p, time, S::center(B, c, j));
+            for (int n = 0; n < S::nq; ++n) qsel(B, src)[idx + n * B.sc] = q(n, 0);
+            for (int n = 0; n < S::na; ++n) B.aux[idx + n * B.sc] = av(n, 0);
+        }
+    }
+}
+
+// Per-workgroup partial sums of h over valid cells, written in item order
+// (deterministic); the host adds the partials in order.
+template <class M>
+__global__ void __launch_bounds__(NT) mass_partial_kernel(const BoxDesc* __restrict__ boxes,
+                                                          const WorkItem* __restrict__ items, int nitems, int src,
+                                                          double* partial, int halo) {
+    __shared__ double s_red[NT / 64];
+    const int t = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const BoxView B = view(boxes, w.box);
+        const int c = w.i0 - halo + t;
+        double s = 0.0;
+        if ((t >= halo) && (t < NT - halo) && (c <= B.hi0))
+            for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+                const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+                s += qsel(B, src)[idx + 1 * B.sc];
+            }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((t & 63) == 0) s_red[t >> 6] = s;
+        __syncthreads();
+        if (t == 0) {
+            double m = 0.0;
+            for (int k = 0; k < NT / 64; ++k) m += s_red[k];
+            partial[it] = m * B.dx0 * B.dx1;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace zamr

@@ -1,0 +1,112 @@
+#pragma once
+// AMReX plotfile writer for one level (WriteMultiLevelPlotfile's on-disk
+// format, "HyperCLaw-V1.1"): Header, Level_0/Cell_H and one Cell_D_00000 data
+// file holding every box as a FAB (one header line, then little-endian
+// float64, component-major, i fastest).  Variables are var0.. then aux0..
+// (Source/ZoomyAmr.cpp:851-857); this is the layout deliverable.py:33-87 reads.
+#include <sys/stat.h>
+#include <sys/types.h>
+
+#include <cerrno>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace zamr {
+
+struct PlotBox {
+    int lo[2], hi[2];
+    std::vector<double> data;  // (ncomp, ny_box, nx_box)
+};
+
+inline bool make_dir(const std::string& d) {
+    if (mkdir(d.c_str(), 0755) == 0) return true;
+    return errno == EEXIST;
+}
+
+// Concatenate(Concatenate("plt_", identifier), step, 5).
+inline std::string plotfile_name(int identifier, int step) {
+    std::ostringstream s;
+    s << "plt_" << identifier << std::setw(5) << std::setfill('0') << step;
+    return s.str();
+}
+
+inline bool write_plotfile_level0(const std::string& path, const std::vector<std::string>& names, double time,
+                                  const double plo[2], const double phi[2], int nx, int ny, const double dx[2],
+                                  int step, const std::vector<PlotBox>& boxes) {
+    const int ncomp = (int)names.size();
+    if (!make_dir(path) || !make_dir(path + "/Level_0")) return false;
+    {
+        std::ofstream h(path + "/Header");
+        if (!h) return false;
+        h << std::setprecision(17);
+        h << "HyperCLaw-V1.1\n" << ncomp << "\n";
+        for (auto& n : names) h << n << "\n";
+        h << 2 << "\n" << time << "\n" << 0 << "\n";
+        h << plo[0] << " " << plo[1] << "\n" << phi[0] << " " << phi[1] << "\n";
+        h << "\n";  // no refinement ratios on a single level
+        h << "((0,0) (" << nx - 1 << "," << ny - 1 << ") (0,0))\n";
+        h << step << "\n";
+        h << dx[0] << " " << dx[1] << "\n";
+        h << 0 << "\n" << 0 << "\n";
+        h << 0 << " " << boxes.size() << " " << time << "\n" << step << "\n";
+        for (auto& b : boxes) {
+            h << plo[0] + b.lo[0] * dx[0] << " " << plo[0] + (b.hi[0] + 1) * dx[0] << "\n";
+            h << plo[1] + b.lo[1] * dx[1] << " " << plo[1] + (b.hi[1] + 1) * dx[1] << "\n";
+        }
+        h << "Level_0/Cell\n";
+    }
+    std::vector<long long> offsets;
+    std::vector<double> mins, maxs;
+    {
+        std::ofstream d(path + "/Level_0/Cell_D_00000", std::ios::binary);
+        if (!d) return false;
+        for (auto& b : boxes) {
+            offsets.push_back((long long)d.tellp());
+            std::ostringstream fh;
+            fh << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))((" << b.lo[0] << "," << b.lo[1]
+               << ") (" << b.hi[0] << "," << b.hi[1] << ") (0,0)) " << ncomp << "\n";
+            const std::string s = fh.str();
+            d.write(s.data(), (std::streamsize)s.size());
+            d.write(reinterpret_cast<const char*>(b.data.data()), (std::streamsize)(b.data.size() * sizeof(double)));
+            const size_t per = b.data.size() / (size_t)ncomp;
+            for (int c = 0; c < ncomp; ++c) {
+                double mn = std::numeric_limits<double>::infinity(), mx = -mn;
+                for (size_t k = 0; k < per; ++k) {
+                    const double v = b.data[c * per + k];
+                    mn = v < mn ? v : mn;
+                    mx = v > mx ? v : mx;
+                }
+                mins.push_back(mn);
+                maxs.push_back(mx);
+            }
+        }
+        if (!d) return false;
+    }
+    std::ofstream c(path + "/Level_0/Cell_H");
+    if (!c) return false;
+    c << std::setprecision(17);
+    c << 1 << "\n" << 1 << "\n" << ncomp << "\n" << 0 << "\n";
+    c << "(" << boxes.size() << " 0\n";
+    for (auto& b : boxes)
+        c << "((" << b.lo[0] << "," << b.lo[1] << ") (" << b.hi[0] << "," << b.hi[1] << ") (0,0))\n";
+    c << ")\n" << boxes.size() << "\n";
+    for (size_t k = 0; k < boxes.size(); ++k) c << "FabOnDisk: Cell_D_00000 " << offsets[k] << "\n";
+    c << "\n" << boxes.size() << "," << ncomp << "\n";
+    for (size_t k = 0; k < boxes.size(); ++k) {
+        for (int n = 0; n < ncomp; ++n) c << mins[k * ncomp + n] << ",";
+        c << "\n";
+    }
+    c << "\n" << boxes.size() << "," << ncomp << "\n";
+    for (size_t k = 0; k < boxes.size(); ++k) {
+        for (int n = 0; n < ncomp; ++n) c << maxs[k * ncomp + n] << ",";
+        c << "\n";
+    }
+    return (bool)c;
+}
+
+}  // namespace zamr

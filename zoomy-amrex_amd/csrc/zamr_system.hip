@@ -1,0 +1,303 @@
+// C-ABI of include/zamr.h for one generated system.  Compiled once per system
+// with -I <system dir> so that "Model.H" / "Numerics.H" are that system's
+// printer output; the MarchConstants values come from a per-variant object
+// (zamr_march.cpp) linked beside this one.
+#include "Model.H"
+#include "Numerics.H"
+#include "zamr.h"
+#include "zamr_driver.hpp"
+#include "zamr_plotfile.hpp"
+
+extern "C" void zamr_internal_march(int* dim, int* deg);
+
+using Sol = zamr::Solver<Model, Numerics>;
+
+struct zamr_ctx {
+    Sol s;
+    std::string err;
+};
+
+namespace {
+
+template <class F>
+int guard(zamr_ctx* c, F&& f) {
+    try {
+        f();
+        return ZAMR_OK;
+    } catch (const zamr::Error& e) {
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        if (c) c->err = e.what();
+        return ZAMR_EINVAL;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return ZAMR_EHIP;
+    }
+}
+
+int copy_name(const std::vector<std::string>& v, int k, char* buf, int cap) {
+    if (k < 0 || k >= (int)v.size() || !buf || cap <= 0) return ZAMR_EINVAL;
+    const int n = std::min<int>((int)v[k].size(), cap - 1);
+    std::memcpy(buf, v[k].data(), n);
+    buf[n] = 0;
+    return ZAMR_OK;
+}
+
+}  // namespace
+
+template <class M, class N>
+void zamr::Solver<M, N>::write_plotfile(const std::string& dir, int step, double time) {
+    ensure_aux();
+    std::vector<std::string> names;
+    for (int n = 0; n < nq; ++n) names.push_back("var" + std::to_string(n));
+    for (int n = 0; n < na; ++n) names.push_back("aux" + std::to_string(n));
+    const int ncomp = nq + na;
+    std::vector<PlotBox> boxes;
+    std::vector<double> Q((size_t)nq * nx * ny), A((size_t)na * nx * ny);
+    copy_level(0, 0, Q.data(), Q.size(), false);
+    copy_level(0, 1, A.data(), A.size(), false);
+    for (auto& b : hb) {
+        PlotBox pb;
+        const int bx = b.hi[0] - b.lo[0] + 1, by = b.hi[1] - b.lo[1] + 1;
+        pb.lo[0] = b.lo[0];
+        pb.lo[1] = b.lo[1];
+        pb.hi[0] = b.hi[0];
+        pb.hi[1] = b.hi[1];
+        pb.data.resize((size_t)ncomp * bx * by);
+        for (int c = 0; c < ncomp; ++c)
+            for (int j = 0; j < by; ++j)
+                for (int i = 0; i < bx; ++i) {
+                    const size_t g = ((size_t)(j + b.lo[1])) * nx + (i + b.lo[0]);
+                    const double v = c < nq ? Q[(size_t)c * nx * ny + g] : A[(size_t)(c - nq) * nx * ny + g];
+                    pb.data[((size_t)c * by + j) * bx + i] = v;
+                }
+        boxes.push_back(std::move(pb));
+    }
+    const std::string path = (dir.empty() ? std::string() : dir + "/") + plotfile_name(identifier, step);
+    const double dx[2] = {(phi[0] - plo[0]) / nx, (phi[1] - plo[1]) / ny};
+    if (!write_plotfile_level0(path, names, time, plo, phi, nx, ny, dx, step, boxes))
+        throw Error(ZAMR_EIO, "cannot write plotfile " + path);
+}
+
+extern "C" {
+
+int zamr_model_info(int* nq, int* na, int* np, int* dim, int* ntags) {
+    if (nq) *nq = Model::n_dof_q;
+    if (na) *na = Model::n_dof_qaux;
+    if (np) *np = Model::n_parameters;
+    if (dim) *dim = Model::dimension;
+    if (ntags) *ntags = (int)Model::get_boundary_tags().size();
+    return ZAMR_OK;
+}
+
+int zamr_tag_name(int k, char* buf, int cap) { return copy_name(Model::get_boundary_tags(), k, buf, cap); }
+int zamr_param_name(int k, char* buf, int cap) { return copy_name(Model::parameter_names(), k, buf, cap); }
+int zamr_default_param(int k, double* v) {
+    auto d = Model::default_parameters();
+    if (k < 0 || k >= (int)d.size() || !v) return ZAMR_EINVAL;
+    *v = d[k];
+    return ZAMR_OK;
+}
+int zamr_march_constants(int* d, int* k) {
+    zamr_internal_march(d, k);
+    return ZAMR_OK;
+}
+
+int zamr_create(const char* text, zamr_ctx** out) {
+    if (!text || !out) return ZAMR_EINVAL;
+    *out = nullptr;
+    auto* c = new zamr_ctx();
+    zamr_internal_march(&c->s.c_cfl_dimension, &c->s.c_cfl_degree_factor);
+    int rc = guard(c, [&] { c->s.configure_deck(text); });
+    if (rc != ZAMR_OK) {
+        // keep the context so the caller can read the message, but nothing else works
+        c->s.release();
+    }
+    *out = c;
+    return rc;
+}
+
+int zamr_create_boxlist(const zamr_box* boxes, int nbox, const char* text, zamr_ctx** out) {
+    if (!boxes || !out) return ZAMR_EINVAL;
+    *out = nullptr;
+    auto* c = new zamr_ctx();
+    zamr_internal_march(&c->s.c_cfl_dimension, &c->s.c_cfl_degree_factor);
+    int rc = guard(c, [&] { c->s.configure_boxlist(boxes, nbox, text ? text : ""); });
+    if (rc != ZAMR_OK) c->s.release();
+    *out = c;
+    return rc;
+}
+
+int zamr_destroy(zamr_ctx* c) {
+    delete c;
+    return ZAMR_OK;
+}
+
+const char* zamr_last_error(const zamr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int zamr_init_data(zamr_ctx* c) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.init_data(); });
+}
+
+int zamr_level_shape(const zamr_ctx* c, int lev, int* nx, int* ny, int* nboxes) {
+    if (!c) return ZAMR_EINVAL;
+    if (c->s.batched) {
+        if (lev < 0 || lev >= (int)c->s.hb.size()) return ZAMR_EINVAL;
+        auto& b = c->s.hb[lev];
+        if (nx) *nx = b.hi[0] - b.lo[0] + 1;
+        if (ny) *ny = b.hi[1] - b.lo[1] + 1;
+    } else {
+        if (lev != 0) return ZAMR_EINVAL;
+        if (nx) *nx = c->s.nx;
+        if (ny) *ny = c->s.ny;
+    }
+    if (nboxes) *nboxes = (int)c->s.hb.size();
+    return ZAMR_OK;
+}
+
+int zamr_set_state(zamr_ctx* c, int lev, int field, const double* host, size_t n) {
+    if (!c || !host || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.copy_level(lev, field, const_cast<double*>(host), n, true); });
+}
+int zamr_get_state(zamr_ctx* c, int lev, int field, double* host, size_t n) {
+    if (!c || !host || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.copy_level(lev, field, host, n, false); });
+}
+
+int zamr_compute_dt(zamr_ctx* c, int lev, double* dt) {
+    if (!c || !dt || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        *dt = c->s.compute_dt_host();
+    });
+}
+int zamr_update_state(zamr_ctx* c, int lev, double time) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        c->s.update_state_host(time);
+    });
+}
+int zamr_fill_patch(zamr_ctx* c, int lev, double time) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        (void)time;
+        c->s.fill_boundary(c->s.cur);
+        ZAMR_HIP(hipStreamSynchronize(c->s.stream));
+    });
+}
+int zamr_stage(zamr_ctx* c, int lev, int order, double time, double dt) {
+    if (!c || !c->s.d_pool || (order != 1 && order != 2)) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        c->s.stage_host(order, time, dt);
+    });
+}
+int zamr_advance(zamr_ctx* c, int lev, double time, double dt) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        c->s.advance_host(time, dt);
+    });
+}
+int zamr_time_step(zamr_ctx* c, int lev, double time, int iteration, double* dt_taken) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    (void)iteration;
+    return guard(c, [&] {
+        c->s.level_box_check(lev);
+        const double dt = c->s.time_step_host(time);
+        if (dt_taken) *dt_taken = dt;
+    });
+}
+int zamr_run_steps(zamr_ctx* c, int nsteps, int* done, double* time) {
+    if (!c || !c->s.d_pool || nsteps < 0) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.run_steps(nsteps, done, time); });
+}
+int zamr_evolve(zamr_ctx* c) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.evolve(); });
+}
+int zamr_total_mass(zamr_ctx* c, double* m) {
+    if (!c || !m || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { *m = c->s.total_mass(); });
+}
+int zamr_get_time(const zamr_ctx* cc, int lev, double* time, int* step, double* dt) {
+    auto* c = const_cast<zamr_ctx*>(cc);
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.download_groups();
+        const int g = c->s.batched ? lev : 0;
+        if (g < 0 || g >= c->s.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
+        if (time) *time = c->s.h_groups[g].time;
+        if (step) *step = c->s.h_groups[g].step;
+        if (dt) *dt = c->s.h_groups[g].dt;
+    });
+}
+int zamr_reset_clock(zamr_ctx* c) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] { c->s.reset_clock(); });
+}
+int zamr_write_plotfile(zamr_ctx* c, const char* dir, int step, double time) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        if (c->s.batched) throw zamr::Error(ZAMR_EUNSUP, "plotfiles are written for single-level runs only");
+        c->s.write_plotfile(dir ? dir : "", step, time);
+    });
+}
+int zamr_device_ptr(zamr_ctx* c, int lev, int box, int field, void** ptr, long long* rs, long long* cs, int* ng,
+                    void** stream) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        int b = box;
+        if (c->s.batched) {
+            b = lev;
+        } else if (lev != 0) {
+            throw zamr::Error(ZAMR_EINVAL, "only level 0 exists in this build");
+        }
+        if (b < 0 || b >= (int)c->s.hb.size()) throw zamr::Error(ZAMR_EINVAL, "box index out of range");
+        if (field != 0 && field != 1) throw zamr::Error(ZAMR_EINVAL, "field must be 0 (Q) or 1 (Qaux)");
+        auto& B = c->s.hb[b];
+        if (field == 1) c->s.ensure_aux();
+        if (ptr) *ptr = c->s.d_pool + (field == 0 ? B.off_q[c->s.cur] : B.off_aux);
+        if (rs) *rs = B.gx;
+        if (cs) *cs = (long long)B.gx * B.gy;
+        if (ng) *ng = B.ng;
+        if (stream) *stream = (void*)c->s.stream;
+    });
+}
+int zamr_set_stream(zamr_ctx* c, void* stream) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        ZAMR_HIP(hipStreamSynchronize(c->s.stream));
+        if (c->s.own_stream) hipStreamDestroy(c->s.stream);
+        c->s.own_stream = false;
+        c->s.stream = (hipStream_t)stream;
+        c->s.invalidate_graph();
+    });
+}
+int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
+    if (!c || rows < 0) return ZAMR_EINVAL;
+    c->s.rows_per_item = rows;
+    c->s.per_box_launch = per_box;
+    c->s.xcd_remap = xcd;
+    c->s.invalidate_graph();
+    return ZAMR_OK;
+}
+int zamr_set_profiling(zamr_ctx* c, int on) {
+    if (!c) return ZAMR_EINVAL;
+    c->s.profiling = on != 0;
+    c->s.invalidate_graph();
+    return ZAMR_OK;
+}
+int zamr_kernel_times(zamr_ctx* c, double* out, int* n) {
+    if (!c || !out) return ZAMR_EINVAL;
+    const int k = c->s.prof_steps;
+    for (int i = 0; i < 3; ++i) out[i] = k ? c->s.prof_ms[i] / k : 0.0;
+    if (n) *n = k;
+    return ZAMR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+"""Seeded synthetic workloads.
+
+Random numbers come from SplitMix64 (Steele, Lea & Flood 2014): a 64-bit
+counter advanced by the golden-ratio increment and finalised by two
+xor-shift-multiply rounds; U[0,1) takes the top 53 bits.  The same arrays
+feed the HIP library and the CPU oracle.
+"""
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    def __init__(self, seed=0):
+        self.state = np.uint64(seed)
+
+    def next_u64(self, n):
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + k * _GOLDEN
+            self.state = z[-1] if n else self.state
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            return z ^ (z >> np.uint64(31))
+
+    def uniform(self, n, lo=0.0, hi=1.0):
+        u = (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+
+def cell_centers(nx, ny, plo=(0.0, 0.0), phi=(1.0, 1.0)):
+    dx = (phi[0] - plo[0]) / nx
+    dy = (phi[1] - plo[1]) / ny
+    x = plo[0] + (np.arange(nx) + 0.5) * dx
+    y = plo[1] + (np.arange(ny) + 0.5) * dy
+    return x, y
+
+
+def gaussian_bumps(x, y, rng, nbumps, amp=(0.0, 0.8), sigma=(0.02, 0.1)):
+    """Sum of nbumps Gaussians A exp(-|X-c|^2 / (2 s^2)), separable in x and y."""
+    A = rng.uniform(nbumps, *amp)
+    s = rng.uniform(nbumps, *sigma)
+    cx = rng.uniform(nbumps)
+    cy = rng.uniform(nbumps)
+    b = np.zeros((len(y), len(x)))
+    for k in range(nbumps):
+        gx = np.exp(-((x - cx[k]) ** 2) / (2 * s[k] ** 2))
+        gy = np.exp(-((y - cy[k]) ** 2) / (2 * s[k] ** 2))
+        b += A[k] * np.outer(gy, gx)
+    return b
+
+
+def surface_perturbation(x, y, rng, nmodes=8, kmax=8, amplitude=1e-2):
+    """Random-phase sum of nmodes plane waves with integer wave vectors
+    |k_x|, |k_y| <= kmax, scaled so that its maximum magnitude is amplitude."""
+    kx = np.floor(rng.uniform(nmodes, -kmax, kmax + 1)).astype(int)
+    ky = np.floor(rng.uniform(nmodes, -kmax, kmax + 1)).astype(int)
+    ph = rng.uniform(nmodes, 0.0, 2 * np.pi)
+    a = rng.uniform(nmodes, 0.5, 1.0)
+    eta = np.zeros((len(y), len(x)))
+    for m in range(nmodes):
+        # sin(a+b) expanded so the 2-D field is built from 1-D factors
+        sx, cx = np.sin(2 * np.pi * kx[m] * x), np.cos(2 * np.pi * kx[m] * x)
+        sy, cy = np.sin(2 * np.pi * ky[m] * y + ph[m]), np.cos(2 * np.pi * ky[m] * y + ph[m])
+        eta += a[m] * (np.outer(cy, sx) + np.outer(sy, cx))
+    mx = np.abs(eta).max()
+    return eta * (amplitude / mx) if mx > 0 else eta
+
+
+def dam_break_2d(n=100, time_end=0.1):
+    """The reference's default deck (Exec/inputs): 2-D dam break on [0,1]^2,
+    h = 2 left of x = 0.5 and 1 right, flat bed, order 2, CFL 0.45,
+    extrapolation, explicit source, t_end = 0.1.  The state is init_solution's."""
+    case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=64, spatial_order=2,
+                implicit_source=False, well_balanced=False, clamp_positivity=True, cfl=0.45, dtmin=1e-7,
+                dtmax=0.1, time_end=time_end, max_step=-1, bc=("", "", "", ""), params={})
+    return case, None
+
+
+def swe_bumps(n=4096, seed=0, nbumps=32, steps=100, manning=0.03, max_grid_size=None):
+    """Single-level 2-D shallow water over nbumps seeded Gaussian bumps:
+    eta = 1 + a 1e-2 random-phase perturbation, h = max(0, eta - b) (dry
+    where b > eta), zero velocity, Manning friction through the point-implicit
+    source, well-balanced reconstruction, order 2, `steps` SSP-RK2 steps."""
+    rng = SplitMix64(seed)
+    x, y = cell_centers(n, n)
+    b = gaussian_bumps(x, y, rng, nbumps)
+    eta = 1.0 + surface_perturbation(x, y, rng)
+    h = np.maximum(0.0, eta - b)
+    z = np.zeros_like(h)
+    Q = np.stack([b, h, z, z])
+    case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0),
+                max_grid_size=max_grid_size or n, spatial_order=2, implicit_source=True,
+                well_balanced=True, clamp_positivity=True, cfl=0.45, dtmin=1e-7, dtmax=1.0,
+                time_end=1.0e30, max_step=steps, bc=("", "", "", ""), params={"n_m": manning})
+    return case, Q

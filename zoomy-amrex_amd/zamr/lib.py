@@ -1,0 +1,233 @@
+"""ctypes binding of include/zamr.h and a ZoomyAmr-shaped Python handle.
+
+The product path is the compiled HIP library only: if the variant's .so is
+missing this module raises; there is no CPU fallback.
+"""
+import ctypes
+import pathlib
+
+import numpy as np
+
+from . import compile as _compile
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+
+ERRORS = {1: "EINVAL", 2: "ETAG", 3: "ERASTER", 4: "EHIP", 5: "EUNSUP", 6: "EIO"}
+
+
+class ZamrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"zamr error {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ZamrBox(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_int * 2), ("hi", ctypes.c_int * 2), ("dx", ctypes.c_double * 2),
+                ("prob_lo", ctypes.c_double * 2), ("group", ctypes.c_int), ("bc", ctypes.c_char_p * 4)]
+
+
+_cache = {}
+
+
+def load_library(path):
+    path = str(pathlib.Path(path).resolve())
+    if path in _cache:
+        return _cache[path]
+    if not pathlib.Path(path).exists():
+        raise FileNotFoundError(f"zamr library {path} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    sig = {
+        "zamr_model_info": [_I, _I, _I, _I, _I],
+        "zamr_tag_name": [ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+        "zamr_param_name": [ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+        "zamr_default_param": [ctypes.c_int, _D],
+        "zamr_march_constants": [_I, _I],
+        "zamr_create": [ctypes.c_char_p, ctypes.POINTER(_VP)],
+        "zamr_create_boxlist": [ctypes.POINTER(ZamrBox), ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(_VP)],
+        "zamr_destroy": [_VP],
+        "zamr_init_data": [_VP],
+        "zamr_level_shape": [_VP, ctypes.c_int, _I, _I, _I],
+        "zamr_set_state": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t],
+        "zamr_get_state": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t],
+        "zamr_compute_dt": [_VP, ctypes.c_int, _D],
+        "zamr_update_state": [_VP, ctypes.c_int, ctypes.c_double],
+        "zamr_fill_patch": [_VP, ctypes.c_int, ctypes.c_double],
+        "zamr_stage": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
+        "zamr_advance": [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double],
+        "zamr_time_step": [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int, _D],
+        "zamr_run_steps": [_VP, ctypes.c_int, _I, _D],
+        "zamr_evolve": [_VP],
+        "zamr_total_mass": [_VP, _D],
+        "zamr_get_time": [_VP, ctypes.c_int, _D, _I, _D],
+        "zamr_reset_clock": [_VP],
+        "zamr_write_plotfile": [_VP, ctypes.c_char_p, ctypes.c_int, ctypes.c_double],
+        "zamr_device_ptr": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP),
+                            ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _I,
+                            ctypes.POINTER(_VP)],
+        "zamr_set_stream": [_VP, _VP],
+        "zamr_set_tuning": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "zamr_set_profiling": [_VP, ctypes.c_int],
+        "zamr_kernel_times": [_VP, _D, _I],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    lib.zamr_last_error.argtypes = [_VP]
+    lib.zamr_last_error.restype = ctypes.c_char_p
+    _cache[path] = lib
+    return lib
+
+
+def library_for(system, dim, order, contract=True):
+    return load_library(_compile.lib_path(system, dim, order, contract))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class ZoomyAmr:
+    """One context of a system library, mirroring the reference driver class
+    (Source/ZoomyAmr.H:14-166): InitData, ComputeDt, Advance, TimeStep,
+    Evolve, ComputeTotalMass, WritePlotFile, plus state I/O."""
+
+    def __init__(self, inputs_text, system="swe2d", dim=None, order=None, contract=True, lib=None, boxes=None):
+        if lib is None:
+            if order is None:
+                order = _deck_int(inputs_text, "solver.spatial_order", 1)
+            if dim is None:
+                dim = 1 if system.endswith("1d") else 2
+            lib = library_for(system, dim, order, contract)
+        self.lib = lib
+        self.ctx = _VP()
+        if boxes is None:
+            rc = lib.zamr_create(inputs_text.encode(), ctypes.byref(self.ctx))
+        else:
+            arr = (ZamrBox * len(boxes))(*boxes)
+            self._boxes_keepalive = (arr, boxes)
+            rc = lib.zamr_create_boxlist(arr, len(boxes), inputs_text.encode(), ctypes.byref(self.ctx))
+        if rc != 0:
+            msg = lib.zamr_last_error(self.ctx).decode() if self.ctx else "create failed"
+            self.close()
+            raise ZamrError(rc, msg)
+        nq, na, np_, d, nt = (ctypes.c_int() for _ in range(5))
+        lib.zamr_model_info(*(ctypes.byref(v) for v in (nq, na, np_, d, nt)))
+        self.nq, self.na, self.nparams, self.dim = nq.value, na.value, np_.value, d.value
+        buf = ctypes.create_string_buffer(128)
+        self.tags = []
+        for k in range(nt.value):
+            lib.zamr_tag_name(k, buf, 128)
+            self.tags.append(buf.value.decode())
+        self.param_names = []
+        for k in range(self.nparams):
+            lib.zamr_param_name(k, buf, 128)
+            self.param_names.append(buf.value.decode())
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ZamrError(rc, self.lib.zamr_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.zamr_destroy(self.ctx)
+        self.ctx = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, lev=0):
+        nx, ny, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.zamr_level_shape(self.ctx, lev, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nb)))
+        return nx.value, ny.value, nb.value
+
+    def init_data(self):
+        self._check(self.lib.zamr_init_data(self.ctx))
+
+    def set_state(self, arr, field=0, lev=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._check(self.lib.zamr_set_state(self.ctx, lev, field, _dp(a), a.size))
+
+    def get_state(self, field=0, lev=0):
+        nx, ny, _ = self.shape(lev)
+        nc = self.nq if field == 0 else self.na
+        a = np.empty((nc, ny, nx))
+        self._check(self.lib.zamr_get_state(self.ctx, lev, field, _dp(a), a.size))
+        return a
+
+    def compute_dt(self, lev=0):
+        dt = ctypes.c_double()
+        self._check(self.lib.zamr_compute_dt(self.ctx, lev, ctypes.byref(dt)))
+        return dt.value
+
+    def update_state(self, time=0.0, lev=0):
+        self._check(self.lib.zamr_update_state(self.ctx, lev, time))
+
+    def stage(self, order, time, dt, lev=0):
+        self._check(self.lib.zamr_stage(self.ctx, lev, order, time, dt))
+
+    def advance(self, time, dt, lev=0):
+        self._check(self.lib.zamr_advance(self.ctx, lev, time, dt))
+
+    def time_step(self, time, iteration=0, lev=0):
+        dt = ctypes.c_double()
+        self._check(self.lib.zamr_time_step(self.ctx, lev, time, iteration, ctypes.byref(dt)))
+        return dt.value
+
+    def run_steps(self, n):
+        done, t = ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.zamr_run_steps(self.ctx, n, ctypes.byref(done), ctypes.byref(t)))
+        return done.value, t.value
+
+    def evolve(self):
+        self._check(self.lib.zamr_evolve(self.ctx))
+
+    def total_mass(self):
+        m = ctypes.c_double()
+        self._check(self.lib.zamr_total_mass(self.ctx, ctypes.byref(m)))
+        return m.value
+
+    def get_time(self, group=0):
+        t, s, dt = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.zamr_get_time(self.ctx, group, ctypes.byref(t), ctypes.byref(s), ctypes.byref(dt)))
+        return t.value, s.value, dt.value
+
+    def reset_clock(self):
+        self._check(self.lib.zamr_reset_clock(self.ctx))
+
+    def write_plotfile(self, directory, step, time):
+        self._check(self.lib.zamr_write_plotfile(self.ctx, str(directory).encode(), step, time))
+
+    def device_ptr(self, field=0, box=0, lev=0):
+        p, rs, cs, ng, st = _VP(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int(), _VP()
+        self._check(self.lib.zamr_device_ptr(self.ctx, lev, box, field, ctypes.byref(p), ctypes.byref(rs),
+                                             ctypes.byref(cs), ctypes.byref(ng), ctypes.byref(st)))
+        return p.value, rs.value, cs.value, ng.value, st.value
+
+    def set_stream(self, stream_ptr):
+        self._check(self.lib.zamr_set_stream(self.ctx, _VP(stream_ptr)))
+
+    def set_tuning(self, rows=0, per_box_launch=0, xcd_remap=1):
+        self._check(self.lib.zamr_set_tuning(self.ctx, rows, per_box_launch, xcd_remap))
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.zamr_set_profiling(self.ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        out = np.zeros(3)
+        n = ctypes.c_int()
+        self._check(self.lib.zamr_kernel_times(self.ctx, _dp(out), ctypes.byref(n)))
+        return out, n.value
+
+
+def _deck_int(text, key, default):
+    for line in text.splitlines():
+        line = line.split("#", 1)[0]
+        if "=" in line and line.split("=", 1)[0].strip() == key:
+            return int(float(line.split("=", 1)[1].split()[0]))
+    return default

@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the run, copy a 1 GiB buffer with a plain device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("this benchmark runs on one GPU only; the multi-GPU decomposition is a design in DESIGN.md",
@@ -131,6 +133,14 @@ def main():
                      else 0.0,
                      "timed_launches": nk},
     }
+    if args.calibrate_copy:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * a.numel() * 8
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         del z

@@ -109,6 +109,27 @@ struct StageArgs {
     int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
 };
 
+// The bitwise build (-DZAMR_BITWISE, FMA contraction off) performs every
+// division the reference performs, in the reference's order, and agrees with
+// the CPU oracle bit for bit.  The throughput build multiplies by reciprocals
+// where the reference divides by a cell size or a pivot (results within a few
+// ulps; the parity bar is 1e-12 relative).
+#ifdef ZAMR_BITWISE
+constexpr bool kBitwise = true;
+#else
+constexpr bool kBitwise = false;
+#endif
+
+// 1/x: the hardware estimate refined by two Newton steps (normal range).
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ T dev_max(T a, T b) { return (a < b) ? b : a; }
 
@@ -218,8 +239,14 @@ struct Sys {
             const double bf = dev_max(bL, bR);
             hLs = dev_max(0.0, qL(1, 0) + bL - bf);
             hRs = dev_max(0.0, qR(1, 0) + bR - bf);
-            const double sL = (qL(1, 0) > 0.0) ? hLs / qL(1, 0) : 0.0;
-            const double sR = (qR(1, 0) > 0.0) ? hRs / qR(1, 0) : 0.0;
+            double sL, sR;
+            if constexpr (kBitwise) {
+                sL = (qL(1, 0) > 0.0) ? hLs / qL(1, 0) : 0.0;
+                sR = (qR(1, 0) > 0.0) ? hRs / qR(1, 0) : 0.0;
+            } else {
+                sL = (qL(1, 0) > 0.0) ? hLs * recip(qL(1, 0)) : 0.0;
+                sR = (qR(1, 0) > 0.0) ? hRs * recip(qR(1, 0)) : 0.0;
+            }
 #pragma unroll
             for (int d = 0; d < dim; ++d) {
                 qL(2 + d, 0) *= sL;
@@ -253,10 +280,14 @@ struct Sys {
             rhs[r] = dt * S(r, 0);
 #pragma unroll
             for (int c = 0; c < nq; ++c) {
-                double term = dS_dQ(r * nq + c, 0);
+                // The chain-rule term is summed first and added once.  The
+                // printer emits dA/dQ == 0 (to_amrex.py:478-480), so the sum is
+                // +-0 or NaN and the entry equals make_rhs.H's running sum bit
+                // for bit, while the compiler can share the sum across c.
+                double chain = 0.0;
 #pragma unroll
-                for (int k = 0; k < na; ++k) term += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
-                A[r][c] = (r == c ? 1.0 : 0.0) - dt * term;
+                for (int k = 0; k < na; ++k) chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
+                A[r][c] = (r == c ? 1.0 : 0.0) - dt * (dS_dQ(r * nq + c, 0) + chain);
             }
         }
         bool ok = true;
@@ -271,27 +302,41 @@ struct Sys {
                 if (!(best > 0.0) || !isfinite(best)) {
                     ok = false;
                 } else {
-                    // Row swap by predication: registers cannot be indexed.
+                    // Row swap by predication (registers cannot be indexed),
+                    // skipped as a whole when no lane of the wave pivots.
+                    if (__any(piv != k)) {
 #pragma unroll
-                    for (int r = k + 1; r < nq; ++r) {
-                        if (r == piv) {
+                        for (int r = k + 1; r < nq; ++r) {
+                            if (r == piv) {
 #pragma unroll
-                            for (int c = k; c < nq; ++c) {
-                                const double t = A[k][c];
-                                A[k][c] = A[r][c];
-                                A[r][c] = t;
+                                for (int c = k; c < nq; ++c) {
+                                    const double t = A[k][c];
+                                    A[k][c] = A[r][c];
+                                    A[r][c] = t;
+                                }
+                                const double t = rhs[k];
+                                rhs[k] = rhs[r];
+                                rhs[r] = t;
                             }
-                            const double t = rhs[k];
-                            rhs[k] = rhs[r];
-                            rhs[r] = t;
                         }
                     }
+                    if constexpr (kBitwise) {
 #pragma unroll
-                    for (int r = k + 1; r < nq; ++r) {
-                        const double f = A[r][k] / A[k][k];
+                        for (int r = k + 1; r < nq; ++r) {
+                            const double f = A[r][k] / A[k][k];
 #pragma unroll
-                        for (int c = k; c < nq; ++c) A[r][c] -= f * A[k][c];
-                        rhs[r] -= f * rhs[k];
+                            for (int c = k; c < nq; ++c) A[r][c] -= f * A[k][c];
+                            rhs[r] -= f * rhs[k];
+                        }
+                    } else {
+                        const double inv = recip(A[k][k]);
+#pragma unroll
+                        for (int r = k + 1; r < nq; ++r) {
+                            const double f = A[r][k] * inv;
+#pragma unroll
+                            for (int c = k; c < nq; ++c) A[r][c] -= f * A[k][c];
+                            rhs[r] -= f * rhs[k];
+                        }
                     }
                 }
             }
@@ -304,7 +349,8 @@ struct Sys {
                 double sum = 0.0;
 #pragma unroll
                 for (int c = r + 1; c < nq; ++c) sum += A[r][c] * dq[c];
-                dq[r] = (rhs[r] - sum) / A[r][r];
+                if constexpr (kBitwise) dq[r] = (rhs[r] - sum) / A[r][r];
+                else dq[r] = (rhs[r] - sum) * recip(A[r][r]);
                 if (!isfinite(dq[r])) ok = false;
             }
             if (ok) {
@@ -375,15 +421,32 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
         GroupState* G = gs + B.group;
-        if (!G->active) continue;
+        const int c = w.i0 - HX + t;  // this thread's column
+        if (!G->active) {
+            // An order-1 step ping-pongs buffers; a finished group carries its
+            // state across so that every group's current buffer stays in step.
+            if constexpr (EPI == 2) {
+                if ((t >= HX) && (t < HX + TXC) && (c <= B.hi0)) {
+                    const double* __restrict__ qs = a.src ? B.q1 : B.q0;
+                    double* __restrict__ qd = a.dst ? B.q1 : B.q0;
+                    for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+                        const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+#pragma unroll
+                        for (int n = 0; n < nq; ++n) qd[idx + n * B.sc] = qs[idx + n * B.sc];
+                    }
+                }
+            }
+            continue;
+        }
         const double dt = G->dt;
         const double time = G->time;
+        const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;             // uniform per item
+        const double hgx = 0.5 * p(0, 0) * rdx, hgy = 0.5 * p(0, 0) * rdy;
         // select, never index: a runtime index would push the descriptor to scratch
         const double* __restrict__ qs = a.src ? B.q1 : B.q0;
         double* __restrict__ qd = a.dst ? B.q1 : B.q0;
         const double* __restrict__ qo = a.old ? B.q1 : B.q0;
 
-        const int c = w.i0 - HX + t;                       // this thread's column
         const bool col_in = c <= B.hi0 + HX;               // inside the grown box
         const bool own = (t >= HX) && (t < HX + TXC) && (c <= B.hi0);  // assembles cells
         const bool xf = (t >= HX) && (t <= HX + TXC) && (c <= B.hi0 + 1);  // computes face c-1/2
@@ -495,12 +558,18 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
                     const double Fx = -(fR[n] - fxL(n, 0)) - dmR[n] - dxL(n, 0);
                     double Fy = 0.0;
                     if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
-                    double rhs = Fx / B.dx0 + Fy / B.dx1;
+                    double rhs = kBitwise ? Fx / B.dx0 + Fy / B.dx1 : Fx * rdx + Fy * rdy;
                     if (!a.impl) rhs += src(n, 0);
                     if (a.wb) {
-                        if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
-                        if constexpr (dim == 2)
-                            if (n == 3) rhs += 0.5 * p(0, 0) * (hyT * hyT - hyB * hyB) / B.dx1;
+                        if constexpr (kBitwise) {
+                            if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
+                            if constexpr (dim == 2)
+                                if (n == 3) rhs += 0.5 * p(0, 0) * (hyT * hyT - hyB * hyB) / B.dx1;
+                        } else {
+                            if (n == 2) rhs += (hL_xr * hL_xr - hxL * hxL) * hgx;
+                            if constexpr (dim == 2)
+                                if (n == 3) rhs += (hyT * hyT - hyB * hyB) * hgy;
+                        }
                     }
                     qn(n, 0) = qc(n, 0) + dt * rhs;
                 }
@@ -517,7 +586,8 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
                     for (int d = 0; d < dim; ++d) {
                         VN nh{};
                         nh(d, 0) = 1.0;
-                        const double inv = N::local_max_abs_eigenvalue(qn, af, p, nh)(0, 0) / (d == 0 ? B.dx0 : B.dx1);
+                        const double ev = N::local_max_abs_eigenvalue(qn, af, p, nh)(0, 0);
+                        const double inv = kBitwise ? ev / (d == 0 ? B.dx0 : B.dx1) : ev * (d == 0 ? rdx : rdy);
                         if (inv > lam) lam = inv;
                     }
                 }

@@ -71,7 +71,7 @@ def compile_system(system_dir, name, variants, contract=True, objdir=OBJDIR, lib
     libdir.mkdir(parents=True, exist_ok=True)
     tag = "" if contract else "_nofma"
     obj = objdir / f"{name}{tag}.o"
-    fp = ["-ffp-contract=fast"] if contract else ["-ffp-contract=off"]
+    fp = ["-ffp-contract=fast"] if contract else ["-ffp-contract=off", "-DZAMR_BITWISE"]
     _run([HIPCC, *HIP_FLAGS, *fp, f"-I{COMPAT}", f"-I{system_dir}", f"-I{INCLUDE}", f"-I{CSRC}",
           "-c", str(CSRC / "zamr_system.hip"), "-o", str(obj)])
     outs = []
@@ -83,9 +83,18 @@ def compile_system(system_dir, name, variants, contract=True, objdir=OBJDIR, lib
         _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-x", "c++", f"-I{COMPAT}", f"-I{vdir}", "-c",
               str(CSRC / "zamr_march.cpp"), "-o", str(mobj)])
         out = libdir / f"libzamr_{name}_d{dim}o{order}{tag}.so"
-        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", str(obj), str(mobj), "-o", str(out)])
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", f"-Wl,-soname,{out.name}", str(obj), str(mobj),
+              "-o", str(out)])
         outs.append(out)
+        # the drop-in executable `main_<variant>.ex inputs` (Source/main.cpp counterpart)
+        exe = libdir / f"main_{name}_d{dim}o{order}{tag}.ex"
+        _run([shutil.which("g++") or "g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(CSRC / "zamr_main.cpp"),
+              str(out), "-Wl,-rpath,$ORIGIN", "-o", str(exe)])
     return outs
+
+
+def main_path(system, dim, order, contract=True):
+    return LIBDIR / f"main_{system}_d{dim}o{order}{'' if contract else '_nofma'}.ex"
 
 
 def build_all(jobs=4, with_nofma=True):

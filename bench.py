@@ -134,13 +134,14 @@ def main():
                      "timed_launches": nk},
     }
     if args.calibrate_copy:
+        # a plain fill (writes 1 GiB) and a plain sum (reads 1 GiB)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         a.fill_(1.0)
-        b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        s = a.sum()
         torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = 2 * a.numel() * 8
+        out["calibration_bytes"] = a.numel() * 8
+        out["calibration_sum"] = float(s)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         del z

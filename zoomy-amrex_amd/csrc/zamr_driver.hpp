@@ -401,7 +401,7 @@ class Solver {
         if (it.d) hipFree(it.d), it.d = nullptr;
         std::vector<WorkItem> v;
         it.box_begin.clear();
-        const int txc = NT - 2 * hx;
+        const int txc = WV - 2 * hx;  // one wave64 per item, HX halo columns each side
         for (size_t b = 0; b < hb.size(); ++b) {
             it.box_begin.push_back((int)v.size());
             const HostBox& B = hb[b];
@@ -566,6 +566,7 @@ class Solver {
 
     // ---------------- kernels ----------------
     int hx_of(int order) const { return order == 2 ? 2 : 1; }
+    static int aux_grid(int nitems) { return (nitems + NT / WV - 1) / (NT / WV); }
 
     StageArgs stage_args(int src, int dst, int old) const {
         StageArgs a;
@@ -593,11 +594,11 @@ class Solver {
             for (size_t b = 0; b + 1 < it.box_begin.size(); ++b) {
                 const int n0 = it.box_begin[b], n = it.box_begin[b + 1] - n0;
                 if (n <= 0) continue;
-                hipLaunchKernelGGL(kern, dim3(n), dim3(NT), 0, stream, d_boxes, it.d + n0, n, a, d_groups, 0);
+                hipLaunchKernelGGL(kern, dim3(slots_for(n)), dim3(NT), 0, stream, d_boxes, it.d + n0, n, a, d_groups, 0);
             }
         } else {
-            const int grid = ((it.n + 7) / 8) * 8;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, d_boxes, it.d, it.n, a, d_groups, xcd_remap);
+            hipLaunchKernelGGL(kern, dim3(slots_for(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n, a, d_groups,
+                               xcd_remap);
         }
         ZAMR_HIP(hipGetLastError());
     }
@@ -631,8 +632,8 @@ class Solver {
         for (auto& g : h_groups) g.maxinv_bits = 0ull;
         upload_groups();
         Items& it = items(hx_of(spatial_order));
-        hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur, prm,
-                           d_groups, it.hx);
+        hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
+                           cur, prm, d_groups, WV - 2 * it.hx);
         ZAMR_HIP(hipGetLastError());
         maxinv_fresh = true;
     }
@@ -644,8 +645,8 @@ class Solver {
     void materialize_aux() {
         Items& it = items(hx_of(spatial_order));
         download_groups();
-        hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur,
-                           prm, h_groups[0].time, it.hx);
+        hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d,
+                           it.n, cur, prm, h_groups[0].time, WV - 2 * it.hx);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipStreamSynchronize(stream));
         aux_stale = false;
@@ -795,8 +796,8 @@ class Solver {
 
     void update_state_host(double time) {
         Items& it = items(hx_of(spatial_order));
-        hipLaunchKernelGGL((update_state_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur, prm,
-                           time, (clamp_positivity && positivity != "mood") ? 1 : 0, it.hx);
+        hipLaunchKernelGGL((update_state_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
+                           cur, prm, time, (clamp_positivity && positivity != "mood") ? 1 : 0, WV - 2 * it.hx);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipStreamSynchronize(stream));
         maxinv_fresh = false;
@@ -804,8 +805,8 @@ class Solver {
 
     double total_mass() {  // ComputeTotalMass, level 0 (deterministic item-order sum)
         Items& it = items(hx_of(spatial_order));
-        hipLaunchKernelGGL((mass_partial_kernel<M>), dim3(it.n), dim3(NT), 0, stream, d_boxes, it.d, it.n, cur,
-                           d_partial, it.hx);
+        hipLaunchKernelGGL((mass_partial_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
+                           cur, d_partial, WV - 2 * it.hx);
         ZAMR_HIP(hipGetLastError());
         std::vector<double> part(it.n);
         ZAMR_HIP(hipMemcpyAsync(part.data(), d_partial, it.n * sizeof(double), hipMemcpyDeviceToHost, stream));

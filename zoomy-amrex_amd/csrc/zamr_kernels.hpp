@@ -8,15 +8,17 @@
 // point-implicit source solve and (stage 2) the Heun combine, the end-of-step
 // UpdateState and the CFL max-reduction for the next step.
 //
-// Work decomposition: a work item is a strip of NT-2*HX columns of one box and
-// a segment of up to TY rows.  A 256-thread workgroup owns NT = 256 columns
-// (the strip plus an HX-column halo on each side) and marches up the segment
-// row by row.  Every thread keeps a (2*HX+1)-row register window of its own
-// column, so each y-face is evaluated once, by the thread that owns the
-// column, and carried to the next row; each x-face is evaluated once, by the
-// thread right of it, and handed to the left neighbour through LDS.  State is
-// read row by row with consecutive lanes on consecutive cells (coalesced SoA),
-// and every cell is loaded once per workgroup.
+// Work decomposition: a work item is a strip of 64-2*HX columns of one box and
+// a segment of up to `rows` rows, and one wave64 owns one item: its 64 lanes
+// hold 64 consecutive columns (the strip plus an HX-column halo on each side)
+// and march up the segment row by row.  Every lane keeps a (2*HX+1)-row
+// register window of its own column, so each y-face is evaluated once, by the
+// lane that owns the column, and carried to the next row; each x-face is
+// evaluated once, by the lane right of it, and its left-cell share is handed
+// over with a cross-lane DPP shift.  No LDS and no barriers: the four waves
+// of a workgroup are independent.  State is read row by row with consecutive
+// lanes on consecutive cells (coalesced SoA); the next row's load is issued
+// one row ahead.
 //
 // The Model / Numerics headers are the reference's generated device-function
 // API (to_amrex.py:326-516), used unchanged.  Models that declare
@@ -29,7 +31,8 @@
 namespace zamr {
 
 constexpr int kMaxParams = 32;
-constexpr int NT = 256;  // threads (columns) per workgroup
+constexpr int NT = 256;  // threads per workgroup
+constexpr int WV = 64;   // lanes per wave = columns per work item
 
 struct Params {
     double v[kMaxParams];
@@ -176,27 +179,46 @@ struct Sys {
         return q;
     }
 
-    // Value of cell (c, j) as the face pass sees it: interior/ghost storage
-    // after UpdateState, or, outside the physical domain, the mirror-index
-    // interior cell after UpdateState pushed through Model::boundary_conditions
-    // (ZoomyAmr.cpp:385-441; corner ghosts take the x-side tag).
-    __device__ static __forceinline__ VQ load(const BoxView& B, const double* __restrict__ q, int c, int j,
-                                              const VP& p, int clamp, double time) {
-        const bool ox = (c < B.dlo0) || (c > B.dhi0);
-        const bool oy = (dim == 2) && ((j < B.dlo1) || (j > B.dhi1));
-        int ci = c, cj = j;
+    // Mirror index of (c, j) about the physical domain (ZoomyAmr.cpp:399-406):
+    // only the outside coordinate is reflected.
+    __device__ static __forceinline__ void mirror(const BoxView& B, int c, int j, int& ci, int& cj) {
+        ci = c;
+        cj = j;
         if (c < B.dlo0) ci = 2 * B.dlo0 - 1 - c;
         else if (c > B.dhi0) ci = 2 * B.dhi0 + 1 - c;
         if (dim == 2) {
             if (j < B.dlo1) cj = 2 * B.dlo1 - 1 - j;
             else if (j > B.dhi1) cj = 2 * B.dhi1 + 1 - j;
         }
-        const long long idx = (long long)(ci - B.gl0) + (long long)(cj - B.gl1) * B.sy;
+    }
+
+    struct Raw {
         VQ v;
+    };
+
+    // The stored value behind cell (c, j): itself, or its mirror-index cell.
+    __device__ static __forceinline__ Raw fetch(const BoxView& B, const double* __restrict__ q, int c, int j) {
+        int ci, cj;
+        mirror(B, c, j, ci, cj);
+        const long long idx = (long long)(ci - B.gl0) + (long long)(cj - B.gl1) * B.sy;
+        Raw r;
 #pragma unroll
-        for (int n = 0; n < nq; ++n) v(n, 0) = q[idx + n * B.sc];
-        v = update_state(v, p, clamp);
+        for (int n = 0; n < nq; ++n) r.v(n, 0) = q[idx + n * B.sc];
+        return r;
+    }
+
+    // Value of cell (c, j) as the face pass sees it: interior/ghost storage
+    // after UpdateState, or, outside the physical domain, the mirror-index
+    // interior cell after UpdateState pushed through Model::boundary_conditions
+    // (ZoomyAmr.cpp:385-441; corner ghosts take the x-side tag).
+    __device__ static __forceinline__ VQ finish(const BoxView& B, const Raw& r, int c, int j, const VP& p, int clamp,
+                                                double time) {
+        const bool ox = (c < B.dlo0) || (c > B.dhi0);
+        const bool oy = (dim == 2) && ((j < B.dlo1) || (j > B.dhi1));
+        VQ v = update_state(r.v, p, clamp);
         if (!ox && !oy) return v;
+        int ci, cj;
+        mirror(B, c, j, ci, cj);
         // Side index -> tag by arithmetic on all four tags: a select between
         // two field loads would be folded into a load through a selected
         // address, which pins the descriptor in scratch memory.
@@ -212,6 +234,11 @@ struct Sys {
         }
         const VA ai = aux_of(v, p, time, center(B, ci, cj));
         return M::boundary_conditions(bc, v, ai, nh, center(B, c, j), time, B.dx0);
+    }
+
+    __device__ static __forceinline__ VQ load(const BoxView& B, const double* __restrict__ q, int c, int j,
+                                              const VP& p, int clamp, double time) {
+        return finish(B, fetch(B, q, c, j), c, j, p, clamp, time);
     }
 
     // eval_face (make_rhs.H:31-131) from the four cells straddling the face.
@@ -376,15 +403,31 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Whole-wave lane shifts (DPP wave_shr / wave_shl): lane i receives lane
+// i-1 (from_left) or lane i+1 (from_right); the edge lane receives 0.
+__device__ __forceinline__ double from_left(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // XCD-aware item order: the hardware deals consecutive workgroups round-robin
-// over the 8 XCDs; give each XCD a contiguous run of items so that row-segment
-// neighbours (which share halo rows) hit the same L2.
-__device__ __forceinline__ int remap_item(int b, int nitems) {
-    const int per = (nitems + 7) / 8;
+// over the 8 XCDs; give each XCD a contiguous run of workgroup slots so that
+// row-segment neighbours (which share halo rows) hit the same L2.
+__device__ __forceinline__ int remap_slot(int b, int nslots) {
+    const int per = (nslots + 7) / 8;
     const int x = b & 7, k = b >> 3;
     const int it = x * per + k;
-    return (it < nitems) ? it : -1;
+    return (it < nslots) ? it : -1;
 }
+
+// Number of workgroup slots for n items (4 waves per workgroup), padded to 8.
+__host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems + 3) / 4 + 7) / 8) * 8; }
 
 // The fused stage.  EPI: 0 = plain stage (writes dst), 1 = stage + Heun with
 // buffer `old` + end-of-step UpdateState + CFL max (order 2, second stage),
@@ -402,31 +445,36 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
     constexpr int nq = S::nq;
     constexpr int dim = S::dim;
     constexpr int HX = (ORDER == 2) ? 2 : 1;
-    constexpr int TXC = NT - 2 * HX;
-    constexpr int WR = (dim == 2) ? 2 * HX + 1 : 1;  // window rows
-    constexpr int FW = 2 * nq + 1;                    // face words handed left
-
-    __shared__ double s_row[nq][NT];
-    __shared__ double s_face[FW][NT];
-    __shared__ double s_red[NT / 64];
+    constexpr int TXW = WV - 2 * HX;                  // cells per item row
+    // Window of rows j-HX+1 .. j+HX at row j (the bottom face of row j was
+    // evaluated one row earlier, so row j-HX is no longer needed).
+    constexpr int WR = (dim == 2) ? 2 * HX : 1;
+    constexpr int WC = (dim == 2) ? HX - 1 : 0;  // window slot of row j
 
     VP p;
 #pragma unroll
     for (int k = 0; k < S::np; ++k) p(k, 0) = a.p.v[k];
 
-    const int t = threadIdx.x;
-    for (int blk = blockIdx.x; blk < ((nitems + 7) / 8) * 8; blk += gridDim.x) {
-        const int it = xcd_remap ? remap_item(blk, nitems) : blk;
-        if (it < 0 || it >= nitems) continue;
+    const int t = threadIdx.x & (WV - 1);
+    const int wv = threadIdx.x / WV;
+    const int nslots = slots_for(nitems);
+    for (int blk = blockIdx.x; blk < nslots; blk += gridDim.x) {
+        const int slot = xcd_remap ? remap_slot(blk, nslots) : blk;
+        if (slot < 0) continue;
+        const int it = slot * (NT / WV) + wv;
+        if (it >= nitems) continue;
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
         GroupState* G = gs + B.group;
-        const int c = w.i0 - HX + t;  // this thread's column
+        const int c = w.i0 - HX + t;  // this lane's column
+        const bool own = (t >= HX) && (t < HX + TXW) && (c <= B.hi0);       // assembles cells
+        const bool xf = (t >= HX) && (t <= HX + TXW) && (c <= B.hi0 + 1);  // computes face c-1/2
+        const bool col_in = c <= B.hi0 + HX;                                // inside the grown box
         if (!G->active) {
             // An order-1 step ping-pongs buffers; a finished group carries its
             // state across so that every group's current buffer stays in step.
             if constexpr (EPI == 2) {
-                if ((t >= HX) && (t < HX + TXC) && (c <= B.hi0)) {
+                if (own) {
                     const double* __restrict__ qs = a.src ? B.q1 : B.q0;
                     double* __restrict__ qd = a.dst ? B.q1 : B.q0;
                     for (int j = w.j0; j < w.j0 + w.nj; ++j) {
@@ -440,114 +488,102 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
         }
         const double dt = G->dt;
         const double time = G->time;
-        const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;             // uniform per item
+        const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;  // uniform per item
         const double hgx = 0.5 * p(0, 0) * rdx, hgy = 0.5 * p(0, 0) * rdy;
         // select, never index: a runtime index would push the descriptor to scratch
         const double* __restrict__ qs = a.src ? B.q1 : B.q0;
         double* __restrict__ qd = a.dst ? B.q1 : B.q0;
         const double* __restrict__ qo = a.old ? B.q1 : B.q0;
 
-        const bool col_in = c <= B.hi0 + HX;               // inside the grown box
-        const bool own = (t >= HX) && (t < HX + TXC) && (c <= B.hi0);  // assembles cells
-        const bool xf = (t >= HX) && (t <= HX + TXC) && (c <= B.hi0 + 1);  // computes face c-1/2
-
         VQ win[WR];
         const int jstart = w.j0;
         const int jend = w.j0 + w.nj;
-        // prologue: rows jstart-HX .. jstart+HX-1
-#pragma unroll
-        for (int r = 0; r < WR - 1; ++r) {
-            const int j = jstart - (WR == 1 ? 0 : HX) + r;
-            if (col_in) win[r] = S::load(B, qs, c, j, p, a.clamp, time);
-            else
-#pragma unroll
-                for (int n = 0; n < nq; ++n) win[r](n, 0) = 0.0;
-        }
-        // bottom y-face of the first row (face jstart - 1/2)
+        // bottom y-face of the first row (face jstart - 1/2) from rows
+        // jstart-HX .. jstart+HX-1, which then become the window minus its lead
         VQ fyB;
         VF dyB;
         double hyB = 0.0;
         if constexpr (dim == 2) {
+            VQ pro[2 * HX];
+#pragma unroll
+            for (int r = 0; r < 2 * HX; ++r) {
+                if (col_in) pro[r] = S::load(B, qs, c, jstart - HX + r, p, a.clamp, time);
+                else
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) pro[r](n, 0) = 0.0;
+            }
             if (own) {
                 double hl;
-                const VQ& Lm = win[(HX == 2) ? 0 : 0];
-                const VQ& L = win[HX - 1];
-                const VQ& R = win[HX];
-                const VQ& Rp = win[(HX == 2) ? HX + 1 : HX];
-                S::template face<ORDER, 1, N>(Lm, L, R, Rp, S::center(B, c, jstart - 1), S::center(B, c, jstart), p,
-                                              time, a.wb, fyB, dyB, hl, hyB);
+                S::template face<ORDER, 1, N>(pro[0], pro[HX - 1], pro[HX], pro[2 * HX - 1],
+                                              S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
+                                              a.wb, fyB, dyB, hl, hyB);
             }
+#pragma unroll
+            for (int r = 0; r < WR - 1; ++r) win[r] = pro[r + 1];
         }
+        // first leading row
+        const int jlead0 = (dim == 2) ? jstart + HX : jstart;
+        typename S::Raw nxt;
+        if (col_in) nxt = S::fetch(B, qs, c, jlead0);
         double lam = 0.0;
         for (int j = jstart; j < jend; ++j) {
-            // stream in the leading row
-            if constexpr (WR > 1) {
-                if (col_in) win[WR - 1] = S::load(B, qs, c, j + HX, p, a.clamp, time);
-                else
+            const int jl = (dim == 2) ? j + HX : j;
+            VQ lead;
+            if (col_in) lead = S::finish(B, nxt, c, jl, p, a.clamp, time);
+            else
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) win[WR - 1](n, 0) = 0.0;
-            } else {
-                if (col_in) win[0] = S::load(B, qs, c, j, p, a.clamp, time);
-                else
+                for (int n = 0; n < nq; ++n) lead(n, 0) = 0.0;
+            if (col_in && j + 1 < jend) nxt = S::fetch(B, qs, c, jl + 1);  // one row ahead
+            win[WR - 1] = lead;
+            const VQ& qc = win[WC];
+            // x-face c-1/2 (left face of this column), neighbours by lane shifts
+            VQ Lm, L, Rp;
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) win[0](n, 0) = 0.0;
+            for (int n = 0; n < nq; ++n) {
+                L(n, 0) = from_left(qc(n, 0));
+                if constexpr (ORDER == 2) {
+                    Lm(n, 0) = from_left(L(n, 0));
+                    Rp(n, 0) = from_right(qc(n, 0));
+                } else {
+                    Lm(n, 0) = 0.0;
+                    Rp(n, 0) = 0.0;
+                }
             }
-            const VQ& qc = win[WR / 2];
-            // share row j across the workgroup
-#pragma unroll
-            for (int n = 0; n < nq; ++n) s_row[n][t] = qc(n, 0);
-            __syncthreads();
-            // x-face c-1/2 (left face of this column)
             VQ fxL;
             VF dxL;
-            double hxL = 0.0;
+            double hxL = 0.0, hl = 0.0;
             if (xf) {
-                VQ Lm, L, Rp;
-#pragma unroll
-                for (int n = 0; n < nq; ++n) {
-                    L(n, 0) = s_row[n][t - 1];
-                    if constexpr (ORDER == 2) {
-                        Lm(n, 0) = s_row[n][t - 2];
-                        Rp(n, 0) = s_row[n][t + 1];
-                    } else {
-                        Lm(n, 0) = 0.0;
-                        Rp(n, 0) = 0.0;
-                    }
-                }
-                double hl;
                 S::template face<ORDER, 0, N>(Lm, L, qc, Rp, S::center(B, c - 1, j), S::center(B, c, j), p, time,
                                               a.wb, fxL, dxL, hl, hxL);
-                // the left cell needs flux, D- and its own reconstructed depth
+            } else {
 #pragma unroll
-                for (int n = 0; n < nq; ++n) {
-                    s_face[n][t] = fxL(n, 0);
-                    s_face[nq + n][t] = dxL(nq + n, 0);
-                }
-                s_face[2 * nq][t] = hl;
+                for (int n = 0; n < nq; ++n) fxL(n, 0) = 0.0;
+#pragma unroll
+                for (int n = 0; n < 2 * nq; ++n) dxL(n, 0) = 0.0;
             }
-            // top y-face j+1/2 (needs rows j-1 .. j+2)
+            // the left cell needs this face's flux, D- and its own reconstructed
+            // depth; fold them into the x part of the divergence right away
+            double Fx[nq];
+#pragma unroll
+            for (int n = 0; n < nq; ++n) {
+                const double fR = from_right(fxL(n, 0));
+                const double dmR = from_right(dxL(nq + n, 0));
+                Fx[n] = -(fR - fxL(n, 0)) - dmR - dxL(n, 0);  // face_divergence, make_rhs.H:142-147
+            }
+            const double hL_xr = from_right(hl);
+            // top y-face j+1/2 (needs rows j-HX+1 .. j+HX)
             VQ fyT;
             VF dyT;
             double hyT = 0.0, hyTnext = 0.0;
             if constexpr (dim == 2) {
                 if (own) {
-                    const VQ& Lm = win[(HX == 2) ? 1 : 1];
-                    const VQ& L = win[HX];
-                    const VQ& R = win[HX + 1];
-                    const VQ& Rp = win[(HX == 2) ? HX + 2 : HX + 1];
-                    S::template face<ORDER, 1, N>(Lm, L, R, Rp, S::center(B, c, j), S::center(B, c, j + 1), p, time,
-                                                  a.wb, fyT, dyT, hyT, hyTnext);
+                    const VQ& Lm = win[(HX == 2) ? 0 : 0];
+                    const VQ& Rp = win[WR - 1];
+                    S::template face<ORDER, 1, N>(Lm, win[WC], win[WC + 1], Rp, S::center(B, c, j),
+                                                  S::center(B, c, j + 1), p, time, a.wb, fyT, dyT, hyT, hyTnext);
                 }
             }
-            __syncthreads();
             if (own) {
-                double fR[nq], dmR[nq];
-#pragma unroll
-                for (int n = 0; n < nq; ++n) {
-                    fR[n] = s_face[n][t + 1];
-                    dmR[n] = s_face[nq + n][t + 1];
-                }
-                const double hL_xr = s_face[2 * nq][t + 1];
                 const typename S::VX X = S::center(B, c, j);
                 const VA ac = S::aux_of(qc, p, time, X);
                 VQ src{};
@@ -555,10 +591,9 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
                 VQ qn;
 #pragma unroll
                 for (int n = 0; n < nq; ++n) {
-                    const double Fx = -(fR[n] - fxL(n, 0)) - dmR[n] - dxL(n, 0);
                     double Fy = 0.0;
                     if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
-                    double rhs = kBitwise ? Fx / B.dx0 + Fy / B.dx1 : Fx * rdx + Fy * rdy;
+                    double rhs = kBitwise ? Fx[n] / B.dx0 + Fy / B.dx1 : Fx[n] * rdx + Fy * rdy;
                     if (!a.impl) rhs += src(n, 0);
                     if (a.wb) {
                         if constexpr (kBitwise) {
@@ -605,18 +640,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
         }
         if constexpr (EPI != 0) {
             lam = wave_max(lam);
-            if ((t & 63) == 0) s_red[t >> 6] = lam;
-            __syncthreads();
-            if (t == 0) {
-                double m = s_red[0];
-#pragma unroll
-                for (int k = 1; k < NT / 64; ++k) m = dev_max(m, s_red[k]);
-                if (m > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(m));
-            }
+            if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
         }
-        __syncthreads();
     }
 }
+
+// Lane t of a wave owns column i0 + t of its item (t < txw) in the auxiliary
+// kernels below, which walk the same item list without a halo.
+#define ZAMR_ITEM_LOOP(nitems)                                                            \
+    const int t = threadIdx.x & (WV - 1);                                                 \
+    const int wv = threadIdx.x / WV;                                                      \
+    for (int it = blockIdx.x * (NT / WV) + wv; it < (nitems); it += gridDim.x * (NT / WV))
 
 // ComputeDt's reduction (ZoomyAmr.cpp:465-489) over the state and aux AS
 // STORED — used for a step whose dt must see the stored aux (the first step
@@ -624,21 +658,18 @@ __global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ b
 template <class M, class N>
 __global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__ boxes,
                                                      const WorkItem* __restrict__ items, int nitems, int src,
-                                                     Params pp, GroupState* __restrict__ gs, int halo) {
+                                                     Params pp, GroupState* __restrict__ gs, int txw) {
     using S = Sys<M>;
     constexpr int nq = S::nq, na = S::na, dim = S::dim;
-    __shared__ double s_red[NT / 64];
     typename S::VP p;
     for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
-    const int t = threadIdx.x;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
         GroupState* G = gs + B.group;
-        const int c = w.i0 - halo + t;
-        const bool own = (t >= halo) && (t < NT - halo) && (c <= B.hi0);
+        const int c = w.i0 + t;
         double lam = 0.0;
-        if (own) {
+        if (t < txw && c <= B.hi0) {
             for (int j = w.j0; j < w.j0 + w.nj; ++j) {
                 const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
                 typename S::VQ q;
@@ -649,21 +680,16 @@ __global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__
                 for (int d = 0; d < dim; ++d) {
                     typename S::VN nh{};
                     nh(d, 0) = 1.0;
-                    const double inv = N::local_max_abs_eigenvalue(q, av, p, nh)(0, 0) / (d == 0 ? B.dx0 : B.dx1);
+                    // same arithmetic as the stage-2 epilogue, so both paths give one dt
+                    const double ev = N::local_max_abs_eigenvalue(q, av, p, nh)(0, 0);
+                    const double inv = kBitwise ? ev / (d == 0 ? B.dx0 : B.dx1) : ev * (1.0 / (d == 0 ? B.dx0 : B.dx1));
                     if (inv > mi) mi = inv;
                 }
                 lam = dev_max(lam, mi);
             }
         }
         lam = wave_max(lam);
-        if ((t & 63) == 0) s_red[t >> 6] = lam;
-        __syncthreads();
-        if (t == 0) {
-            double m = s_red[0];
-            for (int k = 1; k < NT / 64; ++k) m = dev_max(m, s_red[k]);
-            if (m > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(m));
-        }
-        __syncthreads();
+        if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
     }
 }
 
@@ -741,17 +767,17 @@ __global__ void fill_boundary_kernel(const BoxDesc* __restrict__ boxes, const Co
 // Qaux as the reference holds it after UpdateState (ZoomyAmr.cpp:328):
 // update_aux_variables(Q, Qaux_stored) over the valid cells.
 template <class M>
-__global__ void materialize_aux_kernel(const BoxDesc* __restrict__ boxes, const WorkItem* __restrict__ items,
-                                       int nitems, int src, Params pp, double time, int halo) {
+__global__ void __launch_bounds__(NT) materialize_aux_kernel(const BoxDesc* __restrict__ boxes,
+                                                             const WorkItem* __restrict__ items, int nitems, int src,
+                                                             Params pp, double time, int txw) {
     using S = Sys<M>;
     typename S::VP p;
     for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
-    const int t = threadIdx.x;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
-        const int c = w.i0 - halo + t;
-        if (!((t >= halo) && (t < NT - halo) && (c <= B.hi0))) continue;
+        const int c = w.i0 + t;
+        if (!(t < txw && c <= B.hi0)) continue;
         for (int j = w.j0; j < w.j0 + w.nj; ++j) {
             const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
             typename S::VQ q;
@@ -766,17 +792,17 @@ __global__ void materialize_aux_kernel(const BoxDesc* __restrict__ boxes, const 
 
 // UpdateState over the valid cells of buffer src (in place) plus the aux.
 template <class M>
-__global__ void update_state_kernel(const BoxDesc* __restrict__ boxes, const WorkItem* __restrict__ items,
-                                    int nitems, int src, Params pp, double time, int clamp, int halo) {
+__global__ void __launch_bounds__(NT) update_state_kernel(const BoxDesc* __restrict__ boxes,
+                                                          const WorkItem* __restrict__ items, int nitems, int src,
+                                                          Params pp, double time, int clamp, int txw) {
     using S = Sys<M>;
     typename S::VP p;
     for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
-    const int t = threadIdx.x;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
-        const int c = w.i0 - halo + t;
-        if (!((t >= halo) && (t < NT - halo) && (c <= B.hi0))) continue;
+        const int c = w.i0 + t;
+        if (!(t < txw && c <= B.hi0)) continue;
         for (int j = w.j0; j < w.j0 + w.nj; ++j) {
             const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
             typename S::VQ q;
@@ -792,34 +818,25 @@ __global__ void update_state_kernel(const BoxDesc* __restrict__ boxes, const Wor
     }
 }
 
-// Per-workgroup partial sums of h over valid cells, written in item order
+// Per-item partial sums of h over valid cells, written in item order
 // (deterministic); the host adds the partials in order.
 template <class M>
 __global__ void __launch_bounds__(NT) mass_partial_kernel(const BoxDesc* __restrict__ boxes,
                                                           const WorkItem* __restrict__ items, int nitems, int src,
-                                                          double* partial, int halo) {
-    __shared__ double s_red[NT / 64];
-    const int t = threadIdx.x;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+                                                          double* partial, int txw) {
+    ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
-        const int c = w.i0 - halo + t;
+        const int c = w.i0 + t;
         double s = 0.0;
-        if ((t >= halo) && (t < NT - halo) && (c <= B.hi0))
+        if (t < txw && c <= B.hi0)
             for (int j = w.j0; j < w.j0 + w.nj; ++j) {
                 const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
                 s += qsel(B, src)[idx + 1 * B.sc];
             }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if ((t & 63) == 0) s_red[t >> 6] = s;
-        __syncthreads();
-        if (t == 0) {
-            double m = 0.0;
-            for (int k = 0; k < NT / 64; ++k) m += s_red[k];
-            partial[it] = m * B.dx0 * B.dx1;
-        }
-        __syncthreads();
+        if (t == 0) partial[it] = s * B.dx0 * B.dx1;
     }
 }
 

@@ -79,6 +79,53 @@ def dam_break_2d(n=100, time_end=0.1):
     return case, None
 
 
+def box_list(seed=0, n_amr=600, n_members=64, steps=200, member_range=(256, 1024)):
+    """A batched box list: n_amr patches of a 3-level hierarchy (refinement
+    ratio 2, side lengths per axis drawn from {16, 32, 64, 128} with weights
+    0.4/0.3/0.2/0.1, level l with dx = 2^-l / 1024, smooth seeded wet states),
+    plus n_members dam-break ensemble members of sizes uniform in
+    member_range^2 (h_left ~ U[1.5, 2.5], h_right = 1, dx = 1/nx).
+
+    Every box is its own physical domain with extrapolation boundaries.
+    Time-step groups: one per AMR level (boxes of a level share dt, the
+    segmented max of their CFL speeds) and one per ensemble member.
+    Returns (boxes, states, solver_keys): boxes are dicts with lo, hi, dx,
+    prob_lo, group; states are (4, ny, nx) arrays."""
+    rng = SplitMix64(seed)
+    sides = np.array([16, 32, 64, 128])
+    cum = np.cumsum([0.4, 0.3, 0.2, 0.1])
+    boxes, states = [], []
+    u = rng.uniform(3 * n_amr)
+    for k in range(n_amr):
+        nx = int(sides[np.searchsorted(cum, u[3 * k], side="right")])
+        ny = int(sides[np.searchsorted(cum, u[3 * k + 1], side="right")])
+        lev = min(int(u[3 * k + 2] * 3), 2)
+        dx = 2.0 ** (-lev) / 1024
+        ph = rng.uniform(4, 0, 2 * np.pi)
+        x = (np.arange(nx) + 0.5) * dx
+        y = (np.arange(ny) + 0.5) * dx
+        X, Y = np.meshgrid(x, y, indexing="xy")
+        kx, ky = 2 * np.pi / (nx * dx), 2 * np.pi / (ny * dx)
+        h = 1.0 + 0.05 * np.sin(kx * X + ph[0]) * np.cos(ky * Y + ph[1])
+        hu = 0.1 * h * np.sin(ky * Y + ph[2])
+        hv = 0.1 * h * np.cos(kx * X + ph[3])
+        boxes.append(dict(lo=(0, 0), hi=(nx - 1, ny - 1), dx=(dx, dx), prob_lo=(0.0, 0.0), group=lev))
+        states.append(np.stack([np.zeros_like(h), h, hu, hv]))
+    lo_m, hi_m = member_range
+    dims = np.floor(rng.uniform(2 * n_members, lo_m, hi_m + 1)).astype(int)
+    hl = rng.uniform(n_members, 1.5, 2.5)
+    for m in range(n_members):
+        nx, ny = int(dims[2 * m]), int(dims[2 * m + 1])
+        x = (np.arange(nx) + 0.5) / nx
+        h = np.where(x < 0.5, hl[m], 1.0)[None, :].repeat(ny, axis=0)
+        z = np.zeros_like(h)
+        boxes.append(dict(lo=(0, 0), hi=(nx - 1, ny - 1), dx=(1.0 / nx, 1.0 / ny), prob_lo=(0.0, 0.0), group=3 + m))
+        states.append(np.stack([z, h, z, z]))
+    keys = dict(spatial_order=2, implicit_source=False, well_balanced=False, clamp_positivity=True, cfl=0.45,
+                dtmin=1e-12, dtmax=1.0, time_end=1e30, max_step=steps)
+    return boxes, states, keys
+
+
 def swe_bumps(n=4096, seed=0, nbumps=32, steps=100, manning=0.03, max_grid_size=None):
     """Single-level 2-D shallow water over nbumps seeded Gaussian bumps:
     eta = 1 + a 1e-2 random-phase perturbation, h = max(0, eta - b) (dry

@@ -686,7 +686,9 @@ class Solver {
         ZAMR_HIP(hipEventRecord(ev[idx], stream));
     }
 
-    bool graph_ok() const { return spatial_order == 2 && !profiling; }
+    // Graph replay for the device-clock loop; the per-box launch mode stays
+    // eager, as the baseline it stands for (one launch per box per pass).
+    bool graph_ok() const { return spatial_order == 2 && !profiling && !per_box_launch; }
 
     void run_steps(int nsteps, int* done, double* time) {
         download_groups();

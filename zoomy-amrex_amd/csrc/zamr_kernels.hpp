@@ -432,8 +432,19 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 // The fused stage.  EPI: 0 = plain stage (writes dst), 1 = stage + Heun with
 // buffer `old` + end-of-step UpdateState + CFL max (order 2, second stage),
 // 2 = stage + end-of-step UpdateState + CFL max (order 1 step).
+// Occupancy request for the stage kernel (waves per SIMD); 0 leaves the
+// register budget to the compiler.
+#ifndef ZAMR_STAGE_WAVES
+#define ZAMR_STAGE_WAVES 0
+#endif
+#if ZAMR_STAGE_WAVES > 0
+#define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_STAGE_WAVES, ZAMR_STAGE_WAVES)))
+#else
+#define ZAMR_STAGE_ATTR
+#endif
+
 template <class M, class N, int ORDER, int EPI>
-__global__ void __launch_bounds__(NT) stage_kernel(const BoxDesc* __restrict__ boxes,
+__global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc* __restrict__ boxes,
                                                    const WorkItem* __restrict__ items, int nitems,
                                                    StageArgs a, GroupState* __restrict__ gs, int xcd_remap) {
     using S = Sys<M>;
@@ -703,10 +714,7 @@ __global__ void begin_step_kernel(GroupState* gs, int ngroups) {
     G.active = (G.time < G.time_end) && (G.max_step < 0 || G.step < G.max_step);
     const double m = __longlong_as_double((long long)G.maxinv_bits);
     G.maxinv_bits = 0ull;
-    if (!G.active) {
-        G.dt = 0.0;
-        return;
-    }
+    if (!G.active) return;  // dt keeps the group's last step
     double dt;
     if (m < 1e-14) {
         dt = G.dtmax;

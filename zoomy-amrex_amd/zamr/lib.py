@@ -225,6 +225,47 @@ class ZoomyAmr:
         return out, n.value
 
 
+def solver_deck(keys):
+    """solver.* / params.* deck lines from a dictionary of solver keys."""
+    lines = []
+    for k in ("time_end", "max_step", "cfl", "dtmin", "dtmax", "spatial_order"):
+        if k in keys:
+            lines.append(f"solver.{k} = {keys[k]!r}")
+    for k in ("implicit_source", "well_balanced", "clamp_positivity"):
+        if k in keys:
+            lines.append(f"solver.{k} = {'true' if keys[k] else 'false'}")
+    for k, v in (keys.get("params") or {}).items():
+        lines.append(f"params.{k} = {v!r}")
+    return "\n".join(lines) + "\n"
+
+
+def from_boxes(boxes, states, keys, system="swe2d", contract=True, lib=None):
+    """A batched context over independent boxes (see zamr_create_boxlist):
+    boxes are dicts with lo, hi, dx, prob_lo, group and optional bc (four tag
+    names); states the (n_q, ny, nx) initial state of each box."""
+    zb = []
+    keep = []
+    for b in boxes:
+        z = ZamrBox()
+        z.lo[:] = list(b["lo"])
+        z.hi[:] = list(b["hi"])
+        z.dx[:] = list(b["dx"])
+        z.prob_lo[:] = list(b["prob_lo"])
+        z.group = int(b["group"])
+        names = [s.encode() for s in b.get("bc", ("", "", "", ""))]
+        keep.append(names)
+        for k in range(4):
+            z.bc[k] = names[k]
+        zb.append(z)
+    order = int(keys.get("spatial_order", 2))
+    ctx = ZoomyAmr(solver_deck(keys), system=system, dim=2, order=order, contract=contract, lib=lib, boxes=zb)
+    ctx._bc_keepalive = keep
+    ctx.init_data()
+    for k, s in enumerate(states):
+        ctx.set_state(s, lev=k)
+    return ctx
+
+
 def _deck_int(text, key, default):
     for line in text.splitlines():
         line = line.split("#", 1)[0]

@@ -577,6 +577,7 @@ class Solver {
         a.wb = well_balanced ? 1 : 0;
         a.impl = implicit_source ? 1 : 0;
         a.clamp = (clamp_positivity && positivity != "mood") ? 1 : 0;
+        a.pre = nullptr;
         return a;
     }
 

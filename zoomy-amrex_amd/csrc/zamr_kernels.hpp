@@ -110,6 +110,8 @@ struct StageArgs {
     Params p;
     int src, dst, old;           // buffer indices: read, write, Heun's Q^n
     int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
+    double* pre;                 // optional: the pre-implicit state Q + dt RHS of a
+                                 // single box (box layout), kept for the adjoint
 };
 
 // The bitwise build (-DZAMR_BITWISE, FMA contraction off) performs every
@@ -168,6 +170,21 @@ struct Sys {
 
     __device__ static __forceinline__ VA aux_of(const VQ& q, const VP& p, double time, const VX& X) {
         VA a0{};
+        return M::update_aux_variables(q, a0, p, time, X);
+    }
+
+    // Aux of cell (c, j): the model's stored per-cell input rows
+    // (M::zamr_aux_in, e.g. a Manning field) read from HBM, the live rows
+    // recomputed from q.  Only the source and its Jacobians read input rows.
+    __device__ static __forceinline__ VA aux_at(const BoxView& B, int c, int j, const VQ& q, const VP& p,
+                                                double time, const VX& X) {
+        VA a0{};
+        if constexpr (M::zamr_n_aux_in > 0) {
+            const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+#pragma unroll
+            for (int k = 0; k < M::zamr_n_aux_in; ++k)
+                a0(M::zamr_aux_in[k], 0) = B.aux[idx + M::zamr_aux_in[k] * B.sc];
+        }
         return M::update_aux_variables(q, a0, p, time, X);
     }
 
@@ -596,7 +613,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             }
             if (own) {
                 const typename S::VX X = S::center(B, c, j);
-                const VA ac = S::aux_of(qc, p, time, X);
+                const VA ac = S::aux_at(B, c, j, qc, p, time, X);
                 VQ src{};
                 if (!a.impl) src = M::source(qc, ac, p, time, X);
                 VQ qn;
@@ -619,15 +636,19 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                     qn(n, 0) = qc(n, 0) + dt * rhs;
                 }
-                if (a.impl) qn = S::implicit_source(qn, ac, dt, p, time, X);
                 const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+                if (a.pre) {
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) a.pre[idx + n * B.sc] = qn(n, 0);
+                }
+                if (a.impl) qn = S::implicit_source(qn, ac, dt, p, time, X);
                 if constexpr (EPI == 1) {
 #pragma unroll
                     for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qo[idx + n * B.sc] + 0.5 * qn(n, 0);
                 }
                 if constexpr (EPI != 0) {
                     qn = S::update_state(qn, p, a.clamp);
-                    const VA af = S::aux_of(qn, p, time, X);
+                    const VA af = S::aux_at(B, c, j, qn, p, time, X);
 #pragma unroll
                     for (int d = 0; d < dim; ++d) {
                         VN nh{};

@@ -50,6 +50,8 @@ struct Model {
     static constexpr bool has_diffusion = false;
     static constexpr int n_boundary_tags = 5;
     static constexpr bool zamr_aux_is_local = true;
+    static constexpr int zamr_n_aux_in = 0;          // aux rows read from storage (per-cell inputs)
+    static constexpr int zamr_aux_in[1] = {0};
     static constexpr int zamr_idx_hinv = 6;
     static const std::vector<std::string> get_boundary_tags() { return { "East", "North", "South", "West", "wall" }; }
     static const std::vector<std::string> parameter_names() { return { "g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s" }; }

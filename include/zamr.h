@@ -132,6 +132,42 @@ int zamr_set_tuning(zamr_ctx* ctx, int rows_per_item, int per_box_launch, int xc
 int zamr_set_profiling(zamr_ctx* ctx, int on);
 int zamr_kernel_times(zamr_ctx* ctx, double* out_ms, int* nsteps);
 
+/* (dt, time after the step) of the steps taken on the device clock since the
+ * last zamr_reset_clock, up to cap entries — the values the reference prints
+ * per step (ZoomyAmr.cpp:804-809).  *n receives the count. */
+int zamr_dt_history(zamr_ctx* ctx, int cap, double* dts, double* times, int* n);
+
+/* ---- Adjoint (gradients) -------------------------------------------------
+ * No counterpart in the reference, which has no gradients.  Systems that ship
+ * an Adjoint.H beside their Model.H (swe2d_nf: per-cell Manning field in aux
+ * row 1) support these; other systems return ZAMR_EUNSUP.  Scope: one box
+ * covering the domain, order 2, point-implicit source, no well-balanced
+ * reconstruction, no positivity clamp (ZAMR_EUNSUP otherwise).
+ *
+ * zamr_adj_setup: nsteps SSP-RK2 steps with the frozen step sizes dts[nsteps]
+ * (time t_k = dts[0] + ... + dts[k-1]), a checkpoint every seg_steps steps,
+ * and ngauges distinct gauge cells (gi[g], gj[g]) whose depth h is recorded
+ * after every step.
+ * zamr_adj_forward: advances the current state by the nsteps frozen steps
+ * (the solver clock is not moved) and writes the gauge depths to
+ * gauge_h[k * ngauges + g] (after step k), if gauge_h is not NULL.
+ * zamr_adj_backward: given the adjoint of every gauge depth (same layout),
+ * recomputes each segment from its checkpoint and runs the reverse stages.
+ * zamr_adj_gradient: the adjoint of the initial state (n_dof_q * ny * nx)
+ * and of the friction aux row (ny * nx), raster order; either may be NULL. */
+int zamr_adj_setup(zamr_ctx* ctx, int nsteps, int seg_steps, const double* dts, int ngauges, const int* gi,
+                   const int* gj);
+int zamr_adj_forward(zamr_ctx* ctx, double* gauge_h);
+int zamr_adj_backward(zamr_ctx* ctx, const double* gauge_hbar);
+int zamr_adj_gradient(zamr_ctx* ctx, double* qbar0, size_t n_qbar, double* auxbar, size_t n_auxbar);
+/* Frozen limiter, for derivative checks: with on = 1 the next
+ * zamr_adj_forward records which argument every minmod of every face and
+ * stage returned, and later forwards replay those choices instead of
+ * comparing, so that finite differences see the same one-sided derivative
+ * the adjoint takes at slope-limiter switching points.  on = 0 turns it off.
+ * A backward must follow a forward that computed its limiter choices. */
+int zamr_adj_freeze_limiter(zamr_ctx* ctx, int on);
+
 #ifdef __cplusplus
 }
 #endif

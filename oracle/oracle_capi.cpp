@@ -88,6 +88,7 @@ void orc_stage(void* h, int ord, double time, double dt) { static_cast<OSolver*>
 void orc_advance(void* h, double time, double dt) { static_cast<OSolver*>(h)->advance(time, dt); }
 double orc_total_mass(void* h) { return static_cast<OSolver*>(h)->total_mass(); }
 long orc_mood_redos(void* h) { return static_cast<OSolver*>(h)->mood_redos; }
+void orc_set_tape(void* h, int mode) { static_cast<OSolver*>(h)->set_tape(mode); }
 
 // Runs the Evolve loop; dts/masses (capacity cap) receive per-step values.
 int orc_evolve(void* h, double* time_out, double* dts, double* masses, int cap, int step_limit) {

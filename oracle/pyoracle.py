@@ -61,6 +61,8 @@ def load(system):
     lib.orc_advance.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     lib.orc_mood_redos.restype = ctypes.c_long
     lib.orc_mood_redos.argtypes = [ctypes.c_void_p]
+    lib.orc_set_tape.restype = None
+    lib.orc_set_tape.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_evolve.restype = ctypes.c_int
     lib.orc_evolve.argtypes = [ctypes.c_void_p, _D, _D, _D, ctypes.c_int, ctypes.c_int]
     lib.orc_ge_solve.restype = ctypes.c_int
@@ -172,6 +174,11 @@ class Oracle:
 
     def mood_redos(self):
         return self.lib.orc_mood_redos(self.h)
+
+    def set_tape(self, mode):
+        """Limiter tape: 1 records every minmod choice of the following stages,
+        2 replays the recorded choices from the start, 0 turns it off."""
+        self.lib.orc_set_tape(self.h, mode)
 
     def evolve(self, step_limit=-1, cap=100000):
         t = np.zeros(1)

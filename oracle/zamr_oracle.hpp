@@ -341,10 +341,43 @@ struct Solver {
         fill_boundary(Qaux);
     }
 
+    // Limiter tape (derivative checks only): in record mode every minmod of
+    // eval_face appends which argument it returned (0 none, 1 first, 2
+    // second); in replay mode the choices are read back in the same order
+    // instead of compared, freezing the limiter.  One tape per box, so the
+    // OpenMP loop over boxes keeps each tape in its own box's order.
+    struct Tape {
+        std::vector<unsigned char> code;
+        size_t pos = 0;
+    };
+    int tape_mode = 0;  // 0 off, 1 record, 2 replay
+    mutable std::vector<Tape> tapes;
+
+    void set_tape(int mode) {
+        tape_mode = mode;
+        tapes.resize(boxes.size());
+        for (auto& t : tapes) {
+            if (mode == 1) t.code.clear();
+            t.pos = 0;
+        }
+    }
+
+    Real limited(Real a, Real b, Tape* tp) const {
+        if (!tp) return minmod(a, b);
+        unsigned char k;
+        if (tape_mode == 2) {
+            k = tp->code.at(tp->pos++);
+        } else {
+            k = (a * b <= 0.0) ? 0 : ((std::abs(a) < std::abs(b)) ? 1 : 2);
+            tp->code.push_back(k);
+        }
+        return k == 1 ? a : (k == 2 ? b : Real(0.0));
+    }
+
     // make_rhs.H:31-131.
     void eval_face(const Fab& fq, const Fab& fa, int iL, int jL, int iR, int jR, int dir, int order,
                    VQ& flux, amrex::SmallMatrix<Real, 2 * nq, 1>& fluct, Real time, bool wb,
-                   Real* hLstar, Real* hRstar) const {
+                   Real* hLstar, Real* hRstar, Tape* tp = nullptr) const {
         VQ qL, qR;
         VA aL, aR;
         VN n_hat{};
@@ -355,13 +388,13 @@ struct Solver {
                 qL(n, 0) = fq(iL, jL, n);
                 qR(n, 0) = fq(iR, jR, n);
             } else if (dir == 0) {
-                Real gL = minmod(fq(iL, jL, n) - fq(iL - 1, jL, n), fq(iL + 1, jL, n) - fq(iL, jL, n));
-                Real gR = minmod(fq(iR, jR, n) - fq(iR - 1, jR, n), fq(iR + 1, jR, n) - fq(iR, jR, n));
+                Real gL = limited(fq(iL, jL, n) - fq(iL - 1, jL, n), fq(iL + 1, jL, n) - fq(iL, jL, n), tp);
+                Real gR = limited(fq(iR, jR, n) - fq(iR - 1, jR, n), fq(iR + 1, jR, n) - fq(iR, jR, n), tp);
                 qL(n, 0) = fq(iL, jL, n) + 0.5 * gL;
                 qR(n, 0) = fq(iR, jR, n) - 0.5 * gR;
             } else {
-                Real gL = minmod(fq(iL, jL, n) - fq(iL, jL - 1, n), fq(iL, jL + 1, n) - fq(iL, jL, n));
-                Real gR = minmod(fq(iR, jR, n) - fq(iR, jR - 1, n), fq(iR, jR + 1, n) - fq(iR, jR, n));
+                Real gL = limited(fq(iL, jL, n) - fq(iL, jL - 1, n), fq(iL, jL + 1, n) - fq(iL, jL, n), tp);
+                Real gR = limited(fq(iR, jR, n) - fq(iR, jR - 1, n), fq(iR, jR + 1, n) - fq(iR, jR, n), tp);
                 qL(n, 0) = fq(iL, jL, n) + 0.5 * gL;
                 qR(n, 0) = fq(iR, jR, n) - 0.5 * gR;
             }
@@ -412,11 +445,12 @@ struct Solver {
         VQ fxr, fxl, fyr{}, fyl{};
         amrex::SmallMatrix<Real, 2 * nq, 1> dxr, dxl, dyr{}, dyl{};
         Real hL_xr = 0.0, hR_xl = 0.0, hL_yr = 0.0, hR_yl = 0.0, dum;
-        eval_face(fq, fa, i, j, i + 1, j, 0, order, fxr, dxr, time, s.well_balanced, &hL_xr, &dum);
-        eval_face(fq, fa, i - 1, j, i, j, 0, order, fxl, dxl, time, s.well_balanced, &dum, &hR_xl);
+        Tape* tp = tape_mode ? &tapes[b] : nullptr;
+        eval_face(fq, fa, i, j, i + 1, j, 0, order, fxr, dxr, time, s.well_balanced, &hL_xr, &dum, tp);
+        eval_face(fq, fa, i - 1, j, i, j, 0, order, fxl, dxl, time, s.well_balanced, &dum, &hR_xl, tp);
         if constexpr (dim == 2) {
-            eval_face(fq, fa, i, j, i, j + 1, 1, order, fyr, dyr, time, s.well_balanced, &hL_yr, &dum);
-            eval_face(fq, fa, i, j - 1, i, j, 1, order, fyl, dyl, time, s.well_balanced, &dum, &hR_yl);
+            eval_face(fq, fa, i, j, i, j + 1, 1, order, fyr, dyr, time, s.well_balanced, &hL_yr, &dum, tp);
+            eval_face(fq, fa, i, j - 1, i, j, 1, order, fyl, dyl, time, s.well_balanced, &dum, &hR_yl, tp);
         }
         VQ qc, src;
         VA ac;
@@ -444,7 +478,8 @@ struct Solver {
         Real hLs = 0.0, hRs = 0.0;
         const int iL = (dir == 0) ? i - 1 : i;
         const int jL = (dir == 1) ? j - 1 : j;
-        eval_face(fq, fa, iL, jL, i, j, dir, order, flux, fl, time, s.well_balanced, &hLs, &hRs);
+        eval_face(fq, fa, iL, jL, i, j, dir, order, flux, fl, time, s.well_balanced, &hLs, &hRs,
+                  tape_mode ? &tapes[b] : nullptr);
         for (int n = 0; n < nq; ++n) F(i, j, n) = flux(n, 0);
         for (int n = 0; n < 2 * nq; ++n) F(i, j, nq + n) = fl(n, 0);
         F(i, j, 3 * nq) = hLs;

@@ -311,26 +311,7 @@ class Solver {
         ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
         ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));
         std::vector<BoxDesc> bd(hb.size());
-        for (size_t k = 0; k < hb.size(); ++k) {
-            const HostBox& b = hb[k];
-            BoxDesc& d = bd[k];
-            d.q[0] = d_pool + b.off_q[0];
-            d.q[1] = d_pool + b.off_q[1];
-            d.aux = d_pool + b.off_aux;
-            for (int i = 0; i < 2; ++i) {
-                d.lo[i] = b.lo[i];
-                d.hi[i] = b.hi[i];
-                d.gl[i] = b.lo[i] - b.ng;
-                d.dlo[i] = b.dlo[i];
-                d.dhi[i] = b.dhi[i];
-                d.dx[i] = b.dx[i];
-                d.plo[i] = b.plo[i];
-            }
-            d.sy = b.gx;
-            d.sc = (long long)b.gx * b.gy;
-            d.group = b.group;
-            for (int s = 0; s < 4; ++s) d.bc[s] = b.bc[s];
-        }
+        for (size_t k = 0; k < hb.size(); ++k) bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1]);
         ZAMR_HIP(hipMalloc(&d_boxes, bd.size() * sizeof(BoxDesc)));
         ZAMR_HIP(hipMemcpyAsync(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, stream));
         h_groups.assign(ngroups, GroupState{});
@@ -352,6 +333,28 @@ class Solver {
         ZAMR_HIP(hipMalloc(&d_hist, 2 * (size_t)hist_cap * sizeof(double)));
         build_rects();
         ZAMR_HIP(hipStreamSynchronize(stream));
+    }
+
+    // Device descriptor of box b with the given state buffers.
+    BoxDesc desc_of(const HostBox& b, double* q0, double* q1) const {
+        BoxDesc d;
+        d.q[0] = q0;
+        d.q[1] = q1;
+        d.aux = d_pool + b.off_aux;
+        for (int i = 0; i < 2; ++i) {
+            d.lo[i] = b.lo[i];
+            d.hi[i] = b.hi[i];
+            d.gl[i] = b.lo[i] - b.ng;
+            d.dlo[i] = b.dlo[i];
+            d.dhi[i] = b.dhi[i];
+            d.dx[i] = b.dx[i];
+            d.plo[i] = b.plo[i];
+        }
+        d.sy = b.gx;
+        d.sc = (long long)b.gx * b.gy;
+        d.group = b.group;
+        for (int s = 0; s < 4; ++s) d.bc[s] = b.bc[s];
+        return d;
     }
 
     void upload_groups() {
@@ -578,6 +581,8 @@ class Solver {
         a.impl = implicit_source ? 1 : 0;
         a.clamp = (clamp_positivity && positivity != "mood") ? 1 : 0;
         a.pre = nullptr;
+        a.dstp = nullptr;
+        a.sw = nullptr;
         return a;
     }
 

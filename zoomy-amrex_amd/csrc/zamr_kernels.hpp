@@ -112,6 +112,10 @@ struct StageArgs {
     int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
     double* pre;                 // optional: the pre-implicit state Q + dt RHS of a
                                  // single box (box layout), kept for the adjoint
+    double* dstp;                // optional: output buffer of a single box in place
+                                 // of buffer `dst` (the adjoint's per-step storage)
+    unsigned* sw;                // limiter switch table of a single box (record /
+                                 // replay builds of the stage kernel only)
 };
 
 // The bitwise build (-DZAMR_BITWISE, FMA contraction off) performs every
@@ -142,6 +146,12 @@ __device__ __forceinline__ T dev_max(T a, T b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double minmod(double a, double b) {
     if (a * b <= 0.0) return 0.0;
     return (fabs(a) < fabs(b)) ? a : b;
+}
+
+// Which argument minmod returns: 0 none (zero), 1 the first, 2 the second.
+__device__ __forceinline__ unsigned minmod_code(double a, double b) {
+    if (a * b <= 0.0) return 0u;
+    return (fabs(a) < fabs(b)) ? 1u : 2u;
 }
 
 template <class M>
@@ -259,10 +269,15 @@ struct Sys {
     }
 
     // eval_face (make_rhs.H:31-131) from the four cells straddling the face.
-    template <int ORDER, int DIR, class N>
+    // SW = 1 also reports which argument each minmod returned (2 bits per
+    // call, 4 per component, in *bits); SW = 2 takes those choices from *bits
+    // instead of comparing (a forward with the limiter frozen).
+    template <int ORDER, int DIR, class N, int SW = 0>
     __device__ static __forceinline__ void face(const VQ& Lm, const VQ& L, const VQ& R, const VQ& Rp,
                                                 const VX& XL, const VX& XR, const VP& p, double time, int wb,
-                                                VQ& flux, VF& fluct, double& hLs, double& hRs) {
+                                                VQ& flux, VF& fluct, double& hLs, double& hRs,
+                                                unsigned* bits = nullptr) {
+        static_assert(SW == 0 || nq <= 4, "the switch table holds 16 bits per face");
         VQ qL, qR;
 #pragma unroll
         for (int n = 0; n < nq; ++n) {
@@ -270,8 +285,18 @@ struct Sys {
                 qL(n, 0) = L(n, 0);
                 qR(n, 0) = R(n, 0);
             } else {
-                const double gL = minmod(L(n, 0) - Lm(n, 0), R(n, 0) - L(n, 0));
-                const double gR = minmod(R(n, 0) - L(n, 0), Rp(n, 0) - R(n, 0));
+                const double a1 = L(n, 0) - Lm(n, 0), b1 = R(n, 0) - L(n, 0), b2 = Rp(n, 0) - R(n, 0);
+                double gL, gR;
+                if constexpr (SW == 2) {
+                    const unsigned sl = (*bits >> (4 * n)) & 3u, sr = (*bits >> (4 * n + 2)) & 3u;
+                    gL = (sl == 1u) ? a1 : ((sl == 2u) ? b1 : 0.0);
+                    gR = (sr == 1u) ? b1 : ((sr == 2u) ? b2 : 0.0);
+                } else {
+                    gL = minmod(a1, b1);
+                    gR = minmod(b1, b2);
+                    if constexpr (SW == 1)
+                        *bits |= (minmod_code(a1, b1) << (4 * n)) | (minmod_code(b1, b2) << (4 * n + 2));
+                }
                 qL(n, 0) = L(n, 0) + 0.5 * gL;
                 qR(n, 0) = R(n, 0) - 0.5 * gR;
             }
@@ -460,7 +485,10 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 #define ZAMR_STAGE_ATTR
 #endif
 
-template <class M, class N, int ORDER, int EPI>
+// SW: 0 = plain; 1 = record every face's limiter choices into a.sw (one
+// uint32 per cell: its left x-face in the low 16 bits, its top y-face in the
+// high 16); 2 = replay them (single box, for derivative checks).
+template <class M, class N, int ORDER, int EPI, int SW = 0>
 __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc* __restrict__ boxes,
                                                    const WorkItem* __restrict__ items, int nitems,
                                                    StageArgs a, GroupState* __restrict__ gs, int xcd_remap) {
@@ -520,7 +548,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const double hgx = 0.5 * p(0, 0) * rdx, hgy = 0.5 * p(0, 0) * rdy;
         // select, never index: a runtime index would push the descriptor to scratch
         const double* __restrict__ qs = a.src ? B.q1 : B.q0;
-        double* __restrict__ qd = a.dst ? B.q1 : B.q0;
+        double* __restrict__ qd = a.dstp ? a.dstp : (a.dst ? B.q1 : B.q0);
         const double* __restrict__ qo = a.old ? B.q1 : B.q0;
 
         VQ win[WR];
@@ -542,9 +570,15 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             }
             if (own) {
                 double hl;
-                S::template face<ORDER, 1, N>(pro[0], pro[HX - 1], pro[HX], pro[2 * HX - 1],
-                                              S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
-                                              a.wb, fyB, dyB, hl, hyB);
+                unsigned bb = 0u;
+                const long long ib = (long long)(c - B.gl0) + (long long)(jstart - 1 - B.gl1) * B.sy;
+                if constexpr (SW == 2) bb = a.sw[ib] >> 16;
+                S::template face<ORDER, 1, N, SW>(pro[0], pro[HX - 1], pro[HX], pro[2 * HX - 1],
+                                                  S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
+                                                  a.wb, fyB, dyB, hl, hyB, &bb);
+                // the face below the box is nobody's top face: its first item keeps it
+                if constexpr (SW == 1)
+                    if (jstart == B.lo1) a.sw[ib] = bb << 16;
             }
 #pragma unroll
             for (int r = 0; r < WR - 1; ++r) win[r] = pro[r + 1];
@@ -580,9 +614,12 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             VQ fxL;
             VF dxL;
             double hxL = 0.0, hl = 0.0;
+            unsigned bx = 0u, by = 0u;
+            const long long isw = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
             if (xf) {
-                S::template face<ORDER, 0, N>(Lm, L, qc, Rp, S::center(B, c - 1, j), S::center(B, c, j), p, time,
-                                              a.wb, fxL, dxL, hl, hxL);
+                if constexpr (SW == 2) bx = a.sw[isw] & 0xFFFFu;
+                S::template face<ORDER, 0, N, SW>(Lm, L, qc, Rp, S::center(B, c - 1, j), S::center(B, c, j), p,
+                                                  time, a.wb, fxL, dxL, hl, hxL, &bx);
             } else {
 #pragma unroll
                 for (int n = 0; n < nq; ++n) fxL(n, 0) = 0.0;
@@ -607,9 +644,15 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 if (own) {
                     const VQ& Lm = win[(HX == 2) ? 0 : 0];
                     const VQ& Rp = win[WR - 1];
-                    S::template face<ORDER, 1, N>(Lm, win[WC], win[WC + 1], Rp, S::center(B, c, j),
-                                                  S::center(B, c, j + 1), p, time, a.wb, fyT, dyT, hyT, hyTnext);
+                    if constexpr (SW == 2) by = a.sw[isw] >> 16;
+                    S::template face<ORDER, 1, N, SW>(Lm, win[WC], win[WC + 1], Rp, S::center(B, c, j),
+                                                      S::center(B, c, j + 1), p, time, a.wb, fyT, dyT, hyT, hyTnext,
+                                                      &by);
                 }
+            }
+            if constexpr (SW == 1) {
+                if (own) a.sw[isw] = bx | (by << 16);
+                else if (xf && c == B.hi0 + 1) a.sw[isw] = bx;
             }
             if (own) {
                 const typename S::VX X = S::center(B, c, j);

@@ -7,14 +7,33 @@
 #include "zamr.h"
 #include "zamr_driver.hpp"
 #include "zamr_plotfile.hpp"
+#include <memory>
+#if __has_include("Adjoint.H")
+#include "Adjoint.H"
+#include "zamr_adjoint.hpp"
+#define ZAMR_HAS_ADJOINT 1
+#else
+#define ZAMR_HAS_ADJOINT 0
+#endif
 
 extern "C" void zamr_internal_march(int* dim, int* deg);
 
 using Sol = zamr::Solver<Model, Numerics>;
+#if ZAMR_HAS_ADJOINT
+using AdjDrv = zamr::AdjointDriver<Model, Numerics, Adjoint>;
+#endif
 
 struct zamr_ctx {
     Sol s;
     std::string err;
+#if ZAMR_HAS_ADJOINT
+    std::unique_ptr<AdjDrv> adj;
+#endif
+    ~zamr_ctx() {
+#if ZAMR_HAS_ADJOINT
+        adj.reset();  // before the solver frees its stream
+#endif
+    }
 };
 
 namespace {
@@ -299,5 +318,77 @@ int zamr_kernel_times(zamr_ctx* c, double* out, int* n) {
     if (n) *n = k;
     return ZAMR_OK;
 }
+
+int zamr_dt_history(zamr_ctx* c, int cap, double* dts, double* times, int* n) {
+    if (!c || !c->s.d_pool || cap < 0) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        c->s.download_groups();
+        const int k = std::min({c->s.h_groups[0].step, c->s.hist_cap, cap});
+        std::vector<double> h(2 * (size_t)std::max(k, 1));
+        if (k > 0) {
+            ZAMR_HIP(hipMemcpyAsync(h.data(), c->s.d_hist, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->s.stream));
+            ZAMR_HIP(hipStreamSynchronize(c->s.stream));
+        }
+        for (int i = 0; i < k; ++i) {
+            if (dts) dts[i] = h[2 * i];
+            if (times) times[i] = h[2 * i + 1];
+        }
+        if (n) *n = k;
+    });
+}
+
+#if ZAMR_HAS_ADJOINT
+int zamr_adj_setup(zamr_ctx* c, int nsteps, int seg_steps, const double* dts, int ngauges, const int* gi,
+                   const int* gj) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        if (!c->adj) c->adj = std::make_unique<AdjDrv>(c->s);
+        c->adj->setup(nsteps, seg_steps, dts, ngauges, gi, gj);
+    });
+}
+int zamr_adj_forward(zamr_ctx* c, double* gauge_h) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->adj) {
+        c->err = "adjoint: call zamr_adj_setup first";
+        return ZAMR_EINVAL;
+    }
+    return guard(c, [&] { c->adj->forward(gauge_h); });
+}
+int zamr_adj_backward(zamr_ctx* c, const double* gauge_hbar) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->adj) {
+        c->err = "adjoint: call zamr_adj_setup first";
+        return ZAMR_EINVAL;
+    }
+    return guard(c, [&] { c->adj->backward(gauge_hbar); });
+}
+int zamr_adj_gradient(zamr_ctx* c, double* qbar0, size_t nq_cells, double* auxbar, size_t ncells) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->adj) {
+        c->err = "adjoint: call zamr_adj_setup first";
+        return ZAMR_EINVAL;
+    }
+    return guard(c, [&] { c->adj->gradient(qbar0, nq_cells, auxbar, ncells); });
+}
+int zamr_adj_freeze_limiter(zamr_ctx* c, int on) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->adj) {
+        c->err = "adjoint: call zamr_adj_setup first";
+        return ZAMR_EINVAL;
+    }
+    return guard(c, [&] { c->adj->freeze_limiter(on != 0); });
+}
+#else
+static int no_adjoint(zamr_ctx* c) {
+    if (c) c->err = "this system has no adjoint (no Adjoint.H beside its Model.H)";
+    return ZAMR_EUNSUP;
+}
+int zamr_adj_setup(zamr_ctx* c, int, int, const double*, int, const int*, const int*) { return no_adjoint(c); }
+int zamr_adj_forward(zamr_ctx* c, double*) { return no_adjoint(c); }
+int zamr_adj_backward(zamr_ctx* c, const double*) { return no_adjoint(c); }
+int zamr_adj_gradient(zamr_ctx* c, double*, size_t, double*, size_t) { return no_adjoint(c); }
+int zamr_adj_freeze_limiter(zamr_ctx* c, int) { return no_adjoint(c); }
+#endif
 
 }  // extern "C"

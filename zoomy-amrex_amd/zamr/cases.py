@@ -143,3 +143,67 @@ def swe_bumps(n=4096, seed=0, nbumps=32, steps=100, manning=0.03, max_grid_size=
                 well_balanced=True, clamp_positivity=True, cfl=0.45, dtmin=1e-7, dtmax=1.0,
                 time_end=1.0e30, max_step=steps, bc=("", "", "", ""), params={"n_m": manning})
     return case, Q
+
+
+def box_filter(a, passes=3, width=5):
+    """`passes` passes of a width x width moving average, edges replicated."""
+    r = width // 2
+    for _ in range(passes):
+        for ax in (0, 1):
+            p = np.pad(a, [(r, r) if k == ax else (0, 0) for k in range(2)], mode="edge")
+            c = np.cumsum(p, axis=ax)
+            c = np.concatenate([np.zeros_like(np.take(c, [0], axis=ax)), c], axis=ax)
+            hi = np.take(c, np.arange(width, width + a.shape[ax]), axis=ax)
+            lo = np.take(c, np.arange(0, a.shape[ax]), axis=ax)
+            a = (hi - lo) / width
+    return a
+
+
+def swe_adjoint(n=1024, seed=0, steps=200, ngauges=256, nfd=32, seg=20, bc=("", "", "", "")):
+    """Gauge-misfit inversion on the per-cell-Manning shallow-water system.
+
+    Base: bathymetry b0 = 16 seeded Gaussian bumps (A ~ U[0, 0.25], sigma ~
+    U[0.05, 0.15]), Manning n0 = 0.03 everywhere.  Truth: b = b0 + 8 bumps of
+    amplitude 0.1 (sigma ~ U[0.02, 0.06]); n = 0.03 + U[-0.01, 0.01] per
+    cell, smoothed by 3 passes of a 5x5 box filter.  Initial free surface
+    eta0 = 1 + 0.1 exp(-|x - (1/2, 1/2)|^2 / (2 0.05^2)), h0 = max(0, eta0 - b),
+    zero velocity.  Order 2, Rusanov, point-implicit Manning, no
+    well-balancing and no positivity clamp (everything stays wet), `steps`
+    SSP-RK2 steps with the dt sequence of the base run frozen; `ngauges`
+    distinct seeded gauge cells; `nfd` seeded finite-difference cells, each a
+    gauge shifted by up to 3 cells.  Returns a dict."""
+    rng = SplitMix64(seed)
+    x, y = cell_centers(n, n)
+    b0 = gaussian_bumps(x, y, rng, 16, amp=(0.0, 0.25), sigma=(0.05, 0.15))
+    bt = b0 + gaussian_bumps(x, y, rng, 8, amp=(0.1, 0.1), sigma=(0.02, 0.06))
+    nt = box_filter(0.03 + rng.uniform(n * n, -0.01, 0.01).reshape(n, n), passes=3, width=5)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    eta0 = 1.0 + 0.1 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / (2 * 0.05 ** 2))
+    cells = []
+    seen = set()
+    while len(cells) < ngauges:
+        u = rng.uniform(2)
+        c = (int(u[0] * n), int(u[1] * n))
+        if c not in seen:
+            seen.add(c)
+            cells.append(c)
+    gi = np.array([c[0] for c in cells], dtype=np.int32)
+    gj = np.array([c[1] for c in cells], dtype=np.int32)
+    fi, fj = [], []
+    for _ in range(nfd):
+        u = rng.uniform(3)
+        g = min(int(u[0] * ngauges), ngauges - 1)
+        off = np.floor(rng.uniform(2, -3, 4)).astype(int)
+        fi.append(int(np.clip(gi[g] + off[0], 0, n - 1)))
+        fj.append(int(np.clip(gj[g] + off[1], 0, n - 1)))
+    case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=n, spatial_order=2,
+                implicit_source=True, well_balanced=False, clamp_positivity=False, cfl=0.45, dtmin=1e-12,
+                dtmax=1.0, time_end=1.0e30, max_step=steps, bc=tuple(bc), params={})
+    return dict(case=case, b0=b0, b_truth=bt, n0=np.full((n, n), 0.03), n_truth=nt, eta0=eta0,
+                gauges=(gi, gj), fd_cells=(np.array(fi), np.array(fj)), steps=steps, seg=seg)
+
+
+def initial_state(b, eta0):
+    """[b, max(0, eta0 - b), 0, 0]."""
+    z = np.zeros_like(b)
+    return np.stack([b, np.maximum(0.0, eta0 - b), z, z])

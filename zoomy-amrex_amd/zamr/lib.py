@@ -70,6 +70,12 @@ def load_library(path):
         "zamr_set_tuning": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "zamr_set_profiling": [_VP, ctypes.c_int],
         "zamr_kernel_times": [_VP, _D, _I],
+        "zamr_dt_history": [_VP, ctypes.c_int, _D, _D, _I],
+        "zamr_adj_setup": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I, _I],
+        "zamr_adj_forward": [_VP, _D],
+        "zamr_adj_backward": [_VP, _D],
+        "zamr_adj_gradient": [_VP, _D, ctypes.c_size_t, _D, ctypes.c_size_t],
+        "zamr_adj_freeze_limiter": [_VP, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -223,6 +229,45 @@ class ZoomyAmr:
         n = ctypes.c_int()
         self._check(self.lib.zamr_kernel_times(self.ctx, _dp(out), ctypes.byref(n)))
         return out, n.value
+
+    def dt_history(self, cap=1 << 16):
+        """(dts, times) of the steps taken on the device clock since the last reset."""
+        d, t, n = np.zeros(cap), np.zeros(cap), ctypes.c_int()
+        self._check(self.lib.zamr_dt_history(self.ctx, cap, _dp(d), _dp(t), ctypes.byref(n)))
+        return d[:n.value].copy(), t[:n.value].copy()
+
+    # ---- adjoint (systems with an Adjoint.H; see include/zamr.h) ----
+    def adj_setup(self, dts, gauges_i, gauges_j, seg_steps=20):
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        gi = np.ascontiguousarray(gauges_i, dtype=np.int32)
+        gj = np.ascontiguousarray(gauges_j, dtype=np.int32)
+        self._adj = (len(dts), len(gi))
+        self._check(self.lib.zamr_adj_setup(self.ctx, len(dts), seg_steps, _dp(dts), len(gi),
+                                            gi.ctypes.data_as(_I), gj.ctypes.data_as(_I)))
+
+    def adj_forward(self):
+        """Frozen-dt forward from the current state; returns the gauge depths
+        after every step, shape (nsteps, ngauges)."""
+        ns, ng = self._adj
+        gh = np.zeros((ns, ng))
+        self._check(self.lib.zamr_adj_forward(self.ctx, _dp(gh)))
+        return gh
+
+    def adj_backward(self, gbar):
+        ns, ng = self._adj
+        gbar = np.ascontiguousarray(gbar, dtype=np.float64).reshape(ns, ng)
+        self._check(self.lib.zamr_adj_backward(self.ctx, _dp(gbar)))
+
+    def adj_freeze_limiter(self, on=True):
+        self._check(self.lib.zamr_adj_freeze_limiter(self.ctx, 1 if on else 0))
+
+    def adj_gradient(self):
+        """(adjoint of the initial state (n_q, ny, nx), adjoint of the friction aux row (ny, nx))."""
+        nx, ny, _ = self.shape()
+        qb = np.zeros((self.nq, ny, nx))
+        fb = np.zeros((ny, nx))
+        self._check(self.lib.zamr_adj_gradient(self.ctx, _dp(qb), qb.size, _dp(fb), fb.size))
+        return qb, fb
 
 
 def solver_deck(keys):

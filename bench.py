@@ -96,6 +96,62 @@ def bench_boxlist(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(reps=3):
+    """Adjoint cost on the 1024^2, 200-step gauge-misfit inversion (per-cell
+    Manning system, checkpoint every 20 steps): wall time of the backward
+    (segment recompute + reverse stages) and of the checkpointed forward,
+    over the wall time of the plain graph-captured forward of the same 200
+    steps from the same state.  Host copies of the gauge arrays are inside
+    the timed calls; building the input arrays is not."""
+    import numpy as np
+    import torch
+    from zamr import cases
+    from zamr.adjoint import GaugeRun, frozen_dts
+    from zamr.deck import deck_from_case
+    from zamr.lib import ZoomyAmr
+
+    c = cases.swe_adjoint(n=1024, seed=0, steps=200, ngauges=256, nfd=32, seg=20)
+    Q0 = cases.initial_state(c["b0"], c["eta0"])
+    dts = frozen_dts(c["case"], Q0, c["n0"], 200)
+    run = GaugeRun(c["case"], c["gauges"], dts, seg=20)
+    truth = run.forward(cases.initial_state(c["b_truth"], c["eta0"]), c["n_truth"])
+
+    plain = ZoomyAmr(deck_from_case(dict(c["case"], max_step=10 ** 9)), system="swe2d_nf", dim=2, order=2)
+    plain.init_data()
+    aux = plain.get_state(field=1)
+    aux[1] = c["n0"]
+    tf, tc, tb = [], [], []
+    for r in range(reps + 1):
+        plain.set_state(Q0)
+        plain.set_state(aux, field=1)
+        plain.reset_clock()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done, _ = plain.run_steps(200)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        assert done == 200
+        run.set_inputs(Q0, c["n0"])
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        G = run.z.adj_forward()
+        t3 = time.perf_counter()
+        run.z.adj_backward(G - truth)
+        t4 = time.perf_counter()
+        if r:  # the first round loads kernels and builds the graph
+            tf.append(t1 - t0)
+            tc.append(t3 - t2)
+            tb.append(t4 - t3)
+    qb, nb = run.z.adj_gradient()
+    assert np.isfinite(qb).all() and np.isfinite(nb).all()
+    f, ck, bw = min(tf), min(tc), min(tb)
+    return {"adjoint_over_forward": bw / f, "gradient_over_forward": (ck + bw) / f,
+            "forward_ms": 1e3 * f, "checkpointed_forward_ms": 1e3 * ck, "backward_ms": 1e3 * bw,
+            "workload": "2-D SWE 1024x1024, per-cell Manning, 200 SSP-RK2 steps (frozen dt), 256 gauges, "
+                        "checkpoint every 20 steps; gradients w.r.t. the initial state and the Manning field",
+            "timing": f"best of {reps} after one warm-up round"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,9 +160,11 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist"],
+    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint"],
                     help="swe4096: the single-level 4096^2 workload (default); boxlist: 600 AMR patches + "
-                         "64 ensemble members, batched launch vs one launch per box")
+                         "64 ensemble members, batched launch vs one launch per box; adjoint: the 1024^2 "
+                         "gauge-inversion gradient alone")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg of the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the run, copy a 1 GiB buffer with a plain device copy (counter calibration)")
     args = ap.parse_args()
@@ -123,6 +181,10 @@ def main():
 
     if args.workload == "boxlist":
         return bench_boxlist(args)
+    if args.workload == "adjoint":
+        print(json.dumps({"metric": "adjoint/forward time ratio", "unit": "x", "higher_is_better": False,
+                          "n_gpus": 1, "dtype": "f64", "data": "synthetic", **bench_adjoint()}), flush=True)
+        return
 
     case, Q0 = cases.swe_bumps(n=args.n, seed=0, steps=args.warmup + args.steps)
     z = ZoomyAmr(deck_from_case(case), system="swe2d", dim=2, order=2)
@@ -188,9 +250,11 @@ def main():
         torch.cuda.synchronize()
         out["calibration_bytes"] = a.numel() * 8
         out["calibration_sum"] = float(s)
+    del z
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        del z
         out["cpu_baseline"] = cpu_baseline(case, Q0, threads)
     print(json.dumps(out), flush=True)
 

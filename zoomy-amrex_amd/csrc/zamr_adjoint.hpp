@@ -326,6 +326,16 @@ struct AdjointDriver {
     static constexpr int ORDER = 2;
     static constexpr int HX = ORDER + 1;
 
+    // Per-step storage of one segment: stage inputs X[i] (X[len] = the end),
+    // Y[i], pre-implicit states V1[i], V2[i], and a descriptor per step
+    // (q0 = X[i], q1 = Y[i]).  Two slots: the backward recomputes segment
+    // m-1 into one (second stream) while it reverses segment m from the other.
+    struct Slot {
+        std::vector<double*> X, Y, V1, V2;
+        BoxDesc* d_desc = nullptr;
+        int held = -1;  // segment whose states the slot holds
+    };
+
     Sol& s;
     int nsteps = 0, seg = 0, nseg = 0, ng = 0;
     std::vector<double> dts, times;
@@ -335,12 +345,13 @@ struct AdjointDriver {
     GroupState* d_groups = nullptr;
     long long* d_gidx = nullptr;
     double *d_gh = nullptr, *d_gbar = nullptr;
-    BoxDesc* d_desc = nullptr;  // per step of a segment: q0 = X[i], q1 = Y[i]
     WorkItem* d_items = nullptr;
     int nitems = 0;
-    std::vector<double*> ck, X, Y, V1, V2;
+    std::vector<double*> ck;
+    Slot slot[2];
     double *Ab = nullptr, *Bb = nullptr, *nbar = nullptr;
-    int stored = -1;     // segment whose per-step states are in storage
+    hipStream_t side = nullptr;  // recompute stream (default priority)
+    hipEvent_t ev_go = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     bool have_forward = false;
     bool have_backward = false;
     // frozen limiter: the first forward after freeze_limiter(1) records every
@@ -352,19 +363,28 @@ struct AdjointDriver {
     explicit AdjointDriver(Sol& sol) : s(sol) {}
     ~AdjointDriver() { release(); }
     void release() {
+        if (side) hipStreamSynchronize(side);
+        if (s.stream) hipStreamSynchronize(s.stream);
         if (pool) hipFree(pool);
         if (d_groups) hipFree(d_groups);
         if (d_gidx) hipFree(d_gidx);
-        if (d_desc) hipFree(d_desc);
+        for (auto& sl : slot) {
+            if (sl.d_desc) hipFree(sl.d_desc);
+            sl = Slot{};
+        }
         if (d_items) hipFree(d_items);
         if (d_sw) hipFree(d_sw);
+        for (hipEvent_t* e : {&ev_go, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
+            if (*e) hipEventDestroy(*e), *e = nullptr;
+        if (side) hipStreamDestroy(side);
+        side = nullptr;
         d_sw = nullptr;
         freeze = sw_recorded = replayed = false;
         pool = nullptr;
         d_groups = nullptr;
         d_gidx = nullptr;
-        d_desc = nullptr;
         d_items = nullptr;
+        have_forward = have_backward = false;
     }
 
     void check_config() const {
@@ -413,7 +433,8 @@ struct AdjointDriver {
         cells = (size_t)b.gx * b.gy;
         auto al = [](size_t x) { return (x + 31) & ~size_t(31); };
         bufn = al(cells * nq);
-        const size_t nbuf = (size_t)nseg + (size_t)(seg + 1) + 3 * (size_t)seg + 2;
+        const size_t per_slot = (size_t)(seg + 1) + 3 * (size_t)seg;
+        const size_t nbuf = (size_t)nseg + 2 * per_slot + 2;
         const size_t total = nbuf * bufn + al(cells) + 2 * al((size_t)nsteps * std::max(ng, 1));
         ZAMR_HIP(hipMalloc(&pool, total * sizeof(double)));
         double* p = pool;
@@ -423,15 +444,13 @@ struct AdjointDriver {
             return r;
         };
         ck.clear();
-        X.clear();
-        Y.clear();
-        V1.clear();
-        V2.clear();
         for (int m = 0; m < nseg; ++m) ck.push_back(take(bufn));
-        for (int i = 0; i <= seg; ++i) X.push_back(take(bufn));
-        for (int i = 0; i < seg; ++i) Y.push_back(take(bufn));
-        for (int i = 0; i < seg; ++i) V1.push_back(take(bufn));
-        for (int i = 0; i < seg; ++i) V2.push_back(take(bufn));
+        for (auto& sl : slot) {
+            for (int i = 0; i <= seg; ++i) sl.X.push_back(take(bufn));
+            for (int i = 0; i < seg; ++i) sl.Y.push_back(take(bufn));
+            for (int i = 0; i < seg; ++i) sl.V1.push_back(take(bufn));
+            for (int i = 0; i < seg; ++i) sl.V2.push_back(take(bufn));
+        }
         Ab = take(bufn);
         Bb = take(bufn);
         nbar = take(al(cells));
@@ -460,18 +479,23 @@ struct AdjointDriver {
             ZAMR_HIP(hipMalloc(&d_gidx, ng * sizeof(long long)));
             ZAMR_HIP(hipMemcpyAsync(d_gidx, gidx.data(), ng * sizeof(long long), hipMemcpyHostToDevice, s.stream));
         }
-        std::vector<BoxDesc> dd(seg);
-        for (int i = 0; i < seg; ++i) {
-            dd[i] = s.desc_of(b, X[i], Y[i]);
-            dd[i].group = 0;
+        for (auto& sl : slot) {
+            std::vector<BoxDesc> dd(seg);
+            for (int i = 0; i < seg; ++i) {
+                dd[i] = s.desc_of(b, sl.X[i], sl.Y[i]);
+                dd[i].group = 0;
+            }
+            ZAMR_HIP(hipMalloc(&sl.d_desc, dd.size() * sizeof(BoxDesc)));
+            ZAMR_HIP(hipMemcpyAsync(sl.d_desc, dd.data(), dd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice,
+                                    s.stream));
+            ZAMR_HIP(hipStreamSynchronize(s.stream));  // dd leaves scope
         }
-        ZAMR_HIP(hipMalloc(&d_desc, dd.size() * sizeof(BoxDesc)));
-        ZAMR_HIP(hipMemcpyAsync(d_desc, dd.data(), dd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, s.stream));
         build_items();
+        ZAMR_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_go, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
+            ZAMR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        s.items(HX - 1);  // the forward stage's work items, built before any second-stream use
         ZAMR_HIP(hipStreamSynchronize(s.stream));
-        stored = -1;
-        have_forward = false;
-        have_backward = false;
     }
 
     void build_items() {
@@ -484,27 +508,28 @@ struct AdjointDriver {
         nitems = (int)v.size();
         ZAMR_HIP(hipMalloc(&d_items, v.size() * sizeof(WorkItem)));
         ZAMR_HIP(hipMemcpyAsync(d_items, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s.stream));
+        ZAMR_HIP(hipStreamSynchronize(s.stream));
     }
 
     int seg_len(int m) const { return std::min(seg, nsteps - m * seg); }
     double* state_buf(int k) const { return s.d_pool + s.hb[0].off_q[k]; }
 
-    // Stage `st` (0, 1) of step `step`; sw = 0 plain, 1 record, 2 replay.
+    // Stage of step `step` on stream st; sw = 0 plain, 1 record, 2 replay.
     template <int EPI>
-    void stage(const BoxDesc* boxes, StageArgs a, int step, int sw) {
+    void stage(const BoxDesc* boxes, StageArgs a, int step, int sw, hipStream_t st) {
         auto& it = s.items(HX - 1);
         const dim3 grid(slots_for(it.n)), block(NT);
         if (sw == 0) {
-            hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 0>), grid, block, 0, s.stream, boxes, it.d, it.n, a,
+            hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 0>), grid, block, 0, st, boxes, it.d, it.n, a,
                                d_groups + step, s.xcd_remap);
         } else {
             a.sw = d_sw + ((size_t)2 * step + (EPI == 0 ? 0 : 1)) * cells;
             if (sw == 1)
-                hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 1>), grid, block, 0, s.stream, boxes, it.d, it.n,
-                                   a, d_groups + step, s.xcd_remap);
+                hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 1>), grid, block, 0, st, boxes, it.d, it.n, a,
+                                   d_groups + step, s.xcd_remap);
             else
-                hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 2>), grid, block, 0, s.stream, boxes, it.d, it.n,
-                                   a, d_groups + step, s.xcd_remap);
+                hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 2>), grid, block, 0, st, boxes, it.d, it.n, a,
+                                   d_groups + step, s.xcd_remap);
         }
         ZAMR_HIP(hipGetLastError());
     }
@@ -528,27 +553,27 @@ struct AdjointDriver {
         ZAMR_HIP(hipGetLastError());
     }
 
-    void copy(double* dst, const double* src) {
-        ZAMR_HIP(hipMemcpyAsync(dst, src, cells * nq * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    void copy(double* dst, const double* src, hipStream_t st) {
+        ZAMR_HIP(hipMemcpyAsync(dst, src, cells * nq * sizeof(double), hipMemcpyDeviceToDevice, st));
     }
 
-    // Steps of segment m from checkpoint ck[m] with every stage input and
-    // pre-implicit state kept (X[i], Y[i], V1[i], V2[i]); X[len] = the end.
-    void run_stored(int m, bool with_gauges, int sw) {
-        copy(X[0], ck[m]);
+    // Steps of segment m from checkpoint ck[m] into slot k, on stream st.
+    void run_stored(int m, int k, bool with_gauges, int sw, hipStream_t st) {
+        Slot& sl = slot[k];
+        copy(sl.X[0], ck[m], st);
         const int len = seg_len(m);
         for (int i = 0; i < len; ++i) {
-            const int k = m * seg + i;
+            const int step = m * seg + i;
             StageArgs a1 = s.stage_args(0, 1, 0);
-            a1.pre = V1[i];
-            stage<0>(d_desc + i, a1, k, sw);
+            a1.pre = sl.V1[i];
+            stage<0>(sl.d_desc + i, a1, step, sw, st);
             StageArgs a2 = s.stage_args(1, 0, 0);
-            a2.pre = V2[i];
-            a2.dstp = X[i + 1];
-            stage<1>(d_desc + i, a2, k, sw);
-            if (with_gauges) sample(X[i + 1], k);
+            a2.pre = sl.V2[i];
+            a2.dstp = sl.X[i + 1];
+            stage<1>(sl.d_desc + i, a2, step, sw, st);
+            if (with_gauges) sample(sl.X[i + 1], step);
         }
-        stored = m;
+        sl.held = m;
     }
 
     // Forward from the solver's current state with the frozen dt sequence;
@@ -558,18 +583,19 @@ struct AdjointDriver {
         check_config();
         s.ensure_aux();
         const int sw = freeze ? (sw_recorded ? 2 : 1) : 0;
+        slot[0].held = slot[1].held = -1;
         for (int m = 0; m < nseg; ++m) {
-            copy(ck[m], state_buf(s.cur));
+            copy(ck[m], state_buf(s.cur), s.stream);
             if (m + 1 < nseg) {
                 for (int i = 0; i < seg_len(m); ++i) {
                     const int k = m * seg + i;
-                    stage<0>(s.d_boxes, s.stage_args(s.cur, 1 - s.cur, s.cur), k, sw);
-                    stage<1>(s.d_boxes, s.stage_args(1 - s.cur, s.cur, s.cur), k, sw);
+                    stage<0>(s.d_boxes, s.stage_args(s.cur, 1 - s.cur, s.cur), k, sw, s.stream);
+                    stage<1>(s.d_boxes, s.stage_args(1 - s.cur, s.cur, s.cur), k, sw, s.stream);
                     sample(state_buf(s.cur), k);
                 }
             } else {
-                run_stored(m, true, sw);
-                copy(state_buf(s.cur), X[seg_len(m)]);
+                run_stored(m, 0, true, sw, s.stream);
+                copy(state_buf(s.cur), slot[0].X[seg_len(m)], s.stream);
             }
         }
         if (sw == 1) sw_recorded = true;
@@ -584,7 +610,7 @@ struct AdjointDriver {
         have_backward = false;
     }
 
-    void reverse_stage(const double* U, const double* V, const double* Wb, double wscale, double* out,
+    void reverse_stage(const Slot& sl, const double* U, const double* V, const double* Wb, double wscale, double* out,
                        const double* heun, int k) {
         AdjArgs a;
         a.p = s.prm;
@@ -599,11 +625,29 @@ struct AdjointDriver {
         a.time = times[k];
         a.clamp = 0;
         hipLaunchKernelGGL((adjoint_stage_kernel<M, AD, ORDER>), dim3((nitems + NT / WV - 1) / (NT / WV)), dim3(NT), 0,
-                           s.stream, d_desc, d_items, nitems, a);
+                           s.stream, sl.d_desc, d_items, nitems, a);
         ZAMR_HIP(hipGetLastError());
     }
 
+    // Reverse of segment m from slot k (main stream).
+    void reverse_segment(int m, int k) {
+        const Slot& sl = slot[k];
+        for (int i = seg_len(m) - 1; i >= 0; --i) {
+            const int step = m * seg + i;
+            if (ng > 0) {
+                hipLaunchKernelGGL(gauge_inject_kernel, dim3((ng + 255) / 256), dim3(256), 0, s.stream, Ab,
+                                   (long long)cells, d_gidx, ng, d_gbar + (size_t)step * ng);
+                ZAMR_HIP(hipGetLastError());
+            }
+            reverse_stage(sl, sl.Y[i], sl.V2[i], Ab, 0.5, Bb, nullptr, step);
+            reverse_stage(sl, sl.X[i], sl.V1[i], Bb, 1.0, Ab, Ab, step);
+        }
+    }
+
     // Backward over all steps; gbar = adjoint of the gauge depths (nsteps * ng).
+    // Segment m is reversed on the main stream while the second stream
+    // recomputes segment m-1 from its checkpoint into the other slot; events
+    // order slot reuse both ways.
     void backward(const double* gbar) {
         if (!have_forward) throw Error(ZAMR_EINVAL, "adjoint: run zamr_adj_forward first");
         if (replayed)
@@ -617,21 +661,29 @@ struct AdjointDriver {
         ZAMR_HIP(hipMemsetAsync(Ab, 0, bufn * sizeof(double), s.stream));
         ZAMR_HIP(hipMemsetAsync(Bb, 0, bufn * sizeof(double), s.stream));
         ZAMR_HIP(hipMemsetAsync(nbar, 0, cells * sizeof(double), s.stream));
+        s.items(HX - 1);  // (re)built on the main stream, before the recompute stream reads it
+        // the recompute stream starts after everything queued so far (the forward)
+        ZAMR_HIP(hipEventRecord(ev_go, s.stream));
+        ZAMR_HIP(hipStreamWaitEvent(side, ev_go, 0));
+        int k = (slot[0].held == nseg - 1) ? 0 : ((slot[1].held == nseg - 1) ? 1 : -1);
+        if (k < 0) {
+            k = 0;
+            run_stored(nseg - 1, 0, false, fwd_sw, s.stream);
+        }
+        ZAMR_HIP(hipEventRecord(ev_rev[1 - k], s.stream));  // the other slot is free
         for (int m = nseg - 1; m >= 0; --m) {
-            // the recompute runs the build of the stage kernel the forward ran,
-            // so that it reproduces the forward bit for bit (FMA contraction
-            // may differ between builds)
-            if (stored != m) run_stored(m, false, fwd_sw);
-            for (int i = seg_len(m) - 1; i >= 0; --i) {
-                const int k = m * seg + i;
-                if (ng > 0) {
-                    hipLaunchKernelGGL(gauge_inject_kernel, dim3((ng + 255) / 256), dim3(256), 0, s.stream, Ab,
-                                       (long long)cells, d_gidx, ng, d_gbar + (size_t)k * ng);
-                    ZAMR_HIP(hipGetLastError());
-                }
-                reverse_stage(Y[i], V2[i], Ab, 0.5, Bb, nullptr, k);
-                reverse_stage(X[i], V1[i], Bb, 1.0, Ab, Ab, k);
+            if (m > 0) {
+                // the recompute runs the build of the stage kernel the forward
+                // ran, so that it reproduces the forward bit for bit (FMA
+                // contraction may differ between builds)
+                ZAMR_HIP(hipStreamWaitEvent(side, ev_rev[1 - k], 0));
+                run_stored(m - 1, 1 - k, false, fwd_sw, side);
+                ZAMR_HIP(hipEventRecord(ev_rec[1 - k], side));
             }
+            reverse_segment(m, k);
+            ZAMR_HIP(hipEventRecord(ev_rev[k], s.stream));
+            if (m > 0) ZAMR_HIP(hipStreamWaitEvent(s.stream, ev_rec[1 - k], 0));
+            k = 1 - k;
         }
         ZAMR_HIP(hipStreamSynchronize(s.stream));
         have_backward = true;

@@ -142,6 +142,12 @@ __device__ __forceinline__ double recip(double x) {
 template <class T>
 __device__ __forceinline__ T dev_max(T a, T b) { return (a < b) ? b : a; }
 
+template <class V>
+__device__ __forceinline__ void vq_zero(V& v) {
+#pragma unroll
+    for (int n = 0; n < V::row_size * V::column_size; ++n) v(n, 0) = 0.0;
+}
+
 // make_rhs.H:9-13
 __device__ __forceinline__ double minmod(double a, double b) {
     if (a * b <= 0.0) return 0.0;
@@ -268,39 +274,31 @@ struct Sys {
         return finish(B, fetch(B, q, c, j), c, j, p, clamp, time);
     }
 
-    // eval_face (make_rhs.H:31-131) from the four cells straddling the face.
-    // SW = 1 also reports which argument each minmod returned (2 bits per
-    // call, 4 per component, in *bits); SW = 2 takes those choices from *bits
-    // instead of comparing (a forward with the limiter frozen).
-    template <int ORDER, int DIR, class N, int SW = 0>
-    __device__ static __forceinline__ void face(const VQ& Lm, const VQ& L, const VQ& R, const VQ& Rp,
-                                                const VX& XL, const VX& XR, const VP& p, double time, int wb,
-                                                VQ& flux, VF& fluct, double& hLs, double& hRs,
-                                                unsigned* bits = nullptr) {
-        static_assert(SW == 0 || nq <= 4, "the switch table holds 16 bits per face");
-        VQ qL, qR;
-#pragma unroll
-        for (int n = 0; n < nq; ++n) {
-            if constexpr (ORDER == 1) {
-                qL(n, 0) = L(n, 0);
-                qR(n, 0) = R(n, 0);
-            } else {
-                const double a1 = L(n, 0) - Lm(n, 0), b1 = R(n, 0) - L(n, 0), b2 = Rp(n, 0) - R(n, 0);
-                double gL, gR;
-                if constexpr (SW == 2) {
-                    const unsigned sl = (*bits >> (4 * n)) & 3u, sr = (*bits >> (4 * n + 2)) & 3u;
-                    gL = (sl == 1u) ? a1 : ((sl == 2u) ? b1 : 0.0);
-                    gR = (sr == 1u) ? b1 : ((sr == 2u) ? b2 : 0.0);
-                } else {
-                    gL = minmod(a1, b1);
-                    gR = minmod(b1, b2);
-                    if constexpr (SW == 1)
-                        *bits |= (minmod_code(a1, b1) << (4 * n)) | (minmod_code(b1, b2) << (4 * n + 2));
-                }
-                qL(n, 0) = L(n, 0) + 0.5 * gL;
-                qR(n, 0) = R(n, 0) - 0.5 * gR;
-            }
+    // Limited slope of a cell from its two neighbours along one direction,
+    // minmod(q - qm, qp - q) (make_rhs.H:57-75): the cell's face values are
+    // q +- slope/2.  Every slope is shared by the two faces of the cell.
+    // SW = 1 also reports which argument minmod returned (2 bits at bit 2n of
+    // *code); SW = 2 takes that choice from *code instead of comparing (a
+    // forward with the limiter frozen).
+    template <int SW>
+    __device__ static __forceinline__ double slope(double qm, double q, double qp, unsigned* code, int n) {
+        const double a = q - qm, b = qp - q;
+        if constexpr (SW == 2) {
+            const unsigned k = (*code >> (2 * n)) & 3u;
+            return (k == 1u) ? a : ((k == 2u) ? b : 0.0);
+        } else {
+            if constexpr (SW == 1) *code |= minmod_code(a, b) << (2 * n);
+            return minmod(a, b);
         }
+    }
+
+    // The rest of eval_face (make_rhs.H:76-131) on the reconstructed face
+    // states qL, qR: Audusse hydrostatic reconstruction, face aux refresh,
+    // numerical flux and fluctuations.
+    template <int DIR, class N>
+    __device__ static __forceinline__ void face_states(VQ qL, VQ qR, const VX& XL, const VX& XR, const VP& p,
+                                                       double time, int wb, VQ& flux, VF& fluct, double& hLs,
+                                                       double& hRs) {
         hLs = qL(1, 0);
         hRs = qR(1, 0);
         if (wb) {  // Audusse hydrostatic reconstruction, make_rhs.H:84-98
@@ -485,9 +483,10 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 #define ZAMR_STAGE_ATTR
 #endif
 
-// SW: 0 = plain; 1 = record every face's limiter choices into a.sw (one
-// uint32 per cell: its left x-face in the low 16 bits, its top y-face in the
-// high 16); 2 = replay them (single box, for derivative checks).
+// SW: 0 = plain; 1 = record every cell's limiter choices into a.sw, read
+// as bytes: the four 2-bit codes of the cell's x slope at byte idx and of its
+// y slope at byte sc + idx (idx = the cell's index in the box layout, sc the
+// component stride); 2 = replay them (single box, for derivative checks).
 template <class M, class N, int ORDER, int EPI, int SW = 0>
 __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc* __restrict__ boxes,
                                                    const WorkItem* __restrict__ items, int nitems,
@@ -501,11 +500,8 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
     constexpr int nq = S::nq;
     constexpr int dim = S::dim;
     constexpr int HX = (ORDER == 2) ? 2 : 1;
-    constexpr int TXW = WV - 2 * HX;                  // cells per item row
-    // Window of rows j-HX+1 .. j+HX at row j (the bottom face of row j was
-    // evaluated one row earlier, so row j-HX is no longer needed).
-    constexpr int WR = (dim == 2) ? 2 * HX : 1;
-    constexpr int WC = (dim == 2) ? HX - 1 : 0;  // window slot of row j
+    constexpr int TXW = WV - 2 * HX;  // cells per item row
+    static_assert(SW == 0 || nq <= 4, "the switch table holds four 2-bit codes per slope");
 
     VP p;
 #pragma unroll
@@ -550,38 +546,65 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const double* __restrict__ qs = a.src ? B.q1 : B.q0;
         double* __restrict__ qd = a.dstp ? a.dstp : (a.dst ? B.q1 : B.q0);
         const double* __restrict__ qo = a.old ? B.q1 : B.q0;
+        unsigned char *swx = nullptr, *swy = nullptr;
+        if constexpr (SW != 0) {
+            swx = reinterpret_cast<unsigned char*>(a.sw);
+            swy = swx + B.sc;
+        }
+        // lanes whose x slope a face uses, and where the table holds it
+        const bool sx_used = (t >= HX - 1) && (t <= HX + TXW) && (c >= B.lo0 - 1) && (c <= B.hi0 + 1);
+        const bool sx_rec = own || (t == HX - 1 && c == B.lo0 - 1) || (xf && c == B.hi0 + 1);
 
-        VQ win[WR];
         const int jstart = w.j0;
         const int jend = w.j0 + w.nj;
-        // bottom y-face of the first row (face jstart - 1/2) from rows
-        // jstart-HX .. jstart+HX-1, which then become the window minus its lead
+        // y state: W0 = row j, W1 = row j+1 (order 2), sy0 = limited y slope
+        // of row j, and the bottom face j-1/2 of row j (evaluated one row
+        // earlier, or here for the first row from rows jstart-HX .. jstart+HX-1)
+        VQ W0, W1, sy0;
+        vq_zero(W0);
+        vq_zero(W1);
+        vq_zero(sy0);
         VQ fyB;
         VF dyB;
         double hyB = 0.0;
         if constexpr (dim == 2) {
-            VQ pro[2 * HX];
+            VQ P[2 * HX];
 #pragma unroll
             for (int r = 0; r < 2 * HX; ++r) {
-                if (col_in) pro[r] = S::load(B, qs, c, jstart - HX + r, p, a.clamp, time);
-                else
-#pragma unroll
-                    for (int n = 0; n < nq; ++n) pro[r](n, 0) = 0.0;
+                if (col_in) P[r] = S::load(B, qs, c, jstart - HX + r, p, a.clamp, time);
+                else vq_zero(P[r]);
             }
             if (own) {
-                double hl;
-                unsigned bb = 0u;
-                const long long ib = (long long)(c - B.gl0) + (long long)(jstart - 1 - B.gl1) * B.sy;
-                if constexpr (SW == 2) bb = a.sw[ib] >> 16;
-                S::template face<ORDER, 1, N, SW>(pro[0], pro[HX - 1], pro[HX], pro[2 * HX - 1],
-                                                  S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
-                                                  a.wb, fyB, dyB, hl, hyB, &bb);
-                // the face below the box is nobody's top face: its first item keeps it
-                if constexpr (SW == 1)
-                    if (jstart == B.lo1) a.sw[ib] = bb << 16;
-            }
+                VQ qL, qR;
+                if constexpr (ORDER == 2) {
+                    unsigned cm = 0u, c0 = 0u;
+                    const long long im = (long long)(c - B.gl0) + (long long)(jstart - 1 - B.gl1) * B.sy;
+                    const long long i0 = im + B.sy;
+                    if constexpr (SW == 2) {
+                        cm = swy[im];
+                        c0 = swy[i0];
+                    }
 #pragma unroll
-            for (int r = 0; r < WR - 1; ++r) win[r] = pro[r + 1];
+                    for (int n = 0; n < nq; ++n) {
+                        const double sm = S::template slope<SW>(P[0](n, 0), P[1](n, 0), P[2](n, 0), &cm, n);
+                        sy0(n, 0) = S::template slope<SW>(P[1](n, 0), P[2](n, 0), P[3](n, 0), &c0, n);
+                        qL(n, 0) = P[1](n, 0) + 0.5 * sm;
+                        qR(n, 0) = P[2](n, 0) - 0.5 * sy0(n, 0);
+                    }
+                    if constexpr (SW == 1) {
+                        if (jstart == B.lo1) swy[im] = (unsigned char)cm;  // the ghost row below the box
+                        swy[i0] = (unsigned char)c0;
+                    }
+                } else {
+                    qL = P[0];
+                    qR = P[1];
+                }
+                double hl;
+                S::template face_states<1, N>(qL, qR, S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
+                                              a.wb, fyB, dyB, hl, hyB);
+            }
+            W0 = P[HX];
+            if constexpr (ORDER == 2) W1 = P[HX + 1];
         }
         // first leading row
         const int jlead0 = (dim == 2) ? jstart + HX : jstart;
@@ -592,37 +615,45 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             const int jl = (dim == 2) ? j + HX : j;
             VQ lead;
             if (col_in) lead = S::finish(B, nxt, c, jl, p, a.clamp, time);
-            else
-#pragma unroll
-                for (int n = 0; n < nq; ++n) lead(n, 0) = 0.0;
+            else vq_zero(lead);
             if (col_in && j + 1 < jend) nxt = S::fetch(B, qs, c, jl + 1);  // one row ahead
-            win[WR - 1] = lead;
-            const VQ& qc = win[WC];
-            // x-face c-1/2 (left face of this column), neighbours by lane shifts
-            VQ Lm, L, Rp;
+            if constexpr (dim == 1) W0 = lead;
+            const VQ& qc = W0;
+            const long long isw = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+            // x slopes of every lane's cell, then the x-face c-1/2 (left face of
+            // this column) from the left lane's right face value and this
+            // lane's left face value
+            VQ eR, eL;
+            {
+                unsigned cx = 0u;
+                if constexpr (SW == 2)
+                    if (sx_used) cx = swx[isw];
 #pragma unroll
-            for (int n = 0; n < nq; ++n) {
-                L(n, 0) = from_left(qc(n, 0));
-                if constexpr (ORDER == 2) {
-                    Lm(n, 0) = from_left(L(n, 0));
-                    Rp(n, 0) = from_right(qc(n, 0));
-                } else {
-                    Lm(n, 0) = 0.0;
-                    Rp(n, 0) = 0.0;
+                for (int n = 0; n < nq; ++n) {
+                    if constexpr (ORDER == 2) {
+                        const double qm = from_left(qc(n, 0)), qp = from_right(qc(n, 0));
+                        const double sx = S::template slope<SW>(qm, qc(n, 0), qp, &cx, n);
+                        eR(n, 0) = qc(n, 0) + 0.5 * sx;
+                        eL(n, 0) = qc(n, 0) - 0.5 * sx;
+                    } else {
+                        eR(n, 0) = qc(n, 0);
+                        eL(n, 0) = qc(n, 0);
+                    }
                 }
+                if constexpr (SW == 1)
+                    if (sx_rec) swx[isw] = (unsigned char)cx;
             }
+            VQ qLx;
+#pragma unroll
+            for (int n = 0; n < nq; ++n) qLx(n, 0) = from_left(eR(n, 0));
             VQ fxL;
             VF dxL;
             double hxL = 0.0, hl = 0.0;
-            unsigned bx = 0u, by = 0u;
-            const long long isw = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
             if (xf) {
-                if constexpr (SW == 2) bx = a.sw[isw] & 0xFFFFu;
-                S::template face<ORDER, 0, N, SW>(Lm, L, qc, Rp, S::center(B, c - 1, j), S::center(B, c, j), p,
-                                                  time, a.wb, fxL, dxL, hl, hxL, &bx);
+                S::template face_states<0, N>(qLx, eL, S::center(B, c - 1, j), S::center(B, c, j), p, time, a.wb,
+                                              fxL, dxL, hl, hxL);
             } else {
-#pragma unroll
-                for (int n = 0; n < nq; ++n) fxL(n, 0) = 0.0;
+                vq_zero(fxL);
 #pragma unroll
                 for (int n = 0; n < 2 * nq; ++n) dxL(n, 0) = 0.0;
             }
@@ -636,23 +667,35 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 Fx[n] = -(fR - fxL(n, 0)) - dmR - dxL(n, 0);  // face_divergence, make_rhs.H:142-147
             }
             const double hL_xr = from_right(hl);
-            // top y-face j+1/2 (needs rows j-HX+1 .. j+HX)
-            VQ fyT;
+            // top y-face j+1/2
+            VQ fyT, sy1;
             VF dyT;
             double hyT = 0.0, hyTnext = 0.0;
+            vq_zero(sy1);
             if constexpr (dim == 2) {
                 if (own) {
-                    const VQ& Lm = win[(HX == 2) ? 0 : 0];
-                    const VQ& Rp = win[WR - 1];
-                    if constexpr (SW == 2) by = a.sw[isw] >> 16;
-                    S::template face<ORDER, 1, N, SW>(Lm, win[WC], win[WC + 1], Rp, S::center(B, c, j),
-                                                      S::center(B, c, j + 1), p, time, a.wb, fyT, dyT, hyT, hyTnext,
-                                                      &by);
+                    VQ qL, qR;
+                    if constexpr (ORDER == 2) {
+                        unsigned cy = 0u;
+                        const long long iy = isw + B.sy;  // row j+1
+                        if constexpr (SW == 2) cy = swy[iy];
+#pragma unroll
+                        for (int n = 0; n < nq; ++n) {
+                            sy1(n, 0) = S::template slope<SW>(W0(n, 0), W1(n, 0), lead(n, 0), &cy, n);
+                            qL(n, 0) = W0(n, 0) + 0.5 * sy0(n, 0);
+                            qR(n, 0) = W1(n, 0) - 0.5 * sy1(n, 0);
+                        }
+                        // the next item's prologue records its first row; the
+                        // last item records the ghost row above the box
+                        if constexpr (SW == 1)
+                            if (j + 1 < jend || jend == B.hi1 + 1) swy[iy] = (unsigned char)cy;
+                    } else {
+                        qL = W0;
+                        qR = lead;
+                    }
+                    S::template face_states<1, N>(qL, qR, S::center(B, c, j), S::center(B, c, j + 1), p, time, a.wb,
+                                                  fyT, dyT, hyT, hyTnext);
                 }
-            }
-            if constexpr (SW == 1) {
-                if (own) a.sw[isw] = bx | (by << 16);
-                else if (xf && c == B.hi0 + 1) a.sw[isw] = bx;
             }
             if (own) {
                 const typename S::VX X = S::center(B, c, j);
@@ -679,7 +722,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                     qn(n, 0) = qc(n, 0) + dt * rhs;
                 }
-                const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+                const long long idx = isw;
                 if (a.pre) {
 #pragma unroll
                     for (int n = 0; n < nq; ++n) a.pre[idx + n * B.sc] = qn(n, 0);
@@ -704,13 +747,18 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
 #pragma unroll
                 for (int n = 0; n < nq; ++n) __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);
             }
-            // carry the top face down and shift the window
+            // carry the top face and the slope down and shift the window
             if constexpr (dim == 2) {
                 fyB = fyT;
                 dyB = dyT;
                 hyB = hyTnext;
-#pragma unroll
-                for (int r = 0; r < WR - 1; ++r) win[r] = win[r + 1];
+                if constexpr (ORDER == 2) {
+                    sy0 = sy1;
+                    W0 = W1;
+                    W1 = lead;
+                } else {
+                    W0 = lead;
+                }
             }
         }
         if constexpr (EPI != 0) {

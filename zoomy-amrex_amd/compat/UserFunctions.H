@@ -7,6 +7,7 @@
 // reduction turn into a visible non-finite result instead of a silent one.
 #include <limits>
 #include "AMReX_SmallMatrix.H"
+#include "ZamrMath.H"
 
 namespace amrex_user {
 template <class... Args>

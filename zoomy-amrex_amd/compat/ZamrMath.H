@@ -1,0 +1,49 @@
+#pragma once
+// sqrt(x) and a / sqrt(x) for the model headers.
+//
+// Host code and the bitwise build (ZAMR_BITWISE) use std::sqrt and a true
+// division, correctly rounded, so that the CPU oracle and the bitwise GPU
+// build agree bit for bit.
+//
+// Device code of the throughput build starts from the hardware estimate y of
+// 1/sqrt(x) (v_rsq_f64, relative error near 2^-23):
+//   sqrt:     one coupled Newton step on s = x y and h = y/2
+//             (r = 1/2 - s h; s += s r; h += h r), then s += (x - s^2) h;
+//             y is capped at 2^600 so that x = 0 gives 0;
+//   a/sqrt:   two Newton steps y += y (1/2 - (x/2) y^2), then a y.
+// No intermediate squares y^2, so every finite x >= 0, subnormals included,
+// stays in range; the results land within an ulp or two of the correctly
+// rounded values in under half the instructions.  Negative and NaN
+// arguments give NaN as std::sqrt does; an infinite argument gives NaN where
+// std::sqrt gives inf (a state that has blown up either way).
+#include <cmath>
+#include "AMReX_SmallMatrix.H"
+
+namespace zamr_math {
+
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
+    const double y = fmin(__builtin_amdgcn_rsq(x), 0x1p600);
+    double s = x * y, h = 0.5 * y;
+    const double r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    return fma(fma(-s, s, x), h, s);
+#else
+    return std::sqrt(x);
+#endif
+}
+
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double div_sqrt(double a, double x) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return a * y;
+#else
+    return a / std::sqrt(x);
+#endif
+}
+
+}  // namespace zamr_math

@@ -14,7 +14,7 @@ struct Numerics {
     {
         amrex::SmallMatrix<amrex::Real, 1, 1> res{};
         const amrex::Real un = Q(2, 0) * n(0, 0) * Qaux(3, 0);
-        res(0, 0) = std::fabs(un) + std::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
+        res(0, 0) = std::fabs(un) + zamr_math::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
         return res;
     }
 
@@ -31,8 +31,8 @@ struct Numerics {
         const amrex::Real unR = qnR * aR(3, 0);
         const amrex::Real phL = 0.5 * g * qL(1, 0) * qL(1, 0);
         const amrex::Real phR = 0.5 * g * qR(1, 0) * qR(1, 0);
-        const amrex::Real sL = std::fabs(unL) + std::sqrt(g * amrex::max(qL(1, 0), amrex::Real(0.0)));
-        const amrex::Real sR = std::fabs(unR) + std::sqrt(g * amrex::max(qR(1, 0), amrex::Real(0.0)));
+        const amrex::Real sL = std::fabs(unL) + zamr_math::sqrt(g * amrex::max(qL(1, 0), amrex::Real(0.0)));
+        const amrex::Real sR = std::fabs(unR) + zamr_math::sqrt(g * amrex::max(qR(1, 0), amrex::Real(0.0)));
         const amrex::Real hs = 0.5 * amrex::max(sL, sR);
         res(0, 0) = 0.0;
         res(1, 0) = 0.5 * (qnL + qnR) - hs * (qR(1, 0) - qL(1, 0));

@@ -84,7 +84,7 @@ struct Model {
     {
         const amrex::Real h4 = amrex::Math::powi<4>(h);
         const amrex::Real e4 = amrex::Math::powi<4>(eps);
-        return 1.4142135623730951 * h / std::sqrt(h4 + amrex::max(h4, e4));
+        return zamr_math::div_sqrt(1.4142135623730951 * h, h4 + amrex::max(h4, e4));
     }
 
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 8, 1> flux(
@@ -166,7 +166,7 @@ struct Model {
     {
         auto res = VQ{};
         const amrex::Real un = (Q(2, 0) * n(0, 0) + Q(3, 0) * n(1, 0)) * Qaux(6, 0);
-        const amrex::Real c = std::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
+        const amrex::Real c = zamr_math::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
         res(0, 0) = 0.0;
         res(1, 0) = un - c;
         res(2, 0) = un;
@@ -194,7 +194,7 @@ struct Model {
         auto res = VQ{};
         const amrex::Real u = Q(2, 0) * Qaux(6, 0);
         const amrex::Real v = Q(3, 0) * Qaux(6, 0);
-        const amrex::Real speed = std::sqrt(u * u + v * v);
+        const amrex::Real speed = zamr_math::sqrt(u * u + v * v);
         const amrex::Real r = zamr_rcbrt(amrex::max(Q(1, 0), p(5, 0)));
         const amrex::Real k = p(0, 0) * p(7, 0) * p(7, 0) * speed * amrex::Math::powi<4>(r);
         res(0, 0) = 0.0;
@@ -213,7 +213,7 @@ struct Model {
         const amrex::Real hinv = Qaux(6, 0);
         const amrex::Real u = Q(2, 0) * hinv;
         const amrex::Real v = Q(3, 0) * hinv;
-        const amrex::Real speed = std::sqrt(u * u + v * v);
+        const amrex::Real speed = zamr_math::sqrt(u * u + v * v);
         const amrex::Real hf = amrex::max(Q(1, 0), p(5, 0));
         const amrex::Real r = zamr_rcbrt(hf);
         const amrex::Real c = p(0, 0) * p(7, 0) * p(7, 0) * amrex::Math::powi<4>(r);
@@ -237,7 +237,7 @@ struct Model {
         VQ const& Q, VA const& Qaux, VP const& p)
     {
         auto res = amrex::SmallMatrix<amrex::Real, 28, 1>{};
-        const amrex::Real m = std::sqrt(Q(2, 0) * Q(2, 0) + Q(3, 0) * Q(3, 0));
+        const amrex::Real m = zamr_math::sqrt(Q(2, 0) * Q(2, 0) + Q(3, 0) * Q(3, 0));
         const amrex::Real r = zamr_rcbrt(amrex::max(Q(1, 0), p(5, 0)));
         const amrex::Real c = p(0, 0) * p(7, 0) * p(7, 0) * amrex::Math::powi<4>(r);
         res(2 * 7 + 6, 0) = -c * m * Q(2, 0);

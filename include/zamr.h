@@ -132,6 +132,11 @@ int zamr_set_tuning(zamr_ctx* ctx, int rows_per_item, int per_box_launch, int xc
 int zamr_set_profiling(zamr_ctx* ctx, int on);
 int zamr_kernel_times(zamr_ctx* ctx, double* out_ms, int* nsteps);
 
+/* solver.positivity = mood (ZoomyAmr.cpp:699-720): troubled cells of the last
+ * step's order-2 candidate in time-step group `group` (0 for a single level)
+ * and the number of steps that group has redone at order 1 since creation. */
+int zamr_mood_stats(zamr_ctx* ctx, int group, int* troubled, long long* redos);
+
 /* (dt, time after the step) of the steps taken on the device clock since the
  * last zamr_reset_clock, up to cap entries — the values the reference prints
  * per step (ZoomyAmr.cpp:804-809).  *n receives the count. */

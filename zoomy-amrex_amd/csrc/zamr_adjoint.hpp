@@ -394,6 +394,7 @@ struct AdjointDriver {
         if (s.well_balanced) throw Error(ZAMR_EUNSUP, "the adjoint does not cover solver.well_balanced = true");
         if (!s.implicit_source) throw Error(ZAMR_EUNSUP, "the adjoint is built for solver.implicit_source = true");
         if (s.clamp_positivity) throw Error(ZAMR_EUNSUP, "the adjoint needs solver.clamp_positivity = false");
+        if (s.positivity != "none") throw Error(ZAMR_EUNSUP, "the adjoint needs solver.positivity = none");
         const HostBox& b = s.hb[0];
         if (b.lo[0] != b.dlo[0] || b.hi[0] != b.dhi[0] || b.lo[1] != b.dlo[1] || b.hi[1] != b.dhi[1])
             throw Error(ZAMR_EUNSUP, "the adjoint box must be the whole domain");

@@ -43,7 +43,7 @@ struct HostBox {
     double dx[2], plo[2];
     int group;
     int bc[4];
-    size_t off_q[2], off_aux;  // element offsets into the pool
+    size_t off_q[3], off_aux;  // element offsets into the pool (a third state buffer with MOOD)
     int gx, gy;                // grown extents
 };
 
@@ -189,9 +189,7 @@ class Solver {
         if (spatial_order != 1 && spatial_order != 2) throw Error(1, "solver.spatial_order must be 1 or 2");
         if (implicit_global)
             throw Error(5, "solver.implicit_global = true (matrix-free JFNK source solve) is not part of this build");
-        if (positivity == "mood")
-            throw Error(5, "solver.positivity = mood (a-posteriori order-1 redo) is not part of this build yet");
-        if (positivity != "none") throw Error(1, "solver.positivity must be none or mood");
+        if (positivity != "none" && positivity != "mood") throw Error(1, "solver.positivity must be none or mood");
         auto defs = M::default_parameters();
         for (int i = 0; i < np; ++i) prm.v[i] = defs[i];
         auto names = M::parameter_names();
@@ -303,6 +301,11 @@ class Solver {
             off = align(off + cells * nq);
             b.off_q[1] = off;
             off = align(off + cells * nq);
+            b.off_q[2] = b.off_q[1];
+            if (mood()) {  // Q^n survives the order-2 candidate for the redo
+                b.off_q[2] = off;
+                off = align(off + cells * nq);
+            }
             b.off_aux = off;
             off = align(off + cells * std::max(na, 1));
         }
@@ -310,7 +313,8 @@ class Solver {
         ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
         ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));
         std::vector<BoxDesc> bd(hb.size());
-        for (size_t k = 0; k < hb.size(); ++k) bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1]);
+        for (size_t k = 0; k < hb.size(); ++k)
+            bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1], d_pool + hb[k].off_q[2]);
         ZAMR_HIP(hipMalloc(&d_boxes, bd.size() * sizeof(BoxDesc)));
         ZAMR_HIP(hipMemcpyAsync(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, stream));
         h_groups.assign(ngroups, GroupState{});
@@ -335,10 +339,11 @@ class Solver {
     }
 
     // Device descriptor of box b with the given state buffers.
-    BoxDesc desc_of(const HostBox& b, double* q0, double* q1) const {
+    BoxDesc desc_of(const HostBox& b, double* q0, double* q1, double* q2 = nullptr) const {
         BoxDesc d;
         d.q[0] = q0;
         d.q[1] = q1;
+        d.q[2] = q2 ? q2 : q1;
         d.aux = d_pool + b.off_aux;
         for (int i = 0; i < 2; ++i) {
             d.lo[i] = b.lo[i];
@@ -576,6 +581,8 @@ class Solver {
         next_plot_time = 0.0;
     }
 
+    bool mood() const { return positivity == "mood" && spatial_order == 2; }
+
     // ---------------- kernels ----------------
     int hx_of(int order) const { return order == 2 ? 2 : 1; }
     static int aux_grid(int nitems) { return (nitems + NT / WV - 1) / (NT / WV); }
@@ -589,6 +596,7 @@ class Solver {
         a.wb = well_balanced ? 1 : 0;
         a.impl = implicit_source ? 1 : 0;
         a.clamp = (clamp_positivity && positivity != "mood") ? 1 : 0;
+        a.mood = 0;
         a.pre = nullptr;
         a.dstp = nullptr;
         a.sw = nullptr;
@@ -675,6 +683,30 @@ class Solver {
             fill_boundary(cur);
             launch_stage_dyn(1, 2, stage_args(cur, 1 - cur, cur));
             cur = 1 - cur;
+        } else if (mood()) {
+            // the order-2 candidate goes to a third buffer so that Q^n stays for
+            // an order-1 redo of the groups whose candidate has troubled cells
+            // (ZoomyAmr.cpp:699-720); the buffers rotate by one per step
+            const int t1 = (cur + 1) % 3, t2 = (cur + 2) % 3;
+            fill_boundary(cur);
+            record(0);
+            launch_stage_dyn(2, 0, stage_args(cur, t1, cur));
+            record(1);
+            fill_boundary(t1);
+            StageArgs a2 = stage_args(t1, t2, cur);
+            a2.mood = 1;
+            launch_stage_dyn(2, 1, a2);
+            record(2);
+            hipLaunchKernelGGL(mood_reset_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, d_groups, ngroups);
+            ZAMR_HIP(hipGetLastError());
+            StageArgs r1 = stage_args(cur, t1, cur);
+            r1.mood = 2;
+            launch_stage_dyn(1, 0, r1);
+            fill_boundary(t1);
+            StageArgs r2 = stage_args(t1, t2, cur);
+            r2.mood = 2;
+            launch_stage_dyn(1, 1, r2);
+            cur = t2;
         } else {
             fill_boundary(cur);
             record(0);
@@ -712,7 +744,9 @@ class Solver {
         prof_ms[0] = prof_ms[1] = prof_ms[2] = 0.0;
         prof_steps = 0;
         int k = 0;
-        const int chunk = profiling ? std::max(nsteps, 1) : 16;
+        // MOOD rotates three state buffers, so a captured chunk is a multiple of 3 steps
+        const int chunk = profiling ? std::max(nsteps, 1) : (mood() ? 15 : 16);
+        if (mood()) items(1);  // the redo's work items exist before any capture
         while (k < nsteps) {
             const int n = std::min(chunk, nsteps - k);
             if (graph_ok() && n == chunk) {
@@ -805,8 +839,9 @@ class Solver {
         materialize_aux();  // do_stage's UpdateState leaves Qaux = update_aux(stage-start state)
         set_clock(dt, time);
         fill_boundary(cur);
-        launch_stage_dyn(order, 0, stage_args(cur, 1 - cur, cur));
-        cur = 1 - cur;
+        const int nxt = (cur + 1) % (mood() ? 3 : 2);
+        launch_stage_dyn(order, 0, stage_args(cur, nxt, cur));
+        cur = nxt;
         ZAMR_HIP(hipStreamSynchronize(stream));
         maxinv_fresh = false;
     }
@@ -861,6 +896,9 @@ class Solver {
             double tnew = time;
             run_steps(1, &done, &tnew);
             if (done == 0) break;
+            if (mood() && h_groups[0].troubled > 0)  // ZoomyAmr.cpp:712-713
+                std::cout << "  [MOOD] lev 0 troubled cells: " << (long)h_groups[0].troubled
+                          << " -> whole-step order-1 redo\n";
             const double dt0 = h_groups[0].dt;
             time = tnew;
             step++;

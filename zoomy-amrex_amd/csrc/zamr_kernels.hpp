@@ -41,7 +41,7 @@ struct Params {
 // One box (patch) on the device: AMReX Array4 layout, i fastest, then j, then
 // component; base pointers address the grown-box corner (lo - ng).
 struct BoxDesc {
-    double* q[2];    // two state buffers
+    double* q[3];    // state buffers (the third only with a-posteriori MOOD)
     double* aux;     // stored Qaux (na comps, same layout)
     int lo[2], hi[2];
     int gl[2];       // grown lower corner
@@ -60,7 +60,7 @@ struct WorkItem {
 // The descriptor as kernels hold it: scalars only, so no member is ever
 // indexed at run time (which would demote the whole struct to scratch).
 struct BoxView {
-    double *q0, *q1, *aux;
+    double *q0, *q1, *q2, *aux;
     int lo0, lo1, hi0, hi1, gl0, gl1, sy;
     long long sc;
     int dlo0, dlo1, dhi0, dhi1;
@@ -73,6 +73,7 @@ __device__ __forceinline__ BoxView view(const BoxDesc* __restrict__ boxes, int b
     BoxView v;
     v.q0 = d.q[0];
     v.q1 = d.q[1];
+    v.q2 = d.q[2];
     v.aux = d.aux;
     v.lo0 = d.lo[0];
     v.lo1 = d.lo[1];
@@ -103,13 +104,17 @@ __device__ __forceinline__ BoxView view(const BoxDesc* __restrict__ boxes, int b
 struct GroupState {
     double dt, time, time_end, cfl, dtmin, dtmax, divisor;
     unsigned long long maxinv_bits;  // max over cells of |lambda_d| / dx_d (bit pattern, >= 0)
-    int step, max_step, active, pad;
+    int step, max_step, active;
+    int troubled;                    // MOOD: troubled cells of this step's order-2 candidate
+    long long redos;                 // MOOD: steps redone at order 1
 };
 
 struct StageArgs {
     Params p;
     int src, dst, old;           // buffer indices: read, write, Heun's Q^n
     int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
+    int mood;                    // 0; 1 = count the troubled cells of the candidate
+                                 // (stage 2); 2 = order-1 redo, troubled groups only
     double* pre;                 // optional: the pre-implicit state Q + dt RHS of a
                                  // single box (box layout), kept for the adjoint
     double* dstp;                // optional: output buffer of a single box in place
@@ -435,7 +440,7 @@ struct Sys {
     }
 };
 
-__device__ __forceinline__ double* qsel(const BoxView& B, int k) { return k ? B.q1 : B.q0; }
+__device__ __forceinline__ double* qsel(const BoxView& B, int k) { return k == 0 ? B.q0 : (k == 1 ? B.q1 : B.q2); }
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -527,8 +532,8 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             // state across so that every group's current buffer stays in step.
             if constexpr (EPI == 2) {
                 if (own) {
-                    const double* __restrict__ qs = a.src ? B.q1 : B.q0;
-                    double* __restrict__ qd = a.dst ? B.q1 : B.q0;
+                    const double* __restrict__ qs = qsel(B, a.src);
+                    double* __restrict__ qd = qsel(B, a.dst);
                     for (int j = w.j0; j < w.j0 + w.nj; ++j) {
                         const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
 #pragma unroll
@@ -538,14 +543,15 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             }
             continue;
         }
+        if (a.mood == 2 && G->troubled == 0) continue;  // the candidate stands
         const double dt = G->dt;
         const double time = G->time;
         const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;  // uniform per item
         const double hgx = 0.5 * p(0, 0) * rdx, hgy = 0.5 * p(0, 0) * rdy;
         // select, never index: a runtime index would push the descriptor to scratch
-        const double* __restrict__ qs = a.src ? B.q1 : B.q0;
-        double* __restrict__ qd = a.dstp ? a.dstp : (a.dst ? B.q1 : B.q0);
-        const double* __restrict__ qo = a.old ? B.q1 : B.q0;
+        const double* __restrict__ qs = qsel(B, a.src);
+        double* __restrict__ qd = a.dstp ? a.dstp : qsel(B, a.dst);
+        const double* __restrict__ qo = qsel(B, a.old);
         unsigned char *swx = nullptr, *swy = nullptr;
         if constexpr (SW != 0) {
             swx = reinterpret_cast<unsigned char*>(a.sw);
@@ -611,6 +617,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         typename S::Raw nxt;
         if (col_in) nxt = S::fetch(B, qs, c, jlead0);
         double lam = 0.0;
+        int nbad = 0;  // MOOD: troubled cells among this lane's candidate cells
         for (int j = jstart; j < jend; ++j) {
             const int jl = (dim == 2) ? j + HX : j;
             VQ lead;
@@ -731,6 +738,9 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 if constexpr (EPI == 1) {
 #pragma unroll
                     for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qo[idx + n * B.sc] + 0.5 * qn(n, 0);
+                    // troubled: h < 0, NaN (fails h >= 0) or +-inf (fails |h| < 1e300)
+                    // (ZoomyAmr.cpp:700-711), on the candidate before UpdateState
+                    if (a.mood == 1) nbad += (!(qn(1, 0) >= 0.0) || !(qn(1, 0) < 1.0e300)) ? 1 : 0;
                 }
                 if constexpr (EPI != 0) {
                     qn = S::update_state(qn, p, a.clamp);
@@ -764,6 +774,13 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         if constexpr (EPI != 0) {
             lam = wave_max(lam);
             if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
+        }
+        if constexpr (EPI == 1) {
+            if (a.mood == 1) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) nbad += __shfl_xor(nbad, off, 64);
+                if (t == 0 && nbad > 0) atomicAdd(&G->troubled, nbad);
+            }
         }
     }
 }
@@ -826,6 +843,7 @@ __global__ void begin_step_kernel(GroupState* gs, int ngroups) {
     G.active = (G.time < G.time_end) && (G.max_step < 0 || G.step < G.max_step);
     const double m = __longlong_as_double((long long)G.maxinv_bits);
     G.maxinv_bits = 0ull;
+    G.troubled = 0;
     if (!G.active) return;  // dt keeps the group's last step
     double dt;
     if (m < 1e-14) {
@@ -852,6 +870,18 @@ __global__ void end_step_kernel(GroupState* gs, int ngroups, double* hist, int h
     if (hist && g == 0 && s < hist_cap) {
         hist[2 * s] = G.dt;
         hist[2 * s + 1] = G.time;
+    }
+}
+
+// MOOD (ZoomyAmr.cpp:699-720): a group whose order-2 candidate has troubled
+// cells redoes the step at order 1; its CFL max is recomputed by the redo.
+__global__ void mood_reset_kernel(GroupState* gs, int ngroups) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    GroupState& G = gs[g];
+    if (G.active && G.troubled > 0) {
+        G.maxinv_bits = 0ull;
+        G.redos += 1;
     }
 }
 

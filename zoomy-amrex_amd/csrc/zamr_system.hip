@@ -319,6 +319,16 @@ int zamr_kernel_times(zamr_ctx* c, double* out, int* n) {
     return ZAMR_OK;
 }
 
+int zamr_mood_stats(zamr_ctx* c, int group, int* troubled, long long* redos) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        if (group < 0 || group >= c->s.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
+        c->s.download_groups();
+        if (troubled) *troubled = c->s.h_groups[group].troubled;
+        if (redos) *redos = c->s.h_groups[group].redos;
+    });
+}
+
 int zamr_dt_history(zamr_ctx* c, int cap, double* dts, double* times, int* n) {
     if (!c || !c->s.d_pool || cap < 0) return ZAMR_EINVAL;
     return guard(c, [&] {

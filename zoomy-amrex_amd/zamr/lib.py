@@ -71,6 +71,7 @@ def load_library(path):
         "zamr_set_profiling": [_VP, ctypes.c_int],
         "zamr_kernel_times": [_VP, _D, _I],
         "zamr_dt_history": [_VP, ctypes.c_int, _D, _D, _I],
+        "zamr_mood_stats": [_VP, ctypes.c_int, _I, ctypes.POINTER(ctypes.c_longlong)],
         "zamr_adj_setup": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I, _I],
         "zamr_adj_forward": [_VP, _D],
         "zamr_adj_backward": [_VP, _D],
@@ -230,6 +231,12 @@ class ZoomyAmr:
         self._check(self.lib.zamr_kernel_times(self.ctx, _dp(out), ctypes.byref(n)))
         return out, n.value
 
+    def mood_stats(self, group=0):
+        """(troubled cells of the last order-2 candidate, steps redone at order 1)."""
+        t, r = ctypes.c_int(), ctypes.c_longlong()
+        self._check(self.lib.zamr_mood_stats(self.ctx, group, ctypes.byref(t), ctypes.byref(r)))
+        return t.value, r.value
+
     def dt_history(self, cap=1 << 16):
         """(dts, times) of the steps taken on the device clock since the last reset."""
         d, t, n = np.zeros(cap), np.zeros(cap), ctypes.c_int()
@@ -279,6 +286,8 @@ def solver_deck(keys):
     for k in ("implicit_source", "well_balanced", "clamp_positivity"):
         if k in keys:
             lines.append(f"solver.{k} = {'true' if keys[k] else 'false'}")
+    if "positivity" in keys:
+        lines.append(f"solver.positivity = {keys['positivity']}")
     for k, v in (keys.get("params") or {}).items():
         lines.append(f"params.{k} = {v!r}")
     return "\n".join(lines) + "\n"

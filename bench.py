@@ -152,6 +152,42 @@ def bench_adjoint(reps=3):
             "timing": f"best of {reps} after one warm-up round"}
 
 
+def bench_sme(steps=50, warmup=3):
+    """The level-2 shallow-moment workload at 2048^2 (8 state rows: b, h, three
+    moments of each velocity component), slip + Newtonian friction through the
+    8x8 point-implicit solve, order 2, no well-balancing: cell-updates/s per RK
+    stage and the stage-2 kernel's HBM fraction (algorithmic bytes 16 n_q and
+    24 n_q per cell for stages 1 and 2, n_q = 8)."""
+    import numpy as np
+    import torch
+    from zamr import cases
+    from zamr.deck import deck_from_case
+    from zamr.lib import ZoomyAmr
+
+    n = 2048
+    case, Q0 = cases.sme_bumps(n=n, seed=0, steps=warmup + steps)
+    z = ZoomyAmr(deck_from_case(case), system="sme2d", dim=2, order=2)
+    z.init_data()
+    z.set_state(Q0)
+    z.run_steps(warmup)
+    torch.cuda.synchronize()
+    z.set_profiling(True)
+    t0 = time.perf_counter()
+    done, _ = z.run_steps(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kt, nk = z.kernel_times()
+    assert done == steps and np.isfinite(z.get_state()).all()
+    cells = n * n
+    t1, t2 = kt[0] * 1e-3, kt[1] * 1e-3
+    return {"workload": f"level-2 shallow moments {n}x{n}, 16 seeded bumps, moments U[-0.05,0.05] h, slip + "
+                        "Newtonian friction point-implicit (8x8), order-2 path-conservative Rusanov, SSP-RK2",
+            "value": 2 * steps * cells / el, "unit": "cell-updates/s", "ms_per_step": 1e3 * el / steps,
+            "steps": steps, "stage1_avg_launch_ms": kt[0], "stage2_avg_launch_ms": kt[1],
+            "stage2_frac": (24 * 8 * cells / t2) / HBM_PEAK if t2 > 0 else 0.0,
+            "stage1_frac": (16 * 8 * cells / t1) / HBM_PEAK if t1 > 0 else 0.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,11 +196,12 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint"],
+    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme"],
                     help="swe4096: the single-level 4096^2 workload (default); boxlist: 600 AMR patches + "
                          "64 ensemble members, batched launch vs one launch per box; adjoint: the 1024^2 "
                          "gauge-inversion gradient alone")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg of the default workload")
+    ap.add_argument("--no-sme", action="store_true", help="skip the shallow-moment leg of the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the run, copy a 1 GiB buffer with a plain device copy (counter calibration)")
     args = ap.parse_args()
@@ -181,6 +218,10 @@ def main():
 
     if args.workload == "boxlist":
         return bench_boxlist(args)
+    if args.workload == "sme":
+        print(json.dumps({"metric": "cell-updates/s per RK stage (FP64) + % HBM roofline", "n_gpus": 1,
+                          "dtype": "f64", "data": "synthetic", "higher_is_better": True, **bench_sme()}), flush=True)
+        return
     if args.workload == "adjoint":
         print(json.dumps({"metric": "adjoint/forward time ratio", "unit": "x", "higher_is_better": False,
                           "n_gpus": 1, "dtype": "f64", "data": "synthetic", **bench_adjoint()}), flush=True)
@@ -253,6 +294,8 @@ def main():
     del z
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint()
+    if not args.no_sme:
+        out["sme"] = bench_sme()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(case, Q0, threads)

@@ -207,3 +207,26 @@ def initial_state(b, eta0):
     """[b, max(0, eta0 - b), 0, 0]."""
     z = np.zeros_like(b)
     return np.stack([b, np.maximum(0.0, eta0 - b), z, z])
+
+
+def sme_bumps(n=2048, seed=0, steps=50, nu=0.01, lambda_s=0.1, max_grid_size=None, bc=("", "", "", "")):
+    """Level-2 shallow-moment workload: bathymetry = 16 seeded Gaussian bumps
+    (A ~ U[0, 0.5], sigma ~ U[0.05, 0.15]), h0 = 1 - b (all wet: b < 0.87 for
+    seed 0), every velocity moment hu_k, hv_k (k = 0, 1, 2) = U[-0.05, 0.05] h0
+    per cell, slip (lambda_s) + Newtonian (nu) friction through the
+    point-implicit source, order-2 MUSCL / path-conservative Rusanov, no
+    well-balancing (its row scaling is written for the shallow-water layout),
+    `steps` SSP-RK2 steps.  State rows [b, h, hu0, hu1, hu2, hv0, hv1, hv2]."""
+    rng = SplitMix64(seed)
+    x, y = cell_centers(n, n)
+    b = gaussian_bumps(x, y, rng, 16, amp=(0.0, 0.5), sigma=(0.05, 0.15))
+    h = 1.0 - b
+    Q = np.zeros((8, n, n))
+    Q[0], Q[1] = b, h
+    for k in (2, 3, 4, 5, 6, 7):
+        Q[k] = rng.uniform(n * n, -0.05, 0.05).reshape(n, n) * h
+    case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=max_grid_size or n,
+                spatial_order=2, implicit_source=True, well_balanced=False, clamp_positivity=True, cfl=0.45,
+                dtmin=1e-9, dtmax=1.0, time_end=1.0e30, max_step=steps, bc=tuple(bc),
+                params={"nu": nu, "lambda_s": lambda_s})
+    return case, Q

@@ -27,7 +27,7 @@ ARCH = os.environ.get("ZAMR_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # Shipped variants: (system, mesh dimension, spatial order).
-VARIANTS = [("swe1d", 1, 1), ("swe2d", 2, 1), ("swe2d", 2, 2), ("swe2d_nf", 2, 2)]
+VARIANTS = [("swe1d", 1, 1), ("swe2d", 2, 1), ("swe2d", 2, 2), ("swe2d_nf", 2, 2), ("sme2d", 2, 2)]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -107,6 +107,7 @@ def build_all(jobs=4, with_nofma=True):
     if with_nofma:
         tasks.append(("swe2d", [(2, 2), (2, 1)], False))
         tasks.append(("swe2d_nf", [(2, 2)], False))
+        tasks.append(("sme2d", [(2, 2)], False))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(compile_system, SYSTEMS / s, s, v, c) for s, v, c in tasks]
         return [p for f in futs for p in f.result()]

@@ -147,6 +147,11 @@ __device__ __forceinline__ double recip(double x) {
 template <class T>
 __device__ __forceinline__ T dev_max(T a, T b) { return (a < b) ? b : a; }
 
+// True only for a value the compiler knows to be the constant 0.0 (a literal
+// zero entry of a model function's result, seen through inlining); false for
+// every value computed at run time.
+__device__ __forceinline__ bool zero_literal(double v) { return __builtin_constant_p(v) && v == 0.0; }
+
 template <class V>
 __device__ __forceinline__ void vq_zero(V& v) {
 #pragma unroll
@@ -349,17 +354,22 @@ struct Sys {
         double rhs[nq];
 #pragma unroll
         for (int r = 0; r < nq; ++r) {
-            rhs[r] = dt * S(r, 0);
+            rhs[r] = zero_literal(S(r, 0)) ? 0.0 : dt * S(r, 0);
 #pragma unroll
             for (int c = 0; c < nq; ++c) {
-                // The chain-rule term is summed first and added once.  The
-                // printer emits dA/dQ == 0 (to_amrex.py:478-480), so the sum is
-                // +-0 or NaN and the entry equals make_rhs.H's running sum bit
-                // for bit, while the compiler can share the sum across c.
+                // The chain-rule term is summed first and added once.  Entries
+                // the model returns as literal zeros (the printer emits dA/dQ
+                // == 0, to_amrex.py:478-480, and sparse Jacobians) are known
+                // to the compiler after inlining and skipped: for finite
+                // values every entry and the elimination below equal
+                // make_rhs.H's arithmetic bit for bit, and the identity rows
+                // of I - dt J fold away.
                 double chain = 0.0;
 #pragma unroll
-                for (int k = 0; k < na; ++k) chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
-                A[r][c] = (r == c ? 1.0 : 0.0) - dt * (dS_dQ(r * nq + c, 0) + chain);
+                for (int k = 0; k < na; ++k)
+                    if (!zero_literal(dA_dQ(k * nq + c, 0))) chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
+                const double jc = dS_dQ(r * nq + c, 0) + chain;
+                A[r][c] = zero_literal(jc) ? (r == c ? 1.0 : 0.0) : (r == c ? 1.0 : 0.0) - dt * jc;
             }
         }
         bool ok = true;

@@ -626,6 +626,15 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const int jlead0 = (dim == 2) ? jstart + HX : jstart;
         typename S::Raw nxt;
         if (col_in) nxt = S::fetch(B, qs, c, jlead0);
+        // stage 2: Q^n of the row for the Heun combine, also fetched one row ahead
+        VQ qn_next;
+        if constexpr (EPI == 1) {
+            if (own) {
+                const long long i0 = (long long)(c - B.gl0) + (long long)(jstart - B.gl1) * B.sy;
+#pragma unroll
+                for (int n = 0; n < nq; ++n) qn_next(n, 0) = qo[i0 + n * B.sc];
+            }
+        }
         double lam = 0.0;
         int nbad = 0;  // MOOD: troubled cells among this lane's candidate cells
         for (int j = jstart; j < jend; ++j) {
@@ -634,6 +643,15 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             if (col_in) lead = S::finish(B, nxt, c, jl, p, a.clamp, time);
             else vq_zero(lead);
             if (col_in && j + 1 < jend) nxt = S::fetch(B, qs, c, jl + 1);  // one row ahead
+            VQ qn_row;
+            if constexpr (EPI == 1) {
+                qn_row = qn_next;
+                if (own && j + 1 < jend) {
+                    const long long i1 = (long long)(c - B.gl0) + (long long)(j + 1 - B.gl1) * B.sy;
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) qn_next(n, 0) = qo[i1 + n * B.sc];
+                }
+            }
             if constexpr (dim == 1) W0 = lead;
             const VQ& qc = W0;
             const long long isw = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
@@ -747,7 +765,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 if (a.impl) qn = S::implicit_source(qn, ac, dt, p, time, X);
                 if constexpr (EPI == 1) {
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qo[idx + n * B.sc] + 0.5 * qn(n, 0);
+                    for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qn_row(n, 0) + 0.5 * qn(n, 0);
                     // troubled: h < 0, NaN (fails h >= 0) or +-inf (fails |h| < 1e300)
                     // (ZoomyAmr.cpp:700-711), on the candidate before UpdateState
                     if (a.mood == 1) nbad += (!(qn(1, 0) >= 0.0) || !(qn(1, 0) < 1.0e300)) ? 1 : 0;

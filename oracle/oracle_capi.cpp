@@ -10,11 +10,43 @@
 //       [14] c_cfl_dimension [15] c_cfl_degree_factor [16] OpenMP threads
 // dset: [0] prob_lo x [1] prob_lo y [2] prob_hi x [3] prob_hi y [4] cfl
 //       [5] dtmin [6] dtmax [7] time_end
+//
+// AMR (orc_amr_*): ai: [0] max_level [1] ref_ratio [2] blocking_factor
+//       [3] n_error_buf [4] fine max_grid_size; ad: [0] grid_eff
+//       [1] tagging.threshold [2] tagging.b_max.
 #include "Model.H"
 #include "Numerics.H"
 #include "zamr_oracle.hpp"
+#include "zamr_oracle_amr.hpp"
 
 using OSolver = zorc::Solver<Model, Numerics>;
+using OAmr = zorc::Amr<Model, Numerics>;
+
+static zorc::Settings settings_from(const int* is, const double* ds) {
+    zorc::Settings s;
+    s.nx = is[0];
+    s.ny = is[1];
+    s.max_grid_size = is[2];
+    s.spatial_order = is[3];
+    s.implicit_source = is[4] != 0;
+    s.well_balanced = is[5] != 0;
+    s.clamp_positivity = is[6] != 0;
+    s.mood = is[7] != 0;
+    s.face_based_rhs = is[8] != 0;
+    s.max_step = is[9];
+    for (int k = 0; k < 4; ++k) s.bc_idx[k] = is[10 + k];
+    s.cfl_dimension = is[14];
+    s.cfl_degree_factor = is[15];
+    s.plo[0] = ds[0];
+    s.plo[1] = ds[1];
+    s.phi[0] = ds[2];
+    s.phi[1] = ds[3];
+    s.cfl = ds[4];
+    s.dtmin = ds[5];
+    s.dtmax = ds[6];
+    s.time_end = ds[7];
+    return s;
+}
 
 extern "C" {
 
@@ -41,28 +73,7 @@ void orc_default_params(double* out) {
 }
 
 void* orc_create(const int* is, const double* ds, const double* params) {
-    zorc::Settings s;
-    s.nx = is[0];
-    s.ny = is[1];
-    s.max_grid_size = is[2];
-    s.spatial_order = is[3];
-    s.implicit_source = is[4] != 0;
-    s.well_balanced = is[5] != 0;
-    s.clamp_positivity = is[6] != 0;
-    s.mood = is[7] != 0;
-    s.face_based_rhs = is[8] != 0;
-    s.max_step = is[9];
-    for (int k = 0; k < 4; ++k) s.bc_idx[k] = is[10 + k];
-    s.cfl_dimension = is[14];
-    s.cfl_degree_factor = is[15];
-    s.plo[0] = ds[0];
-    s.plo[1] = ds[1];
-    s.phi[0] = ds[2];
-    s.phi[1] = ds[3];
-    s.cfl = ds[4];
-    s.dtmin = ds[5];
-    s.dtmax = ds[6];
-    s.time_end = ds[7];
+    const zorc::Settings s = settings_from(is, ds);
     auto* o = new OSolver(s, params);
     o->nthreads = is[16] > 0 ? is[16] : 1;
     return o;
@@ -121,5 +132,123 @@ int orc_ge_solve(int n, const double* A, const double* rhs, double* x) {
     if (n == 4) return run(std::integral_constant<int, 4>{});
     if (n == 8) return run(std::integral_constant<int, 8>{});
     return -1;
+}
+
+// ---------------- AMR ----------------
+void* orc_amr_create(const int* is, const double* ds, const double* params, const int* ai, const double* ad) {
+    zorc::AmrParams a;
+    a.max_level = ai[0];
+    a.ref_ratio = ai[1];
+    a.blocking_factor = ai[2];
+    a.n_error_buf = ai[3];
+    a.fine_max_grid_size = ai[4];
+    a.grid_eff = ad[0];
+    a.tag_threshold = ad[1];
+    a.tag_b_max = ad[2];
+    auto* o = new OAmr(settings_from(is, ds), params, a);
+    o->nthreads = is[16] > 0 ? is[16] : 1;
+    o->L0->nthreads = o->nthreads;
+    return o;
+}
+void orc_amr_destroy(void* h) { delete static_cast<OAmr*>(h); }
+void orc_amr_init_solution(void* h) {
+    auto* o = static_cast<OAmr*>(h);
+    o->L0->init_solution(o->L0->Q);
+    o->L0->init_solution(o->L0->Qaux);
+}
+void orc_amr_init_levels(void* h) { static_cast<OAmr*>(h)->init_levels(); }
+int orc_amr_finest(void* h) { return static_cast<OAmr*>(h)->finest_level(); }
+// Boxes of level lev as (lo0, lo1, hi0, hi1) quadruples; returns the count.
+int orc_amr_boxes(void* h, int lev, int* out, int cap) {
+    auto* o = static_cast<OAmr*>(h);
+    if (lev > o->finest_level()) return 0;
+    auto& bx = o->level(lev).boxes;
+    for (int k = 0; k < (int)bx.size() && k < cap; ++k) {
+        out[4 * k + 0] = bx[k].lo[0];
+        out[4 * k + 1] = bx[k].lo[1];
+        out[4 * k + 2] = bx[k].hi[0];
+        out[4 * k + 3] = bx[k].hi[1];
+    }
+    return (int)bx.size();
+}
+// Dense (ncomp, ny_lev, nx_lev) copy of a level's valid cells (field 0 = Q,
+// 1 = Qaux); cells no box covers are left as the caller filled them.
+static void amr_copy(OAmr* o, int lev, int field, double* a, bool to_level) {
+    auto& L = o->level(lev);
+    auto& mf = field == 0 ? L.Q : L.Qaux;
+    const int nc = field == 0 ? OSolver::nq : OSolver::na;
+    const long nx = L.s.nx, ny = L.s.ny;
+    for (auto& f : mf.fabs)
+        for (int n = 0; n < nc; ++n)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i) {
+                    double& g = a[((long)n * ny + j) * nx + i];
+                    if (to_level) f(i, j, n) = g;
+                    else g = f(i, j, n);
+                }
+}
+void orc_amr_get(void* h, int lev, int field, double* a) { amr_copy(static_cast<OAmr*>(h), lev, field, a, false); }
+void orc_amr_set(void* h, int lev, int field, const double* a) {
+    amr_copy(static_cast<OAmr*>(h), lev, field, const_cast<double*>(a), true);
+}
+void orc_amr_time_step(void* h, double time, int iteration, double* dt) {
+    auto* o = static_cast<OAmr*>(h);
+    o->time_step(0, time, iteration);
+    if (dt) *dt = o->dt_level[0];
+}
+double orc_amr_total_mass(void* h) { return static_cast<OAmr*>(h)->total_mass(); }
+void orc_amr_regrid(void* h, double time) { static_cast<OAmr*>(h)->regrid(time); }
+void orc_amr_average_down(void* h) { static_cast<OAmr*>(h)->average_down(); }
+long orc_amr_counts(void* h, int which) {
+    auto* o = static_cast<OAmr*>(h);
+    if (which == 0) return o->regrids;
+    if (which == 1) return o->remakes;
+    return o->L1 ? o->L1->mood_redos : 0;
+}
+// ErrorEst of level 0 as it stands, one byte per cell (raster order).
+void orc_amr_error_est(void* h, unsigned char* tags) {
+    auto t = static_cast<OAmr*>(h)->error_est();
+    std::memcpy(tags, t.data(), t.size());
+}
+// Level-1 boxes MakeNewGrids derives from a level-0 tag map.
+int orc_amr_make_grids(void* h, const unsigned char* tags, int* out, int cap) {
+    auto* o = static_cast<OAmr*>(h);
+    std::vector<unsigned char> t(tags, tags + (size_t)o->s0.nx * o->s0.ny);
+    auto bx = o->make_grids(t);
+    for (int k = 0; k < (int)bx.size() && k < cap; ++k) {
+        out[4 * k + 0] = bx[k].lo[0];
+        out[4 * k + 1] = bx[k].lo[1];
+        out[4 * k + 2] = bx[k].hi[0];
+        out[4 * k + 3] = bx[k].hi[1];
+    }
+    return (int)bx.size();
+}
+// cell_cons_interp of level 0 at level-1 cell (i, j).
+void orc_amr_interp(void* h, int i, int j, double* out) { static_cast<OAmr*>(h)->interp(i, j, out); }
+// Berger-Rigoutsos clusters of n points (x, y pairs) at efficiency eff.
+int orc_cluster(const int* pts, int n, double eff, int* out, int cap) {
+    std::vector<zorc::IV> v(n);
+    for (int k = 0; k < n; ++k) v[k] = zorc::IV{pts[2 * k], pts[2 * k + 1]};
+    auto bx = zorc::cluster_tags(v, eff);
+    for (int k = 0; k < (int)bx.size() && k < cap; ++k) {
+        out[4 * k + 0] = bx[k].lo[0];
+        out[4 * k + 1] = bx[k].lo[1];
+        out[4 * k + 2] = bx[k].hi[0];
+        out[4 * k + 3] = bx[k].hi[1];
+    }
+    return (int)bx.size();
+}
+// Runs the Evolve loop over the hierarchy; per-step dt, mass and level count.
+int orc_amr_evolve(void* h, double* time_out, double* dts, double* masses, int* nlev, int cap, int step_limit) {
+    auto* o = static_cast<OAmr*>(h);
+    std::vector<double> d, m;
+    std::vector<int> l;
+    int n = o->evolve(time_out, &d, &m, &l, step_limit);
+    for (int k = 0; k < n && k < cap; ++k) {
+        if (dts) dts[k] = d[k];
+        if (masses) masses[k] = m[k];
+        if (nlev) nlev[k] = l[k];
+    }
+    return n;
 }
 }

@@ -67,6 +67,39 @@ def load(system):
     lib.orc_evolve.argtypes = [ctypes.c_void_p, _D, _D, _D, ctypes.c_int, ctypes.c_int]
     lib.orc_ge_solve.restype = ctypes.c_int
     lib.orc_ge_solve.argtypes = [ctypes.c_int, _D, _D, _D]
+    _U8 = ctypes.POINTER(ctypes.c_ubyte)
+    V = ctypes.c_void_p
+    lib.orc_amr_create.restype = V
+    lib.orc_amr_create.argtypes = [_I, _D, _D, _I, _D]
+    for name in ("orc_amr_destroy", "orc_amr_init_solution", "orc_amr_init_levels", "orc_amr_average_down"):
+        getattr(lib, name).restype = None
+        getattr(lib, name).argtypes = [V]
+    lib.orc_amr_finest.restype = ctypes.c_int
+    lib.orc_amr_finest.argtypes = [V]
+    lib.orc_amr_boxes.restype = ctypes.c_int
+    lib.orc_amr_boxes.argtypes = [V, ctypes.c_int, _I, ctypes.c_int]
+    lib.orc_amr_get.restype = None
+    lib.orc_amr_get.argtypes = [V, ctypes.c_int, ctypes.c_int, _D]
+    lib.orc_amr_set.restype = None
+    lib.orc_amr_set.argtypes = [V, ctypes.c_int, ctypes.c_int, _D]
+    lib.orc_amr_time_step.restype = None
+    lib.orc_amr_time_step.argtypes = [V, ctypes.c_double, ctypes.c_int, _D]
+    lib.orc_amr_total_mass.restype = ctypes.c_double
+    lib.orc_amr_total_mass.argtypes = [V]
+    lib.orc_amr_regrid.restype = None
+    lib.orc_amr_regrid.argtypes = [V, ctypes.c_double]
+    lib.orc_amr_counts.restype = ctypes.c_long
+    lib.orc_amr_counts.argtypes = [V, ctypes.c_int]
+    lib.orc_amr_error_est.restype = None
+    lib.orc_amr_error_est.argtypes = [V, _U8]
+    lib.orc_amr_make_grids.restype = ctypes.c_int
+    lib.orc_amr_make_grids.argtypes = [V, _U8, _I, ctypes.c_int]
+    lib.orc_amr_interp.restype = None
+    lib.orc_amr_interp.argtypes = [V, ctypes.c_int, ctypes.c_int, _D]
+    lib.orc_cluster.restype = ctypes.c_int
+    lib.orc_cluster.argtypes = [_I, ctypes.c_int, ctypes.c_double, _I, ctypes.c_int]
+    lib.orc_amr_evolve.restype = ctypes.c_int
+    lib.orc_amr_evolve.argtypes = [V, _D, _D, _D, _I, ctypes.c_int, ctypes.c_int]
     _libs[system] = lib
     return lib
 
@@ -78,7 +111,7 @@ def _dp(a):
 class Oracle:
     """One single-level CPU run of `system` ("swe1d", "swe2d", "sme2")."""
 
-    def __init__(self, system, case, threads=1):
+    def __init__(self, system, case, threads=1, amr=None):
         self.lib = lib = load(system)
         c = dict(DEFAULTS)
         c.update(case)
@@ -113,8 +146,18 @@ class Oracle:
                          *bcs, cfl_dim, cfl_deg, threads], dtype=np.int32)
         dset = np.array([*c["prob_lo"], *c["prob_hi"], c["cfl"], c["dtmin"], c["dtmax"], c["time_end"]],
                         dtype=np.float64)
-        self.h = lib.orc_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p))
         self.nx, self.ny = int(c["nx"]), int(c["ny"])
+        self.amr = amr
+        if amr is None:
+            self.h = lib.orc_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p))
+        else:
+            a = dict(AMR_DEFAULTS)
+            a.update(amr)
+            self.amr = a
+            ai = np.array([a["max_level"], a["ref_ratio"], a["blocking_factor"], a["n_error_buf"],
+                           a["fine_max_grid_size"]], dtype=np.int32)
+            ad = np.array([a["grid_eff"], a["threshold"], a["b_max"]], dtype=np.float64)
+            self.h = lib.orc_amr_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p), ai.ctypes.data_as(_I), _dp(ad))
 
     _NAMES = ("g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s")
 
@@ -124,7 +167,7 @@ class Oracle:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.orc_destroy(self.h)
+            (self.lib.orc_destroy if self.amr is None else self.lib.orc_amr_destroy)(self.h)
             self.h = None
 
     __del__ = close
@@ -186,6 +229,105 @@ class Oracle:
         ms = np.zeros(cap)
         n = self.lib.orc_evolve(self.h, _dp(t), _dp(dts), _dp(ms), cap, step_limit)
         return n, float(t[0]), dts[:min(n, cap)].copy(), ms[:min(n, cap)].copy()
+
+
+AMR_DEFAULTS = dict(max_level=1, ref_ratio=2, blocking_factor=2, n_error_buf=1, fine_max_grid_size=64,
+                    grid_eff=0.7, threshold=0.01, b_max=1.0e30)
+
+
+class AmrOracle(Oracle):
+    """A two-level run (levels 0 and 1) of the AMR TimeStep; amr keys:
+    max_level, ref_ratio, blocking_factor, n_error_buf, fine_max_grid_size,
+    grid_eff, threshold (tagging.threshold), b_max (tagging.b_max)."""
+
+    def __init__(self, system, case, threads=1, amr=None):
+        super().__init__(system, case, threads=threads, amr=dict(amr or {}))
+
+    def init_solution(self):
+        self.lib.orc_amr_init_solution(self.h)
+
+    def init_levels(self):
+        self.lib.orc_amr_init_levels(self.h)
+
+    def finest_level(self):
+        return self.lib.orc_amr_finest(self.h)
+
+    def level_shape(self, lev):
+        r = self.amr["ref_ratio"] ** lev
+        return self.ny * r, self.nx * r
+
+    def boxes(self, lev):
+        out = np.zeros(4 * 4096, dtype=np.int32)
+        n = self.lib.orc_amr_boxes(self.h, lev, out.ctypes.data_as(_I), 4096)
+        return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(min(n, 4096))]
+
+    def _get(self, lev, field, fill):
+        nc = self.nq if field == 0 else self.na
+        a = np.full((nc, *self.level_shape(lev)), fill)
+        self.lib.orc_amr_get(self.h, lev, field, _dp(a))
+        return a
+
+    def get_state(self, lev=0, fill=np.nan):
+        return self._get(lev, 0, fill)
+
+    def get_aux(self, lev=0, fill=np.nan):
+        return self._get(lev, 1, fill)
+
+    def set_state(self, Q, lev=0):
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(self.nq, *self.level_shape(lev))
+        self.lib.orc_amr_set(self.h, lev, 0, _dp(Q))
+
+    def time_step(self, time, iteration):
+        dt = np.zeros(1)
+        self.lib.orc_amr_time_step(self.h, time, iteration, _dp(dt))
+        return float(dt[0])
+
+    def total_mass(self):
+        return self.lib.orc_amr_total_mass(self.h)
+
+    def regrid(self, time=0.0):
+        self.lib.orc_amr_regrid(self.h, time)
+
+    def average_down(self):
+        self.lib.orc_amr_average_down(self.h)
+
+    def counts(self):
+        """(regrids, remakes, level-1 MOOD redos)."""
+        return tuple(self.lib.orc_amr_counts(self.h, k) for k in range(3))
+
+    def error_est(self):
+        t = np.zeros((self.ny, self.nx), dtype=np.uint8)
+        self.lib.orc_amr_error_est(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return t
+
+    def make_grids(self, tags):
+        t = np.ascontiguousarray(tags, dtype=np.uint8)
+        out = np.zeros(4 * 65536, dtype=np.int32)
+        n = self.lib.orc_amr_make_grids(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                        out.ctypes.data_as(_I), 65536)
+        return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n)]
+
+    def interp(self, i, j):
+        v = np.zeros(self.nq)
+        self.lib.orc_amr_interp(self.h, i, j, _dp(v))
+        return v
+
+    def evolve(self, step_limit=-1, cap=100000):
+        t = np.zeros(1)
+        dts = np.zeros(cap)
+        ms = np.zeros(cap)
+        nl = np.zeros(cap, dtype=np.int32)
+        n = self.lib.orc_amr_evolve(self.h, _dp(t), _dp(dts), _dp(ms), nl.ctypes.data_as(_I), cap, step_limit)
+        k = min(n, cap)
+        return n, float(t[0]), dts[:k].copy(), ms[:k].copy(), nl[:k].copy()
+
+
+def cluster(points, eff=0.7, system="swe2d"):
+    """Berger-Rigoutsos boxes (lo0, lo1, hi0, hi1) of integer points (x, y)."""
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+    out = np.zeros(4 * (len(pts) + 1), dtype=np.int32)
+    n = load(system).orc_cluster(pts.ctypes.data_as(_I), len(pts), eff, out.ctypes.data_as(_I), len(pts) + 1)
+    return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n)]
 
 
 def ge_solve(A, rhs):

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -169,12 +170,20 @@ struct Solver {
     int nthreads = 1;
     long mood_redos = 0;
 
-    explicit Solver(const Settings& st, const Real* params) : s(st) {
+    // Coarse-fine ghost source of a refined level (FillPatchTwoLevels,
+    // ZoomyAmr.cpp:266-274): writes the nq components at ghost cell (i, j)
+    // of a region inside the domain that no box of this level covers.
+    std::function<void(int, int, Real*)> coarse_fill;
+
+    explicit Solver(const Settings& st, const Real* params) : Solver(st, params, chop_domain(st.nx, st.ny, st.max_grid_size)) {}
+
+    // A level with the given boxes over the domain (0, 0) .. (nx-1, ny-1).
+    Solver(const Settings& st, const Real* params, const std::vector<Box>& bxs) : s(st) {
         for (int i = 0; i < np; ++i) p(i, 0) = params[i];
         dx[0] = (s.phi[0] - s.plo[0]) / s.nx;
         dx[1] = (s.phi[1] - s.plo[1]) / s.ny;
         ng = (s.spatial_order == 2) ? 2 : 1;  // ZoomyAmr.H:152
-        boxes = chop_domain(s.nx, s.ny, s.max_grid_size);
+        boxes = bxs;
         // A 2-D AMReX build grows boxes in y too, even for a 1-D model.
         for (auto* mf : {&Q, &Qtmp, &Qaux}) mf->fabs.resize(boxes.size());
         Qold.fabs.resize(boxes.size());
@@ -333,9 +342,33 @@ struct Solver {
         }
     }
 
-    // ZoomyAmr::FillPatch (lev 0) + the FillBoundary pair at :571-572.
+    // Ghost cells inside the domain that no box of this level covers take
+    // the coarse level's interpolant.
+    void fill_coarse_fine() {
+        const int nb = (int)boxes.size();
+        Real v[nq];
+        for (int b = 0; b < nb; ++b) {
+            Fab& f = Q.fabs[b];
+            for (int j = f.gl[1]; j <= f.gh[1]; ++j)
+                for (int i = f.gl[0]; i <= f.gh[0]; ++i) {
+                    if (i < 0 || i >= s.nx || j < 0 || j >= s.ny) continue;
+                    bool covered = false;
+                    for (int o = 0; o < nb && !covered; ++o) {
+                        const Box& B = boxes[o];
+                        covered = i >= B.lo[0] && i <= B.hi[0] && j >= B.lo[1] && j <= B.hi[1];
+                    }
+                    if (covered) continue;
+                    coarse_fill(i, j, v);
+                    for (int n = 0; n < nq; ++n) f(i, j, n) = v[n];
+                }
+        }
+    }
+
+    // ZoomyAmr::FillPatch (ZoomyAmr.cpp:258-277) + the FillBoundary pair at
+    // :571-572; a refined level also takes coarse-fine ghosts from below.
     void fill_patch(Real time) {
         fill_boundary(Q);
+        if (coarse_fill) fill_coarse_fine();
         fill_physical_bc(time);
         fill_boundary(Q);
         fill_boundary(Qaux);

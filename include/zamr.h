@@ -34,7 +34,7 @@ enum {
     ZAMR_ETAG = 2,      /* boundary tag not declared by the model (ZoomyAmr.cpp:361-372) */
     ZAMR_ERASTER = 3,   /* raster missing or too small (init_solution.cpp:20-40) */
     ZAMR_EHIP = 4,      /* HIP runtime failure */
-    ZAMR_EUNSUP = 5,    /* a deck option outside this build (e.g. amr.max_level > 0) */
+    ZAMR_EUNSUP = 5,    /* a deck option outside this build (e.g. amr.max_level > 1) */
     ZAMR_EIO = 6        /* plotfile / file write failure */
 };
 
@@ -60,8 +60,11 @@ int zamr_default_param(int k, double* value);
 int zamr_march_constants(int* c_cfl_dimension, int* c_cfl_degree_factor);
 
 /* ZoomyAmr::ZoomyAmr() (ZoomyAmr.cpp:27-107): reads the ParmParse deck text
- * written by build.py:write_inputs (build.py:55-120).  Keys outside this build
- * (amr.max_level > 0, solver.implicit_global = true) fail with ZAMR_EUNSUP. */
+ * written by build.py:write_inputs (build.py:55-120).  amr.max_level = 1 sets
+ * up the two-level hierarchy (AmrCore with ref_ratio 2, a 2-D model; the
+ * amr.blocking_factor / n_error_buf / grid_eff / max_grid_size and tagging.*
+ * keys apply).  Keys outside this build (amr.max_level > 1, ref_ratio != 2,
+ * solver.implicit_global = true) fail with ZAMR_EUNSUP. */
 int zamr_create(const char* inputs_text, zamr_ctx** out);
 /* Batched mode: nbox independent boxes sharing one launch per RK stage.
  * inputs_text supplies the solver.* / params.* keys (geometry keys ignored). */
@@ -70,13 +73,17 @@ int zamr_destroy(zamr_ctx* ctx);
 const char* zamr_last_error(const zamr_ctx* ctx);
 
 /* ZoomyAmr::InitData (ZoomyAmr.cpp:112-156): init_solution on Q and Qaux
- * (init_solution.cpp:78-115), then the init.* rasters. */
+ * (init_solution.cpp:78-115), then the init.* rasters, then (AMR) the level
+ * loop of :137-155 that tags level 0 and fills level 1 from it. */
 int zamr_init_data(zamr_ctx* ctx);
 
-/* Geometry of level 0 (or of box `box` in batched mode). */
+/* Geometry of a level: its domain in its own index space and its box count
+ * (in batched mode, box `lev`). */
 int zamr_level_shape(const zamr_ctx* ctx, int lev, int* nx, int* ny, int* nboxes);
 
-/* Whole-level state copy in/out (field 0 = Q, 1 = Qaux), dense (ncomp, ny, nx).
+/* Whole-level state copy in/out (field 0 = Q, 1 = Qaux), dense (ncomp, ny, nx)
+ * over the level's domain; on a refined level only the cells its boxes cover
+ * are read or written (the rest of the array is left as it is).
  * In batched mode lev is the box index. n = number of doubles in host. */
 int zamr_set_state(zamr_ctx* ctx, int lev, int field, const double* host, size_t n);
 int zamr_get_state(zamr_ctx* ctx, int lev, int field, double* host, size_t n);
@@ -93,8 +100,9 @@ int zamr_fill_patch(zamr_ctx* ctx, int lev, double time);
 int zamr_stage(zamr_ctx* ctx, int lev, int order, double time, double dt);
 /* ZoomyAmr::Advance (ZoomyAmr.cpp:538-731) with a caller-given dt. */
 int zamr_advance(zamr_ctx* ctx, int lev, double time, double dt);
-/* ZoomyAmr::TimeStep (ZoomyAmr.cpp:733-762): ComputeDt, time_end cap, Advance.
- * On return *dt_taken holds the step size. */
+/* ZoomyAmr::TimeStep (ZoomyAmr.cpp:733-762): (AMR: regrid when iteration > 0)
+ * ComputeDt, time_end cap, Advance (AMR: then two level-1 substeps and
+ * average_down).  On return *dt_taken holds the level-0 step size. */
 int zamr_time_step(zamr_ctx* ctx, int lev, double time, int iteration, double* dt_taken);
 /* Up to nsteps iterations of the Evolve loop body (ZoomyAmr.cpp:782-810) with
  * dt computed and applied on the device; steps are captured in a hipGraph.
@@ -103,7 +111,8 @@ int zamr_run_steps(zamr_ctx* ctx, int nsteps, int* steps_done, double* time);
 /* ZoomyAmr::Evolve (ZoomyAmr.cpp:767-816) including plotfiles and the
  * "Initial total mass" / "Step" / "Total evolution time" lines on stdout. */
 int zamr_evolve(zamr_ctx* ctx);
-/* ZoomyAmr::ComputeTotalMass (ZoomyAmr.cpp:517-536). */
+/* ZoomyAmr::ComputeTotalMass (ZoomyAmr.cpp:517-536): every level, level-0
+ * cells under level 1 left out. */
 int zamr_total_mass(zamr_ctx* ctx, double* mass);
 /* Time, step count and last dt of level 0 (of group `lev` in batched mode). */
 int zamr_get_time(const zamr_ctx* ctx, int lev, double* time, int* step, double* dt);
@@ -133,9 +142,28 @@ int zamr_set_profiling(zamr_ctx* ctx, int on);
 int zamr_kernel_times(zamr_ctx* ctx, double* out_ms, int* nsteps);
 
 /* solver.positivity = mood (ZoomyAmr.cpp:699-720): troubled cells of the last
- * step's order-2 candidate in time-step group `group` (0 for a single level)
- * and the number of steps that group has redone at order 1 since creation. */
+ * step's order-2 candidate in time-step group `group` (0 for a single level,
+ * the level index with AMR) and the number of steps that group has redone at
+ * order 1 since creation. */
 int zamr_mood_stats(zamr_ctx* ctx, int group, int* troubled, long long* redos);
+
+/* ---- AMR hierarchy (amr.max_level = 1) ---------------------------------- */
+/* finest_level (AmrCore): 1 while level 1 has boxes, else 0. */
+int zamr_finest_level(zamr_ctx* ctx, int* finest);
+/* BoxArray of level lev (boxArray(lev)) as (lo0, lo1, hi0, hi1) quadruples in
+ * the level's index space, up to cap boxes; *nboxes receives the count. */
+int zamr_level_boxes(zamr_ctx* ctx, int lev, int* lohi, int cap, int* nboxes);
+/* The level loop of InitData (ZoomyAmr.cpp:137-155) on the current level-0
+ * state, for a state set with zamr_set_state after zamr_init_data. */
+int zamr_init_levels(zamr_ctx* ctx);
+/* AmrCore::regrid(0, time) (ZoomyAmr.cpp:736): ErrorEst on level 0, new
+ * level-1 grids, RemakeLevel / MakeNewLevelFromCoarse / ClearLevel. */
+int zamr_regrid(zamr_ctx* ctx, double time);
+/* ZoomyAmr::ErrorEst (ZoomyAmr.cpp:225-253) on level 0 as it stands: one byte
+ * per cell (1 = tagged), raster order, n = nx * ny. */
+int zamr_error_est(zamr_ctx* ctx, unsigned char* tags, size_t n);
+/* Regrids run and levels remade since creation. */
+int zamr_amr_stats(zamr_ctx* ctx, long long* regrids, long long* remakes);
 
 /* (dt, time after the step) of the steps taken on the device clock since the
  * last zamr_reset_clock, up to cap entries — the values the reference prints

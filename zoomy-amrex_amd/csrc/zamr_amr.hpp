@@ -1,0 +1,891 @@
+#pragma once
+// Two-level AMR advance of Source/ZoomyAmr.cpp on the device: ErrorEst
+// tagging (:225-253), regrid with RemakeLevel / MakeNewLevelFromCoarse /
+// ClearLevel (:161-219, AmrCore::regrid), the coarse-fine part of FillPatch
+// and FillCoarsePatch with cell_cons_interp (:258-288), the subcycled
+// TimeStep with average_down (:733-762) and the composite ComputeTotalMass
+// (:517-536).  Level 0 is the single-level Solver; level 1 is a second
+// Solver over the refined domain whose boxes follow the tags.
+//
+// The AMR machinery is AMReX's (external to the reference, not vendored);
+// its published algorithms are restated here in device code and host code:
+// CellConservativeLinear with linear limiting, average_down, TagBoxArray
+// buffering and coarsening, Berger-Rigoutsos clustering (ClusterList::chop).
+//
+// Where the reference reads memory it never wrote, this build fixes a value:
+// coarse cells beyond the domain in the interpolation stencil repeat the edge
+// cell (the foextrap its BCRec declares; its fill functor is NullFill);
+// ErrorEst before any stage reads ghost cells as 0; a new or remade fine
+// level's Qaux is update_aux_variables(Q, 0) (the reference leaves it unset
+// and ComputeDt reads it next).
+//
+// Data movement: the coarse state is fixed while the fine level subcycles
+// (FillPatchTwoLevels is handed the coarse level at t_new only), so the
+// coarse-fine ghost cells are interpolated once per coarse step into every
+// fine state buffer; stage kernels never write ghost cells, so they stay.
+#include <cstring>
+#include <memory>
+
+#include "zamr_driver.hpp"
+
+namespace zamr {
+
+// Level 0 is a tensor-product grid of boxes over its domain (max_grid_size
+// chopping); xbox / ybox give the box column / row of each cell index.
+struct CoarseGrid {
+    const BoxDesc* boxes;
+    const int* xbox;
+    const int* ybox;
+    int nbx, nx, ny;
+};
+
+__device__ __forceinline__ int coarse_box_index(const CoarseGrid& g, int ic, int jc) {
+    return g.ybox[jc] * g.nbx + g.xbox[ic];
+}
+
+// Cells (ilo..ilo+ni-1, jlo..jlo+nj-1) of box `box` (and source box `src`
+// for level-to-level copies); `start` is the rectangle's first cell in the
+// launch's flat cell numbering.
+struct CellRect {
+    int box, src, ilo, jlo, ni, nj;
+    long long start;
+};
+
+__device__ __forceinline__ int find_rect(const CellRect* __restrict__ r, int n, long long k) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (r[mid].start <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Coarse value of component n at (ic, jc), clamped into the domain.
+__device__ __forceinline__ double coarse_value(const CoarseGrid& g, int buf, int ic, int jc, int n) {
+    ic = ic < 0 ? 0 : (ic >= g.nx ? g.nx - 1 : ic);
+    jc = jc < 0 ? 0 : (jc >= g.ny ? g.ny - 1 : jc);
+    const BoxView C = view(g.boxes, coarse_box_index(g, ic, jc));
+    const long long idx = (long long)(ic - C.gl0) + (long long)(jc - C.gl1) * C.sy;
+    return qsel(C, buf)[idx + n * C.sc];
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+
+// cell_cons_interp (ratio 2) of the level-0 state into the listed fine
+// cells, written to every fine state buffer in `bufmask`:
+//   sx = central difference, limited by the one-sided differences times 2;
+//   one factor per direction, the least over components of limited/central;
+//   fine = crse + xoff * (sx * fx) + yoff * (sy * fy), xoff, yoff = -+1/4.
+template <class M>
+__global__ void __launch_bounds__(256) cf_interp_kernel(const BoxDesc* __restrict__ fine,
+                                                        const CellRect* __restrict__ rects, int nrects,
+                                                        long long total, CoarseGrid g, int cbuf, unsigned bufmask) {
+#pragma clang fp contract(off)
+    constexpr int nq = M::n_dof_q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const long long cell = k - R.start;
+    const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
+    const int ic = i >> 1, jc = j >> 1;
+    double u0[nq], sx[nq], sy[nq];
+    double fx = 1.0, fy = 1.0;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+        const double u = coarse_value(g, cbuf, ic, jc, n);
+        u0[n] = u;
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double um = d == 0 ? coarse_value(g, cbuf, ic - 1, jc, n) : coarse_value(g, cbuf, ic, jc - 1, n);
+            const double up = d == 0 ? coarse_value(g, cbuf, ic + 1, jc, n) : coarse_value(g, cbuf, ic, jc + 1, n);
+            const double dc = 0.5 * (up - um);
+            const double df = 2.0 * (up - u);
+            const double db = 2.0 * (u - um);
+            double sl = (df * db >= 0.0) ? dmin(fabs(df), fabs(db)) : 0.0;
+            sl = copysign(1.0, dc) * dmin(sl, fabs(dc));
+            if (dc != 0.0) {
+                if (d == 0) fx = dmin(fx, sl / dc);
+                else fy = dmin(fy, sl / dc);
+            }
+            if (d == 0) sx[n] = dc;
+            else sy[n] = dc;
+        }
+    }
+    const double xoff = ((i - 2 * ic) + 0.5) / 2.0 - 0.5;
+    const double yoff = ((j - 2 * jc) + 0.5) / 2.0 - 0.5;
+    const BoxView F = view(fine, R.box);
+    const long long idx = (long long)(i - F.gl0) + (long long)(j - F.gl1) * F.sy;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+        const double v = u0[n] + xoff * (sx[n] * fx) + yoff * (sy[n] * fy);
+        if (bufmask & 1u) F.q0[idx + n * F.sc] = v;
+        if (bufmask & 2u) F.q1[idx + n * F.sc] = v;
+        if (bufmask & 4u) F.q2[idx + n * F.sc] = v;
+    }
+}
+
+// average_down (ratio 2): every level-0 cell under a fine box takes the mean
+// of its four children, summed j-outer, i-inner, times 1/4.  Rectangles are
+// in level-0 index space, one per fine box.
+template <class M>
+__global__ void __launch_bounds__(256) average_down_kernel(const BoxDesc* __restrict__ fine,
+                                                           const CellRect* __restrict__ rects, int nrects,
+                                                           long long total, CoarseGrid g, int fbuf, int cbuf) {
+#pragma clang fp contract(off)
+    constexpr int nq = M::n_dof_q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const long long cell = k - R.start;
+    const int ic = R.ilo + (int)(cell % R.ni), jc = R.jlo + (int)(cell / R.ni);
+    const BoxView F = view(fine, R.box);
+    const BoxView C = view(g.boxes, coarse_box_index(g, ic, jc));
+    const double* __restrict__ qf = qsel(F, fbuf);
+    double* __restrict__ qc = qsel(C, cbuf);
+    const long long f00 = (long long)(2 * ic - F.gl0) + (long long)(2 * jc - F.gl1) * F.sy;
+    const long long ci = (long long)(ic - C.gl0) + (long long)(jc - C.gl1) * C.sy;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+        const long long o = f00 + n * F.sc;
+        double sum = 0.0;
+        sum += qf[o];
+        sum += qf[o + 1];
+        sum += qf[o + F.sy];
+        sum += qf[o + F.sy + 1];
+        qc[ci + n * C.sc] = 0.25 * sum;
+    }
+}
+
+// Level-to-level copy (RemakeLevel's FillPatch from the old fine level):
+// cells of new box `box` from old box `src`, valid cells of the old level.
+template <class M>
+__global__ void __launch_bounds__(256) copy_levels_kernel(const BoxDesc* __restrict__ dst, int dbuf,
+                                                          const BoxDesc* __restrict__ src, int sbuf,
+                                                          const CellRect* __restrict__ rects, int nrects,
+                                                          long long total) {
+    constexpr int nq = M::n_dof_q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const long long cell = k - R.start;
+    const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
+    const BoxView D = view(dst, R.box);
+    const BoxView S = view(src, R.src);
+    const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy;
+    const long long si = (long long)(i - S.gl0) + (long long)(j - S.gl1) * S.sy;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) qsel(D, dbuf)[di + n * D.sc] = qsel(S, sbuf)[si + n * S.sc];
+}
+
+// ErrorEst on level 0 (ZoomyAmr.cpp:225-253): cell (i, j) is tagged when its
+// bed is below b_max and |grad h| (undivided central differences) exceeds
+// the threshold or h > 1e-2.  Neighbours outside the cell's box read the
+// box's ghost cells as the last stage's FillPatch left them (buffer `last`;
+// S::load reproduces the stored ghosts and the physical-BC ghosts), or 0
+// before any stage ran (last < 0).
+template <class M>
+__global__ void __launch_bounds__(256) error_est_kernel(CoarseGrid g, int cur, int last, Params pp, int clamp,
+                                                        double time_last, double threshold, double b_max,
+                                                        unsigned char* __restrict__ tags) {
+#pragma clang fp contract(off)
+    using S = Sys<M>;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)g.nx * g.ny) return;
+    const int i = (int)(k % g.nx), j = (int)(k / g.nx);
+    typename S::VP p;
+    for (int q = 0; q < S::np; ++q) p(q, 0) = pp.v[q];
+    const BoxView B = view(g.boxes, coarse_box_index(g, i, j));
+    auto h_at = [&](int c, int r) -> double {
+        if (c >= B.lo0 && c <= B.hi0 && r >= B.lo1 && r <= B.hi1) {
+            const long long idx = (long long)(c - B.gl0) + (long long)(r - B.gl1) * B.sy;
+            return qsel(B, cur)[idx + 1 * B.sc];
+        }
+        if (last < 0) return 0.0;
+        return S::load(B, qsel(B, last), c, r, p, clamp, time_last)(1, 0);
+    };
+    const long long idx = (long long)(i - B.gl0) + (long long)(j - B.gl1) * B.sy;
+    unsigned char t = 0;
+    if (qsel(B, cur)[idx] < b_max) {
+        double grad = 0.0;
+        const double dhdx = fabs(h_at(i + 1, j) - h_at(i - 1, j));
+        grad += dhdx * dhdx;
+        if constexpr (S::dim >= 2) {
+            const double dhdy = fabs(h_at(i, j + 1) - h_at(i, j - 1));
+            grad += dhdy * dhdy;
+        }
+        if (sqrt(grad) > threshold || qsel(B, cur)[idx + B.sc] > 1.0e-2) t = 1;
+    }
+    tags[k] = t;
+}
+
+// TagBoxArray buffer + coarsen: coarsened cell (I, J) of factor bf is tagged
+// when a tag lies within nbuf cells (in both directions) of its bf x bf block.
+__global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __restrict__ tags, int nx, int ny,
+                                                         int nbuf, int bf, unsigned char* __restrict__ out, int cnx,
+                                                         int cny) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)cnx * cny) return;
+    const int I = (int)(k % cnx), J = (int)(k / cnx);
+    const int i0 = max(0, I * bf - nbuf), i1 = min(nx - 1, (I + 1) * bf - 1 + nbuf);
+    const int j0 = max(0, J * bf - nbuf), j1 = min(ny - 1, (J + 1) * bf - 1 + nbuf);
+    unsigned char t = 0;
+    for (int j = j0; j <= j1 && !t; ++j)
+        for (int i = i0; i <= i1; ++i)
+            if (tags[(long long)j * nx + i]) {
+                t = 1;
+                break;
+            }
+    out[k] = t;
+}
+
+__global__ void clear_max_kernel(GroupState* gs, int ngroups) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups) gs[g].maxinv_bits = 0ull;
+}
+
+// The fine level's substeps start at the coarse step's start time
+// (TimeStep(lev + 1, time + (i - 1) * dt_level[lev + 1]), ZoomyAmr.cpp:756-757).
+__global__ void sync_clock_kernel(const GroupState* coarse, GroupState* fine) { fine->time = coarse->time; }
+
+// ---------------- host: Berger-Rigoutsos clustering ----------------
+// Points live in one array; a cluster is a [begin, end) range of it and its
+// bounding box.  Splitting partitions the range in place (as Cluster::chop
+// does with std::partition); the final cluster boxes depend only on the
+// point set.
+struct ClusterRange {
+    size_t b, e;
+    int lo[2], hi[2];
+};
+
+inline void cluster_bound(const std::vector<int>& px, const std::vector<int>& py, ClusterRange& c) {
+    c.lo[0] = c.hi[0] = px[c.b];
+    c.lo[1] = c.hi[1] = py[c.b];
+    for (size_t k = c.b + 1; k < c.e; ++k) {
+        c.lo[0] = std::min(c.lo[0], px[k]);
+        c.hi[0] = std::max(c.hi[0], px[k]);
+        c.lo[1] = std::min(c.lo[1], py[k]);
+        c.hi[1] = std::max(c.hi[1], py[k]);
+    }
+}
+
+// FindCut (AMReX_Cluster.cpp): status 0 hole, 1 inflection, 2 bisection,
+// 3 none; returns the cut index (cells below it stay).
+inline int br_find_cut(const int* hist, int len, int lo, int& status) {
+    status = 3;
+    if (len <= 1) return lo;
+    const int mid = len / 2;
+    int cut = -1;
+    for (int i = 0; i < len; ++i)
+        if (hist[i] == 0) {
+            status = 0;
+            if (std::abs(cut - mid) > std::abs(i - mid)) {
+                cut = i;
+                if (i > mid) break;
+            }
+        }
+    if (status == 0) return lo + cut;
+    // second differences (0 at both ends), then the largest sign change at
+    // least 2 cells from either end, nearest the middle among equals
+    int best = -1;
+    std::vector<int> d2(len, 0);
+    for (int i = 1; i < len - 1; ++i) d2[i] = hist[i + 1] - 2 * hist[i] + hist[i - 1];
+    for (int i = 2; i < len - 2; ++i) {
+        const int a = d2[i - 1], b = d2[i], dif = std::abs(a - b);
+        if (a * b < 0 && dif >= best) {
+            if (dif > best) {
+                status = 1;
+                cut = i;
+                best = dif;
+            } else if (std::abs(i - mid) < std::abs(cut - mid)) {
+                cut = i;
+            }
+        }
+    }
+    if (best <= 2) {
+        cut = mid;
+        status = 2;
+    }
+    return lo + cut;
+}
+
+// ClusterList::chop to efficiency `eff` over the points (px[k], py[k]);
+// returns the cluster boxes (lo0, lo1, hi0, hi1).
+inline std::vector<std::array<int, 4>> br_cluster(std::vector<int>& px, std::vector<int>& py, double eff) {
+    std::vector<std::array<int, 4>> out;
+    if (px.empty()) return out;
+    std::vector<ClusterRange> work;
+    ClusterRange all{0, px.size(), {0, 0}, {0, 0}};
+    cluster_bound(px, py, all);
+    work.push_back(all);
+    std::vector<int> hist[2];
+    while (!work.empty()) {
+        ClusterRange c = work.back();
+        work.pop_back();
+        const long long n = (long long)(c.e - c.b);
+        const double vol = (double)(c.hi[0] - c.lo[0] + 1) * (double)(c.hi[1] - c.lo[1] + 1);
+        if ((double)n / vol >= eff) {
+            out.push_back({c.lo[0], c.lo[1], c.hi[0], c.hi[1]});
+            continue;
+        }
+        int len[2], cut[2], st[2];
+        for (int d = 0; d < 2; ++d) {
+            len[d] = c.hi[d] - c.lo[d] + 1;
+            hist[d].assign(len[d], 0);
+        }
+        for (size_t k = c.b; k < c.e; ++k) {
+            hist[0][px[k] - c.lo[0]]++;
+            hist[1][py[k] - c.lo[1]]++;
+        }
+        int best = 3;
+        for (int d = 0; d < 2; ++d) {
+            cut[d] = br_find_cut(hist[d].data(), len[d], c.lo[d], st[d]);
+            best = std::min(best, st[d]);
+        }
+        int dir = -1, minlen = -1;
+        for (int d = 0; d < 2; ++d) {
+            if (st[d] != best) continue;
+            const int l = std::min(cut[d] - c.lo[d], c.hi[d] - cut[d]);
+            if (l >= minlen) {
+                dir = d;
+                minlen = l;
+            }
+        }
+        // partition [b, e) into coordinate < cut, then the rest
+        std::vector<int>& pc = dir == 0 ? px : py;
+        size_t m = c.b;
+        for (size_t k = c.b; k < c.e; ++k)
+            if (pc[k] < cut[dir]) {
+                std::swap(px[k], px[m]);
+                std::swap(py[k], py[m]);
+                ++m;
+            }
+        ClusterRange a{c.b, m, {0, 0}, {0, 0}}, b{m, c.e, {0, 0}, {0, 0}};
+        cluster_bound(px, py, a);
+        cluster_bound(px, py, b);
+        work.push_back(b);
+        work.push_back(a);
+    }
+    return out;
+}
+
+// ---------------- host: the hierarchy ----------------
+template <class M, class N>
+class AmrDriver {
+   public:
+    using Sol = Solver<M, N>;
+    static constexpr int nq = M::n_dof_q;
+    static constexpr int na = M::n_dof_qaux;
+
+    Sol& L0;
+    std::unique_ptr<Sol> L1;
+    int max_level = 1, ref_ratio = 2, blocking_factor = 8, n_error_buf = 1;
+    double grid_eff = 0.7, tag_threshold = 0.01, tag_b_max = 1.0e30;
+    int fine_max_grid_size = 64;
+
+    // level-0 grid tables and tag maps
+    int* d_xbox = nullptr;
+    int* d_ybox = nullptr;
+    int nbx = 0;
+    unsigned char* d_tags = nullptr;   // ErrorEst, one byte per level-0 cell
+    unsigned char* d_ctags = nullptr;  // buffered and coarsened
+    unsigned char* d_cover = nullptr;  // level-0 cells under level 1
+    int cnx = 0, cny = 0, bf_lev = 1;
+    std::vector<unsigned char> h_ctags, last_ctags;
+    std::vector<std::array<int, 4>> last_boxes;
+
+    // level-1 launch lists (device, with their cell totals)
+    CellRect* d_cf = nullptr;  // coarse-fine ghost cells
+    int n_cf = 0;
+    long long cf_total = 0;
+    CellRect* d_avg = nullptr;  // level-0 cells under each fine box
+    int n_avg = 0;
+    long long avg_total = 0;
+
+    bool l0_stage_ran = false;  // level-0 ghosts hold a FillPatch
+    double t_last_stage = 0.0;  // the time that FillPatch ran at (its step's start)
+    bool l0_max_stale = true;   // level-0 CFL max must be reduced from the stored state
+    bool l1_max_stale = true;
+    long long regrids = 0, remakes = 0;
+    int plot_step = 0;
+    double next_plot_time = 0.0;
+
+    AmrDriver(Sol& l0, const std::string& deck_text) : L0(l0) {
+        Deck pp(deck_text);
+        pp.query("amr.max_level", max_level);
+        pp.query("amr.ref_ratio", ref_ratio);
+        pp.query("amr.blocking_factor", blocking_factor);
+        pp.query("amr.n_error_buf", n_error_buf);
+        pp.query("amr.grid_eff", grid_eff);
+        pp.query("amr.max_grid_size", fine_max_grid_size);
+        pp.query("tagging.threshold", tag_threshold);
+        pp.query("tagging.b_max", tag_b_max);
+        if (M::dimension != 2) throw Error(5, "AMR (amr.max_level > 0) is built for two-dimensional models only");
+        if (ref_ratio != 2) throw Error(5, "amr.ref_ratio must be 2 in this build");
+        if (blocking_factor < 1 || n_error_buf < 0 || !(grid_eff > 0.0) || grid_eff > 1.0)
+            throw Error(1, "bad amr.blocking_factor / amr.n_error_buf / amr.grid_eff");
+        if (L0.nx % blocking_factor != 0 || L0.ny % blocking_factor != 0)
+            throw Error(1, "amr.n_cell must be divisible by amr.blocking_factor");
+        bf_lev = std::max(1, blocking_factor / ref_ratio);
+        cnx = (L0.nx + bf_lev - 1) / bf_lev;
+        cny = (L0.ny + bf_lev - 1) / bf_lev;
+        // level-0 tensor tables (configure_deck builds hb y-major from chop_axis)
+        auto xs = chop_axis(L0.nx, L0.max_grid_size), ys = chop_axis(L0.ny, L0.max_grid_size);
+        nbx = (int)xs.size();
+        std::vector<int> xb(L0.nx), yb(L0.ny);
+        for (int k = 0; k < (int)xs.size(); ++k)
+            for (int i = xs[k].first; i <= xs[k].second; ++i) xb[i] = k;
+        for (int k = 0; k < (int)ys.size(); ++k)
+            for (int j = ys[k].first; j <= ys[k].second; ++j) yb[j] = k;
+        ZAMR_HIP(hipMalloc(&d_xbox, xb.size() * sizeof(int)));
+        ZAMR_HIP(hipMalloc(&d_ybox, yb.size() * sizeof(int)));
+        ZAMR_HIP(hipMemcpy(d_xbox, xb.data(), xb.size() * sizeof(int), hipMemcpyHostToDevice));
+        ZAMR_HIP(hipMemcpy(d_ybox, yb.data(), yb.size() * sizeof(int), hipMemcpyHostToDevice));
+        const size_t ncell = (size_t)L0.nx * L0.ny;
+        ZAMR_HIP(hipMalloc(&d_tags, ncell));
+        ZAMR_HIP(hipMalloc(&d_ctags, (size_t)cnx * cny));
+        ZAMR_HIP(hipMalloc(&d_cover, ncell));
+        ZAMR_HIP(hipMemset(d_cover, 0, ncell));
+        h_ctags.resize((size_t)cnx * cny);
+    }
+
+    ~AmrDriver() {
+        L1.reset();
+        for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags, (void*)d_cover, (void*)d_cf,
+                        (void*)d_avg})
+            if (p) hipFree(p);
+    }
+
+    CoarseGrid grid() const { return CoarseGrid{L0.d_boxes, d_xbox, d_ybox, nbx, L0.nx, L0.ny}; }
+    int finest_level() const { return L1 ? 1 : 0; }
+    Sol& level(int l) {
+        if (l == 0) return L0;
+        if (l == 1 && L1) return *L1;
+        throw Error(1, "level " + std::to_string(l) + " does not exist");
+    }
+    hipStream_t stream() const { return L0.stream; }
+
+    static unsigned grid1(long long n) { return (unsigned)((n + 255) / 256); }
+
+    // Upload a rectangle list with its running cell offsets.
+    static void upload_rects(std::vector<CellRect>& r, CellRect*& d, int& n, long long& total, hipStream_t s) {
+        total = 0;
+        for (auto& x : r) {
+            x.start = total;
+            total += (long long)x.ni * x.nj;
+        }
+        if (d) hipFree(d);
+        d = nullptr;
+        n = (int)r.size();
+        if (n) {
+            ZAMR_HIP(hipMalloc(&d, r.size() * sizeof(CellRect)));
+            ZAMR_HIP(hipMemcpyAsync(d, r.data(), r.size() * sizeof(CellRect), hipMemcpyHostToDevice, s));
+            ZAMR_HIP(hipStreamSynchronize(s));
+        }
+    }
+
+    // cell_cons_interp into the listed cells of level F, into buffers bufmask.
+    void interp(Sol& F, CellRect* d, int n, long long total, unsigned bufmask) {
+        if (!n || !total) return;
+        hipLaunchKernelGGL((cf_interp_kernel<M>), dim3(grid1(total)), dim3(256), 0, stream(), F.d_boxes, d, n, total,
+                           grid(), L0.cur, bufmask);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    // The coarse-fine ghost cells of the current level 1: in-domain ghost
+    // cells of each box that no box's valid cells cover, as row runs.
+    void build_fine_lists() {
+        std::vector<CellRect> cf, avg;
+        const auto& hb = L1->hb;
+        for (size_t b = 0; b < hb.size(); ++b) {
+            const HostBox& B = hb[b];
+            for (int j = std::max(0, B.lo[1] - B.ng); j <= std::min(L1->ny - 1, B.hi[1] + B.ng); ++j) {
+                int run0 = -1;
+                const int i0 = std::max(0, B.lo[0] - B.ng), i1 = std::min(L1->nx - 1, B.hi[0] + B.ng);
+                for (int i = i0; i <= i1 + 1; ++i) {
+                    bool need = false;
+                    if (i <= i1) {
+                        need = true;
+                        for (const HostBox& O : hb)
+                            if (i >= O.lo[0] && i <= O.hi[0] && j >= O.lo[1] && j <= O.hi[1]) {
+                                need = false;
+                                break;
+                            }
+                    }
+                    if (need && run0 < 0) run0 = i;
+                    if (!need && run0 >= 0) {
+                        cf.push_back(CellRect{(int)b, (int)b, run0, j, i - run0, 1, 0});
+                        run0 = -1;
+                    }
+                }
+            }
+            avg.push_back(CellRect{(int)b, (int)b, B.lo[0] / 2, B.lo[1] / 2, (B.hi[0] - B.lo[0] + 1) / 2,
+                                   (B.hi[1] - B.lo[1] + 1) / 2, 0});
+        }
+        upload_rects(cf, d_cf, n_cf, cf_total, stream());
+        upload_rects(avg, d_avg, n_avg, avg_total, stream());
+        // level-0 cells under level 1, for the composite mass
+        std::vector<unsigned char> cover((size_t)L0.nx * L0.ny, 0);
+        for (const HostBox& B : hb)
+            for (int j = B.lo[1] / 2; j <= B.hi[1] / 2; ++j)
+                std::memset(&cover[(size_t)j * L0.nx + B.lo[0] / 2], 1, (B.hi[0] - B.lo[0] + 1) / 2);
+        ZAMR_HIP(hipMemcpyAsync(d_cover, cover.data(), cover.size(), hipMemcpyHostToDevice, stream()));
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+    }
+
+    void clear_fine() {
+        L1.reset();
+        ZAMR_HIP(hipMemsetAsync(d_cover, 0, (size_t)L0.nx * L0.ny, stream()));
+        last_boxes.clear();
+    }
+
+    // Fine boxes from the buffered, coarsened tag map: clusters refined by
+    // bf_lev * ref_ratio, cut to amr.max_grid_size in cluster units.
+    std::vector<std::array<int, 4>> make_grids() {
+        std::vector<int> px, py;
+        for (int J = 0; J < cny; ++J)
+            for (int I = 0; I < cnx; ++I)
+                if (h_ctags[(size_t)J * cnx + I]) {
+                    px.push_back(I);
+                    py.push_back(J);
+                }
+        auto cl = br_cluster(px, py, grid_eff);
+        std::sort(cl.begin(), cl.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
+            return a[1] != b[1] ? a[1] < b[1] : a[0] < b[0];
+        });
+        const int r = ref_ratio * bf_lev;
+        const int m = std::max(1, fine_max_grid_size / r);
+        std::vector<std::array<int, 4>> fine;
+        for (auto& c : cl)
+            for (int y0 = c[1]; y0 <= c[3]; y0 += m)
+                for (int x0 = c[0]; x0 <= c[2]; x0 += m) {
+                    const int x1 = std::min(x0 + m - 1, c[2]), y1 = std::min(y0 + m - 1, c[3]);
+                    fine.push_back({x0 * r, y0 * r, (x1 + 1) * r - 1, (y1 + 1) * r - 1});
+                }
+        return fine;
+    }
+
+    // ErrorEst + buffer/coarsen on the device, tag map to the host.  Returns
+    // true when the map differs from the last one clustered.
+    bool tag(int mode_last, double time_last) {
+        const long long n = (long long)L0.nx * L0.ny;
+        hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
+                           L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
+        ZAMR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(tag_buffer_kernel, dim3(grid1((long long)cnx * cny)), dim3(256), 0, stream(), d_tags,
+                           L0.nx, L0.ny, n_error_buf, bf_lev, d_ctags, cnx, cny);
+        ZAMR_HIP(hipGetLastError());
+        ZAMR_HIP(hipMemcpyAsync(h_ctags.data(), d_ctags, h_ctags.size(), hipMemcpyDeviceToHost, stream()));
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+        const bool changed = h_ctags != last_ctags;
+        if (changed) last_ctags = h_ctags;
+        return changed;
+    }
+
+    // Qaux of a new or remade level: update_aux_variables(Q, 0) at `time`
+    // (its aux storage starts zeroed).
+    void fresh_aux(Sol& F, double time) {
+        auto& it = F.items(F.hx_of(F.spatial_order));
+        hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, stream(), F.d_boxes,
+                           it.d, it.n, F.cur, F.prm, time, WV - 2 * it.hx, (const GroupState*)nullptr);
+        ZAMR_HIP(hipGetLastError());
+        F.aux_stale = false;
+    }
+
+    std::unique_ptr<Sol> new_level(const std::vector<std::array<int, 4>>& boxes) {
+        auto F = std::make_unique<Sol>();
+        F->configure_fine(L0, ref_ratio, boxes);
+        return F;
+    }
+
+    // FillCoarsePatch: valid and in-domain ghost cells of every box of F
+    // from level 0, into F's current buffer.
+    void fill_coarse_patch(Sol& F) {
+        std::vector<CellRect> all;
+        for (size_t b = 0; b < F.hb.size(); ++b) {
+            const HostBox& B = F.hb[b];
+            const int i0 = std::max(0, B.lo[0] - B.ng), i1 = std::min(F.nx - 1, B.hi[0] + B.ng);
+            const int j0 = std::max(0, B.lo[1] - B.ng), j1 = std::min(F.ny - 1, B.hi[1] + B.ng);
+            all.push_back(CellRect{(int)b, (int)b, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+        }
+        CellRect* d = nullptr;
+        int n = 0;
+        long long total = 0;
+        upload_rects(all, d, n, total, stream());
+        interp(F, d, n, total, 1u << F.cur);
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+        hipFree(d);
+    }
+
+    // AmrCore::regrid(0, time) for level 1.
+    void regrid(int mode_last, double time_last, double time) {
+        if (max_level < 1) return;
+        ++regrids;
+        if (!tag(mode_last, time_last)) return;  // the same tags give the same grids
+        auto boxes = make_grids();
+        if (boxes.empty()) {
+            clear_fine();
+            return;
+        }
+        if (L1 && boxes == last_boxes) return;
+        auto F = new_level(boxes);
+        fill_coarse_patch(*F);
+        if (L1) {  // RemakeLevel: the old level's valid cells where it covers
+            std::vector<CellRect> cp;
+            for (size_t b = 0; b < F->hb.size(); ++b) {
+                const HostBox& B = F->hb[b];
+                for (size_t o = 0; o < L1->hb.size(); ++o) {
+                    const HostBox& O = L1->hb[o];
+                    const int i0 = std::max(B.lo[0] - B.ng, O.lo[0]), i1 = std::min(B.hi[0] + B.ng, O.hi[0]);
+                    const int j0 = std::max(B.lo[1] - B.ng, O.lo[1]), j1 = std::min(B.hi[1] + B.ng, O.hi[1]);
+                    if (i0 > i1 || j0 > j1) continue;
+                    cp.push_back(CellRect{(int)b, (int)o, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+                }
+            }
+            CellRect* d = nullptr;
+            int n = 0;
+            long long total = 0;
+            upload_rects(cp, d, n, total, stream());
+            if (n)
+                hipLaunchKernelGGL((copy_levels_kernel<M>), dim3(grid1(total)), dim3(256), 0, stream(), F->d_boxes,
+                                   F->cur, L1->d_boxes, L1->cur, d, n, total);
+            ZAMR_HIP(hipGetLastError());
+            ZAMR_HIP(hipStreamSynchronize(stream()));
+            hipFree(d);
+            ++remakes;
+        }
+        fresh_aux(*F, time);
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+        L1 = std::move(F);
+        last_boxes = boxes;
+        l1_max_stale = true;
+        build_fine_lists();
+    }
+
+    // The level loop of InitData (ZoomyAmr.cpp:137-155) on the current
+    // level-0 state; its ghosts read as 0 there.
+    void init_levels() {
+        L0.ensure_aux();
+        last_ctags.clear();
+        for (int it = 0; it < 4; ++it) {
+            if (L1) fill_coarse_patch(*L1);
+            const int old_finest = finest_level();
+            regrid(-1, 0.0, 0.0);
+            if (old_finest == finest_level() && it > 0) break;
+        }
+        if (L1) {
+            fill_coarse_patch(*L1);
+            fresh_aux(*L1, 0.0);
+            ZAMR_HIP(hipStreamSynchronize(stream()));
+            L1->reset_clock();
+        }
+        l0_stage_ran = false;
+        l0_max_stale = true;
+        l1_max_stale = true;
+    }
+
+    // Average-down target, CFL reductions and clocks of one coarse step
+    // (ZoomyAmr::TimeStep for lev 0 and its two level-1 substeps).
+    // `fine_cb` runs after each fine substep (for the MOOD lines of Evolve).
+    template <class CB>
+    void coarse_step(int iteration, CB&& fine_cb) {
+        L0.download_groups();
+        const double t0 = L0.h_groups[0].time;
+        if (iteration > 0 && max_level >= 1) regrid(l0_stage_ran ? L0.last_in : -1, t_last_stage, t0);
+        t_last_stage = t0;
+        hipStream_t s = stream();
+        if (l0_max_stale || !L0.maxinv_fresh) {
+            L0.ensure_aux();
+            hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, s, L0.d_groups, L0.ngroups);
+            auto& it = L0.items(L0.hx_of(L0.spatial_order));
+            hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, s, L0.d_boxes, it.d,
+                               it.n, L0.cur, L0.prm, L0.d_groups, WV - 2 * it.hx);
+            ZAMR_HIP(hipGetLastError());
+        }
+        L0.begin_step();
+        if (L1) hipLaunchKernelGGL(sync_clock_kernel, dim3(1), dim3(1), 0, s, L0.d_groups, L1->d_groups);
+        L0.advance_kernels();
+        if (L1) {
+            // the reference's end-of-Advance UpdateState leaves Qaux = aux(Q^{n+1});
+            // ComputeDt of the next step reads it beside the averaged-down Q
+            auto& it = L0.items(L0.hx_of(L0.spatial_order));
+            hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, s, L0.d_boxes,
+                               it.d, it.n, L0.cur, L0.prm, 0.0, WV - 2 * it.hx, (const GroupState*)L0.d_groups);
+            ZAMR_HIP(hipGetLastError());
+        }
+        L0.end_step();
+        L0.aux_stale = !L1;
+        L0.maxinv_fresh = true;
+        l0_stage_ran = true;
+        l0_max_stale = false;
+        if (L1) {
+            interp(*L1, d_cf, n_cf, cf_total, L1->mood() ? 7u : 3u);
+            if (l1_max_stale) {
+                hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, s, L1->d_groups, L1->ngroups);
+                auto& it = L1->items(L1->hx_of(L1->spatial_order));
+                hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, s, L1->d_boxes,
+                                   it.d, it.n, L1->cur, L1->prm, L1->d_groups, WV - 2 * it.hx);
+                ZAMR_HIP(hipGetLastError());
+            }
+            for (int sub = 0; sub < ref_ratio; ++sub) {
+                L1->begin_step();
+                L1->advance_kernels();
+                L1->end_step();
+                fine_cb(sub);
+            }
+            l1_max_stale = false;
+            L1->maxinv_fresh = true;
+            hipLaunchKernelGGL((average_down_kernel<M>), dim3(grid1(avg_total)), dim3(256), 0, s, L1->d_boxes, d_avg,
+                               n_avg, avg_total, grid(), L1->cur, L0.cur);
+            ZAMR_HIP(hipGetLastError());
+            l0_max_stale = true;
+        }
+    }
+
+    // ZoomyAmr::TimeStep(0, time, iteration) at a caller-given time; returns dt.
+    double time_step(double time, int iteration) {
+        L0.download_groups();
+        if (L0.h_groups[0].time != time) {
+            L0.h_groups[0].time = time;
+            L0.upload_groups();
+        }
+        coarse_step(iteration, [](int) {});
+        L0.download_groups();
+        return L0.h_groups[0].dt;
+    }
+
+    // ErrorEst of level 0 as it stands (ghosts as the last stage left them,
+    // or 0 before any), one byte per cell in raster order.
+    void error_est_raw(int mode_last, double time_last, unsigned char* host) {
+        const long long n = (long long)L0.nx * L0.ny;
+        hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
+                           L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
+        ZAMR_HIP(hipGetLastError());
+        ZAMR_HIP(hipMemcpyAsync(host, d_tags, (size_t)n, hipMemcpyDeviceToHost, stream()));
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+    }
+
+    bool running() {
+        L0.download_groups();
+        const GroupState& g = L0.h_groups[0];
+        return g.time < L0.time_end && (L0.max_step < 0 || g.step < L0.max_step);
+    }
+
+    void run_steps(int nsteps, int* done, double* time) {
+        L0.download_groups();
+        const int step0 = L0.h_groups[0].step;
+        for (int k = 0; k < nsteps && running(); ++k) coarse_step(L0.h_groups[0].step, [](int) {});
+        L0.download_groups();
+        if (done) *done = L0.h_groups[0].step - step0;
+        if (time) *time = L0.h_groups[0].time;
+    }
+
+    double total_mass() {
+        double m = L0.total_mass(L1 ? d_cover : nullptr);
+        if (L1) m += L1->total_mass();
+        return m;
+    }
+
+    // Dense (ncomp, ny_lev, nx_lev) copy of a level's valid cells; cells no
+    // box covers are left as the caller filled them.
+    void copy_level(int lev, int field, double* host, size_t n, bool to_device) {
+        if (lev == 0) {
+            L0.copy_level(0, field, host, n, to_device);
+            if (to_device) {
+                l0_max_stale = true;
+                L0.maxinv_fresh = false;
+            }
+            return;
+        }
+        Sol& F = level(lev);
+        const int nc = field == 0 ? nq : na;
+        if (field != 0 && field != 1) throw Error(1, "field must be 0 (Q) or 1 (Qaux)");
+        if (n != (size_t)nc * F.nx * F.ny) throw Error(1, "array size mismatch: expected ncomp*ny*nx doubles");
+        if (field == 1 || to_device) F.ensure_aux();
+        for (const HostBox& B : F.hb) {
+            const size_t cells = (size_t)B.gx * B.gy;
+            std::vector<double> tmp(cells * nc);
+            double* dptr = F.d_pool + (field == 0 ? B.off_q[F.cur] : B.off_aux);
+            ZAMR_HIP(hipMemcpyAsync(tmp.data(), dptr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream()));
+            ZAMR_HIP(hipStreamSynchronize(stream()));
+            for (int c = 0; c < nc; ++c)
+                for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                    for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                        const size_t li = (size_t)(i - B.lo[0] + B.ng) + (size_t)(j - B.lo[1] + B.ng) * B.gx +
+                                          (size_t)c * cells;
+                        const size_t gi = ((size_t)c * F.ny + j) * F.nx + i;
+                        if (to_device) tmp[li] = host[gi];
+                        else host[gi] = tmp[li];
+                    }
+            if (to_device) {
+                ZAMR_HIP(hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                                        stream()));
+                ZAMR_HIP(hipStreamSynchronize(stream()));
+            }
+        }
+        if (to_device) {
+            l1_max_stale = true;
+            F.maxinv_fresh = false;
+        }
+    }
+
+    void reset_clock() {
+        L0.reset_clock();
+        if (L1) L1->reset_clock();
+        plot_step = 0;
+        next_plot_time = 0.0;
+    }
+
+    void write_plotfile(const std::string& dir, int step, double time);
+
+    void evolve() {  // ZoomyAmr::Evolve over the hierarchy
+        reset_clock();
+        double time = 0.0;
+        int step = 0;
+        next_plot_time = 0.0;
+        const auto tstart = std::chrono::steady_clock::now();
+        const double mass0 = total_mass();
+        std::cout << "Initial total mass = " << std::setprecision(12) << mass0 << "\n";
+        while (time < L0.time_end && (L0.max_step < 0 || step < L0.max_step)) {
+            if (time >= next_plot_time) {
+                write_plotfile("", plot_step, time);
+                next_plot_time += L0.plot_dt_interval;
+                plot_step++;
+            }
+            bool l0_printed = false;
+            auto mood_lines = [&](int) {
+                if (!l0_printed) {  // level 0's Advance came first
+                    l0_printed = true;
+                    L0.download_groups();
+                    if (L0.mood() && L0.h_groups[0].troubled > 0)
+                        std::cout << "  [MOOD] lev 0 troubled cells: " << (long)L0.h_groups[0].troubled
+                                  << " -> whole-step order-1 redo\n";
+                }
+                L1->download_groups();
+                if (L1->mood() && L1->h_groups[0].troubled > 0)
+                    std::cout << "  [MOOD] lev 1 troubled cells: " << (long)L1->h_groups[0].troubled
+                              << " -> whole-step order-1 redo\n";
+            };
+            coarse_step(step, mood_lines);
+            L0.download_groups();
+            if (!l0_printed && L0.mood() && L0.h_groups[0].troubled > 0)
+                std::cout << "  [MOOD] lev 0 troubled cells: " << (long)L0.h_groups[0].troubled
+                          << " -> whole-step order-1 redo\n";
+            const double dt0 = L0.h_groups[0].dt;
+            time = L0.h_groups[0].time;
+            step++;
+            const double mass = total_mass();
+            const double drift = (mass0 != 0.0) ? (mass - mass0) / mass0 : (mass - mass0);
+            std::cout << "Step " << step << " dt: " << std::setprecision(12) << dt0 << " time: "
+                      << std::setprecision(12) << time << "s"
+                      << " levels: " << finest_level() + 1 << " mass: " << std::setprecision(14) << mass
+                      << " drift: " << std::setprecision(4) << drift << "\n";
+        }
+        write_plotfile("", plot_step, time);
+        const double evo = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
+        std::cout << "Total evolution time = " << evo << " seconds\n";
+        std::cout.flush();
+    }
+};
+
+}  // namespace zamr

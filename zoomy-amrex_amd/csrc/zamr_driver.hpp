@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -108,6 +109,8 @@ class Solver {
     double* d_partial = nullptr;
     int partial_cap = 0;
     int cur = 0;                    // buffer holding the current state
+    int last_in = -1;               // input buffer of the last stage run (its ghosts hold that
+                                    // stage's FillPatch), -1 before the first
     bool maxinv_fresh = false;      // group maxinv_bits already hold this state's reduction
     bool aux_stale = false;         // stored Qaux lags the last UpdateState (steps ran since)
     int rows_per_item = 0;          // 0 = autotune
@@ -203,9 +206,8 @@ class Solver {
         Deck pp(text);
         read_solver_keys(pp);
         pp.query("amr.max_level", max_level);
-        if (max_level != 0)
-            throw Error(5, "amr.max_level > 0: the AMR hierarchy (regrid, two-level FillPatch, average_down) is "
-                           "not part of this build; use a single level or the batched box list");
+        if (max_level < 0 || max_level > 1)
+            throw Error(5, "amr.max_level > 1: this build's hierarchy has at most two levels (0 and 1)");
         std::vector<double> nc;
         if (!pp.queryarr("amr.n_cell", nc) || nc.size() < 2) throw Error(1, "amr.n_cell needs at least 2 values");
         nx = (int)nc[0];
@@ -249,6 +251,67 @@ class Solver {
                     throw Error(1, "boxes narrower than the ghost width are not supported");
                 hb.push_back(b);
             }
+        ngroups = 1;
+        allocate();
+    }
+
+    // A refined level (ref_ratio `ratio` over level c): the given boxes of
+    // valid cells in this level's index space, every solver key and the
+    // stream taken from c.  It keeps its own clock; only level 0 stops at
+    // time_end (ZoomyAmr.cpp:746-749).
+    void configure_fine(const Solver& c, int ratio, const std::vector<std::array<int, 4>>& boxes) {
+        spatial_order = c.spatial_order;
+        implicit_source = c.implicit_source;
+        implicit_global = c.implicit_global;
+        well_balanced = c.well_balanced;
+        clamp_positivity = c.clamp_positivity;
+        positivity = c.positivity;
+        face_based_rhs = c.face_based_rhs;
+        cfl = c.cfl;
+        dtmin = c.dtmin;
+        dtmax = c.dtmax;
+        prm = c.prm;
+        c_cfl_dimension = c.c_cfl_dimension;
+        c_cfl_degree_factor = c.c_cfl_degree_factor;
+        identifier = c.identifier;
+        for (int k = 0; k < 4; ++k) bc_name[k] = c.bc_name[k];
+        time_end = 1.0e300;
+        max_step = -1;
+        nx = c.nx * ratio;
+        ny = c.ny * ratio;
+        plo[0] = c.plo[0];
+        plo[1] = c.plo[1];
+        phi[0] = c.phi[0];
+        phi[1] = c.phi[1];
+        rows_per_item = c.rows_per_item;
+        xcd_remap = c.xcd_remap;
+        per_box_launch = c.per_box_launch;
+        stream = c.stream;
+        own_stream = false;
+        int bcidx[4];
+        for (int k = 0; k < 4; ++k) bcidx[k] = tag_index(bc_name[k]);
+        const double dx[2] = {(phi[0] - plo[0]) / nx, (phi[1] - plo[1]) / ny};
+        for (auto& q : boxes) {
+            HostBox b{};
+            b.lo[0] = q[0];
+            b.lo[1] = q[1];
+            b.hi[0] = q[2];
+            b.hi[1] = q[3];
+            b.ng = 2;
+            b.dlo[0] = 0;
+            b.dhi[0] = nx - 1;
+            b.dlo[1] = 0;
+            b.dhi[1] = ny - 1;
+            b.dx[0] = dx[0];
+            b.dx[1] = dx[1];
+            b.plo[0] = plo[0];
+            b.plo[1] = plo[1];
+            b.group = 0;
+            for (int k = 0; k < 4; ++k) b.bc[k] = bcidx[k];
+            if (b.hi[0] - b.lo[0] + 1 < 2 || b.hi[1] - b.lo[1] + 1 < 2)
+                throw Error(1, "refined boxes narrower than the ghost width are not supported");
+            hb.push_back(b);
+        }
         ngroups = 1;
         allocate();
     }
@@ -603,9 +666,14 @@ class Solver {
         return a;
     }
 
-    void fill_boundary(int buf) {
+    int clamp_flag() const { return (clamp_positivity && positivity != "mood") ? 1 : 0; }
+
+    // FillPatch's same-level part before a stage (update: ghosts take the
+    // owners' values after UpdateState) or a plain copy of the stored state.
+    void fill_boundary(int buf, bool update = true) {
         if (nrects == 0) return;
-        hipLaunchKernelGGL(fill_boundary_kernel, dim3(nrects), dim3(256), 0, stream, d_boxes, d_rects, buf, nq);
+        hipLaunchKernelGGL(fill_boundary_kernel<M>, dim3(nrects), dim3(256), 0, stream, d_boxes, d_rects, buf, prm,
+                           clamp_flag(), update ? 1 : 0);
         ZAMR_HIP(hipGetLastError());
     }
 
@@ -669,7 +737,7 @@ class Solver {
         Items& it = items(hx_of(spatial_order));
         download_groups();
         hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d,
-                           it.n, cur, prm, h_groups[0].time, WV - 2 * it.hx);
+                           it.n, cur, prm, h_groups[0].time, WV - 2 * it.hx, (const GroupState*)nullptr);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipStreamSynchronize(stream));
         aux_stale = false;
@@ -682,6 +750,7 @@ class Solver {
         if (spatial_order == 1) {
             fill_boundary(cur);
             launch_stage_dyn(1, 2, stage_args(cur, 1 - cur, cur));
+            last_in = cur;
             cur = 1 - cur;
         } else if (mood()) {
             // the order-2 candidate goes to a third buffer so that Q^n stays for
@@ -706,6 +775,7 @@ class Solver {
             StageArgs r2 = stage_args(t1, t2, cur);
             r2.mood = 2;
             launch_stage_dyn(1, 1, r2);
+            last_in = t1;
             cur = t2;
         } else {
             fill_boundary(cur);
@@ -715,6 +785,7 @@ class Solver {
             fill_boundary(1 - cur);
             launch_stage_dyn(2, 1, stage_args(1 - cur, cur, cur));
             record(2);
+            last_in = 1 - cur;
         }
     }
 
@@ -841,6 +912,7 @@ class Solver {
         fill_boundary(cur);
         const int nxt = (cur + 1) % (mood() ? 3 : 2);
         launch_stage_dyn(order, 0, stage_args(cur, nxt, cur));
+        last_in = cur;
         cur = nxt;
         ZAMR_HIP(hipStreamSynchronize(stream));
         maxinv_fresh = false;
@@ -855,10 +927,12 @@ class Solver {
         maxinv_fresh = false;
     }
 
-    double total_mass() {  // ComputeTotalMass, level 0 (deterministic item-order sum)
+    // ComputeTotalMass of this level (deterministic item-order sum), without
+    // the cells flagged in `cover` (device, one byte per cell, row length nx).
+    double total_mass(const unsigned char* cover = nullptr) {
         Items& it = items(hx_of(spatial_order));
         hipLaunchKernelGGL((mass_partial_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
-                           cur, d_partial, WV - 2 * it.hx);
+                           cur, d_partial, WV - 2 * it.hx, cover, nx);
         ZAMR_HIP(hipGetLastError());
         std::vector<double> part(it.n);
         ZAMR_HIP(hipMemcpyAsync(part.data(), d_partial, it.n * sizeof(double), hipMemcpyDeviceToHost, stream));

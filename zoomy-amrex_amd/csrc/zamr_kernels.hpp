@@ -250,18 +250,24 @@ struct Sys {
         return r;
     }
 
-    // Value of cell (c, j) as the face pass sees it: interior/ghost storage
-    // after UpdateState, or, outside the physical domain, the mirror-index
-    // interior cell after UpdateState pushed through Model::boundary_conditions
-    // (ZoomyAmr.cpp:385-441; corner ghosts take the x-side tag).
+    // Value of cell (c, j) as the face pass sees it (do_stage: UpdateState,
+    // then FillPatch, ZoomyAmr.cpp:563-572): the box's own cells after
+    // UpdateState; ghost cells inside the domain as the stage's FillPatch
+    // stored them (a neighbour box's updated value, or on a refined level the
+    // coarse interpolant, which UpdateState never touches); outside the
+    // physical domain the mirror-index cell's value, so obtained, pushed
+    // through Model::boundary_conditions (ZoomyAmr.cpp:385-441; corner
+    // ghosts take the x-side tag).
     __device__ static __forceinline__ VQ finish(const BoxView& B, const Raw& r, int c, int j, const VP& p, int clamp,
                                                 double time) {
         const bool ox = (c < B.dlo0) || (c > B.dhi0);
         const bool oy = (dim == 2) && ((j < B.dlo1) || (j > B.dhi1));
-        VQ v = update_state(r.v, p, clamp);
-        if (!ox && !oy) return v;
         int ci, cj;
         mirror(B, c, j, ci, cj);
+        const bool own = (ci >= B.lo0) && (ci <= B.hi0) && (cj >= B.lo1) && (cj <= B.hi1);
+        VQ v = update_state(r.v, p, clamp);
+        if (!own) v = r.v;
+        if (!ox && !oy) return v;
         // Side index -> tag by arithmetic on all four tags: a select between
         // two field loads would be folded into a load through a selected
         // address, which pins the descriptor in scratch memory.
@@ -923,37 +929,55 @@ __global__ void set_dt_kernel(GroupState* gs, int ngroups, double dt, double tim
 }
 
 // Same-level ghost exchange (FillPatchSingleLevel / FillBoundary): copy the
-// owning box's valid cells into another box's ghost cells.
+// owning box's valid cells into another box's ghost cells.  With `update`
+// the copy is of the owner's value after UpdateState (do_stage runs
+// UpdateState before FillPatch, ZoomyAmr.cpp:563-567), so that the stage
+// kernel can take every ghost cell as stored.
 struct CopyRect {
     int src_box, dst_box, ilo, jlo, ni, nj;
 };
+template <class M>
 __global__ void fill_boundary_kernel(const BoxDesc* __restrict__ boxes, const CopyRect* __restrict__ rects, int buf,
-                                     int nq) {
+                                     Params pp, int clamp, int update) {
+    using S = Sys<M>;
+    constexpr int nq = S::nq;
+    typename S::VP p;
+    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
     const CopyRect r = rects[blockIdx.x];
-    const BoxView S = view(boxes, r.src_box);
+    const BoxView Sb = view(boxes, r.src_box);
     const BoxView D = view(boxes, r.dst_box);
     const int ncell = r.ni * r.nj;
-    for (int k = threadIdx.x; k < ncell * nq; k += blockDim.x) {
-        const int n = k / ncell, cell = k % ncell;
+    const double* __restrict__ qs = qsel(Sb, buf);
+    double* __restrict__ qd = qsel(D, buf);
+    for (int cell = threadIdx.x; cell < ncell; cell += blockDim.x) {
         const int i = r.ilo + cell % r.ni, j = r.jlo + cell / r.ni;
-        const long long si = (long long)(i - S.gl0) + (long long)(j - S.gl1) * S.sy + n * S.sc;
-        const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy + n * D.sc;
-        qsel(D, buf)[di] = qsel(S, buf)[si];
+        const long long si = (long long)(i - Sb.gl0) + (long long)(j - Sb.gl1) * Sb.sy;
+        const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy;
+        typename S::VQ v;
+#pragma unroll
+        for (int n = 0; n < nq; ++n) v(n, 0) = qs[si + n * Sb.sc];
+        if (update) v = S::update_state(v, p, clamp);
+#pragma unroll
+        for (int n = 0; n < nq; ++n) qd[di + n * D.sc] = v(n, 0);
     }
 }
 
 // Qaux as the reference holds it after UpdateState (ZoomyAmr.cpp:328):
-// update_aux_variables(Q, Qaux_stored) over the valid cells.
+// update_aux_variables(Q, Qaux_stored) over the valid cells, at `time` or,
+// with gs, at the time of each box's group (the step's start when launched
+// before end_step).
 template <class M>
 __global__ void __launch_bounds__(NT) materialize_aux_kernel(const BoxDesc* __restrict__ boxes,
                                                              const WorkItem* __restrict__ items, int nitems, int src,
-                                                             Params pp, double time, int txw) {
+                                                             Params pp, double time0, int txw,
+                                                             const GroupState* __restrict__ gs) {
     using S = Sys<M>;
     typename S::VP p;
     for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
     ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
+        const double time = gs ? gs[B.group].time : time0;
         const int c = w.i0 + t;
         if (!(t < txw && c <= B.hi0)) continue;
         for (int j = w.j0; j < w.j0 + w.nj; ++j) {
@@ -997,11 +1021,14 @@ __global__ void __launch_bounds__(NT) update_state_kernel(const BoxDesc* __restr
 }
 
 // Per-item partial sums of h over valid cells, written in item order
-// (deterministic); the host adds the partials in order.
+// (deterministic); the host adds the partials in order.  Cells flagged in
+// `cover` (one byte per cell of the level, row length cover_nx) are left
+// out: on an AMR level they lie under the next finer level.
 template <class M>
 __global__ void __launch_bounds__(NT) mass_partial_kernel(const BoxDesc* __restrict__ boxes,
                                                           const WorkItem* __restrict__ items, int nitems, int src,
-                                                          double* partial, int txw) {
+                                                          double* partial, int txw,
+                                                          const unsigned char* __restrict__ cover, int cover_nx) {
     ZAMR_ITEM_LOOP(nitems) {
         const WorkItem w = items[it];
         const BoxView B = view(boxes, w.box);
@@ -1009,6 +1036,7 @@ __global__ void __launch_bounds__(NT) mass_partial_kernel(const BoxDesc* __restr
         double s = 0.0;
         if (t < txw && c <= B.hi0)
             for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+                if (cover && cover[(long long)j * cover_nx + c]) continue;
                 const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
                 s += qsel(B, src)[idx + 1 * B.sc];
             }

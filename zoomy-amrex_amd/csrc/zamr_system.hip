@@ -5,6 +5,7 @@
 #include "Model.H"
 #include "Numerics.H"
 #include "zamr.h"
+#include "zamr_amr.hpp"
 #include "zamr_driver.hpp"
 #include "zamr_plotfile.hpp"
 #include <memory>
@@ -19,12 +20,14 @@
 extern "C" void zamr_internal_march(int* dim, int* deg);
 
 using Sol = zamr::Solver<Model, Numerics>;
+using Amr = zamr::AmrDriver<Model, Numerics>;
 #if ZAMR_HAS_ADJOINT
 using AdjDrv = zamr::AdjointDriver<Model, Numerics, Adjoint>;
 #endif
 
 struct zamr_ctx {
     Sol s;
+    std::unique_ptr<Amr> amr;  // amr.max_level >= 1: the hierarchy over s (level 0)
     std::string err;
 #if ZAMR_HAS_ADJOINT
     std::unique_ptr<AdjDrv> adj;
@@ -33,7 +36,9 @@ struct zamr_ctx {
 #if ZAMR_HAS_ADJOINT
         adj.reset();  // before the solver frees its stream
 #endif
+        amr.reset();
     }
+    Sol& level(int lev) { return amr ? amr->level(lev) : s; }
 };
 
 namespace {
@@ -99,6 +104,58 @@ void zamr::Solver<M, N>::write_plotfile(const std::string& dir, int step, double
         throw Error(ZAMR_EIO, "cannot write plotfile " + path);
 }
 
+namespace {
+// Every box of level `lev` as a plot box of Q then Qaux rows.
+void plot_boxes(Sol& L, int lev, int ncomp, std::vector<zamr::PlotBox>& boxes, Amr* amr) {
+    const int nq = Model::n_dof_q, na = Model::n_dof_qaux;
+    std::vector<double> Q((size_t)nq * L.nx * L.ny), A((size_t)na * L.nx * L.ny);
+    if (amr) {
+        amr->copy_level(lev, 0, Q.data(), Q.size(), false);
+        amr->copy_level(lev, 1, A.data(), A.size(), false);
+    } else {
+        L.copy_level(0, 0, Q.data(), Q.size(), false);
+        L.copy_level(0, 1, A.data(), A.size(), false);
+    }
+    for (auto& b : L.hb) {
+        zamr::PlotBox pb;
+        const int bx = b.hi[0] - b.lo[0] + 1, by = b.hi[1] - b.lo[1] + 1;
+        pb.lo[0] = b.lo[0];
+        pb.lo[1] = b.lo[1];
+        pb.hi[0] = b.hi[0];
+        pb.hi[1] = b.hi[1];
+        pb.data.resize((size_t)ncomp * bx * by);
+        for (int cc = 0; cc < ncomp; ++cc)
+            for (int j = 0; j < by; ++j)
+                for (int i = 0; i < bx; ++i) {
+                    const size_t g = ((size_t)(j + b.lo[1])) * L.nx + (i + b.lo[0]);
+                    const double v = cc < nq ? Q[(size_t)cc * L.nx * L.ny + g]
+                                             : A[(size_t)(cc - nq) * L.nx * L.ny + g];
+                    pb.data[((size_t)cc * by + j) * bx + i] = v;
+                }
+        boxes.push_back(std::move(pb));
+    }
+}
+}  // namespace
+
+template <class M, class N>
+void zamr::AmrDriver<M, N>::write_plotfile(const std::string& dir, int step, double time) {
+    std::vector<std::string> names;
+    for (int n = 0; n < nq; ++n) names.push_back("var" + std::to_string(n));
+    for (int n = 0; n < na; ++n) names.push_back("aux" + std::to_string(n));
+    std::vector<zamr::PlotLevel> lv(finest_level() + 1);
+    for (int l = 0; l <= finest_level(); ++l) {
+        Sol& L = level(l);
+        lv[l].nx = L.nx;
+        lv[l].ny = L.ny;
+        lv[l].dx[0] = (L.phi[0] - L.plo[0]) / L.nx;
+        lv[l].dx[1] = (L.phi[1] - L.plo[1]) / L.ny;
+        plot_boxes(L, l, (int)names.size(), lv[l].boxes, this);
+    }
+    const std::string path = (dir.empty() ? std::string() : dir + "/") + plotfile_name(L0.identifier, step);
+    if (!write_plotfile_levels(path, names, time, L0.plo, L0.phi, ref_ratio, step, lv))
+        throw Error(ZAMR_EIO, "cannot write plotfile " + path);
+}
+
 extern "C" {
 
 int zamr_model_info(int* nq, int* na, int* np, int* dim, int* ntags) {
@@ -128,8 +185,12 @@ int zamr_create(const char* text, zamr_ctx** out) {
     *out = nullptr;
     auto* c = new zamr_ctx();
     zamr_internal_march(&c->s.c_cfl_dimension, &c->s.c_cfl_degree_factor);
-    int rc = guard(c, [&] { c->s.configure_deck(text); });
+    int rc = guard(c, [&] {
+        c->s.configure_deck(text);
+        if (c->s.max_level >= 1) c->amr = std::make_unique<Amr>(c->s, text);
+    });
     if (rc != ZAMR_OK) {
+        c->amr.reset();
         // keep the context so the caller can read the message, but nothing else works
         c->s.release();
     }
@@ -157,11 +218,26 @@ const char* zamr_last_error(const zamr_ctx* c) { return c ? c->err.c_str() : "nu
 
 int zamr_init_data(zamr_ctx* c) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.init_data(); });
+    return guard(c, [&] {
+        c->s.init_data();
+        if (c->amr) {
+            c->amr->init_levels();
+            c->amr->reset_clock();
+        }
+    });
 }
 
-int zamr_level_shape(const zamr_ctx* c, int lev, int* nx, int* ny, int* nboxes) {
+int zamr_level_shape(const zamr_ctx* cc, int lev, int* nx, int* ny, int* nboxes) {
+    auto* c = const_cast<zamr_ctx*>(cc);
     if (!c) return ZAMR_EINVAL;
+    if (c->amr) {
+        if (lev < 0 || lev > c->amr->max_level) return ZAMR_EINVAL;
+        const int r = lev == 0 ? 1 : c->amr->ref_ratio;
+        if (nx) *nx = c->s.nx * r;
+        if (ny) *ny = c->s.ny * r;
+        if (nboxes) *nboxes = lev <= c->amr->finest_level() ? (int)c->amr->level(lev).hb.size() : 0;
+        return ZAMR_OK;
+    }
     if (c->s.batched) {
         if (lev < 0 || lev >= (int)c->s.hb.size()) return ZAMR_EINVAL;
         auto& b = c->s.hb[lev];
@@ -178,16 +254,26 @@ int zamr_level_shape(const zamr_ctx* c, int lev, int* nx, int* ny, int* nboxes) 
 
 int zamr_set_state(zamr_ctx* c, int lev, int field, const double* host, size_t n) {
     if (!c || !host || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.copy_level(lev, field, const_cast<double*>(host), n, true); });
+    return guard(c, [&] {
+        if (c->amr) c->amr->copy_level(lev, field, const_cast<double*>(host), n, true);
+        else c->s.copy_level(lev, field, const_cast<double*>(host), n, true);
+    });
 }
 int zamr_get_state(zamr_ctx* c, int lev, int field, double* host, size_t n) {
     if (!c || !host || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.copy_level(lev, field, host, n, false); });
+    return guard(c, [&] {
+        if (c->amr) c->amr->copy_level(lev, field, host, n, false);
+        else c->s.copy_level(lev, field, host, n, false);
+    });
 }
 
 int zamr_compute_dt(zamr_ctx* c, int lev, double* dt) {
     if (!c || !dt || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
+        if (c->amr) {
+            *dt = c->amr->level(lev).compute_dt_host();
+            return;
+        }
         c->s.level_box_check(lev);
         *dt = c->s.compute_dt_host();
     });
@@ -204,7 +290,7 @@ int zamr_fill_patch(zamr_ctx* c, int lev, double time) {
     return guard(c, [&] {
         c->s.level_box_check(lev);
         (void)time;
-        c->s.fill_boundary(c->s.cur);
+        c->s.fill_boundary(c->s.cur, false);
         ZAMR_HIP(hipStreamSynchronize(c->s.stream));
     });
 }
@@ -224,6 +310,13 @@ int zamr_advance(zamr_ctx* c, int lev, double time, double dt) {
 }
 int zamr_time_step(zamr_ctx* c, int lev, double time, int iteration, double* dt_taken) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (c->amr) {
+        return guard(c, [&] {
+            if (lev != 0) throw zamr::Error(ZAMR_EINVAL, "TimeStep is driven from level 0");
+            const double dt = c->amr->time_step(time, iteration);
+            if (dt_taken) *dt_taken = dt;
+        });
+    }
     (void)iteration;
     return guard(c, [&] {
         c->s.level_box_check(lev);
@@ -233,37 +326,48 @@ int zamr_time_step(zamr_ctx* c, int lev, double time, int iteration, double* dt_
 }
 int zamr_run_steps(zamr_ctx* c, int nsteps, int* done, double* time) {
     if (!c || !c->s.d_pool || nsteps < 0) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.run_steps(nsteps, done, time); });
+    return guard(c, [&] {
+        if (c->amr) c->amr->run_steps(nsteps, done, time);
+        else c->s.run_steps(nsteps, done, time);
+    });
 }
 int zamr_evolve(zamr_ctx* c) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.evolve(); });
+    return guard(c, [&] {
+        if (c->amr) c->amr->evolve();
+        else c->s.evolve();
+    });
 }
 int zamr_total_mass(zamr_ctx* c, double* m) {
     if (!c || !m || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { *m = c->s.total_mass(); });
+    return guard(c, [&] { *m = c->amr ? c->amr->total_mass() : c->s.total_mass(); });
 }
 int zamr_get_time(const zamr_ctx* cc, int lev, double* time, int* step, double* dt) {
     auto* c = const_cast<zamr_ctx*>(cc);
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
-        c->s.download_groups();
+        Sol& L = c->amr ? c->amr->level(lev) : c->s;
+        L.download_groups();
         const int g = c->s.batched ? lev : 0;
-        if (g < 0 || g >= c->s.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
-        if (time) *time = c->s.h_groups[g].time;
-        if (step) *step = c->s.h_groups[g].step;
-        if (dt) *dt = c->s.h_groups[g].dt;
+        if (g < 0 || g >= L.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
+        if (time) *time = L.h_groups[g].time;
+        if (step) *step = L.h_groups[g].step;
+        if (dt) *dt = L.h_groups[g].dt;
     });
 }
 int zamr_reset_clock(zamr_ctx* c) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
-    return guard(c, [&] { c->s.reset_clock(); });
+    return guard(c, [&] {
+        if (c->amr) c->amr->reset_clock();
+        else c->s.reset_clock();
+    });
 }
 int zamr_write_plotfile(zamr_ctx* c, const char* dir, int step, double time) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
         if (c->s.batched) throw zamr::Error(ZAMR_EUNSUP, "plotfiles are written for single-level runs only");
-        c->s.write_plotfile(dir ? dir : "", step, time);
+        if (c->amr) c->amr->write_plotfile(dir ? dir : "", step, time);
+        else c->s.write_plotfile(dir ? dir : "", step, time);
     });
 }
 int zamr_device_ptr(zamr_ctx* c, int lev, int box, int field, void** ptr, long long* rs, long long* cs, int* ng,
@@ -271,20 +375,23 @@ int zamr_device_ptr(zamr_ctx* c, int lev, int box, int field, void** ptr, long l
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
         int b = box;
+        Sol* L = &c->s;
         if (c->s.batched) {
             b = lev;
+        } else if (c->amr) {
+            L = &c->amr->level(lev);
         } else if (lev != 0) {
             throw zamr::Error(ZAMR_EINVAL, "only level 0 exists in this build");
         }
-        if (b < 0 || b >= (int)c->s.hb.size()) throw zamr::Error(ZAMR_EINVAL, "box index out of range");
+        if (b < 0 || b >= (int)L->hb.size()) throw zamr::Error(ZAMR_EINVAL, "box index out of range");
         if (field != 0 && field != 1) throw zamr::Error(ZAMR_EINVAL, "field must be 0 (Q) or 1 (Qaux)");
-        auto& B = c->s.hb[b];
-        if (field == 1) c->s.ensure_aux();
-        if (ptr) *ptr = c->s.d_pool + (field == 0 ? B.off_q[c->s.cur] : B.off_aux);
+        auto& B = L->hb[b];
+        if (field == 1) L->ensure_aux();
+        if (ptr) *ptr = L->d_pool + (field == 0 ? B.off_q[L->cur] : B.off_aux);
         if (rs) *rs = B.gx;
         if (cs) *cs = (long long)B.gx * B.gy;
         if (ng) *ng = B.ng;
-        if (stream) *stream = (void*)c->s.stream;
+        if (stream) *stream = (void*)L->stream;
     });
 }
 int zamr_set_stream(zamr_ctx* c, void* stream) {
@@ -295,6 +402,7 @@ int zamr_set_stream(zamr_ctx* c, void* stream) {
         c->s.own_stream = false;
         c->s.stream = (hipStream_t)stream;
         c->s.invalidate_graph();
+        if (c->amr && c->amr->L1) c->amr->L1->stream = (hipStream_t)stream;
     });
 }
 int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
@@ -303,6 +411,11 @@ int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
     c->s.per_box_launch = per_box;
     c->s.xcd_remap = xcd;
     c->s.invalidate_graph();
+    if (c->amr && c->amr->L1) {
+        c->amr->L1->rows_per_item = rows;
+        c->amr->L1->per_box_launch = per_box;
+        c->amr->L1->xcd_remap = xcd;
+    }
     return ZAMR_OK;
 }
 int zamr_set_profiling(zamr_ctx* c, int on) {
@@ -322,10 +435,12 @@ int zamr_kernel_times(zamr_ctx* c, double* out, int* n) {
 int zamr_mood_stats(zamr_ctx* c, int group, int* troubled, long long* redos) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
-        if (group < 0 || group >= c->s.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
-        c->s.download_groups();
-        if (troubled) *troubled = c->s.h_groups[group].troubled;
-        if (redos) *redos = c->s.h_groups[group].redos;
+        Sol& L = c->amr ? c->amr->level(group) : c->s;
+        const int g = c->amr ? 0 : group;
+        if (g < 0 || g >= L.ngroups) throw zamr::Error(ZAMR_EINVAL, "group index out of range");
+        L.download_groups();
+        if (troubled) *troubled = L.h_groups[g].troubled;
+        if (redos) *redos = L.h_groups[g].redos;
     });
 }
 
@@ -348,11 +463,85 @@ int zamr_dt_history(zamr_ctx* c, int cap, double* dts, double* times, int* n) {
     });
 }
 
+int zamr_finest_level(zamr_ctx* c, int* finest) {
+    if (!c || !finest || !c->s.d_pool) return ZAMR_EINVAL;
+    *finest = c->amr ? c->amr->finest_level() : 0;
+    return ZAMR_OK;
+}
+
+int zamr_level_boxes(zamr_ctx* c, int lev, int* lohi, int cap, int* nboxes) {
+    if (!c || !c->s.d_pool || cap < 0) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        const bool exists = c->amr ? lev >= 0 && lev <= c->amr->finest_level() : lev == 0;
+        if (!exists) {
+            if (c->amr && lev >= 0 && lev <= c->amr->max_level) {
+                if (nboxes) *nboxes = 0;
+                return;
+            }
+            throw zamr::Error(ZAMR_EINVAL, "level index out of range");
+        }
+        Sol& L = c->level(lev);
+        const int n = (int)L.hb.size();
+        for (int k = 0; k < n && k < cap && lohi; ++k) {
+            lohi[4 * k + 0] = L.hb[k].lo[0];
+            lohi[4 * k + 1] = L.hb[k].lo[1];
+            lohi[4 * k + 2] = L.hb[k].hi[0];
+            lohi[4 * k + 3] = L.hb[k].hi[1];
+        }
+        if (nboxes) *nboxes = n;
+    });
+}
+
+int zamr_init_levels(zamr_ctx* c) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->amr) {
+        c->err = "zamr_init_levels: amr.max_level is 0";
+        return ZAMR_EUNSUP;
+    }
+    return guard(c, [&] {
+        c->amr->init_levels();
+        c->amr->reset_clock();
+    });
+}
+
+int zamr_regrid(zamr_ctx* c, double time) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->amr) {
+        c->err = "zamr_regrid: amr.max_level is 0";
+        return ZAMR_EUNSUP;
+    }
+    return guard(c, [&] {
+        auto& A = *c->amr;
+        A.regrid(A.l0_stage_ran ? c->s.last_in : -1, A.t_last_stage, time);
+    });
+}
+
+int zamr_error_est(zamr_ctx* c, unsigned char* tags, size_t n) {
+    if (!c || !tags || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->amr) {
+        c->err = "zamr_error_est: amr.max_level is 0";
+        return ZAMR_EUNSUP;
+    }
+    return guard(c, [&] {
+        auto& A = *c->amr;
+        if (n != (size_t)c->s.nx * c->s.ny) throw zamr::Error(ZAMR_EINVAL, "tag array must hold nx*ny bytes");
+        A.error_est_raw(A.l0_stage_ran ? c->s.last_in : -1, A.t_last_stage, tags);
+    });
+}
+
+int zamr_amr_stats(zamr_ctx* c, long long* regrids, long long* remakes) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (regrids) *regrids = c->amr ? c->amr->regrids : 0;
+    if (remakes) *remakes = c->amr ? c->amr->remakes : 0;
+    return ZAMR_OK;
+}
+
 #if ZAMR_HAS_ADJOINT
 int zamr_adj_setup(zamr_ctx* c, int nsteps, int seg_steps, const double* dts, int ngauges, const int* gi,
                    const int* gj) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     return guard(c, [&] {
+        if (c->amr) throw zamr::Error(ZAMR_EUNSUP, "adjoint: single-level runs only (amr.max_level = 0)");
         if (!c->adj) c->adj = std::make_unique<AdjDrv>(c->s);
         c->adj->setup(nsteps, seg_steps, dts, ngauges, gi, gj);
     });

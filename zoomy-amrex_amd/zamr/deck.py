@@ -75,14 +75,24 @@ def write_inputs(path, ncell, dim_mesh, tend, order, plot_dt, cfl=0.45, bc="extr
     return text
 
 
-def deck_from_case(case):
+AMR_KEYS = dict(max_level=1, ref_ratio=2, blocking_factor=2, n_error_buf=1, grid_eff=0.7, threshold=0.01,
+                b_max=1.0e30)
+
+
+def deck_from_case(case, amr=None):
     """Deck text for a case dictionary (keys as in the oracle's Python handle:
     nx, ny, prob_lo, prob_hi, max_grid_size, spatial_order, implicit_source,
     well_balanced, clamp_positivity, positivity, cfl, dtmin, dtmax, time_end,
-    max_step, bc, params)."""
+    max_step, bc, params).  `amr` (a dict over AMR_KEYS) adds the hierarchy:
+    amr.max_level, ref_ratio, blocking_factor, n_error_buf, grid_eff and
+    tagging.threshold / tagging.b_max."""
     c = dict(case)
     bc = c.get("bc", ("", "", "", ""))
-    lines = ["amr.max_level = 0",
+    a = None
+    if amr is not None:
+        a = dict(AMR_KEYS)
+        a.update(amr)
+    lines = [f"amr.max_level = {a['max_level'] if a else 0}",
              f"amr.n_cell = {c['nx']} {c['ny']}",
              f"amr.max_grid_size = {c.get('max_grid_size', 64)}",
              f"geometry.prob_lo = {c.get('prob_lo', (0.0, 0.0))[0]!r} {c.get('prob_lo', (0.0, 0.0))[1]!r}",
@@ -102,4 +112,8 @@ def deck_from_case(case):
             lines.append(f"bc.{side} = {name}")
     for k, v in (c.get("params") or {}).items():
         lines.append(f"params.{k} = {v!r}")
+    if a:
+        lines += [f"amr.ref_ratio = {a['ref_ratio']}", f"amr.blocking_factor = {a['blocking_factor']}",
+                  f"amr.n_error_buf = {a['n_error_buf']}", f"amr.grid_eff = {a['grid_eff']!r}",
+                  f"tagging.threshold = {a['threshold']!r}", f"tagging.b_max = {a['b_max']!r}"]
     return "\n".join(lines) + "\n"

@@ -72,6 +72,12 @@ def load_library(path):
         "zamr_kernel_times": [_VP, _D, _I],
         "zamr_dt_history": [_VP, ctypes.c_int, _D, _D, _I],
         "zamr_mood_stats": [_VP, ctypes.c_int, _I, ctypes.POINTER(ctypes.c_longlong)],
+        "zamr_finest_level": [_VP, _I],
+        "zamr_level_boxes": [_VP, ctypes.c_int, _I, ctypes.c_int, _I],
+        "zamr_init_levels": [_VP],
+        "zamr_regrid": [_VP, ctypes.c_double],
+        "zamr_error_est": [_VP, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_size_t],
+        "zamr_amr_stats": [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
         "zamr_adj_setup": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I, _I],
         "zamr_adj_forward": [_VP, _D],
         "zamr_adj_backward": [_VP, _D],
@@ -160,12 +166,46 @@ class ZoomyAmr:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self._check(self.lib.zamr_set_state(self.ctx, lev, field, _dp(a), a.size))
 
-    def get_state(self, field=0, lev=0):
+    def get_state(self, field=0, lev=0, fill=np.nan):
+        """Dense (ncomp, ny, nx) copy of level `lev`; on a refined level the
+        cells no box covers hold `fill`."""
         nx, ny, _ = self.shape(lev)
         nc = self.nq if field == 0 else self.na
-        a = np.empty((nc, ny, nx))
+        a = np.full((nc, ny, nx), fill) if lev > 0 else np.empty((nc, ny, nx))
         self._check(self.lib.zamr_get_state(self.ctx, lev, field, _dp(a), a.size))
         return a
+
+    # ---- AMR hierarchy ----
+    def finest_level(self):
+        f = ctypes.c_int()
+        self._check(self.lib.zamr_finest_level(self.ctx, ctypes.byref(f)))
+        return f.value
+
+    def level_boxes(self, lev):
+        """Boxes of level `lev` as (lo0, lo1, hi0, hi1) tuples."""
+        n = ctypes.c_int()
+        self._check(self.lib.zamr_level_boxes(self.ctx, lev, None, 0, ctypes.byref(n)))
+        out = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+        self._check(self.lib.zamr_level_boxes(self.ctx, lev, out.ctypes.data_as(_I), n.value, ctypes.byref(n)))
+        return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n.value)]
+
+    def init_levels(self):
+        self._check(self.lib.zamr_init_levels(self.ctx))
+
+    def regrid(self, time=0.0):
+        self._check(self.lib.zamr_regrid(self.ctx, time))
+
+    def error_est(self):
+        nx, ny, _ = self.shape(0)
+        t = np.zeros((ny, nx), dtype=np.uint8)
+        self._check(self.lib.zamr_error_est(self.ctx, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), t.size))
+        return t
+
+    def amr_stats(self):
+        """(regrids, levels remade) since creation."""
+        r, m = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.zamr_amr_stats(self.ctx, ctypes.byref(r), ctypes.byref(m)))
+        return r.value, m.value
 
     def compute_dt(self, lev=0):
         dt = ctypes.c_double()

@@ -114,14 +114,10 @@ def run_case(system, settings, output_dir, ic=None, bc_sides=None, contract=True
 _BOX = re.compile(r"\(\(([-\d,]+)\) \(([-\d,]+)\)")
 
 
-def read_plotfile(path):
-    """(ncomp, ny, nx) level-0 array, variable names and time of a plotfile."""
-    path = pathlib.Path(path)
-    lines = (path / "Header").read_text().splitlines()
-    ncomp = int(lines[1])
-    names = lines[2:2 + ncomp]
-    time = float(lines[2 + ncomp + 1])
-    txt = (path / "Level_0" / "Cell_H").read_text()
+def _read_level(lvl_dir, ncomp):
+    """(ncomp, ny, nx) array over the level's box bounding box (NaN where no
+    box), its (ilo, jlo, nx, ny) and the boxes (lo0, lo1, hi0, hi1)."""
+    txt = (lvl_dir / "Cell_H").read_text()
     boxes = [([int(v) for v in a.split(",")], [int(v) for v in b.split(",")]) for a, b in _BOX.findall(txt)]
     fods = re.findall(r"FabOnDisk:\s+(\S+)\s+(\d+)", txt)
     ilo = min(lo[0] for lo, _ in boxes)
@@ -131,10 +127,43 @@ def read_plotfile(path):
     out = np.full((ncomp, ny, nx), np.nan)
     for (lo, hi), (fname, off) in zip(boxes, fods):
         bx, by = hi[0] - lo[0] + 1, hi[1] - lo[1] + 1
-        with open(path / "Level_0" / fname, "rb") as fb:
+        with open(lvl_dir / fname, "rb") as fb:
             fb.seek(int(off))
             fb.readline()
             n = bx * by * ncomp
             data = np.array(struct.unpack("<%dd" % n, fb.read(8 * n))).reshape(ncomp, by, bx)
         out[:, lo[1] - jlo:hi[1] - jlo + 1, lo[0] - ilo:hi[0] - ilo + 1] = data
+    return out, (ilo, jlo, nx, ny), [(lo[0], lo[1], hi[0], hi[1]) for lo, hi in boxes]
+
+
+def read_plotfile(path):
+    """(ncomp, ny, nx) level-0 array, variable names and time of a plotfile."""
+    path = pathlib.Path(path)
+    lines = (path / "Header").read_text().splitlines()
+    ncomp = int(lines[1])
+    names = lines[2:2 + ncomp]
+    time = float(lines[2 + ncomp + 1])
+    out, _, _ = _read_level(path / "Level_0", ncomp)
     return out, names, time
+
+
+def read_levels(path):
+    """Every level of a plotfile, as deliverable.read_levels (deliverable.py:
+    114-129) returns them: a list of dicts arr / ext / boxes / ref
+    (cumulative refinement), and the header fields ncomp, names,
+    finest_level, prob_lo, prob_hi, refs, time."""
+    path = pathlib.Path(path)
+    lines = (path / "Header").read_text().splitlines()
+    ncomp = int(lines[1])
+    i = 2 + ncomp
+    H = dict(ncomp=ncomp, names=lines[2:i], dim=int(lines[i]), time=float(lines[i + 1]),
+             finest_level=int(lines[i + 2]), prob_lo=[float(v) for v in lines[i + 3].split()],
+             prob_hi=[float(v) for v in lines[i + 4].split()],
+             refs=[int(v) for v in lines[i + 5].split()])
+    levels, cum = [], 1
+    for lev in range(H["finest_level"] + 1):
+        if lev > 0:
+            cum *= H["refs"][lev - 1] if lev - 1 < len(H["refs"]) else 2
+        arr, ext, boxes = _read_level(path / f"Level_{lev}", ncomp)
+        levels.append(dict(arr=arr, ext=ext, boxes=boxes, ref=cum))
+    return levels, H

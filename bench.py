@@ -188,6 +188,65 @@ def bench_sme(steps=50, warmup=3):
             "stage1_frac": (16 * 8 * cells / t1) / HBM_PEAK if t1 > 0 else 0.0}
 
 
+def bench_amr(steps=10, warmup=2, n=2048):
+    """The two-level AMR advance (amr.max_level = 1) on seeded water columns
+    over a dry bed, n^2 at level 0 (level 1 in (2n)^2 index space where the
+    water is): ms per coarse step (regrid, level-0 step, two level-1
+    substeps, average_down), composite cell-updates/s per RK stage (level-0
+    cells + 2 x level-1 cells per coarse step), the host share of regrid, and
+    the same two level-1 steps on the uniform (2n)^2 grid for comparison."""
+    import numpy as np
+    import torch
+    from zamr import cases
+    from zamr.deck import deck_from_case
+    from zamr.lib import ZoomyAmr
+
+    amr = dict(max_level=1, ref_ratio=2, blocking_factor=2, n_error_buf=1, grid_eff=0.7, threshold=0.02,
+               b_max=1.0e30)
+    case, Q0 = cases.dry_columns(n=n, seed=0, steps=warmup + steps + 1)
+    z = ZoomyAmr(deck_from_case(case, amr=amr), system="swe2d", dim=2, order=2)
+    z.init_data()
+    z.set_state(Q0)
+    z.init_levels()
+    z.run_steps(warmup)
+    torch.cuda.synchronize()
+    r0, m0, ms0 = z.amr_stats(with_time=True)
+    fine_cells = []
+    t_total = 0.0
+    for _ in range(steps):
+        fine_cells.append(sum((b[2] - b[0] + 1) * (b[3] - b[1] + 1) for b in z.level_boxes(1)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done, _ = z.run_steps(1)
+        torch.cuda.synchronize()
+        t_total += time.perf_counter() - t0
+        assert done == 1
+    r1, m1, ms1 = z.amr_stats(with_time=True)
+    assert np.isfinite(z.get_state()).all()
+    upd = 2 * sum(n * n + 2 * f for f in fine_cells)
+    frac = float(np.mean(fine_cells)) / (4 * n * n)
+    del z
+    # the same two fine steps on the uniform grid
+    cu, Qu = cases.dry_columns(n=2 * n, seed=0, steps=2 * (steps + warmup) + 1)
+    u = ZoomyAmr(deck_from_case(cu), system="swe2d", dim=2, order=2)
+    u.init_data()
+    u.set_state(Qu)
+    u.run_steps(2 * warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u.run_steps(2 * steps)
+    torch.cuda.synchronize()
+    t_uni = time.perf_counter() - t0
+    del u
+    return {"workload": f"2-D SWE two-level AMR, {n}x{n} level 0, ref_ratio 2, 24 seeded water columns over a dry "
+                        "bed, walls, Manning 0.03 point-implicit, MOOD, order 2, regrid every coarse step",
+            "value": upd / t_total, "unit": "cell-updates/s", "ms_per_coarse_step": 1e3 * t_total / steps,
+            "steps": steps, "level1_fraction_of_domain": frac, "regrids": r1 - r0, "remakes": m1 - m0,
+            "regrid_host_ms_per_step": (ms1 - ms0) / steps,
+            "uniform_fine_ms_per_coarse_step": 1e3 * t_uni / steps,
+            "speedup_vs_uniform_fine": t_uni / t_total}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +255,7 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme"],
+    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme", "amr"],
                     help="swe4096: the single-level 4096^2 workload (default); boxlist: 600 AMR patches + "
                          "64 ensemble members, batched launch vs one launch per box; adjoint: the 1024^2 "
                          "gauge-inversion gradient alone")
@@ -221,6 +280,10 @@ def main():
     if args.workload == "sme":
         print(json.dumps({"metric": "cell-updates/s per RK stage (FP64) + % HBM roofline", "n_gpus": 1,
                           "dtype": "f64", "data": "synthetic", "higher_is_better": True, **bench_sme()}), flush=True)
+        return
+    if args.workload == "amr":
+        print(json.dumps({"metric": "cell-updates/s per RK stage (FP64), two-level AMR", "n_gpus": 1,
+                          "dtype": "f64", "data": "synthetic", "higher_is_better": True, **bench_amr()}), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"metric": "adjoint/forward time ratio", "unit": "x", "higher_is_better": False,

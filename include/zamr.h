@@ -162,8 +162,9 @@ int zamr_regrid(zamr_ctx* ctx, double time);
 /* ZoomyAmr::ErrorEst (ZoomyAmr.cpp:225-253) on level 0 as it stands: one byte
  * per cell (1 = tagged), raster order, n = nx * ny. */
 int zamr_error_est(zamr_ctx* ctx, unsigned char* tags, size_t n);
-/* Regrids run and levels remade since creation. */
-int zamr_amr_stats(zamr_ctx* ctx, long long* regrids, long long* remakes);
+/* Regrids run, levels remade, and the host time spent regridding (ms), since
+ * creation. */
+int zamr_amr_stats(zamr_ctx* ctx, long long* regrids, long long* remakes, double* regrid_ms);
 
 /* (dt, time after the step) of the steps taken on the device clock since the
  * last zamr_reset_clock, up to cap entries — the values the reference prints

@@ -77,10 +77,14 @@ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b 
 //   sx = central difference, limited by the one-sided differences times 2;
 //   one factor per direction, the least over components of limited/central;
 //   fine = crse + xoff * (sx * fx) + yoff * (sy * fy), xoff, yoff = -+1/4.
+// With `cover` (one byte per level-0 cell, set under every fine box) cells
+// that a fine box covers are skipped: FillPatch takes those from the fine
+// level itself.
 template <class M>
 __global__ void __launch_bounds__(256) cf_interp_kernel(const BoxDesc* __restrict__ fine,
                                                         const CellRect* __restrict__ rects, int nrects,
-                                                        long long total, CoarseGrid g, int cbuf, unsigned bufmask) {
+                                                        long long total, CoarseGrid g, int cbuf, unsigned bufmask,
+                                                        const unsigned char* __restrict__ cover) {
 #pragma clang fp contract(off)
     constexpr int nq = M::n_dof_q;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,6 +93,7 @@ __global__ void __launch_bounds__(256) cf_interp_kernel(const BoxDesc* __restric
     const long long cell = k - R.start;
     const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
     const int ic = i >> 1, jc = j >> 1;
+    if (cover && cover[(long long)jc * g.nx + ic]) return;
     double u0[nq], sx[nq], sy[nq];
     double fx = 1.0, fy = 1.0;
 #pragma unroll
@@ -221,9 +226,11 @@ __global__ void __launch_bounds__(256) error_est_kernel(CoarseGrid g, int cur, i
 
 // TagBoxArray buffer + coarsen: coarsened cell (I, J) of factor bf is tagged
 // when a tag lies within nbuf cells (in both directions) of its bf x bf block.
+// `changed` is set when the map differs from `prev` (the last one clustered).
 __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __restrict__ tags, int nx, int ny,
                                                          int nbuf, int bf, unsigned char* __restrict__ out, int cnx,
-                                                         int cny) {
+                                                         int cny, const unsigned char* __restrict__ prev,
+                                                         unsigned* __restrict__ changed) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)cnx * cny) return;
     const int I = (int)(k % cnx), J = (int)(k / cnx);
@@ -237,6 +244,63 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
                 break;
             }
     out[k] = t;
+    if (t != prev[k]) atomicOr(changed, 1u);
+}
+
+// Row and column tag counts of a list of regions of the buffered tag map:
+// hist[offx[r] + i - lo0] counts column i, hist[offy[r] + j - lo1] row j of
+// region r.  One 256-thread block per 64 x 64 tile of a region (tile[r] is
+// the first tile of region r); partial sums in LDS, one atomic per row and
+// column of the tile.
+struct Region {
+    int lo0, lo1, hi0, hi1;
+    int offx, offy, tile;
+};
+__global__ void __launch_bounds__(256) cluster_hist_kernel(const unsigned char* __restrict__ map, int cnx,
+                                                           const Region* __restrict__ reg, int nreg,
+                                                           int* __restrict__ hist) {
+    __shared__ int cs[64], rs[64];
+    int lo = 0, hi = nreg - 1;
+    while (lo < hi) {  // the region owning this tile
+        const int mid = (lo + hi + 1) >> 1;
+        if (reg[mid].tile <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const Region R = reg[lo];
+    const int w = R.hi0 - R.lo0 + 1, h = R.hi1 - R.lo1 + 1;
+    const int tw = (w + 63) / 64;
+    const int t = (int)blockIdx.x - R.tile;
+    const int i0 = R.lo0 + (t % tw) * 64, j0 = R.lo1 + (t / tw) * 64;
+    if (threadIdx.x < 64) {
+        cs[threadIdx.x] = 0;
+        rs[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int ci = threadIdx.x & 63;
+    const int i = i0 + ci;
+    int csum = 0;
+    for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+        const int j = j0 + r;
+        if (i <= R.hi0 && j <= R.hi1 && map[(long long)j * cnx + i]) {
+            ++csum;
+            atomicAdd(&rs[r], 1);
+        }
+    }
+    if (csum) atomicAdd(&cs[ci], csum);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (cs[threadIdx.x]) atomicAdd(&hist[R.offx + (i0 - R.lo0) + threadIdx.x], cs[threadIdx.x]);
+    } else if (threadIdx.x < 128) {
+        const int r = threadIdx.x - 64;
+        if (rs[r]) atomicAdd(&hist[R.offy + (j0 - R.lo1) + r], rs[r]);
+    }
+}
+
+// The level-0 cells under the fine boxes (level-0 index space boxes).
+__global__ void cover_paint_kernel(const int4* __restrict__ boxes, unsigned char* __restrict__ cover, int nx) {
+    const int4 b = boxes[blockIdx.x];
+    const int w = b.z - b.x + 1, n = w * (b.w - b.y + 1);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) cover[(long long)(b.y + k / w) * nx + b.x + k % w] = 1;
 }
 
 __global__ void clear_max_kernel(GroupState* gs, int ngroups) {
@@ -248,27 +312,7 @@ __global__ void clear_max_kernel(GroupState* gs, int ngroups) {
 // (TimeStep(lev + 1, time + (i - 1) * dt_level[lev + 1]), ZoomyAmr.cpp:756-757).
 __global__ void sync_clock_kernel(const GroupState* coarse, GroupState* fine) { fine->time = coarse->time; }
 
-// ---------------- host: Berger-Rigoutsos clustering ----------------
-// Points live in one array; a cluster is a [begin, end) range of it and its
-// bounding box.  Splitting partitions the range in place (as Cluster::chop
-// does with std::partition); the final cluster boxes depend only on the
-// point set.
-struct ClusterRange {
-    size_t b, e;
-    int lo[2], hi[2];
-};
-
-inline void cluster_bound(const std::vector<int>& px, const std::vector<int>& py, ClusterRange& c) {
-    c.lo[0] = c.hi[0] = px[c.b];
-    c.lo[1] = c.hi[1] = py[c.b];
-    for (size_t k = c.b + 1; k < c.e; ++k) {
-        c.lo[0] = std::min(c.lo[0], px[k]);
-        c.hi[0] = std::max(c.hi[0], px[k]);
-        c.lo[1] = std::min(c.lo[1], py[k]);
-        c.hi[1] = std::max(c.hi[1], py[k]);
-    }
-}
-
+// ---------------- host: Berger-Rigoutsos cut selection ----------------
 // FindCut (AMReX_Cluster.cpp): status 0 hole, 1 inflection, 2 bisection,
 // 3 none; returns the cut index (cells below it stay).
 inline int br_find_cut(const int* hist, int len, int lo, int& status) {
@@ -309,66 +353,6 @@ inline int br_find_cut(const int* hist, int len, int lo, int& status) {
     return lo + cut;
 }
 
-// ClusterList::chop to efficiency `eff` over the points (px[k], py[k]);
-// returns the cluster boxes (lo0, lo1, hi0, hi1).
-inline std::vector<std::array<int, 4>> br_cluster(std::vector<int>& px, std::vector<int>& py, double eff) {
-    std::vector<std::array<int, 4>> out;
-    if (px.empty()) return out;
-    std::vector<ClusterRange> work;
-    ClusterRange all{0, px.size(), {0, 0}, {0, 0}};
-    cluster_bound(px, py, all);
-    work.push_back(all);
-    std::vector<int> hist[2];
-    while (!work.empty()) {
-        ClusterRange c = work.back();
-        work.pop_back();
-        const long long n = (long long)(c.e - c.b);
-        const double vol = (double)(c.hi[0] - c.lo[0] + 1) * (double)(c.hi[1] - c.lo[1] + 1);
-        if ((double)n / vol >= eff) {
-            out.push_back({c.lo[0], c.lo[1], c.hi[0], c.hi[1]});
-            continue;
-        }
-        int len[2], cut[2], st[2];
-        for (int d = 0; d < 2; ++d) {
-            len[d] = c.hi[d] - c.lo[d] + 1;
-            hist[d].assign(len[d], 0);
-        }
-        for (size_t k = c.b; k < c.e; ++k) {
-            hist[0][px[k] - c.lo[0]]++;
-            hist[1][py[k] - c.lo[1]]++;
-        }
-        int best = 3;
-        for (int d = 0; d < 2; ++d) {
-            cut[d] = br_find_cut(hist[d].data(), len[d], c.lo[d], st[d]);
-            best = std::min(best, st[d]);
-        }
-        int dir = -1, minlen = -1;
-        for (int d = 0; d < 2; ++d) {
-            if (st[d] != best) continue;
-            const int l = std::min(cut[d] - c.lo[d], c.hi[d] - cut[d]);
-            if (l >= minlen) {
-                dir = d;
-                minlen = l;
-            }
-        }
-        // partition [b, e) into coordinate < cut, then the rest
-        std::vector<int>& pc = dir == 0 ? px : py;
-        size_t m = c.b;
-        for (size_t k = c.b; k < c.e; ++k)
-            if (pc[k] < cut[dir]) {
-                std::swap(px[k], px[m]);
-                std::swap(py[k], py[m]);
-                ++m;
-            }
-        ClusterRange a{c.b, m, {0, 0}, {0, 0}}, b{m, c.e, {0, 0}, {0, 0}};
-        cluster_bound(px, py, a);
-        cluster_bound(px, py, b);
-        work.push_back(b);
-        work.push_back(a);
-    }
-    return out;
-}
-
 // ---------------- host: the hierarchy ----------------
 template <class M, class N>
 class AmrDriver {
@@ -387,12 +371,21 @@ class AmrDriver {
     int* d_xbox = nullptr;
     int* d_ybox = nullptr;
     int nbx = 0;
-    unsigned char* d_tags = nullptr;   // ErrorEst, one byte per level-0 cell
-    unsigned char* d_ctags = nullptr;  // buffered and coarsened
-    unsigned char* d_cover = nullptr;  // level-0 cells under level 1
+    unsigned char* d_tags = nullptr;      // ErrorEst, one byte per level-0 cell
+    unsigned char* d_ctags[2] = {nullptr, nullptr};  // buffered and coarsened: current, last clustered
+    unsigned* d_changed = nullptr;
+    bool have_last = false;               // d_ctags[1] holds the map the current grids came from
+    unsigned char* d_cover = nullptr;     // level-0 cells under level 1
     int cnx = 0, cny = 0, bf_lev = 1;
-    std::vector<unsigned char> h_ctags, last_ctags;
     std::vector<std::array<int, 4>> last_boxes;
+    // clustering scratch: regions and their histograms (device and host)
+    Region* d_reg = nullptr;
+    int* d_hist = nullptr;
+    size_t reg_cap = 0, hist_cap = 0;
+    int4* d_fboxes = nullptr;  // fine boxes in level-0 index space (cover painting)
+    size_t fbox_cap = 0;
+    double* spare_pool = nullptr;  // a retired fine level's pool, reused by the next
+    size_t spare_cap = 0;
 
     // level-1 launch lists (device, with their cell totals)
     CellRect* d_cf = nullptr;  // coarse-fine ghost cells
@@ -407,6 +400,7 @@ class AmrDriver {
     bool l0_max_stale = true;   // level-0 CFL max must be reduced from the stored state
     bool l1_max_stale = true;
     long long regrids = 0, remakes = 0;
+    double regrid_ms = 0.0;  // host wall-clock inside regrid (tags to host, clustering, remake)
     int plot_step = 0;
     double next_plot_time = 0.0;
 
@@ -443,16 +437,20 @@ class AmrDriver {
         ZAMR_HIP(hipMemcpy(d_ybox, yb.data(), yb.size() * sizeof(int), hipMemcpyHostToDevice));
         const size_t ncell = (size_t)L0.nx * L0.ny;
         ZAMR_HIP(hipMalloc(&d_tags, ncell));
-        ZAMR_HIP(hipMalloc(&d_ctags, (size_t)cnx * cny));
+        for (auto& p : d_ctags) {
+            ZAMR_HIP(hipMalloc(&p, (size_t)cnx * cny));
+            ZAMR_HIP(hipMemset(p, 0, (size_t)cnx * cny));
+        }
+        ZAMR_HIP(hipMalloc(&d_changed, sizeof(unsigned)));
         ZAMR_HIP(hipMalloc(&d_cover, ncell));
         ZAMR_HIP(hipMemset(d_cover, 0, ncell));
-        h_ctags.resize((size_t)cnx * cny);
     }
 
     ~AmrDriver() {
         L1.reset();
-        for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags, (void*)d_cover, (void*)d_cf,
-                        (void*)d_avg})
+        for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags[0], (void*)d_ctags[1],
+                        (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_reg, (void*)d_hist,
+                        (void*)d_fboxes, (void*)spare_pool})
             if (p) hipFree(p);
     }
 
@@ -484,72 +482,164 @@ class AmrDriver {
         }
     }
 
-    // cell_cons_interp into the listed cells of level F, into buffers bufmask.
-    void interp(Sol& F, CellRect* d, int n, long long total, unsigned bufmask) {
+    // cell_cons_interp into the listed cells of level F, into buffers
+    // bufmask; with `cover`, cells under a fine box are left alone.
+    void interp(Sol& F, CellRect* d, int n, long long total, unsigned bufmask, const unsigned char* cover) {
         if (!n || !total) return;
         hipLaunchKernelGGL((cf_interp_kernel<M>), dim3(grid1(total)), dim3(256), 0, stream(), F.d_boxes, d, n, total,
-                           grid(), L0.cur, bufmask);
+                           grid(), L0.cur, bufmask, cover);
         ZAMR_HIP(hipGetLastError());
     }
 
-    // The coarse-fine ghost cells of the current level 1: in-domain ghost
-    // cells of each box that no box's valid cells cover, as row runs.
+    // Launch lists of the current level 1: its ghost strips (four per box,
+    // inside the domain; the interpolation skips the cells another fine box
+    // covers), the level-0 cells under each box (average_down), and the
+    // coverage map.
     void build_fine_lists() {
         std::vector<CellRect> cf, avg;
         const auto& hb = L1->hb;
+        const int NX = L1->nx, NY = L1->ny;
+        auto strip = [&](int b, int i0, int j0, int i1, int j1) {
+            i0 = std::max(i0, 0);
+            j0 = std::max(j0, 0);
+            i1 = std::min(i1, NX - 1);
+            j1 = std::min(j1, NY - 1);
+            if (i0 <= i1 && j0 <= j1) cf.push_back(CellRect{b, b, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+        };
+        std::vector<int4> fb;
         for (size_t b = 0; b < hb.size(); ++b) {
             const HostBox& B = hb[b];
-            for (int j = std::max(0, B.lo[1] - B.ng); j <= std::min(L1->ny - 1, B.hi[1] + B.ng); ++j) {
-                int run0 = -1;
-                const int i0 = std::max(0, B.lo[0] - B.ng), i1 = std::min(L1->nx - 1, B.hi[0] + B.ng);
-                for (int i = i0; i <= i1 + 1; ++i) {
-                    bool need = false;
-                    if (i <= i1) {
-                        need = true;
-                        for (const HostBox& O : hb)
-                            if (i >= O.lo[0] && i <= O.hi[0] && j >= O.lo[1] && j <= O.hi[1]) {
-                                need = false;
-                                break;
-                            }
-                    }
-                    if (need && run0 < 0) run0 = i;
-                    if (!need && run0 >= 0) {
-                        cf.push_back(CellRect{(int)b, (int)b, run0, j, i - run0, 1, 0});
-                        run0 = -1;
-                    }
-                }
-            }
+            const int g = B.ng;
+            strip((int)b, B.lo[0] - g, B.lo[1] - g, B.hi[0] + g, B.lo[1] - 1);  // below, corners included
+            strip((int)b, B.lo[0] - g, B.hi[1] + 1, B.hi[0] + g, B.hi[1] + g);  // above
+            strip((int)b, B.lo[0] - g, B.lo[1], B.lo[0] - 1, B.hi[1]);          // left
+            strip((int)b, B.hi[0] + 1, B.lo[1], B.hi[0] + g, B.hi[1]);          // right
             avg.push_back(CellRect{(int)b, (int)b, B.lo[0] / 2, B.lo[1] / 2, (B.hi[0] - B.lo[0] + 1) / 2,
                                    (B.hi[1] - B.lo[1] + 1) / 2, 0});
+            fb.push_back(int4{B.lo[0] / 2, B.lo[1] / 2, B.hi[0] / 2, B.hi[1] / 2});
         }
         upload_rects(cf, d_cf, n_cf, cf_total, stream());
         upload_rects(avg, d_avg, n_avg, avg_total, stream());
-        // level-0 cells under level 1, for the composite mass
-        std::vector<unsigned char> cover((size_t)L0.nx * L0.ny, 0);
-        for (const HostBox& B : hb)
-            for (int j = B.lo[1] / 2; j <= B.hi[1] / 2; ++j)
-                std::memset(&cover[(size_t)j * L0.nx + B.lo[0] / 2], 1, (B.hi[0] - B.lo[0] + 1) / 2);
-        ZAMR_HIP(hipMemcpyAsync(d_cover, cover.data(), cover.size(), hipMemcpyHostToDevice, stream()));
+        if (fb.size() > fbox_cap) {
+            if (d_fboxes) hipFree(d_fboxes);
+            fbox_cap = fb.size() * 2;
+            ZAMR_HIP(hipMalloc(&d_fboxes, fbox_cap * sizeof(int4)));
+        }
+        ZAMR_HIP(hipMemcpyAsync(d_fboxes, fb.data(), fb.size() * sizeof(int4), hipMemcpyHostToDevice, stream()));
+        ZAMR_HIP(hipMemsetAsync(d_cover, 0, (size_t)L0.nx * L0.ny, stream()));
+        hipLaunchKernelGGL(cover_paint_kernel, dim3((unsigned)fb.size()), dim3(256), 0, stream(), d_fboxes, d_cover,
+                           L0.nx);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    // Retire level 1, keeping its state pool for the next one.
+    void retire_fine() {
+        if (!L1) return;
         ZAMR_HIP(hipStreamSynchronize(stream()));
+        if (spare_pool) hipFree(spare_pool);
+        spare_pool = L1->d_pool;
+        spare_cap = L1->pool_cap;
+        L1->d_pool = nullptr;
+        L1.reset();
     }
 
     void clear_fine() {
-        L1.reset();
+        retire_fine();
         ZAMR_HIP(hipMemsetAsync(d_cover, 0, (size_t)L0.nx * L0.ny, stream()));
         last_boxes.clear();
     }
 
-    // Fine boxes from the buffered, coarsened tag map: clusters refined by
-    // bf_lev * ref_ratio, cut to amr.max_grid_size in cluster units.
-    std::vector<std::array<int, 4>> make_grids() {
-        std::vector<int> px, py;
-        for (int J = 0; J < cny; ++J)
-            for (int I = 0; I < cnx; ++I)
-                if (h_ctags[(size_t)J * cnx + I]) {
-                    px.push_back(I);
-                    py.push_back(J);
+    // Berger-Rigoutsos on the device map d_ctags[1], one generation of
+    // clusters at a time: a cluster is the set of tags inside its region, so
+    // the region's row and column counts give its bounding box (the nonzero
+    // range), its efficiency and FindCut's input; a cluster below grid_eff
+    // is cut into two regions for the next generation.
+    std::vector<std::array<int, 4>> gpu_cluster() {
+        std::vector<std::array<int, 4>> out;
+        std::vector<Region> cur{Region{0, 0, cnx - 1, cny - 1, 0, 0, 0}};
+        std::vector<int> h;
+        while (!cur.empty()) {
+            size_t nh = 0;
+            int ntile = 0;
+            for (Region& r : cur) {
+                const int w = r.hi0 - r.lo0 + 1, hh = r.hi1 - r.lo1 + 1;
+                r.offx = (int)nh;
+                r.offy = (int)nh + w;
+                nh += (size_t)w + hh;
+                r.tile = ntile;
+                ntile += ((w + 63) / 64) * ((hh + 63) / 64);
+            }
+            if (cur.size() > reg_cap) {
+                if (d_reg) hipFree(d_reg);
+                reg_cap = cur.size() * 2;
+                ZAMR_HIP(hipMalloc(&d_reg, reg_cap * sizeof(Region)));
+            }
+            if (nh > hist_cap) {
+                if (d_hist) hipFree(d_hist);
+                hist_cap = nh * 2;
+                ZAMR_HIP(hipMalloc(&d_hist, hist_cap * sizeof(int)));
+            }
+            ZAMR_HIP(hipMemcpyAsync(d_reg, cur.data(), cur.size() * sizeof(Region), hipMemcpyHostToDevice, stream()));
+            ZAMR_HIP(hipMemsetAsync(d_hist, 0, nh * sizeof(int), stream()));
+            hipLaunchKernelGGL(cluster_hist_kernel, dim3((unsigned)ntile), dim3(256), 0, stream(), d_ctags[1], cnx,
+                               d_reg, (int)cur.size(), d_hist);
+            ZAMR_HIP(hipGetLastError());
+            h.resize(nh);
+            ZAMR_HIP(hipMemcpyAsync(h.data(), d_hist, nh * sizeof(int), hipMemcpyDeviceToHost, stream()));
+            ZAMR_HIP(hipStreamSynchronize(stream()));
+            std::vector<Region> next;
+            for (const Region& r : cur) {
+                const int* hx = h.data() + r.offx;
+                const int* hy = h.data() + r.offy;
+                const int w = r.hi0 - r.lo0 + 1, hh = r.hi1 - r.lo1 + 1;
+                int a0 = 0, a1 = w - 1, b0 = 0, b1 = hh - 1;
+                while (a0 < w && hx[a0] == 0) ++a0;
+                if (a0 == w) continue;  // no tags
+                while (hx[a1] == 0) --a1;
+                while (hy[b0] == 0) ++b0;
+                while (hy[b1] == 0) --b1;
+                long long count = 0;
+                for (int i = a0; i <= a1; ++i) count += hx[i];
+                const int lo0 = r.lo0 + a0, hi0 = r.lo0 + a1, lo1 = r.lo1 + b0, hi1 = r.lo1 + b1;
+                const double vol = (double)(hi0 - lo0 + 1) * (double)(hi1 - lo1 + 1);
+                if ((double)count / vol >= grid_eff) {
+                    out.push_back({lo0, lo1, hi0, hi1});
+                    continue;
                 }
-        auto cl = br_cluster(px, py, grid_eff);
+                int st[2], cut[2];
+                cut[0] = br_find_cut(hx + a0, a1 - a0 + 1, lo0, st[0]);
+                cut[1] = br_find_cut(hy + b0, b1 - b0 + 1, lo1, st[1]);
+                const int best = std::min(st[0], st[1]);
+                const int lo[2] = {lo0, lo1}, hi[2] = {hi0, hi1};
+                int dir = -1, minlen = -1;
+                for (int d = 0; d < 2; ++d) {
+                    if (st[d] != best) continue;
+                    const int l = std::min(cut[d] - lo[d], hi[d] - cut[d]);
+                    if (l >= minlen) {
+                        dir = d;
+                        minlen = l;
+                    }
+                }
+                Region a{lo0, lo1, hi0, hi1, 0, 0, 0}, b = a;
+                if (dir == 0) {
+                    a.hi0 = cut[0] - 1;
+                    b.lo0 = cut[0];
+                } else {
+                    a.hi1 = cut[1] - 1;
+                    b.lo1 = cut[1];
+                }
+                next.push_back(a);
+                next.push_back(b);
+            }
+            cur.swap(next);
+        }
+        return out;
+    }
+
+    // Fine boxes from the clusters of the buffered, coarsened tag map:
+    // refined by bf_lev * ref_ratio, cut to amr.max_grid_size in cluster units.
+    std::vector<std::array<int, 4>> make_grids() {
+        auto cl = gpu_cluster();
         std::sort(cl.begin(), cl.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
             return a[1] != b[1] ? a[1] < b[1] : a[0] < b[0];
         });
@@ -565,21 +655,25 @@ class AmrDriver {
         return fine;
     }
 
-    // ErrorEst + buffer/coarsen on the device, tag map to the host.  Returns
-    // true when the map differs from the last one clustered.
+    // ErrorEst + buffer/coarsen on the device into d_ctags[0].  Returns true
+    // when the map differs from the one the current grids were made from
+    // (only that flag crosses to the host).
     bool tag(int mode_last, double time_last) {
         const long long n = (long long)L0.nx * L0.ny;
         hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
                            L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
         ZAMR_HIP(hipGetLastError());
+        ZAMR_HIP(hipMemsetAsync(d_changed, 0, sizeof(unsigned), stream()));
         hipLaunchKernelGGL(tag_buffer_kernel, dim3(grid1((long long)cnx * cny)), dim3(256), 0, stream(), d_tags,
-                           L0.nx, L0.ny, n_error_buf, bf_lev, d_ctags, cnx, cny);
+                           L0.nx, L0.ny, n_error_buf, bf_lev, d_ctags[0], cnx, cny, d_ctags[1], d_changed);
         ZAMR_HIP(hipGetLastError());
-        ZAMR_HIP(hipMemcpyAsync(h_ctags.data(), d_ctags, h_ctags.size(), hipMemcpyDeviceToHost, stream()));
+        unsigned changed = 0;
+        ZAMR_HIP(hipMemcpyAsync(&changed, d_changed, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
         ZAMR_HIP(hipStreamSynchronize(stream()));
-        const bool changed = h_ctags != last_ctags;
-        if (changed) last_ctags = h_ctags;
-        return changed;
+        if (have_last && !changed) return false;
+        std::swap(d_ctags[0], d_ctags[1]);  // [1]: the map clustered now, compared against next time
+        have_last = true;
+        return true;
     }
 
     // Qaux of a new or remade level: update_aux_variables(Q, 0) at `time`
@@ -594,6 +688,11 @@ class AmrDriver {
 
     std::unique_ptr<Sol> new_level(const std::vector<std::array<int, 4>>& boxes) {
         auto F = std::make_unique<Sol>();
+        F->hist_cap = 16;  // the step history is level 0's
+        F->given_pool = spare_pool;
+        F->given_cap = spare_cap;
+        spare_pool = nullptr;
+        spare_cap = 0;
         F->configure_fine(L0, ref_ratio, boxes);
         return F;
     }
@@ -612,14 +711,21 @@ class AmrDriver {
         int n = 0;
         long long total = 0;
         upload_rects(all, d, n, total, stream());
-        interp(F, d, n, total, 1u << F.cur);
+        interp(F, d, n, total, 1u << F.cur, nullptr);
         ZAMR_HIP(hipStreamSynchronize(stream()));
         hipFree(d);
     }
 
     // AmrCore::regrid(0, time) for level 1.
+    struct Clock {
+        double& acc;
+        std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+        ~Clock() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+    };
+
     void regrid(int mode_last, double time_last, double time) {
         if (max_level < 1) return;
+        Clock clk{regrid_ms};
         ++regrids;
         if (!tag(mode_last, time_last)) return;  // the same tags give the same grids
         auto boxes = make_grids();
@@ -655,7 +761,7 @@ class AmrDriver {
             ++remakes;
         }
         fresh_aux(*F, time);
-        ZAMR_HIP(hipStreamSynchronize(stream()));
+        retire_fine();
         L1 = std::move(F);
         last_boxes = boxes;
         l1_max_stale = true;
@@ -666,7 +772,7 @@ class AmrDriver {
     // level-0 state; its ghosts read as 0 there.
     void init_levels() {
         L0.ensure_aux();
-        last_ctags.clear();
+        have_last = false;
         for (int it = 0; it < 4; ++it) {
             if (L1) fill_coarse_patch(*L1);
             const int old_finest = finest_level();
@@ -719,7 +825,7 @@ class AmrDriver {
         l0_stage_ran = true;
         l0_max_stale = false;
         if (L1) {
-            interp(*L1, d_cf, n_cf, cf_total, L1->mood() ? 7u : 3u);
+            interp(*L1, d_cf, n_cf, cf_total, L1->mood() ? 7u : 3u, d_cover);
             if (l1_max_stale) {
                 hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, s, L1->d_groups, L1->ngroups);
                 auto& it = L1->items(L1->hx_of(L1->spatial_order));

@@ -99,6 +99,9 @@ class Solver {
     bool own_stream = false;
     double* d_pool = nullptr;
     size_t pool_elems = 0;
+    size_t pool_cap = 0;             // elements allocated (>= pool_elems when handed a larger pool)
+    double* given_pool = nullptr;    // optional pool to use instead of allocating (AMR level reuse)
+    size_t given_cap = 0;
     BoxDesc* d_boxes = nullptr;
     GroupState* d_groups = nullptr;
     std::vector<GroupState> h_groups;
@@ -149,6 +152,8 @@ class Solver {
         if (d_rects) hipFree(d_rects);
         if (d_hist) hipFree(d_hist);
         if (d_partial) hipFree(d_partial);
+        if (given_pool) hipFree(given_pool);
+        given_pool = nullptr;
         d_pool = nullptr;
         d_boxes = nullptr;
         d_groups = nullptr;
@@ -373,7 +378,15 @@ class Solver {
             off = align(off + cells * std::max(na, 1));
         }
         pool_elems = off;
-        ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
+        if (given_pool && given_cap >= pool_elems) {
+            d_pool = given_pool;
+            pool_cap = given_cap;
+        } else {
+            if (given_pool) hipFree(given_pool);
+            ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
+            pool_cap = pool_elems;
+        }
+        given_pool = nullptr;
         ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));
         std::vector<BoxDesc> bd(hb.size());
         for (size_t k = 0; k < hb.size(); ++k)

@@ -529,10 +529,11 @@ int zamr_error_est(zamr_ctx* c, unsigned char* tags, size_t n) {
     });
 }
 
-int zamr_amr_stats(zamr_ctx* c, long long* regrids, long long* remakes) {
+int zamr_amr_stats(zamr_ctx* c, long long* regrids, long long* remakes, double* regrid_ms) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     if (regrids) *regrids = c->amr ? c->amr->regrids : 0;
     if (remakes) *remakes = c->amr ? c->amr->remakes : 0;
+    if (regrid_ms) *regrid_ms = c->amr ? c->amr->regrid_ms : 0.0;
     return ZAMR_OK;
 }
 

@@ -145,6 +145,28 @@ def swe_bumps(n=4096, seed=0, nbumps=32, steps=100, manning=0.03, max_grid_size=
     return case, Q
 
 
+def dry_columns(n=2048, seed=0, ncols=24, radius=(0.02, 0.05), steps=20, max_grid_size=256):
+    """Water columns (depth 1) at seeded positions over a dry flat bed, walls:
+    the AMR workload (amr.max_level = 1, ErrorEst's wet/front tags, so level 1
+    follows the spreading water).  Order 2, Manning 0.03 point-implicit, no
+    well-balancing, MOOD positivity."""
+    rng = SplitMix64(seed)
+    x, y = cell_centers(n, n)
+    X, Y = np.meshgrid(x, y)
+    h = np.zeros((n, n))
+    u = rng.uniform(3 * ncols)
+    for k in range(ncols):
+        cx, cy = 0.1 + 0.8 * u[3 * k], 0.1 + 0.8 * u[3 * k + 1]
+        r = radius[0] + (radius[1] - radius[0]) * u[3 * k + 2]
+        h = np.where((X - cx) ** 2 + (Y - cy) ** 2 < r * r, 1.0, h)
+    z = np.zeros_like(h)
+    case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=max_grid_size,
+                spatial_order=2, implicit_source=True, well_balanced=False, clamp_positivity=False,
+                positivity="mood", cfl=0.45, dtmin=1e-9, dtmax=1.0, time_end=1.0e30, max_step=steps,
+                bc=("wall",) * 4, params={"n_m": 0.03})
+    return case, np.stack([z, h, z, z])
+
+
 def box_filter(a, passes=3, width=5):
     """`passes` passes of a width x width moving average, edges replicated."""
     r = width // 2

@@ -77,7 +77,7 @@ def load_library(path):
         "zamr_init_levels": [_VP],
         "zamr_regrid": [_VP, ctypes.c_double],
         "zamr_error_est": [_VP, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_size_t],
-        "zamr_amr_stats": [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+        "zamr_amr_stats": [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _D],
         "zamr_adj_setup": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I, _I],
         "zamr_adj_forward": [_VP, _D],
         "zamr_adj_backward": [_VP, _D],
@@ -201,11 +201,12 @@ class ZoomyAmr:
         self._check(self.lib.zamr_error_est(self.ctx, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), t.size))
         return t
 
-    def amr_stats(self):
-        """(regrids, levels remade) since creation."""
-        r, m = ctypes.c_longlong(), ctypes.c_longlong()
-        self._check(self.lib.zamr_amr_stats(self.ctx, ctypes.byref(r), ctypes.byref(m)))
-        return r.value, m.value
+    def amr_stats(self, with_time=False):
+        """(regrids, levels remade) since creation, and with with_time the host
+        milliseconds spent regridding."""
+        r, m, t = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        self._check(self.lib.zamr_amr_stats(self.ctx, ctypes.byref(r), ctypes.byref(m), ctypes.byref(t)))
+        return (r.value, m.value, t.value) if with_time else (r.value, m.value)
 
     def compute_dt(self, lev=0):
         dt = ctypes.c_double()

@@ -61,13 +61,16 @@ __device__ __forceinline__ int find_rect(const CellRect* __restrict__ r, int n, 
     return lo;
 }
 
-// Coarse value of component n at (ic, jc), clamped into the domain.
-__device__ __forceinline__ double coarse_value(const CoarseGrid& g, int buf, int ic, int jc, int n) {
+// All nq components of coarse cell (ic, jc), clamped into the domain.
+template <int NQ>
+__device__ __forceinline__ void coarse_cell(const CoarseGrid& g, int buf, int ic, int jc, double* v) {
     ic = ic < 0 ? 0 : (ic >= g.nx ? g.nx - 1 : ic);
     jc = jc < 0 ? 0 : (jc >= g.ny ? g.ny - 1 : jc);
-    const BoxView C = view(g.boxes, coarse_box_index(g, ic, jc));
-    const long long idx = (long long)(ic - C.gl0) + (long long)(jc - C.gl1) * C.sy;
-    return qsel(C, buf)[idx + n * C.sc];
+    const BoxDesc& C = g.boxes[coarse_box_index(g, ic, jc)];
+    const double* __restrict__ q = buf == 0 ? C.q[0] : (buf == 1 ? C.q[1] : C.q[2]);
+    const long long idx = (long long)(ic - C.gl[0]) + (long long)(jc - C.gl[1]) * C.sy;
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) v[n] = q[idx + n * C.sc];
 }
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -94,16 +97,20 @@ __global__ void __launch_bounds__(256) cf_interp_kernel(const BoxDesc* __restric
     const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
     const int ic = i >> 1, jc = j >> 1;
     if (cover && cover[(long long)jc * g.nx + ic]) return;
-    double u0[nq], sx[nq], sy[nq];
+    double u0[nq], sx[nq], sy[nq], xm[nq], xp[nq], ym[nq], yp[nq];
+    coarse_cell<nq>(g, cbuf, ic, jc, u0);
+    coarse_cell<nq>(g, cbuf, ic - 1, jc, xm);
+    coarse_cell<nq>(g, cbuf, ic + 1, jc, xp);
+    coarse_cell<nq>(g, cbuf, ic, jc - 1, ym);
+    coarse_cell<nq>(g, cbuf, ic, jc + 1, yp);
     double fx = 1.0, fy = 1.0;
 #pragma unroll
     for (int n = 0; n < nq; ++n) {
-        const double u = coarse_value(g, cbuf, ic, jc, n);
-        u0[n] = u;
+        const double u = u0[n];
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const double um = d == 0 ? coarse_value(g, cbuf, ic - 1, jc, n) : coarse_value(g, cbuf, ic, jc - 1, n);
-            const double up = d == 0 ? coarse_value(g, cbuf, ic + 1, jc, n) : coarse_value(g, cbuf, ic, jc + 1, n);
+            const double um = d == 0 ? xm[n] : ym[n];
+            const double up = d == 0 ? xp[n] : yp[n];
             const double dc = 0.5 * (up - um);
             const double df = 2.0 * (up - u);
             const double db = 2.0 * (u - um);
@@ -231,8 +238,9 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
                                                          int nbuf, int bf, unsigned char* __restrict__ out, int cnx,
                                                          int cny, const unsigned char* __restrict__ prev,
                                                          unsigned* __restrict__ changed) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)cnx * cny) return;
+    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = k0 < (long long)cnx * cny;
+    const long long k = in ? k0 : 0;  // lanes past the end repeat cell 0 so the wave vote stays whole
     const int I = (int)(k % cnx), J = (int)(k / cnx);
     const int i0 = max(0, I * bf - nbuf), i1 = min(nx - 1, (I + 1) * bf - 1 + nbuf);
     const int j0 = max(0, J * bf - nbuf), j1 = min(ny - 1, (J + 1) * bf - 1 + nbuf);
@@ -243,8 +251,8 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
                 t = 1;
                 break;
             }
-    out[k] = t;
-    if (t != prev[k]) atomicOr(changed, 1u);
+    if (in) out[k] = t;
+    if (__any(in && t != prev[k]) && (threadIdx.x & (WV - 1)) == 0) atomicOr(changed, 1u);  // one atomic per wave
 }
 
 // Row and column tag counts of a list of regions of the buffered tag map:
@@ -381,6 +389,9 @@ class AmrDriver {
     // clustering scratch: regions and their histograms (device and host)
     Region* d_reg = nullptr;
     int* d_hist = nullptr;
+    Region* h_reg = nullptr;  // pinned staging of the same
+    int* h_hist = nullptr;
+    unsigned* h_flag = nullptr;
     size_t reg_cap = 0, hist_cap = 0;
     int4* d_fboxes = nullptr;  // fine boxes in level-0 index space (cover painting)
     size_t fbox_cap = 0;
@@ -442,6 +453,7 @@ class AmrDriver {
             ZAMR_HIP(hipMemset(p, 0, (size_t)cnx * cny));
         }
         ZAMR_HIP(hipMalloc(&d_changed, sizeof(unsigned)));
+        ZAMR_HIP(hipHostMalloc(&h_flag, sizeof(unsigned)));
         ZAMR_HIP(hipMalloc(&d_cover, ncell));
         ZAMR_HIP(hipMemset(d_cover, 0, ncell));
     }
@@ -452,6 +464,8 @@ class AmrDriver {
                         (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_reg, (void*)d_hist,
                         (void*)d_fboxes, (void*)spare_pool})
             if (p) hipFree(p);
+        for (void* p : {(void*)h_reg, (void*)h_hist, (void*)h_flag})
+            if (p) hipHostFree(p);
     }
 
     CoarseGrid grid() const { return CoarseGrid{L0.d_boxes, d_xbox, d_ybox, nbx, L0.nx, L0.ny}; }
@@ -557,7 +571,6 @@ class AmrDriver {
     std::vector<std::array<int, 4>> gpu_cluster() {
         std::vector<std::array<int, 4>> out;
         std::vector<Region> cur{Region{0, 0, cnx - 1, cny - 1, 0, 0, 0}};
-        std::vector<int> h;
         while (!cur.empty()) {
             size_t nh = 0;
             int ntile = 0;
@@ -571,26 +584,31 @@ class AmrDriver {
             }
             if (cur.size() > reg_cap) {
                 if (d_reg) hipFree(d_reg);
+                if (h_reg) hipHostFree(h_reg);
                 reg_cap = cur.size() * 2;
                 ZAMR_HIP(hipMalloc(&d_reg, reg_cap * sizeof(Region)));
+                ZAMR_HIP(hipHostMalloc(&h_reg, reg_cap * sizeof(Region)));
             }
             if (nh > hist_cap) {
                 if (d_hist) hipFree(d_hist);
+                if (h_hist) hipHostFree(h_hist);
                 hist_cap = nh * 2;
                 ZAMR_HIP(hipMalloc(&d_hist, hist_cap * sizeof(int)));
+                ZAMR_HIP(hipHostMalloc(&h_hist, hist_cap * sizeof(int)));
             }
-            ZAMR_HIP(hipMemcpyAsync(d_reg, cur.data(), cur.size() * sizeof(Region), hipMemcpyHostToDevice, stream()));
+            std::memcpy(h_reg, cur.data(), cur.size() * sizeof(Region));
+            ZAMR_HIP(hipMemcpyAsync(d_reg, h_reg, cur.size() * sizeof(Region), hipMemcpyHostToDevice, stream()));
             ZAMR_HIP(hipMemsetAsync(d_hist, 0, nh * sizeof(int), stream()));
             hipLaunchKernelGGL(cluster_hist_kernel, dim3((unsigned)ntile), dim3(256), 0, stream(), d_ctags[1], cnx,
                                d_reg, (int)cur.size(), d_hist);
             ZAMR_HIP(hipGetLastError());
-            h.resize(nh);
-            ZAMR_HIP(hipMemcpyAsync(h.data(), d_hist, nh * sizeof(int), hipMemcpyDeviceToHost, stream()));
+            ZAMR_HIP(hipMemcpyAsync(h_hist, d_hist, nh * sizeof(int), hipMemcpyDeviceToHost, stream()));
             ZAMR_HIP(hipStreamSynchronize(stream()));
+            const int* h = h_hist;
             std::vector<Region> next;
             for (const Region& r : cur) {
-                const int* hx = h.data() + r.offx;
-                const int* hy = h.data() + r.offy;
+                const int* hx = h + r.offx;
+                const int* hy = h + r.offy;
                 const int w = r.hi0 - r.lo0 + 1, hh = r.hi1 - r.lo1 + 1;
                 int a0 = 0, a1 = w - 1, b0 = 0, b1 = hh - 1;
                 while (a0 < w && hx[a0] == 0) ++a0;
@@ -667,9 +685,9 @@ class AmrDriver {
         hipLaunchKernelGGL(tag_buffer_kernel, dim3(grid1((long long)cnx * cny)), dim3(256), 0, stream(), d_tags,
                            L0.nx, L0.ny, n_error_buf, bf_lev, d_ctags[0], cnx, cny, d_ctags[1], d_changed);
         ZAMR_HIP(hipGetLastError());
-        unsigned changed = 0;
-        ZAMR_HIP(hipMemcpyAsync(&changed, d_changed, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
+        ZAMR_HIP(hipMemcpyAsync(h_flag, d_changed, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
         ZAMR_HIP(hipStreamSynchronize(stream()));
+        const unsigned changed = *h_flag;
         if (have_last && !changed) return false;
         std::swap(d_ctags[0], d_ctags[1]);  // [1]: the map clustered now, compared against next time
         have_last = true;

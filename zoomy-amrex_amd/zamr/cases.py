@@ -145,11 +145,12 @@ def swe_bumps(n=4096, seed=0, nbumps=32, steps=100, manning=0.03, max_grid_size=
     return case, Q
 
 
-def dry_columns(n=2048, seed=0, ncols=24, radius=(0.02, 0.05), steps=20, max_grid_size=256):
+def dry_columns(n=2048, seed=0, ncols=24, radius=(0.02, 0.05), steps=20, max_grid_size=256, positivity="clamp"):
     """Water columns (depth 1) at seeded positions over a dry flat bed, walls:
     the AMR workload (amr.max_level = 1, ErrorEst's wet/front tags, so level 1
     follows the spreading water).  Order 2, Manning 0.03 point-implicit, no
-    well-balancing, MOOD positivity."""
+    well-balancing; positivity "clamp" (the reference's default,
+    solver.clamp_positivity) or "mood"."""
     rng = SplitMix64(seed)
     x, y = cell_centers(n, n)
     X, Y = np.meshgrid(x, y)
@@ -161,9 +162,10 @@ def dry_columns(n=2048, seed=0, ncols=24, radius=(0.02, 0.05), steps=20, max_gri
         h = np.where((X - cx) ** 2 + (Y - cy) ** 2 < r * r, 1.0, h)
     z = np.zeros_like(h)
     case = dict(nx=n, ny=n, prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=max_grid_size,
-                spatial_order=2, implicit_source=True, well_balanced=False, clamp_positivity=False,
-                positivity="mood", cfl=0.45, dtmin=1e-9, dtmax=1.0, time_end=1.0e30, max_step=steps,
-                bc=("wall",) * 4, params={"n_m": 0.03})
+                spatial_order=2, implicit_source=True, well_balanced=False,
+                clamp_positivity=positivity == "clamp", positivity="mood" if positivity == "mood" else "none",
+                cfl=0.45, dtmin=1e-9, dtmax=1.0, time_end=1.0e30, max_step=steps, bc=("wall",) * 4,
+                params={"n_m": 0.03})
     return case, np.stack([z, h, z, z])
 
 

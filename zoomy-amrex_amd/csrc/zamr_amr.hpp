@@ -93,8 +93,8 @@ __global__ void __launch_bounds__(256) cf_interp_kernel(const BoxDesc* __restric
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     const CellRect R = rects[find_rect(rects, nrects, k)];
-    const long long cell = k - R.start;
-    const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
+    const int cell = (int)(k - R.start);  // < ni * nj: 32-bit division
+    const int i = R.ilo + cell % R.ni, j = R.jlo + cell / R.ni;
     const int ic = i >> 1, jc = j >> 1;
     if (cover && cover[(long long)jc * g.nx + ic]) return;
     double u0[nq], sx[nq], sy[nq], xm[nq], xp[nq], ym[nq], yp[nq];
@@ -149,8 +149,8 @@ __global__ void __launch_bounds__(256) average_down_kernel(const BoxDesc* __rest
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     const CellRect R = rects[find_rect(rects, nrects, k)];
-    const long long cell = k - R.start;
-    const int ic = R.ilo + (int)(cell % R.ni), jc = R.jlo + (int)(cell / R.ni);
+    const int cell = (int)(k - R.start);
+    const int ic = R.ilo + cell % R.ni, jc = R.jlo + cell / R.ni;
     const BoxView F = view(fine, R.box);
     const BoxView C = view(g.boxes, coarse_box_index(g, ic, jc));
     const double* __restrict__ qf = qsel(F, fbuf);
@@ -180,8 +180,8 @@ __global__ void __launch_bounds__(256) copy_levels_kernel(const BoxDesc* __restr
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
     const CellRect R = rects[find_rect(rects, nrects, k)];
-    const long long cell = k - R.start;
-    const int i = R.ilo + (int)(cell % R.ni), j = R.jlo + (int)(cell / R.ni);
+    const int cell = (int)(k - R.start);  // < ni * nj: 32-bit division
+    const int i = R.ilo + cell % R.ni, j = R.jlo + cell / R.ni;
     const BoxView D = view(dst, R.box);
     const BoxView S = view(src, R.src);
     const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy;
@@ -202,9 +202,9 @@ __global__ void __launch_bounds__(256) error_est_kernel(CoarseGrid g, int cur, i
                                                         unsigned char* __restrict__ tags) {
 #pragma clang fp contract(off)
     using S = Sys<M>;
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)g.nx * g.ny) return;
-    const int i = (int)(k % g.nx), j = (int)(k / g.nx);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // a level's cells fit in 32 bits
+    if (k >= g.nx * g.ny) return;
+    const int i = k % g.nx, j = k / g.nx;
     typename S::VP p;
     for (int q = 0; q < S::np; ++q) p(q, 0) = pp.v[q];
     const BoxView B = view(g.boxes, coarse_box_index(g, i, j));
@@ -238,21 +238,30 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
                                                          int nbuf, int bf, unsigned char* __restrict__ out, int cnx,
                                                          int cny, const unsigned char* __restrict__ prev,
                                                          unsigned* __restrict__ changed) {
-    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = k0 < (long long)cnx * cny;
-    const long long k = in ? k0 : 0;  // lanes past the end repeat cell 0 so the wave vote stays whole
-    const int I = (int)(k % cnx), J = (int)(k / cnx);
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;  // a level's cells fit in 32 bits
+    const bool in = k0 < cnx * cny;
+    const int k = in ? k0 : 0;  // lanes past the end repeat cell 0 so the wave vote stays whole
+    const int I = k % cnx, J = k / cnx;
     const int i0 = max(0, I * bf - nbuf), i1 = min(nx - 1, (I + 1) * bf - 1 + nbuf);
     const int j0 = max(0, J * bf - nbuf), j1 = min(ny - 1, (J + 1) * bf - 1 + nbuf);
-    unsigned char t = 0;
-    for (int j = j0; j <= j1 && !t; ++j)
-        for (int i = i0; i <= i1; ++i)
-            if (tags[(long long)j * nx + i]) {
-                t = 1;
-                break;
-            }
-    if (in) out[k] = t;
-    if (__any(in && t != prev[k]) && (threadIdx.x & (WV - 1)) == 0) atomicOr(changed, 1u);  // one atomic per wave
+    unsigned t = 0;
+    if (bf == 1 && nbuf == 1) {  // the common case: nine independent loads
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int j = min(max(J + dj, j0), j1);
+#pragma unroll
+            for (int di = -1; di <= 1; ++di) t |= tags[(long long)j * nx + min(max(I + di, i0), i1)];
+        }
+    } else {
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) t |= tags[(long long)j * nx + i];
+    }
+    t = t ? 1u : 0u;
+    if (in) out[k] = (unsigned char)t;
+    // one atomic per wave, and none once the flag is up
+    if (__any(in && t != prev[k]) && (threadIdx.x & (WV - 1)) == 0 &&
+        __hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        atomicOr(changed, 1u);
 }
 
 // Row and column tag counts of a list of regions of the buffered tag map:

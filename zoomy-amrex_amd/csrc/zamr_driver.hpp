@@ -107,6 +107,7 @@ class Solver {
     std::vector<GroupState> h_groups;
     CopyRect* d_rects = nullptr;
     int nrects = 0;
+    long long rect_cells = 0;  // ghost cells over all copy rectangles
     double* d_hist = nullptr;
     int hist_cap = 1 << 16;
     double* d_partial = nullptr;
@@ -450,6 +451,7 @@ class Solver {
     // Ghost cells of box b that lie inside the domain and inside another box
     // (same-level FillBoundary).  Batched boxes are independent domains.
     void build_rects() {
+        rect_cells = 0;
         if (batched || hb.size() < 2) return;
         std::vector<CopyRect> rr;
         for (size_t b = 0; b < hb.size(); ++b)
@@ -461,7 +463,8 @@ class Solver {
                 const int ilo = std::max(B.lo[0] - B.ng, O.lo[0]), ihi = std::min(B.hi[0] + B.ng, O.hi[0]);
                 const int jlo = std::max(B.lo[1] - gy, O.lo[1]), jhi = std::min(B.hi[1] + gy, O.hi[1]);
                 if (ilo > ihi || jlo > jhi) continue;
-                rr.push_back(CopyRect{(int)o, (int)b, ilo, jlo, ihi - ilo + 1, jhi - jlo + 1});
+                rr.push_back(CopyRect{(int)o, (int)b, ilo, jlo, ihi - ilo + 1, jhi - jlo + 1, rect_cells});
+                rect_cells += (long long)(ihi - ilo + 1) * (jhi - jlo + 1);
             }
         nrects = (int)rr.size();
         if (nrects) {
@@ -685,8 +688,8 @@ class Solver {
     // owners' values after UpdateState) or a plain copy of the stored state.
     void fill_boundary(int buf, bool update = true) {
         if (nrects == 0) return;
-        hipLaunchKernelGGL(fill_boundary_kernel<M>, dim3(nrects), dim3(256), 0, stream, d_boxes, d_rects, buf, prm,
-                           clamp_flag(), update ? 1 : 0);
+        hipLaunchKernelGGL(fill_boundary_kernel<M>, dim3((unsigned)((rect_cells + 255) / 256)), dim3(256), 0, stream,
+                           d_boxes, d_rects, nrects, rect_cells, buf, prm, clamp_flag(), update ? 1 : 0);
         ZAMR_HIP(hipGetLastError());
     }
 

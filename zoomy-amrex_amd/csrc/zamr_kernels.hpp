@@ -933,33 +933,46 @@ __global__ void set_dt_kernel(GroupState* gs, int ngroups, double dt, double tim
 // the copy is of the owner's value after UpdateState (do_stage runs
 // UpdateState before FillPatch, ZoomyAmr.cpp:563-567), so that the stage
 // kernel can take every ghost cell as stored.
+// One thread per ghost cell over all rectangles (`start`: a rectangle's first
+// cell in the flat numbering), so that many small rectangles share blocks.
 struct CopyRect {
     int src_box, dst_box, ilo, jlo, ni, nj;
+    long long start;
 };
 template <class M>
-__global__ void fill_boundary_kernel(const BoxDesc* __restrict__ boxes, const CopyRect* __restrict__ rects, int buf,
-                                     Params pp, int clamp, int update) {
+__global__ void __launch_bounds__(256) fill_boundary_kernel(const BoxDesc* __restrict__ boxes,
+                                                            const CopyRect* __restrict__ rects, int nrects,
+                                                            long long total, int buf, Params pp, int clamp,
+                                                            int update) {
     using S = Sys<M>;
     constexpr int nq = S::nq;
-    typename S::VP p;
-    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
-    const CopyRect r = rects[blockIdx.x];
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    int lo = 0, hi = nrects - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rects[mid].start <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const CopyRect r = rects[lo];
     const BoxView Sb = view(boxes, r.src_box);
     const BoxView D = view(boxes, r.dst_box);
-    const int ncell = r.ni * r.nj;
+    const int cell = (int)(k - r.start);  // < ni * nj: 32-bit division
+    const int i = r.ilo + cell % r.ni, j = r.jlo + cell / r.ni;
+    const long long si = (long long)(i - Sb.gl0) + (long long)(j - Sb.gl1) * Sb.sy;
+    const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy;
     const double* __restrict__ qs = qsel(Sb, buf);
     double* __restrict__ qd = qsel(D, buf);
-    for (int cell = threadIdx.x; cell < ncell; cell += blockDim.x) {
-        const int i = r.ilo + cell % r.ni, j = r.jlo + cell / r.ni;
-        const long long si = (long long)(i - Sb.gl0) + (long long)(j - Sb.gl1) * Sb.sy;
-        const long long di = (long long)(i - D.gl0) + (long long)(j - D.gl1) * D.sy;
-        typename S::VQ v;
+    typename S::VQ v;
 #pragma unroll
-        for (int n = 0; n < nq; ++n) v(n, 0) = qs[si + n * Sb.sc];
-        if (update) v = S::update_state(v, p, clamp);
-#pragma unroll
-        for (int n = 0; n < nq; ++n) qd[di + n * D.sc] = v(n, 0);
+    for (int n = 0; n < nq; ++n) v(n, 0) = qs[si + n * Sb.sc];
+    if (update) {
+        typename S::VP p;
+        for (int q = 0; q < S::np; ++q) p(q, 0) = pp.v[q];
+        v = S::update_state(v, p, clamp);
     }
+#pragma unroll
+    for (int n = 0; n < nq; ++n) qd[di + n * D.sc] = v(n, 0);
 }
 
 // Qaux as the reference holds it after UpdateState (ZoomyAmr.cpp:328):

@@ -8,6 +8,7 @@
 //       [5] well_balanced [6] clamp_positivity [7] mood [8] face_based_rhs
 //       [9] max_step [10..13] bc tag index x_lo,x_hi,y_lo,y_hi
 //       [14] c_cfl_dimension [15] c_cfl_degree_factor [16] OpenMP threads
+//       [17] implicit_global (JFNK source solve)
 // dset: [0] prob_lo x [1] prob_lo y [2] prob_hi x [3] prob_hi y [4] cfl
 //       [5] dtmin [6] dtmax [7] time_end
 //
@@ -37,6 +38,7 @@ static zorc::Settings settings_from(const int* is, const double* ds) {
     for (int k = 0; k < 4; ++k) s.bc_idx[k] = is[10 + k];
     s.cfl_dimension = is[14];
     s.cfl_degree_factor = is[15];
+    s.implicit_global = is[17] != 0;
     s.plo[0] = ds[0];
     s.plo[1] = ds[1];
     s.phi[0] = ds[2];
@@ -99,6 +101,12 @@ void orc_stage(void* h, int ord, double time, double dt) { static_cast<OSolver*>
 void orc_advance(void* h, double time, double dt) { static_cast<OSolver*>(h)->advance(time, dt); }
 double orc_total_mass(void* h) { return static_cast<OSolver*>(h)->total_mass(); }
 long orc_mood_redos(void* h) { return static_cast<OSolver*>(h)->mood_redos; }
+// Newton and GMRES iterations of the last global source solve.
+void orc_jfnk_iters(void* h, long* newton, long* gmres) {
+    auto* o = static_cast<OSolver*>(h);
+    *newton = o->jfnk_newton;
+    *gmres = o->jfnk_gmres;
+}
 void orc_set_tape(void* h, int mode) { static_cast<OSolver*>(h)->set_tape(mode); }
 
 // Runs the Evolve loop; dts/masses (capacity cap) receive per-step values.
@@ -132,6 +140,38 @@ int orc_ge_solve(int n, const double* A, const double* rhs, double* x) {
     if (n == 4) return run(std::integral_constant<int, 4>{});
     if (n == 8) return run(std::integral_constant<int, 8>{});
     return -1;
+}
+
+// ComputeSourceField (imex_solver.H:21-59) on a dense (nq, ny, nx) state:
+// aux = update_aux_variables(q, 0), then Model::source, cell by cell.
+void orc_source_field(void* h, const double* q, double* S, double time) {
+    auto* o = static_cast<OSolver*>(h);
+    const long nx = o->s.nx, ny = o->s.ny, nc = nx * ny;
+    for (long j = 0; j < ny; ++j)
+        for (long i = 0; i < nx; ++i) {
+            OSolver::VQ qq;
+            OSolver::VA a;
+            for (int n = 0; n < OSolver::nq; ++n) qq(n, 0) = q[n * nc + j * nx + i];
+            for (int n = 0; n < OSolver::na; ++n) a(n, 0) = 0.0;
+            const auto X = o->cell_center((int)i, (int)j);
+            a = Model::update_aux_variables(qq, a, o->p, time, X);
+            const auto Sc = Model::source(qq, a, o->p, time, X);
+            for (int n = 0; n < OSolver::nq; ++n) S[n * nc + j * nx + i] = Sc(n, 0);
+        }
+}
+
+// GMRES (the global source solve's linear solver) on a dense n x n system
+// A x = b, row-major A; returns the iteration count.
+long orc_gmres_dense(int n, const double* A, const double* b, double rtol, double* x) {
+    std::vector<double> bv(b, b + n), xv;
+    auto op = [&](const std::vector<double>& v, std::vector<double>& Av) {
+        Av.assign(n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) Av[i] += A[(size_t)i * n + j] * v[j];
+    };
+    const long its = zorc::gmres(xv, bv, rtol, 0.0, op);
+    for (int i = 0; i < n; ++i) x[i] = xv[i];
+    return its;
 }
 
 // ---------------- AMR ----------------

@@ -27,7 +27,7 @@ _I = ctypes.POINTER(ctypes.c_int)
 
 DEFAULTS = dict(
     prob_lo=(0.0, 0.0), prob_hi=(1.0, 1.0), max_grid_size=64, spatial_order=1,
-    implicit_source=True, well_balanced=False, clamp_positivity=True,
+    implicit_source=True, implicit_global=False, well_balanced=False, clamp_positivity=True,
     positivity="none", face_based_rhs=1, cfl=0.5, dtmin=1.0e-7, dtmax=1.0e-2,
     time_end=1.0, max_step=-1, bc=("", "", "", ""), params=None,
     cfl_dimension=None, cfl_degree_factor=None)
@@ -61,6 +61,12 @@ def load(system):
     lib.orc_advance.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     lib.orc_mood_redos.restype = ctypes.c_long
     lib.orc_mood_redos.argtypes = [ctypes.c_void_p]
+    lib.orc_source_field.restype = None
+    lib.orc_source_field.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_double]
+    lib.orc_gmres_dense.restype = ctypes.c_long
+    lib.orc_gmres_dense.argtypes = [ctypes.c_int, _D, _D, ctypes.c_double, _D]
+    lib.orc_jfnk_iters.restype = None
+    lib.orc_jfnk_iters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     lib.orc_set_tape.restype = None
     lib.orc_set_tape.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_evolve.restype = ctypes.c_int
@@ -143,7 +149,7 @@ class Oracle:
         iset = np.array([c["nx"], c["ny"], c["max_grid_size"], order, int(bool(c["implicit_source"])),
                          int(bool(c["well_balanced"])), int(bool(c["clamp_positivity"])),
                          int(c["positivity"] == "mood"), int(c["face_based_rhs"]), int(c["max_step"]),
-                         *bcs, cfl_dim, cfl_deg, threads], dtype=np.int32)
+                         *bcs, cfl_dim, cfl_deg, threads, int(bool(c["implicit_global"]))], dtype=np.int32)
         dset = np.array([*c["prob_lo"], *c["prob_hi"], c["cfl"], c["dtmin"], c["dtmax"], c["time_end"]],
                         dtype=np.float64)
         self.nx, self.ny = int(c["nx"]), int(c["ny"])
@@ -217,6 +223,19 @@ class Oracle:
 
     def mood_redos(self):
         return self.lib.orc_mood_redos(self.h)
+
+    def source_field(self, Q, time=0.0):
+        """Model::source of every cell with aux = update_aux_variables(q, 0)."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(self.nq, self.ny, self.nx)
+        S = np.empty_like(Q)
+        self.lib.orc_source_field(self.h, _dp(Q), _dp(S), time)
+        return S
+
+    def jfnk_iters(self):
+        """(Newton, GMRES) iterations of the last global source solve."""
+        a, b = ctypes.c_long(), ctypes.c_long()
+        self.lib.orc_jfnk_iters(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def set_tape(self, mode):
         """Limiter tape: 1 records every minmod choice of the following stages,
@@ -328,6 +347,15 @@ def cluster(points, eff=0.7, system="swe2d"):
     out = np.zeros(4 * (len(pts) + 1), dtype=np.int32)
     n = load(system).orc_cluster(pts.ctypes.data_as(_I), len(pts), eff, out.ctypes.data_as(_I), len(pts) + 1)
     return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n)]
+
+
+def gmres_dense(A, b, rtol):
+    """The global source solve's GMRES on a dense system; (x, iterations)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    its = load("swe2d").orc_gmres_dense(len(b), _dp(A), _dp(b), rtol, _dp(x))
+    return x, its
 
 
 def ge_solve(A, rhs):

@@ -81,6 +81,7 @@ struct Settings {
     bool well_balanced = false;
     bool clamp_positivity = true;
     bool mood = false;  // solver.positivity == "mood"
+    bool implicit_global = false;  // solver.implicit_global: JFNK source solve (Source/imex_solver.H)
     bool face_based_rhs = true;
     Real cfl = 0.5, dtmin = 1.0e-7, dtmax = 1.0e-2, time_end = 1.0;
     int max_step = -1;
@@ -146,6 +147,86 @@ inline bool ge_solve(amrex::SmallMatrix<Real, N, N>& A, amrex::SmallMatrix<Real,
         if (!std::isfinite(dq(r, 0))) return false;
     }
     return true;
+}
+
+inline Real vdot(const std::vector<Real>& a, const std::vector<Real>& b) {
+    Real s = 0.0;
+    for (size_t k = 0; k < a.size(); ++k) s += a[k] * b[k];
+    return s;
+}
+
+// Restarted GMRES(30), zero initial guess, no preconditioner (ImpSrcOp's
+// precond is the identity), classical Gram-Schmidt applied twice, Givens
+// rotations; stops when the residual estimate falls below
+// max(rtol * |b|, atol) (amrex::GMRES::solve(x, b, rtol, atol)).
+template <class Op>
+long gmres(std::vector<Real>& x, const std::vector<Real>& b, Real rtol, Real atol, Op&& A) {
+    const int m = 30;
+    const long maxiter = 2000;
+    const size_t N = b.size();
+    x.assign(N, 0.0);
+    const Real r0 = std::sqrt(vdot(b, b));
+    if (r0 == 0.0) return 0;
+    auto conv = [&](Real r) { return r < r0 * rtol || r < atol; };
+    std::vector<Real> r = b, w(N);
+    Real rnorm = r0;
+    long its = 0;
+    std::vector<std::vector<Real>> V(m + 1, std::vector<Real>(N));
+    while (its < maxiter) {
+        for (size_t k = 0; k < N; ++k) V[0][k] = r[k] / rnorm;
+        std::vector<Real> g(m + 1, 0.0), cs(m, 0.0), sn(m, 0.0);
+        std::vector<std::vector<Real>> H(m + 1, std::vector<Real>(m, 0.0));
+        g[0] = rnorm;
+        int kd = 0;
+        bool done = false;
+        for (int k = 0; k < m; ++k) {
+            A(V[k], w);
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<Real> h(k + 1);
+                for (int j = 0; j <= k; ++j) h[j] = vdot(V[j], w);
+                for (int j = 0; j <= k; ++j) {
+                    for (size_t q = 0; q < N; ++q) w[q] -= h[j] * V[j][q];
+                    H[j][k] += h[j];
+                }
+            }
+            const Real hn = std::sqrt(vdot(w, w));
+            H[k + 1][k] = hn;
+            if (hn > 0.0)
+                for (size_t q = 0; q < N; ++q) V[k + 1][q] = w[q] / hn;
+            for (int j = 0; j < k; ++j) {
+                const Real t = cs[j] * H[j][k] + sn[j] * H[j + 1][k];
+                H[j + 1][k] = -sn[j] * H[j][k] + cs[j] * H[j + 1][k];
+                H[j][k] = t;
+            }
+            const Real den = std::hypot(H[k][k], H[k + 1][k]);
+            cs[k] = den > 0.0 ? H[k][k] / den : 1.0;
+            sn[k] = den > 0.0 ? H[k + 1][k] / den : 0.0;
+            H[k][k] = den;
+            H[k + 1][k] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            ++its;
+            kd = k + 1;
+            if (conv(std::abs(g[k + 1])) || hn == 0.0 || its >= maxiter) {
+                done = conv(std::abs(g[k + 1]));
+                break;
+            }
+        }
+        std::vector<Real> y(kd, 0.0);
+        for (int i = kd - 1; i >= 0; --i) {
+            Real t = g[i];
+            for (int j = i + 1; j < kd; ++j) t -= H[i][j] * y[j];
+            y[i] = H[i][i] != 0.0 ? t / H[i][i] : 0.0;
+        }
+        for (int j = 0; j < kd; ++j)
+            for (size_t q = 0; q < N; ++q) x[q] += y[j] * V[j][q];
+        if (done) break;
+        A(x, w);
+        for (size_t q = 0; q < N; ++q) r[q] = b[q] - w[q];
+        rnorm = std::sqrt(vdot(r, r));
+        if (conv(rnorm) || rnorm == 0.0) break;
+    }
+    return its;
 }
 
 template <class Model, class Numerics>
@@ -651,12 +732,102 @@ struct Solver {
             for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)  // ConsUpdate
                 for (int i = vb.lo[0]; i <= vb.hi[0]; ++i)
                     for (int n = 0; n < nq; ++n) fq(i, j, n) += dt * R(i, j, n);
-            if (s.implicit_source) {
+            if (s.implicit_source && !s.implicit_global) {
                 const Fab& fa = Qaux.fabs[b];
                 for (int j = vb.lo[1]; j <= vb.hi[1]; ++j)
                     for (int i = vb.lo[0]; i <= vb.hi[0]; ++i) apply_implicit_source(fq, fa, i, j, dt, time);
             }
         }
+        if (s.implicit_source && s.implicit_global) solve_source_global(dt, time);
+    }
+
+    // ---- SolveImplicitSourceGlobal (Source/imex_solver.H:102-160) ----
+    // Vectors hold the valid cells of every box, box by box, nq components
+    // per cell.  Every source of this build is local (a cell's S reads only
+    // that cell), so ComputeSourceField's ghost refresh (:31-36) does not
+    // enter S.
+    long jfnk_newton = 0, jfnk_gmres = 0;  // iterations of the last solve
+    std::vector<int> vc_i, vc_j;            // cell of each vector entry block
+
+    void vec_cells() {
+        vc_i.clear();
+        vc_j.clear();
+        for (auto& f : Q.fabs)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i) {
+                    vc_i.push_back(i);
+                    vc_j.push_back(j);
+                }
+    }
+    void gather_q(std::vector<Real>& v) const {
+        v.clear();
+        for (auto& f : Q.fabs)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i)
+                    for (int n = 0; n < nq; ++n) v.push_back(f(i, j, n));
+    }
+    void scatter_q(const std::vector<Real>& v) {
+        size_t k = 0;
+        for (auto& f : Q.fabs)
+            for (int j = f.vb.lo[1]; j <= f.vb.hi[1]; ++j)
+                for (int i = f.vb.lo[0]; i <= f.vb.hi[0]; ++i)
+                    for (int n = 0; n < nq; ++n) f(i, j, n) = v[k++];
+    }
+    // ComputeSourceField (:21-59): aux = update_aux_variables(q, 0), then S.
+    void source_field(const std::vector<Real>& q, std::vector<Real>& S, Real time) const {
+        S.assign(q.size(), 0.0);
+        const size_t nc = vc_i.size();
+        for (size_t c = 0; c < nc; ++c) {
+            VQ qq;
+            VA a;
+            for (int n = 0; n < nq; ++n) qq(n, 0) = q[c * nq + n];
+            for (int n = 0; n < na; ++n) a(n, 0) = 0.0;
+            const VX X = cell_center(vc_i[c], vc_j[c]);
+            a = Model::update_aux_variables(qq, a, p, time, X);
+            const VQ Sc = Model::source(qq, a, p, time, X);
+            for (int n = 0; n < nq; ++n) S[c * nq + n] = Sc(n, 0);
+        }
+    }
+    // Newton on F(Q) = Q - Qexp - dt S(Q): at most 12 iterations, stop at
+    // |F| < 1e-12 or |F| < 1e-9 |F_0|; each correction from GMRES (rtol 1e-4)
+    // with the finite-difference Jacobian-vector product
+    // J x = x - (dt/eps) (S(Q + eps x) - S(Q)), eps = 1e-7 (1 + |Q|) / |x|.
+    void solve_source_global(Real dt, Real time) {
+        vec_cells();
+        std::vector<Real> q, qexp, S, F, rhs, delta, qp, Sp;
+        gather_q(q);
+        qexp = q;
+        Real resid0 = -1.0;
+        jfnk_newton = 0;
+        jfnk_gmres = 0;
+        for (int it = 0; it < 12; ++it) {
+            source_field(q, S, time);
+            F.resize(q.size());
+            for (size_t k = 0; k < q.size(); ++k) F[k] = (q[k] - qexp[k]) + (-dt) * S[k];
+            const Real resid = std::sqrt(vdot(F, F));
+            if (it == 0) resid0 = resid;
+            if (resid < 1e-12 || (resid0 > 0 && resid < 1e-9 * resid0)) break;
+            rhs.resize(q.size());
+            for (size_t k = 0; k < q.size(); ++k) rhs[k] = -F[k];
+            const Real nrmQ = std::sqrt(vdot(q, q));
+            auto A = [&](const std::vector<Real>& x, std::vector<Real>& Ax) {
+                Ax.resize(x.size());
+                const Real nx = std::sqrt(vdot(x, x));
+                if (nx == 0.0) {
+                    std::fill(Ax.begin(), Ax.end(), 0.0);
+                    return;
+                }
+                const Real eps = 1e-7 * (1.0 + nrmQ) / nx;
+                qp.resize(x.size());
+                for (size_t k = 0; k < x.size(); ++k) qp[k] = 1.0 * q[k] + eps * x[k];
+                source_field(qp, Sp, time);
+                for (size_t k = 0; k < x.size(); ++k) Ax[k] = (1.0 * x[k] + (-dt / eps) * Sp[k]) + (dt / eps) * S[k];
+            };
+            jfnk_gmres += gmres(delta, rhs, 1e-4, 0.0, A);
+            for (size_t k = 0; k < q.size(); ++k) q[k] += 1.0 * delta[k];
+            ++jfnk_newton;
+        }
+        scatter_q(q);
     }
 
     void heun() {  // ZoomyAmr.cpp:665-675

@@ -63,8 +63,10 @@ int zamr_march_constants(int* c_cfl_dimension, int* c_cfl_degree_factor);
  * written by build.py:write_inputs (build.py:55-120).  amr.max_level = 1 sets
  * up the two-level hierarchy (AmrCore with ref_ratio 2, a 2-D model; the
  * amr.blocking_factor / n_error_buf / grid_eff / max_grid_size and tagging.*
- * keys apply).  Keys outside this build (amr.max_level > 1, ref_ratio != 2,
- * solver.implicit_global = true) fail with ZAMR_EUNSUP. */
+ * keys apply).  solver.implicit_global = true solves the source globally
+ * after each stage (Source/imex_solver.H: Newton + GMRES, single level).
+ * Keys outside this build (amr.max_level > 1, ref_ratio != 2, implicit_global
+ * with MOOD, AMR or a box list) fail with ZAMR_EUNSUP. */
 int zamr_create(const char* inputs_text, zamr_ctx** out);
 /* Batched mode: nbox independent boxes sharing one launch per RK stage.
  * inputs_text supplies the solver.* / params.* keys (geometry keys ignored). */

@@ -1,0 +1,91 @@
+"""solver.implicit_global = true (ZoomyAmr.cpp:647-651, Source/imex_solver.H:
+102-160): after each stage's flux update the source is solved globally by
+Newton + GMRES with a finite-difference Jacobian-vector product.  Through the
+C-ABI, against the oracle's restatement and against closed forms.
+
+Both sides stop Newton at |F| < 1e-9 |F_0| (2-norm over the level) and GMRES
+at 1e-4 relative; their reductions sum in different orders, so the two solves
+agree to that tolerance, not bit for bit: the bar is 1e-10 of the state's
+scale, far below the O(dt^2) gap to the point-implicit step."""
+import numpy as np
+import pytest
+
+from helpers import rel_linf
+from oracle.pyoracle import Oracle
+from zamr import cases
+from zamr.deck import deck_from_case
+from zamr.lib import ZoomyAmr
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n=48, nm=0.08, steps=6, order=2, implicit_global=True, mgs=24):
+    case, Q0 = cases.swe_bumps(n=n, seed=3, nbumps=4, steps=steps, manning=nm)
+    case.update(implicit_global=implicit_global, spatial_order=order, max_grid_size=mgs, well_balanced=False,
+                clamp_positivity=False)
+    Q0 = Q0.copy()
+    x = np.linspace(0, 1, n)
+    Q0[2] = 0.3 * Q0[1] * np.sin(2 * np.pi * x)[None, :]
+    Q0[3] = -0.2 * Q0[1] * np.cos(2 * np.pi * x)[:, None]
+    return case, Q0
+
+
+def _gpu(case, Q0, contract):
+    z = ZoomyAmr(deck_from_case(case), system="swe2d", dim=2, order=case["spatial_order"], contract=contract)
+    z.init_data()
+    z.set_state(Q0)
+    return z
+
+
+@pytest.mark.parametrize("contract,order", [(False, 2), (True, 2), (False, 1)])
+def test_global_source_steps_match_oracle(contract, order):
+    case, Q0 = _case(order=order)
+    print(f"[jfnk] steps vs oracle contract={contract} order={order}", flush=True)
+    o = Oracle("swe2d", case)
+    o.set_state(Q0)
+    n, t, dts, _ = o.evolve(step_limit=6)
+    z = _gpu(case, Q0, contract)
+    done, tg = z.run_steps(6)
+    assert done == n == 6
+    hd, _ = z.dt_history()
+    assert np.allclose(hd[:6], dts, rtol=1e-10, atol=0)
+    Q = z.get_state()
+    assert rel_linf(Q, o.get_state()) <= 1e-10
+    # not the point-implicit run
+    p = Oracle("swe2d", dict(case, implicit_global=False))
+    p.set_state(Q0)
+    p.evolve(step_limit=6)
+    assert rel_linf(Q, p.get_state()) > 1e3 * max(rel_linf(Q, o.get_state()), 1e-16)
+
+
+def test_stage_is_the_closed_form_backward_euler():
+    """One stage through zamr_stage: the momentum is the exact backward-Euler
+    solution of the Manning source per cell (|m| (1 + dt c |m|) = |m_exp|)."""
+    g, nm, eps, dt = 9.81, 0.08, 1e-4, 2e-3
+    case, Q0 = _case(nm=nm)
+    print("[jfnk] closed-form stage", flush=True)
+    f = _gpu(dict(case, params={"n_m": 0.0}), Q0, False)
+    f.stage(2, 0.0, dt)
+    Qexp = f.get_state()
+    z = _gpu(case, Q0, False)
+    z.stage(2, 0.0, dt)
+    Q = z.get_state()
+    h = Qexp[1]
+    hinv = np.sqrt(2.0) * h / np.sqrt(h ** 4 + np.maximum(h ** 4, eps ** 4))
+    c = g * nm * nm * hinv * np.maximum(h, eps) ** (-4.0 / 3.0)
+    s = np.hypot(Qexp[2], Qexp[3])
+    sc = np.where(s > 0, 2.0 / (1.0 + np.sqrt(1.0 + 4.0 * dt * c * s)), 1.0)
+    assert np.array_equal(Q[:2], Qexp[:2])
+    err = max(np.abs(Q[2] - Qexp[2] * sc).max(), np.abs(Q[3] - Qexp[3] * sc).max())
+    corr = np.sqrt(np.sum((Qexp[2] - Qexp[2] * sc) ** 2) + np.sum((Qexp[3] - Qexp[3] * sc) ** 2))
+    assert err <= 1e-9 * corr, (err, corr)
+
+
+def test_no_source_equals_point_implicit_bitwise():
+    case, Q0 = _case(nm=0.0)
+    print("[jfnk] no source", flush=True)
+    a = _gpu(case, Q0, False)
+    a.run_steps(6)
+    b = _gpu(dict(case, implicit_global=False), Q0, False)
+    b.run_steps(6)
+    assert np.array_equal(a.get_state(), b.get_state())

@@ -393,6 +393,8 @@ struct AdjointDriver {
         if (s.spatial_order != ORDER) throw Error(ZAMR_EUNSUP, "the adjoint is built for solver.spatial_order = 2");
         if (s.well_balanced) throw Error(ZAMR_EUNSUP, "the adjoint does not cover solver.well_balanced = true");
         if (!s.implicit_source) throw Error(ZAMR_EUNSUP, "the adjoint is built for solver.implicit_source = true");
+        if (s.implicit_global)
+            throw Error(ZAMR_EUNSUP, "the adjoint reverses the point-implicit source (solver.implicit_global = false)");
         if (s.clamp_positivity) throw Error(ZAMR_EUNSUP, "the adjoint needs solver.clamp_positivity = false");
         if (s.positivity != "none") throw Error(ZAMR_EUNSUP, "the adjoint needs solver.positivity = none");
         const HostBox& b = s.hb[0];

@@ -435,6 +435,7 @@ class AmrDriver {
         pp.query("tagging.threshold", tag_threshold);
         pp.query("tagging.b_max", tag_b_max);
         if (M::dimension != 2) throw Error(5, "AMR (amr.max_level > 0) is built for two-dimensional models only");
+        if (L0.jfnk_mode()) throw Error(5, "AMR with solver.implicit_global = true is not part of this build");
         if (ref_ratio != 2) throw Error(5, "amr.ref_ratio must be 2 in this build");
         if (blocking_factor < 1 || n_error_buf < 0 || !(grid_eff > 0.0) || grid_eff > 1.0)
             throw Error(1, "bad amr.blocking_factor / amr.n_error_buf / amr.grid_eff");

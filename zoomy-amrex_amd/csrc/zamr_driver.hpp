@@ -21,21 +21,11 @@
 #include <vector>
 
 #include "zamr_deck.hpp"
+#include "zamr_error.hpp"
+#include "zamr_jfnk.hpp"
 #include "zamr_kernels.hpp"
 
 namespace zamr {
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define ZAMR_HIP(call)                                                                                  \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            throw ::zamr::Error(4, std::string(#call) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
 
 struct HostBox {
     int lo[2], hi[2];
@@ -92,6 +82,7 @@ class Solver {
 
     bool batched = false;
     std::vector<HostBox> hb;
+    std::unique_ptr<Jfnk<M>> jfnk;  // solver.implicit_global: the global source solve
     int ngroups = 1;
 
     // --- device state ---
@@ -143,6 +134,7 @@ class Solver {
     ~Solver() { release(); }
 
     void release() {
+        jfnk.reset();
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
         for (auto& it : items_)
@@ -196,8 +188,8 @@ class Solver {
         pp.query("solver.face_based_rhs", face_based_rhs);
         pp.query("solver.profile_split", face_based_rhs);
         if (spatial_order != 1 && spatial_order != 2) throw Error(1, "solver.spatial_order must be 1 or 2");
-        if (implicit_global)
-            throw Error(5, "solver.implicit_global = true (matrix-free JFNK source solve) is not part of this build");
+        if (implicit_global && positivity == "mood" && spatial_order == 2)
+            throw Error(5, "solver.implicit_global = true with solver.positivity = mood is not part of this build");
         if (positivity != "none" && positivity != "mood") throw Error(1, "solver.positivity must be none or mood");
         auto defs = M::default_parameters();
         for (int i = 0; i < np; ++i) prm.v[i] = defs[i];
@@ -371,7 +363,7 @@ class Solver {
             b.off_q[1] = off;
             off = align(off + cells * nq);
             b.off_q[2] = b.off_q[1];
-            if (mood()) {  // Q^n survives the order-2 candidate for the redo
+            if (mood() || jfnk_mode()) {  // Q^n survives the order-2 candidate for the redo / the stage-2 solve
                 b.off_q[2] = off;
                 off = align(off + cells * nq);
             }
@@ -412,6 +404,13 @@ class Solver {
         upload_groups();
         ZAMR_HIP(hipMalloc(&d_hist, 2 * (size_t)hist_cap * sizeof(double)));
         build_rects();
+        if (jfnk_mode()) {
+            if (batched) throw Error(5, "solver.implicit_global = true is not part of the batched box list");
+            std::vector<std::array<int, 4>> bx;
+            for (auto& b : hb) bx.push_back({b.lo[0], b.lo[1], b.hi[0], b.hi[1]});
+            jfnk = std::make_unique<Jfnk<M>>();
+            jfnk->setup(stream, d_boxes, bx);
+        }
         ZAMR_HIP(hipStreamSynchronize(stream));
     }
 
@@ -661,6 +660,9 @@ class Solver {
     }
 
     bool mood() const { return positivity == "mood" && spatial_order == 2; }
+    // The source is solved globally after each stage (ZoomyAmr.cpp:647-651).
+    bool jfnk_mode() const { return implicit_source && implicit_global; }
+    int nbuf() const { return (mood() || jfnk_mode()) ? 3 : 2; }
 
     // ---------------- kernels ----------------
     int hx_of(int order) const { return order == 2 ? 2 : 1; }
@@ -673,7 +675,7 @@ class Solver {
         a.dst = dst;
         a.old = old;
         a.wb = well_balanced ? 1 : 0;
-        a.impl = implicit_source ? 1 : 0;
+        a.impl = implicit_source ? (implicit_global ? 2 : 1) : 0;
         a.clamp = (clamp_positivity && positivity != "mood") ? 1 : 0;
         a.mood = 0;
         a.pre = nullptr;
@@ -761,7 +763,50 @@ class Solver {
 
     // The stage kernels of Advance for an order-`order` step on the device
     // clock (dt and time read from the group state).
+    // End of a step whose stages solved the source globally (Heun with
+    // buffer old, or none at order 1; UpdateState; CFL max) in place on dst.
+    void finish_step(int dst, int old) {
+        Items& it = items(hx_of(spatial_order));
+        hipLaunchKernelGGL((finish_step_kernel<M, N>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
+                           dst, old, prm, clamp_flag(), d_groups, WV - 2 * it.hx);
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    // Advance with solver.implicit_global: each stage is the flux update
+    // followed by the global source solve on its output; the host drives
+    // Newton/GMRES, so dt and time come down once per step.
+    void advance_global_source() {
+        aux_stale = true;
+        download_groups();
+        const GroupState& g = h_groups[0];
+        if (!g.active) return;
+        const double dt = g.dt, time = g.time;
+        const int t1 = (cur + 1) % 3, t2 = (cur + 2) % 3;
+        if (spatial_order == 1) {
+            fill_boundary(cur);
+            launch_stage_dyn(1, 0, stage_args(cur, t1, cur));
+            jfnk->solve(t1, dt, time, prm);
+            finish_step(t1, -1);
+            last_in = cur;
+            cur = t1;
+        } else {
+            fill_boundary(cur);
+            launch_stage_dyn(2, 0, stage_args(cur, t1, cur));
+            jfnk->solve(t1, dt, time, prm);
+            fill_boundary(t1);
+            launch_stage_dyn(2, 0, stage_args(t1, t2, cur));
+            jfnk->solve(t2, dt, time, prm);
+            finish_step(t2, cur);
+            last_in = t1;
+            cur = t2;
+        }
+    }
+
     void advance_kernels() {
+        if (jfnk_mode()) {
+            advance_global_source();
+            return;
+        }
         aux_stale = true;
         if (spatial_order == 1) {
             fill_boundary(cur);
@@ -822,7 +867,7 @@ class Solver {
 
     // Graph replay for the device-clock loop; the per-box launch mode stays
     // eager, as the baseline it stands for (one launch per box per pass).
-    bool graph_ok() const { return spatial_order == 2 && !profiling && !per_box_launch; }
+    bool graph_ok() const { return spatial_order == 2 && !profiling && !per_box_launch && !jfnk_mode(); }
 
     void run_steps(int nsteps, int* done, double* time) {
         download_groups();
@@ -926,8 +971,9 @@ class Solver {
         materialize_aux();  // do_stage's UpdateState leaves Qaux = update_aux(stage-start state)
         set_clock(dt, time);
         fill_boundary(cur);
-        const int nxt = (cur + 1) % (mood() ? 3 : 2);
+        const int nxt = (cur + 1) % nbuf();
         launch_stage_dyn(order, 0, stage_args(cur, nxt, cur));
+        if (jfnk_mode()) jfnk->solve(nxt, dt, time, prm);
         last_in = cur;
         cur = nxt;
         ZAMR_HIP(hipStreamSynchronize(stream));

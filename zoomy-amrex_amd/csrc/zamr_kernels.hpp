@@ -112,7 +112,8 @@ struct GroupState {
 struct StageArgs {
     Params p;
     int src, dst, old;           // buffer indices: read, write, Heun's Q^n
-    int wb, impl, clamp;         // well_balanced, implicit_source, clamp (positivity)
+    int wb, impl, clamp;         // well_balanced, implicit_source (1 point-implicit solve here;
+                                 // 2 solved globally after the stage: no source here), clamp
     int mood;                    // 0; 1 = count the troubled cells of the candidate
                                  // (stage 2); 2 = order-1 redo, troubled groups only
     double* pre;                 // optional: the pre-implicit state Q + dt RHS of a
@@ -768,7 +769,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
 #pragma unroll
                     for (int n = 0; n < nq; ++n) a.pre[idx + n * B.sc] = qn(n, 0);
                 }
-                if (a.impl) qn = S::implicit_source(qn, ac, dt, p, time, X);
+                if (a.impl == 1) qn = S::implicit_source(qn, ac, dt, p, time, X);
                 if constexpr (EPI == 1) {
 #pragma unroll
                     for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qn_row(n, 0) + 0.5 * qn(n, 0);

@@ -104,6 +104,7 @@ def deck_from_case(case, amr=None):
              f"solver.dtmax = {c.get('dtmax', 1e-2)!r}",
              f"solver.spatial_order = {c.get('spatial_order', 1)}",
              f"solver.implicit_source = {'true' if c.get('implicit_source', True) else 'false'}",
+             f"solver.implicit_global = {'true' if c.get('implicit_global', False) else 'false'}",
              f"solver.well_balanced = {'true' if c.get('well_balanced', False) else 'false'}",
              f"solver.clamp_positivity = {'true' if c.get('clamp_positivity', True) else 'false'}",
              f"solver.positivity = {c.get('positivity', 'none')}"]

@@ -253,6 +253,46 @@ def bench_amr(steps=10, warmup=2, n=2048):
             "speedup_vs_uniform_fine": t_uni / t_total}
 
 
+def bench_jfnk(steps=10, warmup=2, n=1024):
+    """solver.implicit_global on the seeded-bump workload at n^2 with moving
+    water (Manning 0.03): ms per SSP-RK2 step with the global Newton + GMRES
+    source solve after each stage, against the point-implicit step of the
+    same deck, and the iteration counts."""
+    import numpy as np
+    import torch
+    from zamr import cases
+    from zamr.deck import deck_from_case
+    from zamr.lib import ZoomyAmr
+
+    case, Q0 = cases.swe_bumps(n=n, seed=0, steps=2 * (steps + warmup) + 1)
+    Q0 = Q0.copy()
+    x = np.linspace(0.0, 1.0, n)
+    Q0[2] = 0.3 * Q0[1] * np.sin(2 * np.pi * x)[None, :]
+    Q0[3] = -0.2 * Q0[1] * np.cos(2 * np.pi * x)[:, None]
+    out = {}
+    for label, glob in (("global", True), ("point", False)):
+        z = ZoomyAmr(deck_from_case(dict(case, implicit_global=glob)), system="swe2d", dim=2, order=2)
+        z.init_data()
+        z.set_state(Q0)
+        z.run_steps(warmup)
+        torch.cuda.synchronize()
+        s0 = z.jfnk_stats()
+        t0 = time.perf_counter()
+        done, _ = z.run_steps(steps)
+        torch.cuda.synchronize()
+        out[label] = (time.perf_counter() - t0) / done
+        if glob:
+            s1 = z.jfnk_stats()
+            out["solves"], out["newton"], out["gmres"] = (s1[0] - s0[0], s1[1] - s0[1], s1[2] - s0[2])
+        assert np.isfinite(z.get_state()).all()
+        del z
+    return {"workload": f"2-D SWE {n}x{n}, 32 seeded bumps, moving water, Manning 0.03, well-balanced order 2, "
+                        "solver.implicit_global = true (Newton + GMRES(30) after each stage)",
+            "value": 1e3 * out["global"], "unit": "ms/step", "point_implicit_ms_per_step": 1e3 * out["point"],
+            "steps": steps, "newton_per_solve": out["newton"] / max(out["solves"], 1),
+            "gmres_per_solve": out["gmres"] / max(out["solves"], 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,7 +301,7 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme", "amr"],
+    ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme", "amr", "jfnk"],
                     help="swe4096: the single-level 4096^2 workload (default); boxlist: 600 AMR patches + "
                          "64 ensemble members, batched launch vs one launch per box; adjoint: the 1024^2 "
                          "gauge-inversion gradient alone")
@@ -286,6 +326,10 @@ def main():
     if args.workload == "sme":
         print(json.dumps({"metric": "cell-updates/s per RK stage (FP64) + % HBM roofline", "n_gpus": 1,
                           "dtype": "f64", "data": "synthetic", "higher_is_better": True, **bench_sme()}), flush=True)
+        return
+    if args.workload == "jfnk":
+        print(json.dumps({"metric": "ms per SSP-RK2 step with the global implicit-source solve", "n_gpus": 1,
+                          "dtype": "f64", "data": "synthetic", "higher_is_better": False, **bench_jfnk()}), flush=True)
         return
     if args.workload == "amr":
         print(json.dumps({"metric": "cell-updates/s per RK stage (FP64), two-level AMR", "n_gpus": 1,

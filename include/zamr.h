@@ -168,6 +168,10 @@ int zamr_error_est(zamr_ctx* ctx, unsigned char* tags, size_t n);
  * creation. */
 int zamr_amr_stats(zamr_ctx* ctx, long long* regrids, long long* remakes, double* regrid_ms);
 
+/* solver.implicit_global (Source/imex_solver.H): global source solves run,
+ * and the Newton and GMRES iterations they took, since creation. */
+int zamr_jfnk_stats(zamr_ctx* ctx, long long* solves, long long* newton, long long* gmres);
+
 /* (dt, time after the step) of the steps taken on the device clock since the
  * last zamr_reset_clock, up to cap entries — the values the reference prints
  * per step (ZoomyAmr.cpp:804-809).  *n receives the count. */

@@ -229,7 +229,8 @@ class Jfnk {
     double* d_coef = nullptr;      // kRestart + 1
     double** d_Vsel = nullptr;     // pointer list for one reduction / update
     double* h_pin = nullptr;       // pinned: dots down, coefficients up
-    long newton_its = 0, gmres_its = 0;
+    long newton_its = 0, gmres_its = 0;          // last solve
+    long long total_solves = 0, total_newton = 0, total_gmres = 0;
 
     ~Jfnk() { release(); }
     void release() {
@@ -410,6 +411,9 @@ class Jfnk {
         }
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 1);
         ZAMR_HIP(hipGetLastError());
+        ++total_solves;
+        total_newton += newton_its;
+        total_gmres += gmres_its;
     }
 };
 

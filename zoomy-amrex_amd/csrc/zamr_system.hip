@@ -529,6 +529,15 @@ int zamr_error_est(zamr_ctx* c, unsigned char* tags, size_t n) {
     });
 }
 
+int zamr_jfnk_stats(zamr_ctx* c, long long* solves, long long* newton, long long* gmres) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    const auto* j = c->s.jfnk.get();
+    if (solves) *solves = j ? j->total_solves : 0;
+    if (newton) *newton = j ? j->total_newton : 0;
+    if (gmres) *gmres = j ? j->total_gmres : 0;
+    return ZAMR_OK;
+}
+
 int zamr_amr_stats(zamr_ctx* c, long long* regrids, long long* remakes, double* regrid_ms) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     if (regrids) *regrids = c->amr ? c->amr->regrids : 0;

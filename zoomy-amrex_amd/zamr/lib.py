@@ -73,6 +73,8 @@ def load_library(path):
         "zamr_dt_history": [_VP, ctypes.c_int, _D, _D, _I],
         "zamr_mood_stats": [_VP, ctypes.c_int, _I, ctypes.POINTER(ctypes.c_longlong)],
         "zamr_finest_level": [_VP, _I],
+        "zamr_jfnk_stats": [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                            ctypes.POINTER(ctypes.c_longlong)],
         "zamr_level_boxes": [_VP, ctypes.c_int, _I, ctypes.c_int, _I],
         "zamr_init_levels": [_VP],
         "zamr_regrid": [_VP, ctypes.c_double],
@@ -174,6 +176,12 @@ class ZoomyAmr:
         a = np.full((nc, ny, nx), fill) if lev > 0 else np.empty((nc, ny, nx))
         self._check(self.lib.zamr_get_state(self.ctx, lev, field, _dp(a), a.size))
         return a
+
+    def jfnk_stats(self):
+        """(global source solves, Newton iterations, GMRES iterations) so far."""
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.zamr_jfnk_stats(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     # ---- AMR hierarchy ----
     def finest_level(self):

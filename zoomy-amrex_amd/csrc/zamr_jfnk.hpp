@@ -74,17 +74,29 @@ __global__ void __launch_bounds__(256) jf_move_kernel(const BoxDesc* __restrict_
 }
 
 // S(q) of every cell (mode x == nullptr: out = S(q)), or the Jacobian
-// product out = (x + (-dt/eps) S(q + eps x)) + (dt/eps) Sbase.
+// product out = (x + (-dt/eps) S(q + eps x)) + (dt/eps) Sbase with
+// eps = eps_num / |x|, |x|^2 read from the device (*x2); |x| = 0 gives 0.
 template <class M>
 __global__ void __launch_bounds__(256) jf_source_kernel(const BoxDesc* __restrict__ boxes,
                                                         const long long* __restrict__ bstart, int nb, long long N,
                                                         const double* __restrict__ q, const double* __restrict__ x,
-                                                        double eps, double dt, const double* __restrict__ Sbase,
-                                                        Params pp, double time, double* __restrict__ out) {
+                                                        double eps_num, const double* __restrict__ x2, double dt,
+                                                        const double* __restrict__ Sbase, Params pp, double time,
+                                                        double* __restrict__ out) {
     using S = Sys<M>;
     constexpr int nq = S::nq;
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
+    double eps = 0.0;
+    if (x) {
+        const double nx = sqrt(*x2);
+        if (nx == 0.0) {
+#pragma unroll
+            for (int n = 0; n < nq; ++n) out[n * N + c] = 0.0;
+            return;
+        }
+        eps = eps_num / nx;
+    }
     typename S::VP p;
     for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
     const JfCell e = jf_cell(boxes, bstart, nb, c);
@@ -114,15 +126,23 @@ __global__ void __launch_bounds__(256) jf_lincomb_kernel(long long len, double a
     if (k < len) out[k] = a * x[k] + b * y[k];
 }
 
-// w += sum_j coef[j] V[j]  (coef on the device, m <= 31).
+// w += sign * sum_j coef[j] V[j]  (coef on the device, m <= 31).
 __global__ void __launch_bounds__(256) jf_multi_axpy_kernel(long long len, int m, double* const* __restrict__ V,
-                                                            const double* __restrict__ coef,
+                                                            const double* __restrict__ coef, double sign,
                                                             double* __restrict__ w) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= len) return;
     double acc = w[k];
-    for (int j = 0; j < m; ++j) acc += coef[j] * V[j][k];
+    for (int j = 0; j < m; ++j) acc += (sign * coef[j]) * V[j][k];
     w[k] = acc;
+}
+
+// out = x / sqrt(*x2) when *x2 > 0.
+__global__ void __launch_bounds__(256) jf_normalize_kernel(long long len, const double* __restrict__ x,
+                                                           const double* __restrict__ x2, double* __restrict__ out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double n2 = *x2;
+    if (k < len && n2 > 0.0) out[k] = x[k] / sqrt(n2);
 }
 
 // partial[j * kJfnkBlocks + blk] = this block's share of <V[j], w>.
@@ -144,6 +164,23 @@ __global__ void __launch_bounds__(256) jf_dots_kernel(long long len, int m, doub
         if (threadIdx.x == 0) partial[j * kJfnkBlocks + blockIdx.x] = red[0];
         __syncthreads();
     }
+}
+
+// partial[blk] = this block's share of <a, b> (pointers passed directly).
+__global__ void __launch_bounds__(256) jf_dot_kernel(long long len, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* __restrict__ partial) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < len;
+         k += (long long)gridDim.x * blockDim.x)
+        s += a[k] * b[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 __global__ void jf_sum_partials_kernel(int m, const double* __restrict__ partial, double* __restrict__ out) {
@@ -207,13 +244,18 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
     }
 }
 
-// Host side: the vectors, Newton and GMRES.
+// Host side: the vectors, Newton and GMRES.  Scalars that only steer device
+// work (Gram-Schmidt coefficients, norms for eps and normalisation) stay on
+// the device; one transfer per Arnoldi step brings down what the host needs
+// for the Hessenberg matrix and the convergence test.
 template <class M>
 class Jfnk {
    public:
     static constexpr int nq = M::n_dof_q;
     static constexpr int kRestart = 30;
     static constexpr long kMaxIter = 2000;
+    // device scalar slots
+    static constexpr int kH1 = 0, kH2 = 32, kHN = 64, kNX = 65, kF2 = 66, kQ2 = 67, kR2 = 68, kSlots = 72;
 
     hipStream_t stream = nullptr;
     const BoxDesc* boxes = nullptr;
@@ -221,30 +263,28 @@ class Jfnk {
     long long N = 0, len = 0;  // cells, doubles per vector
     long long* d_bstart = nullptr;
     double *q = nullptr, *qexp = nullptr, *S = nullptr, *F = nullptr, *rhs = nullptr, *x = nullptr, *w = nullptr,
-           *r = nullptr, *tmp = nullptr;
+           *r = nullptr;
     std::vector<double*> V;        // basis vectors, kRestart + 1
-    double** d_V = nullptr;        // device copy of V's pointers
     double* d_partial = nullptr;   // (kRestart + 2) x kJfnkBlocks
-    double* d_dots = nullptr;      // kRestart + 2
-    double* d_coef = nullptr;      // kRestart + 1
-    double** d_Vsel = nullptr;     // pointer list for one reduction / update
-    double* h_pin = nullptr;       // pinned: dots down, coefficients up
+    double* d_sc = nullptr;        // kSlots device scalars
+    double* d_y = nullptr;         // kRestart coefficients (uploaded)
+    double** d_Vsel = nullptr;     // pointer lists, one row per use
+    double* h_pin = nullptr;       // pinned: scalars down, coefficients up
     long newton_its = 0, gmres_its = 0;          // last solve
     long long total_solves = 0, total_newton = 0, total_gmres = 0;
 
     ~Jfnk() { release(); }
     void release() {
-        for (double* p : {q, qexp, S, F, rhs, x, w, r, tmp, d_partial, d_dots, d_coef})
+        for (double* p : {q, qexp, S, F, rhs, x, w, r, d_partial, d_sc, d_y})
             if (p) hipFree(p);
         for (double* p : V)
             if (p) hipFree(p);
         V.clear();
-        if (d_V) hipFree(d_V);
         if (d_Vsel) hipFree(d_Vsel);
         if (d_bstart) hipFree(d_bstart);
         if (h_pin) hipHostFree(h_pin);
-        q = qexp = S = F = rhs = x = w = r = tmp = d_partial = d_dots = d_coef = nullptr;
-        d_V = d_Vsel = nullptr;
+        q = qexp = S = F = rhs = x = w = r = d_partial = d_sc = d_y = nullptr;
+        d_Vsel = nullptr;
         d_bstart = nullptr;
         h_pin = nullptr;
     }
@@ -261,68 +301,57 @@ class Jfnk {
         len = N * nq;
         ZAMR_HIP(hipMalloc(&d_bstart, st.size() * sizeof(long long)));
         ZAMR_HIP(hipMemcpy(d_bstart, st.data(), st.size() * sizeof(long long), hipMemcpyHostToDevice));
-        for (double** p : {&q, &qexp, &S, &F, &rhs, &x, &w, &r, &tmp}) ZAMR_HIP(hipMalloc(p, len * sizeof(double)));
+        for (double** p : {&q, &qexp, &S, &F, &rhs, &x, &w, &r}) ZAMR_HIP(hipMalloc(p, len * sizeof(double)));
         V.assign(kRestart + 1, nullptr);
         for (auto& v : V) ZAMR_HIP(hipMalloc(&v, len * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_V, V.size() * sizeof(double*)));
-        ZAMR_HIP(hipMemcpy(d_V, V.data(), V.size() * sizeof(double*), hipMemcpyHostToDevice));
-        ZAMR_HIP(hipMalloc(&d_Vsel, (kRestart + 2) * sizeof(double*)));
+        // pointer-list rows: row k (k <= kRestart) = V[0..k]
+        ZAMR_HIP(hipMalloc(&d_Vsel, (size_t)(kRestart + 1) * (kRestart + 1) * sizeof(double*)));
+        std::vector<double*> rows((size_t)(kRestart + 1) * (kRestart + 1), nullptr);
+        for (int k = 0; k <= kRestart; ++k)
+            for (int j = 0; j <= k; ++j) rows[(size_t)k * (kRestart + 1) + j] = V[j];
+        ZAMR_HIP(hipMemcpy(d_Vsel, rows.data(), rows.size() * sizeof(double*), hipMemcpyHostToDevice));
         ZAMR_HIP(hipMalloc(&d_partial, (kRestart + 2) * (size_t)kJfnkBlocks * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_dots, (kRestart + 2) * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_coef, (kRestart + 2) * sizeof(double)));
-        ZAMR_HIP(hipHostMalloc(&h_pin, 2 * (kRestart + 2) * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_sc, kSlots * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_y, kRestart * sizeof(double)));
+        ZAMR_HIP(hipHostMalloc(&h_pin, (kSlots + kRestart) * sizeof(double)));
     }
 
     static unsigned g1(long long n) { return (unsigned)((n + 255) / 256); }
+    double* const* basis(int k) const { return d_Vsel + (size_t)k * (kRestart + 1); }  // V[0..k]
 
-    // <V[j], y> for the m vectors vs[0..m-1], one pass over y.
-    std::vector<double> dots(const std::vector<const double*>& vs, const double* y) {
-        const int m = (int)vs.size();
-        ZAMR_HIP(hipMemcpyAsync(d_Vsel, vs.data(), m * sizeof(double*), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(jf_dots_kernel, dim3(kJfnkBlocks), dim3(256), 0, stream, len, m, (double* const*)d_Vsel, y,
-                           d_partial);
-        hipLaunchKernelGGL(jf_sum_partials_kernel, dim3(1), dim3(64), 0, stream, m, d_partial, d_dots);
+    // out[j] = <vs[j], y> for the m vectors of a pointer row, on the device.
+    void dots_dev(double* const* vs, int m, const double* y, double* out) {
+        hipLaunchKernelGGL(jf_dots_kernel, dim3(kJfnkBlocks), dim3(256), 0, stream, len, m, vs, y, d_partial);
+        hipLaunchKernelGGL(jf_sum_partials_kernel, dim3(1), dim3(64), 0, stream, m, d_partial, out);
         ZAMR_HIP(hipGetLastError());
-        ZAMR_HIP(hipMemcpyAsync(h_pin, d_dots, m * sizeof(double), hipMemcpyDeviceToHost, stream));
-        ZAMR_HIP(hipStreamSynchronize(stream));
-        return std::vector<double>(h_pin, h_pin + m);
     }
-    double norm(const double* y) { return std::sqrt(dots({y}, y)[0]); }
-
-    // y += sum_j c[j] vs[j]
-    void axpy(double* y, const std::vector<const double*>& vs, const std::vector<double>& c) {
-        const int m = (int)vs.size();
-        double* hc = h_pin + (kRestart + 2);
-        for (int j = 0; j < m; ++j) hc[j] = c[j];
-        ZAMR_HIP(hipMemcpyAsync(d_coef, hc, m * sizeof(double), hipMemcpyHostToDevice, stream));
-        ZAMR_HIP(hipMemcpyAsync(d_Vsel, vs.data(), m * sizeof(double*), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, m,
-                           (double* const*)d_Vsel, d_coef, y);
+    // <y, y> into a device slot.
+    void norm2_dev(const double* y, double* out) {
+        hipLaunchKernelGGL(jf_dot_kernel, dim3(kJfnkBlocks), dim3(256), 0, stream, len, y, y, d_partial);
+        hipLaunchKernelGGL(jf_sum_partials_kernel, dim3(1), dim3(64), 0, stream, 1, d_partial, out);
         ZAMR_HIP(hipGetLastError());
-        ZAMR_HIP(hipStreamSynchronize(stream));  // the pinned coefficients are reused
+    }
+    // Device scalars [first, first + n) to the host.
+    void fetch(int first, int n) {
+        ZAMR_HIP(hipMemcpyAsync(h_pin + first, d_sc + first, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        ZAMR_HIP(hipStreamSynchronize(stream));
     }
     void lincomb(double* out, double a, const double* xx, double b, const double* yy) {
         hipLaunchKernelGGL(jf_lincomb_kernel, dim3(g1(len)), dim3(256), 0, stream, len, a, xx, b, yy, out);
         ZAMR_HIP(hipGetLastError());
     }
 
-    // out = J v at the frozen iterate (q, S).
+    // out = J v at the frozen iterate (q, S); |v|^2 is computed into slot kNX.
     void apply(const double* v, double* out, double dt, double nrmQ, const Params& p, double time) {
-        const double nx = norm(v);
-        if (nx == 0.0) {
-            ZAMR_HIP(hipMemsetAsync(out, 0, len * sizeof(double), stream));
-            return;
-        }
-        const double eps = 1e-7 * (1.0 + nrmQ) / nx;
+        norm2_dev(v, d_sc + kNX);
         hipLaunchKernelGGL((jf_source_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, q, v,
-                           eps, dt, S, p, time, out);
+                           1e-7 * (1.0 + nrmQ), (const double*)(d_sc + kNX), dt, S, p, time, out);
         ZAMR_HIP(hipGetLastError());
     }
 
-    // x = GMRES(J, b) to rtol (zero initial guess); returns iterations.
-    long gmres(double dt, double nrmQ, const Params& p, double time, double rtol) {
+    // x = GMRES(J, rhs) to rtol (zero initial guess), |rhs| = r0 known.
+    long gmres(double r0, double dt, double nrmQ, const Params& p, double time, double rtol) {
         ZAMR_HIP(hipMemsetAsync(x, 0, len * sizeof(double), stream));
-        const double r0 = norm(rhs);
         if (r0 == 0.0) return 0;
         auto conv = [&](double rr) { return rr < r0 * rtol; };
         const double* rr = rhs;  // the residual of x = 0
@@ -337,19 +366,21 @@ class Jfnk {
             bool done = false;
             for (int k = 0; k < kRestart; ++k) {
                 apply(V[k], w, dt, nrmQ, p, time);
-                std::vector<const double*> basis(V.begin(), V.begin() + k + 1);
-                for (int pass = 0; pass < 2; ++pass) {
-                    std::vector<double> h = dots(basis, w);
-                    std::vector<double> neg(k + 1);
-                    for (int j = 0; j <= k; ++j) {
-                        neg[j] = -h[j];
-                        H[j][k] += h[j];
-                    }
-                    axpy(w, basis, neg);
-                }
-                const double hn = norm(w);
+                // classical Gram-Schmidt, twice, coefficients kept on the device
+                dots_dev(basis(k), k + 1, w, d_sc + kH1);
+                hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, k + 1, basis(k),
+                                   (const double*)(d_sc + kH1), -1.0, w);
+                dots_dev(basis(k), k + 1, w, d_sc + kH2);
+                hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, k + 1, basis(k),
+                                   (const double*)(d_sc + kH2), -1.0, w);
+                norm2_dev(w, d_sc + kHN);
+                hipLaunchKernelGGL(jf_normalize_kernel, dim3(g1(len)), dim3(256), 0, stream, len, w,
+                                   (const double*)(d_sc + kHN), V[k + 1]);
+                ZAMR_HIP(hipGetLastError());
+                fetch(0, kHN + 1);
+                for (int j = 0; j <= k; ++j) H[j][k] = h_pin[kH1 + j] + h_pin[kH2 + j];
+                const double hn = std::sqrt(h_pin[kHN]);
                 H[k + 1][k] = hn;
-                if (hn > 0.0) lincomb(V[k + 1], 1.0 / hn, w, 0.0, w);
                 for (int j = 0; j < k; ++j) {
                     const double t = cs[j] * H[j][k] + sn[j] * H[j + 1][k];
                     H[j + 1][k] = -sn[j] * H[j][k] + cs[j] * H[j + 1][k];
@@ -375,12 +406,19 @@ class Jfnk {
                 for (int j = i + 1; j < kd; ++j) t -= H[i][j] * y[j];
                 y[i] = H[i][i] != 0.0 ? t / H[i][i] : 0.0;
             }
-            axpy(x, std::vector<const double*>(V.begin(), V.begin() + kd), y);
+            for (int j = 0; j < kd; ++j) h_pin[kSlots + j] = y[j];
+            ZAMR_HIP(hipMemcpyAsync(d_y, h_pin + kSlots, kd * sizeof(double), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, kd, basis(kd - 1),
+                               (const double*)d_y, 1.0, x);
+            ZAMR_HIP(hipGetLastError());
+            ZAMR_HIP(hipStreamSynchronize(stream));  // the pinned coefficients are reused
             if (done) break;
             apply(x, w, dt, nrmQ, p, time);
             lincomb(r, 1.0, rhs, -1.0, w);
             rr = r;
-            rnorm = norm(r);
+            norm2_dev(r, d_sc + kR2);
+            fetch(kR2, 1);
+            rnorm = std::sqrt(h_pin[kR2]);
             if (conv(rnorm) || rnorm == 0.0) break;
         }
         return its;
@@ -396,16 +434,20 @@ class Jfnk {
         gmres_its = 0;
         for (int it = 0; it < 12; ++it) {
             hipLaunchKernelGGL((jf_source_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, q,
-                               (const double*)nullptr, 0.0, 0.0, (const double*)nullptr, p, time, S);
+                               (const double*)nullptr, 0.0, (const double*)nullptr, 0.0, (const double*)nullptr, p,
+                               time, S);
             ZAMR_HIP(hipGetLastError());
             lincomb(F, 1.0, q, -1.0, qexp);
             lincomb(F, 1.0, F, -dt, S);
-            const double resid = norm(F);
+            norm2_dev(F, d_sc + kF2);
+            norm2_dev(q, d_sc + kQ2);
+            fetch(kF2, 2);
+            const double resid = std::sqrt(h_pin[kF2]);
             if (it == 0) resid0 = resid;
             if (resid < 1e-12 || (resid0 > 0 && resid < 1e-9 * resid0)) break;
             lincomb(rhs, -1.0, F, 0.0, F);
-            const double nrmQ = norm(q);
-            gmres_its += gmres(dt, nrmQ, p, time, 1e-4);
+            const double nrmQ = std::sqrt(h_pin[kQ2]);
+            gmres_its += gmres(resid, dt, nrmQ, p, time, 1e-4);
             lincomb(q, 1.0, q, 1.0, x);
             ++newton_its;
         }

@@ -118,7 +118,19 @@ __device__ __forceinline__ void vadd(V& a, const V& b) {
 }
 
 template <class M, class AD, int ORDER>
-__global__ void __launch_bounds__(NT) adjoint_stage_kernel(const BoxDesc* __restrict__ boxes,
+// Occupancy request for the reverse stage kernel (waves per SIMD).  Three
+// waves fit its registers with a few bytes of spill and hide the latency of
+// its DPP shifts and of the stored-stage loads better than the two the
+// compiler picks by itself (1024^2 backward: 49.6 against 60.6 ms).
+#ifndef ZAMR_ADJ_WAVES
+#define ZAMR_ADJ_WAVES 3
+#endif
+#if ZAMR_ADJ_WAVES > 0
+#define ZAMR_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_ADJ_WAVES, ZAMR_ADJ_WAVES)))
+#else
+#define ZAMR_ADJ_ATTR
+#endif
+__global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const BoxDesc* __restrict__ boxes,
                                                            const WorkItem* __restrict__ items, int nitems, AdjArgs a) {
     using S = Sys<M>;
     using A = AdjSys<M, AD>;
@@ -134,7 +146,7 @@ __global__ void __launch_bounds__(NT) adjoint_stage_kernel(const BoxDesc* __rest
     for (int k = 0; k < S::np; ++k) p(k, 0) = a.p.v[k];
 
     const int t = threadIdx.x & (WV - 1);
-    const int it = blockIdx.x * (NT / WV) + threadIdx.x / WV;
+    const int it = blockIdx.x * (NT / WV) + wave_in_block();
     if (it >= nitems) return;
     const WorkItem w = items[it];
     const BoxView B = view(boxes, w.box);

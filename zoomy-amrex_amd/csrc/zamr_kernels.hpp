@@ -34,6 +34,11 @@ constexpr int kMaxParams = 32;
 constexpr int NT = 256;  // threads per workgroup
 constexpr int WV = 64;   // lanes per wave = columns per work item
 
+// This wave's index in its workgroup.  The same on every lane; readfirstlane
+// tells the compiler so, which keeps the item, its box descriptor and the row
+// loop in scalar registers and scalar loads instead of one copy per lane.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WV); }
+
 struct Params {
     double v[kMaxParams];
 };
@@ -466,15 +471,16 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Whole-wave lane shifts (DPP wave_shr / wave_shl): lane i receives lane
-// i-1 (from_left) or lane i+1 (from_right); the edge lane receives 0.
+// i-1 (from_left) or lane i+1 (from_right); the edge lane receives 0, by the
+// bound control bit, so no register has to be cleared first.
 __device__ __forceinline__ double from_left(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double from_right(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -530,7 +536,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
     for (int k = 0; k < S::np; ++k) p(k, 0) = a.p.v[k];
 
     const int t = threadIdx.x & (WV - 1);
-    const int wv = threadIdx.x / WV;
+    const int wv = wave_in_block();
     const int nslots = slots_for(nitems);
     for (int blk = blockIdx.x; blk < nslots; blk += gridDim.x) {
         const int slot = xcd_remap ? remap_slot(blk, nslots) : blk;
@@ -543,7 +549,11 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const int c = w.i0 - HX + t;  // this lane's column
         const bool own = (t >= HX) && (t < HX + TXW) && (c <= B.hi0);       // assembles cells
         const bool xf = (t >= HX) && (t <= HX + TXW) && (c <= B.hi0 + 1);  // computes face c-1/2
-        const bool col_in = c <= B.hi0 + HX;                                // inside the grown box
+        // The column this lane loads: its own, or past the grown box the box's
+        // last ghost column (the values of lanes past that column reach no
+        // face of the box, and the loads stay in bounds and in the edge lane's
+        // cache lines)
+        const int cl = min(c, B.hi0 + HX);
         if (!G->active) {
             // An order-1 step ping-pongs buffers; a finished group carries its
             // state across so that every group's current buffer stays in step.
@@ -594,8 +604,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             VQ P[2 * HX];
 #pragma unroll
             for (int r = 0; r < 2 * HX; ++r) {
-                if (col_in) P[r] = S::load(B, qs, c, jstart - HX + r, p, a.clamp, time);
-                else vq_zero(P[r]);
+                P[r] = S::load(B, qs, cl, jstart - HX + r, p, a.clamp, time);
             }
             if (own) {
                 VQ qL, qR;
@@ -632,7 +641,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         // first leading row
         const int jlead0 = (dim == 2) ? jstart + HX : jstart;
         typename S::Raw nxt;
-        if (col_in) nxt = S::fetch(B, qs, c, jlead0);
+        nxt = S::fetch(B, qs, cl, jlead0);
         // stage 2: Q^n of the row for the Heun combine, also fetched one row ahead
         VQ qn_next;
         if constexpr (EPI == 1) {
@@ -646,10 +655,8 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         int nbad = 0;  // MOOD: troubled cells among this lane's candidate cells
         for (int j = jstart; j < jend; ++j) {
             const int jl = (dim == 2) ? j + HX : j;
-            VQ lead;
-            if (col_in) lead = S::finish(B, nxt, c, jl, p, a.clamp, time);
-            else vq_zero(lead);
-            if (col_in && j + 1 < jend) nxt = S::fetch(B, qs, c, jl + 1);  // one row ahead
+            const VQ lead = S::finish(B, nxt, cl, jl, p, a.clamp, time);
+            if (j + 1 < jend) nxt = S::fetch(B, qs, cl, jl + 1);  // one row ahead
             VQ qn_row;
             if constexpr (EPI == 1) {
                 qn_row = qn_next;
@@ -824,7 +831,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
 // kernels below, which walk the same item list without a halo.
 #define ZAMR_ITEM_LOOP(nitems)                                                            \
     const int t = threadIdx.x & (WV - 1);                                                 \
-    const int wv = threadIdx.x / WV;                                                      \
+    const int wv = wave_in_block();                                                       \
     for (int it = blockIdx.x * (NT / WV) + wv; it < (nitems); it += gridDim.x * (NT / WV))
 
 // ComputeDt's reduction (ZoomyAmr.cpp:465-489) over the state and aux AS

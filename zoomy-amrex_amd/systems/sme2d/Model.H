@@ -108,8 +108,10 @@ struct Model {
         const amrex::Real at[2] = {Q(m(e, 1), 0) * a, Q(m(e, 2), 0) * a};
         res(0, 0) = 0.0;
         res(1, 0) = Q(m(d, 0), 0);
-        res(m(d, 0), 0) = h * (un * un + an[0] * an[0] / 3.0 + an[1] * an[1] / 5.0) + 0.5 * p(0, 0) * h * h;
-        res(m(e, 0), 0) = h * (un * ut + an[0] * at[0] / 3.0 + an[1] * at[1] / 5.0);
+        // the Legendre weights 1/3, 1/5 as products (a division by a constant
+        // is a full IEEE division on the device)
+        res(m(d, 0), 0) = h * (un * un + an[0] * an[0] * (1.0 / 3.0) + an[1] * an[1] * (1.0 / 5.0)) + 0.5 * p(0, 0) * h * h;
+        res(m(e, 0), 0) = h * (un * ut + an[0] * at[0] * (1.0 / 3.0) + an[1] * at[1] * (1.0 / 5.0));
         for (int i = 1; i <= 2; ++i) {
             res(m(d, i), 0) = h * (2.0 * un * an[i - 1] + zamr_A(i, an, an));
             res(m(e, i), 0) = h * (un * at[i - 1] + ut * an[i - 1] + zamr_A(i, at, an));
@@ -153,7 +155,7 @@ struct Model {
             const amrex::Real mean = Q(m(c, 0), 0) * a;
             const amrex::Real mk[2] = {Q(m(c, 1), 0) * a, Q(m(c, 2), 0) * a};
             // B_ijk for (i, j, k): B_112 = 1/5, B_121 = -1/5, B_211 = -1, B_222 = -1/7
-            const amrex::Real Bm[2][2] = {{0.2 * mk[1], -0.2 * mk[0]}, {-mk[0], -mk[1] / 7.0}};
+            const amrex::Real Bm[2][2] = {{0.2 * mk[1], -0.2 * mk[0]}, {-mk[0], -mk[1] * (1.0 / 7.0)}};
             for (int i = 1; i <= 2; ++i)
                 for (int j = 1; j <= 2; ++j)
                     B[m(c, i) * 8 + m(d, j)] = ((i == j) ? -mean : 0.0) + Bm[i - 1][j - 1];

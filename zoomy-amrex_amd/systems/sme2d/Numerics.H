@@ -32,8 +32,9 @@ struct Numerics {
             Model::zamr_quasilinear_dir(Q, Qaux, p, d, Ad);
             for (int k = 0; k < 64; ++k) A[k] += n(d, 0) * Ad[k];
         }
-        const amrex::Real c = zamr_math::sqrt(p(0, 0) * amrex::max(Q(1, 0), p(5, 0)));
-        const amrex::Real rc = 1.0 / c;
+        const amrex::Real c2 = p(0, 0) * amrex::max(Q(1, 0), p(5, 0));
+        const amrex::Real c = zamr_math::sqrt(c2);
+        const amrex::Real rc = zamr_math::div_sqrt(1.0, c2);  // 1 / c
         amrex::Real rho = 0.0;
         for (int i = 1; i < 8; ++i) {
             amrex::Real s = 0.0;

@@ -34,6 +34,15 @@ constexpr int kMaxParams = 32;
 constexpr int NT = 256;  // threads per workgroup
 constexpr int WV = 64;   // lanes per wave = columns per work item
 
+// Global-memory views of the state pointers.  The box descriptors hold
+// generic pointers; without this the compiler emits FLAT accesses, which
+// count in both the vector-memory and the LDS/scalar counters, so a wait for
+// a prefetched row also waits for every store issued after it.  GLOBAL
+// accesses count in the in-order vector-memory counter only.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ const gdouble* to_global(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* to_global(double* p) { return (gdouble*)p; }
+
 // This wave's index in its workgroup.  The same on every lane; readfirstlane
 // tells the compiler so, which keeps the item, its box descriptor and the row
 // loop in scalar registers and scalar loads instead of one copy per lane.
@@ -164,6 +173,16 @@ __device__ __forceinline__ void vq_zero(V& v) {
     for (int n = 0; n < V::row_size * V::column_size; ++n) v(n, 0) = 0.0;
 }
 
+// Redefine a loaded vector at this point, through an empty asm statement, so
+// that the wait for its loads is placed here.  A loaded value that crosses a
+// loop edge is copied there, and the copy waits for the load and so, the
+// vector-memory counter being in order, for every store issued after it.
+template <class V>
+__device__ __forceinline__ void land(V& v) {
+#pragma unroll
+    for (int n = 0; n < V::row_size * V::column_size; ++n) asm volatile("" : "+v"(v(n, 0)));
+}
+
 // make_rhs.H:9-13
 __device__ __forceinline__ double minmod(double a, double b) {
     if (a * b <= 0.0) return 0.0;
@@ -246,7 +265,8 @@ struct Sys {
     };
 
     // The stored value behind cell (c, j): itself, or its mirror-index cell.
-    __device__ static __forceinline__ Raw fetch(const BoxView& B, const double* __restrict__ q, int c, int j) {
+    template <class P>
+    __device__ static __forceinline__ Raw fetch(const BoxView& B, P q, int c, int j) {
         int ci, cj;
         mirror(B, c, j, ci, cj);
         const long long idx = (long long)(ci - B.gl0) + (long long)(cj - B.gl1) * B.sy;
@@ -291,8 +311,9 @@ struct Sys {
         return M::boundary_conditions(bc, v, ai, nh, center(B, c, j), time, B.dx0);
     }
 
-    __device__ static __forceinline__ VQ load(const BoxView& B, const double* __restrict__ q, int c, int j,
-                                              const VP& p, int clamp, double time) {
+    template <class P>
+    __device__ static __forceinline__ VQ load(const BoxView& B, P q, int c, int j, const VP& p, int clamp,
+                                              double time) {
         return finish(B, fetch(B, q, c, j), c, j, p, clamp, time);
     }
 
@@ -505,8 +526,13 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 #ifndef ZAMR_STAGE_WAVES
 #define ZAMR_STAGE_WAVES 0
 #endif
+#ifndef ZAMR_STAGE1_WAVES
+#define ZAMR_STAGE1_WAVES 0
+#endif
 #if ZAMR_STAGE_WAVES > 0
 #define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_STAGE_WAVES, ZAMR_STAGE_WAVES)))
+#elif ZAMR_STAGE1_WAVES > 0
+#define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(EPI == 0 ? ZAMR_STAGE1_WAVES : 1)))
 #else
 #define ZAMR_STAGE_ATTR
 #endif
@@ -576,9 +602,10 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;  // uniform per item
         const double hgx = 0.5 * p(0, 0) * rdx, hgy = 0.5 * p(0, 0) * rdy;
         // select, never index: a runtime index would push the descriptor to scratch
-        const double* __restrict__ qs = qsel(B, a.src);
-        double* __restrict__ qd = a.dstp ? a.dstp : qsel(B, a.dst);
-        const double* __restrict__ qo = qsel(B, a.old);
+        const gdouble* __restrict__ qs = to_global(qsel(B, a.src));
+        gdouble* __restrict__ qd = to_global(a.dstp ? a.dstp : qsel(B, a.dst));
+        const gdouble* __restrict__ qo = to_global(qsel(B, a.old));
+        gdouble* __restrict__ pre = to_global(a.pre);
         unsigned char *swx = nullptr, *swy = nullptr;
         if constexpr (SW != 0) {
             swx = reinterpret_cast<unsigned char*>(a.sw);
@@ -640,30 +667,28 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         }
         // first leading row
         const int jlead0 = (dim == 2) ? jstart + HX : jstart;
-        typename S::Raw nxt;
-        nxt = S::fetch(B, qs, cl, jlead0);
-        // stage 2: Q^n of the row for the Heun combine, also fetched one row ahead
-        VQ qn_next;
-        if constexpr (EPI == 1) {
-            if (own) {
-                const long long i0 = (long long)(c - B.gl0) + (long long)(jstart - B.gl1) * B.sy;
-#pragma unroll
-                for (int n = 0; n < nq; ++n) qn_next(n, 0) = qo[i0 + n * B.sc];
-            }
-        }
+        typename S::Raw nxt = S::fetch(B, qs, cl, jlead0);
+        // the window and the first leading row enter the row loop landed
+        land(W0);
+        land(W1);
+        land(nxt.v);
         double lam = 0.0;
         int nbad = 0;  // MOOD: troubled cells among this lane's candidate cells
         for (int j = jstart; j < jend; ++j) {
-            const int jl = (dim == 2) ? j + HX : j;
-            const VQ lead = S::finish(B, nxt, cl, jl, p, a.clamp, time);
-            if (j + 1 < jend) nxt = S::fetch(B, qs, cl, jl + 1);  // one row ahead
+            // The leading row, fetched one row ahead.  The fetch is
+            // unconditional (past the grown box the last row is read again):
+            // a conditional load keeps the old value alive beside the new one.
+            const VQ lead = S::finish(B, nxt, cl, (dim == 2) ? j + HX : j, p, a.clamp, time);
+            nxt = S::fetch(B, qs, cl, (dim == 2) ? min(j + HX + 1, B.hi1 + HX) : j);
+            // stage 2: Q^n of the row for the Heun combine, loaded here and used
+            // at the row's end (a load carried across rows would be copied at
+            // the loop edge, and that copy waits for the row's stores)
             VQ qn_row;
             if constexpr (EPI == 1) {
-                qn_row = qn_next;
-                if (own && j + 1 < jend) {
-                    const long long i1 = (long long)(c - B.gl0) + (long long)(j + 1 - B.gl1) * B.sy;
+                if (own) {
+                    const long long i1 = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) qn_next(n, 0) = qo[i1 + n * B.sc];
+                    for (int n = 0; n < nq; ++n) qn_row(n, 0) = qo[i1 + n * B.sc];
                 }
             }
             if constexpr (dim == 1) W0 = lead;
@@ -746,6 +771,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                                                   fyT, dyT, hyT, hyTnext);
                 }
             }
+            land(nxt.v);  // the prefetch is in before this row's stores
             if (own) {
                 const typename S::VX X = S::center(B, c, j);
                 const VA ac = S::aux_at(B, c, j, qc, p, time, X);
@@ -774,7 +800,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 const long long idx = isw;
                 if (a.pre) {
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) a.pre[idx + n * B.sc] = qn(n, 0);
+                    for (int n = 0; n < nq; ++n) pre[idx + n * B.sc] = qn(n, 0);
                 }
                 if (a.impl == 1) qn = S::implicit_source(qn, ac, dt, p, time, X);
                 if constexpr (EPI == 1) {

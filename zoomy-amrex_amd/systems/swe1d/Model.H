@@ -175,8 +175,9 @@ struct Model {
         const amrex::Real k = c * speed;
         const amrex::Real dh = Q(1, 0) - p(5, 0);
         const amrex::Real heav = (dh > 0.0) ? amrex::Real(1.0) : ((dh < 0.0) ? amrex::Real(0.0) : amrex::Real(0.5));
-        const amrex::Real dk_dh = -(4.0 / 3.0) * k / hf * heav;
-        const amrex::Real ws = (speed > 0.0) ? hinv / speed : amrex::Real(0.0);
+        // 1/hf = r^3 and hinv/speed = hinv/sqrt(|u|^2): no division on the device
+        const amrex::Real dk_dh = -(4.0 / 3.0) * k * (r * r * r) * heav;
+        const amrex::Real ws = (speed > 0.0) ? zamr_math::div_sqrt(hinv, u * u) : amrex::Real(0.0);
         const amrex::Real dk_dhu = c * u * ws;
         res(2 * 3 + 1, 0) = -Q(2, 0) * dk_dh;
         res(2 * 3 + 2, 0) = -k - Q(2, 0) * dk_dhu;

@@ -771,13 +771,12 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                                                   fyT, dyT, hyT, hyTnext);
                 }
             }
-            land(nxt.v);  // the prefetch is in before this row's stores
+            VQ qn;
             if (own) {
                 const typename S::VX X = S::center(B, c, j);
                 const VA ac = S::aux_at(B, c, j, qc, p, time, X);
                 VQ src{};
                 if (!a.impl) src = M::source(qc, ac, p, time, X);
-                VQ qn;
 #pragma unroll
                 for (int n = 0; n < nq; ++n) {
                     double Fy = 0.0;
@@ -822,6 +821,10 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                         if (inv > lam) lam = inv;
                     }
                 }
+            }
+            land(nxt.v);  // the prefetch is in before this row's stores
+            if (own) {
+                const long long idx = isw;
 #pragma unroll
                 for (int n = 0; n < nq; ++n) __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);
             }

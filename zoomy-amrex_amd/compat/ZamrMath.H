@@ -46,4 +46,16 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double div_sqrt(double a, double x) {
 #endif
 }
 
+// Keep the device compiler from moving work across this point (an empty asm
+// statement that "redefines" x): used to stop it from interleaving two large
+// independent evaluations, which costs registers a kernel at one wave per
+// SIMD cannot spare.  No effect on values; nothing on the host.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE void sequence_point(double& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(x));
+#else
+    (void)x;
+#endif
+}
+
 }  // namespace zamr_math

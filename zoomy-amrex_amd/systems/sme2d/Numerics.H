@@ -65,7 +65,8 @@ struct Numerics {
                 fR(r, 0) += n(d, 0) * b(r, 0);
             }
         }
-        const amrex::Real sL = local_max_abs_eigenvalue(qL, aL, p, n)(0, 0);
+        amrex::Real sL = local_max_abs_eigenvalue(qL, aL, p, n)(0, 0);
+        zamr_math::sequence_point(sL);  // one speed bound at a time (registers)
         const amrex::Real sR = local_max_abs_eigenvalue(qR, aR, p, n)(0, 0);
         const amrex::Real hs = 0.5 * amrex::max(sL, sR);
         Model::VQ res{};

@@ -118,12 +118,13 @@ __device__ __forceinline__ void vadd(V& a, const V& b) {
 }
 
 template <class M, class AD, int ORDER>
-// Occupancy request for the reverse stage kernel (waves per SIMD).  Three
-// waves fit its registers with a few bytes of spill and hide the latency of
-// its DPP shifts and of the stored-stage loads better than the two the
-// compiler picks by itself (1024^2 backward: 49.6 against 60.6 ms).
+// Occupancy request for the reverse stage kernel (waves per SIMD); 0 leaves
+// it to the compiler.  With the row prefetch the kernel holds ~210 VGPRs, two
+// waves per SIMD; forcing three spills 160 bytes per lane and is slower
+// (1024^2 backward: 47.5 ms at two waves, 72 ms at three; without the
+// prefetch three waves were best, 49.6 ms).
 #ifndef ZAMR_ADJ_WAVES
-#define ZAMR_ADJ_WAVES 3
+#define ZAMR_ADJ_WAVES 0
 #endif
 #if ZAMR_ADJ_WAVES > 0
 #define ZAMR_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_ADJ_WAVES, ZAMR_ADJ_WAVES)))
@@ -161,29 +162,54 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
     const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;
     const int j0 = w.j0, j1 = w.j0 + w.nj;
 
+    // The loads of the pointwise reverse at (c, r), fetched one row ahead of
+    // their use: Wb, V, the friction coefficient and the friction adjoint so
+    // far.  Lanes or rows outside the domain read an in-domain cell (their
+    // values are not used).
+    struct PointIn {
+        VQ wb, v;
+        double nn, nb;
+    };
+    const int cp = min(max(c, B.dlo0), B.dhi0);
+    auto point_fetch = [&](int r, PointIn& q) {
+        const int rr = min(max(r, B.dlo1), B.dhi1);
+        const long long idx = (long long)(cp - B.gl0) + (long long)(rr - B.gl1) * B.sy;
+#pragma unroll
+        for (int n = 0; n < nq; ++n) {
+            q.wb(n, 0) = a.Wb[idx + n * B.sc];
+            q.v(n, 0) = a.V[idx + n * B.sc];
+        }
+        q.nn = B.aux[idx + AD::friction_row * B.sc];
+        q.nb = a.nbar[idx];
+    };
+    // the prefetched inputs are in (before a row's stores; see land())
+    auto land_point = [&](PointIn& q) {
+        land(q.wb);
+        land(q.v);
+        asm volatile("" : "+v"(q.nn), "+v"(q.nb));
+    };
+
     // pointwise reverse of the implicit source at (c, r): dt * Vb and the
     // direct part Vb + [d aux/dU]^T ab; accumulates the friction adjoint
-    auto point = [&](int r, const VQ& u, VQ& vt, VQ& d) {
+    // (each cell's friction adjoint is updated by one lane once per launch,
+    // so the value fetched a row earlier is current)
+    auto point = [&](int r, const VQ& u, const PointIn& q, VQ& vt, VQ& d) {
         vzero(vt);
         vzero(d);
         const bool vy = (r >= B.dlo1) && (r <= B.dhi1);
         if (vx && vy) {
             const long long idx = (long long)(c - B.gl0) + (long long)(r - B.gl1) * B.sy;
-            VQ wb, v, vb, ub;
+            VQ wb, vb, ub;
 #pragma unroll
-            for (int n = 0; n < nq; ++n) {
-                wb(n, 0) = a.wscale * a.Wb[idx + n * B.sc];
-                v(n, 0) = a.V[idx + n * B.sc];
-            }
-            const double nn = B.aux[idx + AD::friction_row * B.sc];
+            for (int n = 0; n < nq; ++n) wb(n, 0) = a.wscale * q.wb(n, 0);
             double nb;
-            AD::source_reverse(u, v, nn, a.dt, p, wb, vb, ub, nb);
+            AD::source_reverse(u, q.v, q.nn, a.dt, p, wb, vb, ub, nb);
 #pragma unroll
             for (int n = 0; n < nq; ++n) {
                 vt(n, 0) = a.dt * vb(n, 0);
                 d(n, 0) = vb(n, 0) + ub(n, 0);
             }
-            if (own && r >= j0 && r < j1) a.nbar[idx] += nb;
+            if (own && r >= j0 && r < j1) a.nbar[idx] = q.nb + nb;
         }
     };
 
@@ -226,33 +252,52 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
         return G;
     };
 
-    // columns past hi + HX feed no live face; the box is the whole domain,
-    // so every other column and row mirrors into it
-    const bool col_in = c <= B.hi0 + HX;
-    auto load = [&](int r) -> VQ {
-        VQ v;
-        if (col_in) v = S::load(B, a.U, c, r, p, a.clamp, a.time);
-        else vzero(v);
-        return v;
-    };
+    // Columns past hi + HX feed no live face: those lanes load column
+    // hi + HX instead (in bounds, in the edge lane's cache lines).  The box is
+    // the whole domain, so every other column and row mirrors into it.
+    const int cl = min(c, B.hi0 + HX);
+    auto load = [&](int r) -> VQ { return S::load(B, a.U, cl, r, p, a.clamp, a.time); };
 
     VQ U0 = load(j0 - 3), U1 = load(j0 - 2), U2 = load(j0 - 1), U3;
     VQ P0, P1, R0, R1, R2, R3;
+    PointIn pin;
     {
         VQ d;
-        point(j0 - 2, U1, P0, d);
+        point_fetch(j0 - 2, pin);
+        point(j0 - 2, U1, pin, P0, d);
     }
     vzero(R0);
     vzero(R1);
     vzero(R2);
     vzero(R3);
+    // the first row's prefetches: U of row j0 and the point inputs of row j0-1
+    typename S::Raw unext = S::fetch(B, a.U, cl, j0);
+    point_fetch(j0 - 1, pin);
+    land(U0);
+    land(U1);
+    land(U2);
+    land(unext.v);
+    land_point(pin);
     // iteration j: face j+1/2; rows j-1 .. j+2 in U0..U3 and in the
     // accumulators R0..R3; dt*Vb of rows j, j+1 in P0, P1
     for (int j = j0 - 2; j <= j1; ++j) {
-        U3 = load(j + 2);
+        U3 = S::finish(B, unext, cl, j + 2, p, a.clamp, a.time);
+        const PointIn pcur = pin;
+        // the next row's loads (the last row's U prefetch re-reads row hi + HX)
+        unext = S::fetch(B, a.U, cl, min(j + 3, B.hi1 + HX));
+        point_fetch(j + 2, pin);
+        // the Heun part of this row's output, used at the row's end
+        const bool store_row = j - 1 >= j0 && own && vx;
+        VQ hn;
+        vzero(hn);
+        if (a.heun && store_row) {
+            const long long idx = (long long)(c - B.gl0) + (long long)(j - 1 - B.gl1) * B.sy;
+#pragma unroll
+            for (int n = 0; n < nq; ++n) hn(n, 0) = a.heun[idx + n * B.sc];
+        }
         {
             VQ d;
-            point(j + 1, U2, P1, d);
+            point(j + 1, U2, pcur, P1, d);
             vadd(R2, d);
         }
         if (j + 1 >= j0 && j + 1 < j1) vadd(R2, xrow(U2, P1));
@@ -297,12 +342,14 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
                             (sRp == 3 ? cRp(n, 0) : 0.0);
             }
         }
-        if (j - 1 >= j0 && own && vx) {
+        land(unext.v);
+        land_point(pin);
+        if (store_row) {
             const long long idx = (long long)(c - B.gl0) + (long long)(j - 1 - B.gl1) * B.sy;
 #pragma unroll
             for (int n = 0; n < nq; ++n) {
                 double v = R0(n, 0);
-                if (a.heun) v += 0.5 * a.heun[idx + n * B.sc];
+                if (a.heun) v += 0.5 * hn(n, 0);
                 a.out[idx + n * B.sc] = v;
             }
         }

@@ -93,6 +93,7 @@ class Solver {
     size_t pool_cap = 0;             // elements allocated (>= pool_elems when handed a larger pool)
     double* given_pool = nullptr;    // optional pool to use instead of allocating (AMR level reuse)
     size_t given_cap = 0;
+    double pool_headroom = 1.0;      // a new pool holds this many times what the boxes need
     BoxDesc* d_boxes = nullptr;
     GroupState* d_groups = nullptr;
     std::vector<GroupState> h_groups;
@@ -376,8 +377,9 @@ class Solver {
             pool_cap = given_cap;
         } else {
             if (given_pool) hipFree(given_pool);
-            ZAMR_HIP(hipMalloc(&d_pool, pool_elems * sizeof(double)));
-            pool_cap = pool_elems;
+            const size_t want = std::max(pool_elems, (size_t)((double)pool_elems * pool_headroom));
+            ZAMR_HIP(hipMalloc(&d_pool, want * sizeof(double)));
+            pool_cap = want;
         }
         given_pool = nullptr;
         ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));

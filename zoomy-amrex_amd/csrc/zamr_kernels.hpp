@@ -526,13 +526,8 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 #ifndef ZAMR_STAGE_WAVES
 #define ZAMR_STAGE_WAVES 0
 #endif
-#ifndef ZAMR_STAGE1_WAVES
-#define ZAMR_STAGE1_WAVES 0
-#endif
 #if ZAMR_STAGE_WAVES > 0
 #define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_STAGE_WAVES, ZAMR_STAGE_WAVES)))
-#elif ZAMR_STAGE1_WAVES > 0
-#define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(EPI == 0 ? ZAMR_STAGE1_WAVES : 1)))
 #else
 #define ZAMR_STAGE_ATTR
 #endif

@@ -86,7 +86,9 @@ struct Model {
     {
         const amrex::Real h4 = amrex::Math::powi<4>(h);
         const amrex::Real e4 = amrex::Math::powi<4>(eps);
-        return zamr_math::div_sqrt(1.4142135623730951 * h, h4 + amrex::max(h4, e4));
+        // h4 is +0, positive or NaN, so fmax is max(h4, e4) here in every case
+        // (a NaN h gives a NaN hinv either way) in one device instruction
+        return zamr_math::div_sqrt(1.4142135623730951 * h, h4 + std::fmax(h4, e4));
     }
 
     // sum_jk A_ijk x_j y_k for i = 1, 2 (x, y the first and second moments)

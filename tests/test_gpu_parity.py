@@ -193,6 +193,31 @@ def test_decomposition_invariance_gpu():
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+def test_ragged_boxes_run_bitwise():
+    """Boxes that fit no strip or row segment evenly (157 x 83 cells cut at
+    37: widths 37 and 9, heights 37 and 9), walls on two sides and
+    extrapolation on the others, WB, Manning and clamp: 8 steps on the
+    device clock, FMA-off build equal to the oracle bitwise (lanes past a
+    box's last ghost column, the last row's repeated prefetch and the
+    boundary mirrors all enter)."""
+    n, ny = 157, 83
+    Q0 = smooth_state(n, ny, seed=11)
+    Q0[1, 30:36, 40:52] = 0.0  # a dry patch
+    Q0[2:, 30:36, 40:52] = 0.0
+    c = dict(_case(n, ny, 2, True, True, 37, bc=("wall", "", "", "wall")), max_step=8)
+    say("ragged boxes 157x83 / 37")
+    o = orc("swe2d", c, Q0)
+    n_o, t_o, dts, ms = o.evolve()
+    zb = gpu("swe2d", c, Q0, contract=False)
+    n_b, t_b = run_until_done(zb)
+    assert n_b == n_o == 8 and t_b == t_o
+    assert np.array_equal(zb.get_state(), o.get_state())
+    assert np.array_equal(zb.get_state(1), o.get_aux())
+    z = gpu("swe2d", c, Q0)
+    run_until_done(z)
+    assert rel_linf(z.get_state(), o.get_state()) <= 1e-11
+
+
 def test_lake_at_rest_1024():
     n = 1024
     Q = smooth_state(n, n, bed=True, amp=0.0)

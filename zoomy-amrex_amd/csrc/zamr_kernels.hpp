@@ -766,6 +766,10 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                                                   fyT, dyT, hyT, hyTnext);
                 }
             }
+            // The prefetch is landed before this row's stores: stage 1 lands it
+            // before the cell update (fewer live registers), the later stages
+            // just before the stores (more of the row's work covers the load).
+            if constexpr (EPI == 0) land(nxt.v);
             VQ qn;
             if (own) {
                 const typename S::VX X = S::center(B, c, j);
@@ -817,7 +821,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                 }
             }
-            land(nxt.v);  // the prefetch is in before this row's stores
+            if constexpr (EPI != 0) land(nxt.v);
             if (own) {
                 const long long idx = isw;
 #pragma unroll

@@ -474,19 +474,61 @@ class Solver {
         }
     }
 
+    // Waves per SIMD the order's stage kernel runs at (its register use
+    // decides: two for the 4-row systems, one for the 8-row ones), and the
+    // device's CU count; both queried once.
+    mutable int occ_cache[3] = {0, 0, 0};
+    mutable int cu_count = 0;
+    int stage_waves_per_simd(int hx) const {
+        if (!occ_cache[hx]) {
+            int blocks = 0;
+            const void* k = hx == 2 ? (const void*)stage_kernel<M, N, 2, 1> : (const void*)stage_kernel<M, N, 1, 1>;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, NT, 0) != hipSuccess || blocks < 1) blocks = 2;
+            occ_cache[hx] = blocks;  // a block is one wave on each of a CU's 4 SIMDs
+        }
+        if (!cu_count) {
+            int dev = 0, n = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+                n = 256;
+            cu_count = n;
+        }
+        return occ_cache[hx];
+    }
+
     // Rows per work item.  Deeper items re-read fewer halo rows, shallower
-    // ones put more waves on the 1024 SIMDs (256 CUs x 4): take the deepest
+    // ones put more waves on the SIMDs.  At two waves per SIMD: the deepest
     // of 32, 16, 8, 4 rows that still gives every SIMD two waves over the
-    // whole box list (measured: 32 rows at 4096^2, 8 rows at 1024^2).
+    // whole box list (measured: 32 rows at 4096^2, 8 rows at 1024^2).  At one
+    // wave per SIMD the launch runs in rounds of one item per SIMD, and a
+    // last round that fills few SIMDs costs a whole item time: the depth
+    // (8..64) that minimises rounds x (rows + 2), the 2 for the item's
+    // prologue (measured at 2048^2 with 8 rows: 36 rows, 35 x 57 items in two
+    // rounds, 1.25x the rate of 32 rows in three).
     int rows_for(int hx) const {
         if (rows_per_item > 0) return rows_per_item;
         const int txc = WV - 2 * hx;
-        for (int rows : {32, 16, 8}) {
-            long long waves = 0;
+        auto waves = [&](int rows) {
+            long long w = 0;
             for (const HostBox& b : hb)
-                waves += (long long)((b.hi[0] - b.lo[0] + txc) / txc) * ((b.hi[1] - b.lo[1] + rows) / rows);
-            if (waves >= 2 * 1024) return rows;
+                w += (long long)((b.hi[0] - b.lo[0] + txc) / txc) * ((b.hi[1] - b.lo[1] + rows) / rows);
+            return w;
+        };
+        if (stage_waves_per_simd(hx) == 1) {
+            const long long slots = 4LL * cu_count;
+            int best = 32;
+            long long best_cost = -1;
+            for (int rows = 8; rows <= 64; ++rows) {
+                const long long cost = ((waves(rows) + slots - 1) / slots) * (rows + 2);
+                if (best_cost < 0 || cost < best_cost) {
+                    best_cost = cost;
+                    best = rows;
+                }
+            }
+            return best;
         }
+        for (int rows : {32, 16, 8})
+            if (waves(rows) >= 2 * 1024) return rows;
         return 4;
     }
 

@@ -264,52 +264,357 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
         atomicOr(changed, 1u);
 }
 
-// Row and column tag counts of a list of regions of the buffered tag map:
-// hist[offx[r] + i - lo0] counts column i, hist[offy[r] + j - lo1] row j of
-// region r.  One 256-thread block per 64 x 64 tile of a region (tile[r] is
-// the first tile of region r); partial sums in LDS, one atomic per row and
-// column of the tile.
+// ---------------- Berger-Rigoutsos on the device ----------------
+// One generation = the regions still to be clustered: disjoint rectangles of
+// the buffered, coarsened tag map, each holding at least one tag.  Their row
+// and column tag counts sit at hist[offx + i - lo0] (column i) and
+// hist[offy + j - lo1] (row j); `tile` is the first of the region's 64 x 64
+// tiles in the generation's running tile count.
 struct Region {
     int lo0, lo1, hi0, hi1;
     int offx, offy, tile;
 };
-__global__ void __launch_bounds__(256) cluster_hist_kernel(const unsigned char* __restrict__ map, int cnx,
-                                                           const Region* __restrict__ reg, int nreg,
-                                                           int* __restrict__ hist) {
+
+// Control block: generation g works on parity p = g & 1.
+struct BrCtl {
+    int nreg[2], ntile[2], nh[2];
+    int nout;      // clusters found
+    int overflow;  // a capacity was exceeded: the host grows the buffers and starts again
+    int gens;      // generations that had regions
+};
+
+struct BrBufs {
+    Region* reg[2];
+    int* hist[2];
+    Region* child;  // two per region of a generation (the decide kernel's scratch)
+    int* nchild;    // per region: 0 or 2
+    int4* out;      // clusters (lo0, lo1, hi0, hi1)
+    int reg_cap, hist_cap, out_cap;
+};
+
+// Generation 0: the whole map as one region (hist_cap >= cnx + cny).
+__global__ void __launch_bounds__(256) br_init_kernel(BrCtl* ctl, BrBufs b, int cnx, int cny) {
+    const int nh = cnx + cny;
+    for (int k = threadIdx.x; k < nh; k += blockDim.x) b.hist[0][k] = 0;
+    if (threadIdx.x == 0) {
+        b.reg[0][0] = Region{0, 0, cnx - 1, cny - 1, 0, cnx, 0};
+        ctl->nreg[0] = 1;
+        ctl->ntile[0] = ((cnx + 63) / 64) * ((cny + 63) / 64);
+        ctl->nh[0] = nh;
+        ctl->nreg[1] = ctl->ntile[1] = ctl->nh[1] = 0;
+        ctl->nout = 0;
+        ctl->overflow = 0;
+        ctl->gens = 0;
+    }
+}
+
+// Row and column tag counts of generation p's regions: a block per 64 x 64
+// tile (grid-stride over the generation's tiles), partial sums in LDS, one
+// atomic per row and column of the tile.
+__global__ void __launch_bounds__(256) br_hist_kernel(const unsigned char* __restrict__ map, int cnx,
+                                                      const BrCtl* __restrict__ ctl, BrBufs b, int p) {
+    __builtin_amdgcn_s_setprio(3);  // a short serial chain beside a VALU-bound stage kernel: issue first
     __shared__ int cs[64], rs[64];
-    int lo = 0, hi = nreg - 1;
-    while (lo < hi) {  // the region owning this tile
-        const int mid = (lo + hi + 1) >> 1;
-        if (reg[mid].tile <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const Region R = reg[lo];
-    const int w = R.hi0 - R.lo0 + 1, h = R.hi1 - R.lo1 + 1;
-    const int tw = (w + 63) / 64;
-    const int t = (int)blockIdx.x - R.tile;
-    const int i0 = R.lo0 + (t % tw) * 64, j0 = R.lo1 + (t / tw) * 64;
-    if (threadIdx.x < 64) {
-        cs[threadIdx.x] = 0;
-        rs[threadIdx.x] = 0;
-    }
-    __syncthreads();
+    const int nreg = ctl->nreg[p], ntile = ctl->ntile[p];
+    const Region* __restrict__ reg = b.reg[p];
+    int* __restrict__ hist = b.hist[p];
     const int ci = threadIdx.x & 63;
-    const int i = i0 + ci;
-    int csum = 0;
-    for (int r = threadIdx.x >> 6; r < 64; r += 4) {
-        const int j = j0 + r;
-        if (i <= R.hi0 && j <= R.hi1 && map[(long long)j * cnx + i]) {
-            ++csum;
-            atomicAdd(&rs[r], 1);
+    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        int lo = 0, hi = nreg - 1;
+        while (lo < hi) {  // the region owning this tile
+            const int mid = (lo + hi + 1) >> 1;
+            if (reg[mid].tile <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const Region R = reg[lo];
+        const int tw = (R.hi0 - R.lo0 + 64) / 64;
+        const int t = tile - R.tile;
+        const int i0 = R.lo0 + (t % tw) * 64, j0 = R.lo1 + (t / tw) * 64;
+        if (threadIdx.x < 64) {
+            cs[threadIdx.x] = 0;
+            rs[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        const int i = i0 + ci;
+        int csum = 0;
+        for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+            const int j = j0 + r;
+            if (i <= R.hi0 && j <= R.hi1 && map[(long long)j * cnx + i]) {
+                ++csum;
+                atomicAdd(&rs[r], 1);
+            }
+        }
+        if (csum) atomicAdd(&cs[ci], csum);
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            if (cs[threadIdx.x]) atomicAdd(&hist[R.offx + (i0 - R.lo0) + threadIdx.x], cs[threadIdx.x]);
+        } else if (threadIdx.x < 128) {
+            const int r = threadIdx.x - 64;
+            if (rs[r]) atomicAdd(&hist[R.offy + (j0 - R.lo1) + r], rs[r]);
+        }
+        __syncthreads();  // cs / rs are cleared for the next tile
+    }
+}
+
+constexpr int BR_UNROLL = 8;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = WV / 2; o; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = WV / 2; o; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int o) {
+    const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)v, o);
+    const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), o);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+    for (int o = WV / 2; o; o >>= 1) {
+        const unsigned long long w = shfl_xor_u64(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int o = WV / 2; o; o >>= 1) {
+        const unsigned long long w = shfl_xor_u64(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// FindCut (AMReX_Cluster.cpp) over hist[0..len) by one wave; status 0 hole,
+// 1 inflection, 2 bisection, 3 none (len <= 1); returns the cut index (cells
+// below it stay on the low side).  The cut is the zero nearest the middle
+// (the lower index among equals); else, among sign changes of the second
+// difference (0 at both ends) at least 2 cells from either end, the largest
+// jump, nearest the middle, lower index among equals, if that jump exceeds
+// 2; else the middle.  Keys pack that order (indices < 2^20, jumps < 2^21).
+__device__ int br_find_cut_wave(const int* __restrict__ h, int len, int lo, int lane, int& status) {
+    status = 3;
+    if (len <= 1) return lo;
+    const int mid = len / 2;
+    // loads go out BR_UNROLL (or 4) at a time per lane: the scans are
+    // latency-bound walks over counts in L2
+    unsigned long long hole = ~0ull;  // (|i - mid|, i), least
+    for (int base = 0; base < len; base += WV * BR_UNROLL) {
+        int v[BR_UNROLL];
+#pragma unroll
+        for (int u = 0; u < BR_UNROLL; ++u) {
+            const int i = base + u * WV + lane;
+            v[u] = i < len ? h[i] : 1;
+        }
+#pragma unroll
+        for (int u = 0; u < BR_UNROLL; ++u) {
+            const int i = base + u * WV + lane;
+            if (v[u] == 0) {
+                const unsigned long long k = ((unsigned long long)abs(i - mid) << 32) | (unsigned)i;
+                hole = k < hole ? k : hole;
+            }
         }
     }
-    if (csum) atomicAdd(&cs[ci], csum);
+    hole = wave_min_u64(hole);
+    if (hole != ~0ull) {
+        status = 0;
+        return lo + (int)(hole & 0xffffffffu);
+    }
+    unsigned long long infl = 0;  // 1 + (jump, 2^20 - 1 - |i - mid|, 2^20 - 1 - i), greatest; 0: none
+    for (int base = 2; base < len - 2; base += WV * 4) {
+        int m2[4], m1[4], c0[4], p1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = base + u * WV + lane;
+            const bool in = i < len - 2;
+            m2[u] = in ? h[i - 2] : 0;
+            m1[u] = in ? h[i - 1] : 0;
+            c0[u] = in ? h[i] : 0;
+            p1[u] = in ? h[i + 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = base + u * WV + lane;
+            const int a = c0[u] - 2 * m1[u] + m2[u];  // second difference at i - 1
+            const int c = p1[u] - 2 * c0[u] + m1[u];  // at i
+            if (i < len - 2 && a * c < 0) {
+                const unsigned long long k = 1ull + (((unsigned long long)abs(a - c) << 40) |
+                                                     ((unsigned long long)(0xFFFFF - abs(i - mid)) << 20) |
+                                                     (unsigned long long)(0xFFFFF - i));
+                infl = k > infl ? k : infl;
+            }
+        }
+    }
+    infl = wave_max_u64(infl);
+    if (infl == 0 || (int)((infl - 1) >> 40) <= 2) {
+        status = 2;
+        return lo + mid;
+    }
+    status = 1;
+    return lo + (0xFFFFF - (int)((infl - 1) & 0xFFFFF));
+}
+
+// One generation's decisions (a single 512-thread block, a wave per region;
+// two waves per SIMD at its register count still fit beside the stage
+// kernel it overlaps):
+// a region's counts give its cluster (the nonzero row and column ranges)
+// and efficiency; a cluster at grid_eff or better is output, any other is
+// cut (FindCut in both directions, the better status, then the longer
+// shorter side, y among equals) into two regions of the next generation,
+// compacted in region order by a block scan that also lays out their tiles
+// and counts.
+constexpr int BR_DECIDE_THREADS = 512;
+__global__ void __launch_bounds__(BR_DECIDE_THREADS) br_decide_kernel(BrCtl* ctl, BrBufs b, int p,
+                                                                      double grid_eff) {
+    constexpr int T = BR_DECIDE_THREADS;
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ int s_nc[T], s_nt[T], s_nh[T];
+    __shared__ int s_over;
+    const int t = threadIdx.x, lane = t & (WV - 1), wave = wave_in_block();
+    const int nreg = ctl->nreg[p];
+    const Region* __restrict__ reg = b.reg[p];
+    const int* __restrict__ hist = b.hist[p];
+    if (t == 0) s_over = 0;
     __syncthreads();
-    if (threadIdx.x < 64) {
-        if (cs[threadIdx.x]) atomicAdd(&hist[R.offx + (i0 - R.lo0) + threadIdx.x], cs[threadIdx.x]);
-    } else if (threadIdx.x < 128) {
-        const int r = threadIdx.x - 64;
-        if (rs[r]) atomicAdd(&hist[R.offy + (j0 - R.lo1) + r], rs[r]);
+    for (int r = wave; r < nreg; r += T / WV) {
+        const Region R = reg[r];
+        const int w = R.hi0 - R.lo0 + 1, hh = R.hi1 - R.lo1 + 1;
+        const int* hx = hist + R.offx;
+        const int* hy = hist + R.offy;
+        int a0 = 0x7fffffff, a1 = -1, b0 = 0x7fffffff, b1 = -1;
+        unsigned long long cnt = 0;
+        for (int base = 0; base < w; base += WV * BR_UNROLL) {
+            int v[BR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BR_UNROLL; ++u) {
+                const int i = base + u * WV + lane;
+                v[u] = i < w ? hx[i] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < BR_UNROLL; ++u)
+                if (v[u]) {
+                    const int i = base + u * WV + lane;
+                    a0 = min(a0, i);
+                    a1 = max(a1, i);
+                    cnt += (unsigned)v[u];
+                }
+        }
+        for (int base = 0; base < hh; base += WV * BR_UNROLL) {
+            int v[BR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BR_UNROLL; ++u) {
+                const int j = base + u * WV + lane;
+                v[u] = j < hh ? hy[j] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < BR_UNROLL; ++u)
+                if (v[u]) {
+                    const int j = base + u * WV + lane;
+                    b0 = min(b0, j);
+                    b1 = max(b1, j);
+                }
+        }
+        a0 = wave_min_i(a0);
+        a1 = wave_max_i(a1);
+        b0 = wave_min_i(b0);
+        b1 = wave_max_i(b1);
+        for (int o = WV / 2; o; o >>= 1) cnt += shfl_xor_u64(cnt, o);
+        int nch = 0;
+        if (a1 >= 0) {
+            const int lo0 = R.lo0 + a0, hi0 = R.lo0 + a1, lo1 = R.lo1 + b0, hi1 = R.lo1 + b1;
+            const double vol = (double)(hi0 - lo0 + 1) * (double)(hi1 - lo1 + 1);
+            if ((double)(long long)cnt / vol >= grid_eff) {
+                if (lane == 0) {
+                    const int slot = atomicAdd(&ctl->nout, 1);
+                    if (slot < b.out_cap) b.out[slot] = make_int4(lo0, lo1, hi0, hi1);
+                    else atomicOr(&s_over, 1);
+                }
+            } else {
+                int st[2];
+                const int cut0 = br_find_cut_wave(hx + a0, a1 - a0 + 1, lo0, lane, st[0]);
+                const int cut1 = br_find_cut_wave(hy + b0, b1 - b0 + 1, lo1, lane, st[1]);
+                const int best = min(st[0], st[1]);
+                int dir = -1, minlen = -1;
+                if (st[0] == best) {
+                    dir = 0;
+                    minlen = min(cut0 - lo0, hi0 - cut0);
+                }
+                if (st[1] == best && min(cut1 - lo1, hi1 - cut1) >= minlen) dir = 1;
+                Region A{lo0, lo1, hi0, hi1, 0, 0, 0}, B = A;
+                if (dir == 0) {
+                    A.hi0 = cut0 - 1;
+                    B.lo0 = cut0;
+                } else {
+                    A.hi1 = cut1 - 1;
+                    B.lo1 = cut1;
+                }
+                if (lane == 0) {
+                    b.child[2 * r] = A;
+                    b.child[2 * r + 1] = B;
+                }
+                nch = 2;
+            }
+        }
+        if (lane == 0) b.nchild[r] = nch;
+    }
+    __syncthreads();
+    // compaction: each thread a run of regions; inclusive scans over the runs
+    const int C = (nreg + T - 1) / T;
+    const int r0 = min(nreg, t * C), r1 = min(nreg, r0 + C);
+    auto tiles = [](const Region& q) { return ((q.hi0 - q.lo0 + 64) / 64) * ((q.hi1 - q.lo1 + 64) / 64); };
+    auto span = [](const Region& q) { return (q.hi0 - q.lo0 + 1) + (q.hi1 - q.lo1 + 1); };
+    int nc = 0, nt = 0, nh = 0;
+    for (int r = r0; r < r1; ++r)
+        if (b.nchild[r])
+            for (int c = 0; c < 2; ++c) {
+                const Region q = b.child[2 * r + c];
+                ++nc;
+                nt += tiles(q);
+                nh += span(q);
+            }
+    s_nc[t] = nc;
+    s_nt[t] = nt;
+    s_nh[t] = nh;
+    __syncthreads();
+    for (int o = 1; o < T; o <<= 1) {
+        int x = 0, y = 0, z = 0;
+        if (t >= o) {
+            x = s_nc[t - o];
+            y = s_nt[t - o];
+            z = s_nh[t - o];
+        }
+        __syncthreads();
+        s_nc[t] += x;
+        s_nt[t] += y;
+        s_nh[t] += z;
+        __syncthreads();
+    }
+    const int NC = s_nc[T - 1], NT = s_nt[T - 1], NH = s_nh[T - 1];
+    const bool over = s_over != 0 || NC > b.reg_cap || NH > b.hist_cap;
+    if (!over) {
+        int pc = s_nc[t] - nc, pt = s_nt[t] - nt, ph = s_nh[t] - nh;
+        Region* __restrict__ nxt = b.reg[p ^ 1];
+        for (int r = r0; r < r1; ++r)
+            if (b.nchild[r])
+                for (int c = 0; c < 2; ++c) {
+                    Region q = b.child[2 * r + c];
+                    q.offx = ph;
+                    q.offy = ph + (q.hi0 - q.lo0 + 1);
+                    q.tile = pt;
+                    nxt[pc++] = q;
+                    pt += tiles(q);
+                    ph += span(q);
+                }
+        int* __restrict__ hn = b.hist[p ^ 1];
+        for (int k = t; k < NH; k += T) hn[k] = 0;
+    }
+    if (t == 0) {
+        ctl->nreg[p ^ 1] = over ? 0 : NC;
+        ctl->ntile[p ^ 1] = over ? 0 : NT;
+        ctl->nh[p ^ 1] = over ? 0 : NH;
+        if (over) ctl->overflow = 1;
+        if (nreg > 0) ctl->gens += 1;
     }
 }
 
@@ -328,47 +633,6 @@ __global__ void clear_max_kernel(GroupState* gs, int ngroups) {
 // The fine level's substeps start at the coarse step's start time
 // (TimeStep(lev + 1, time + (i - 1) * dt_level[lev + 1]), ZoomyAmr.cpp:756-757).
 __global__ void sync_clock_kernel(const GroupState* coarse, GroupState* fine) { fine->time = coarse->time; }
-
-// ---------------- host: Berger-Rigoutsos cut selection ----------------
-// FindCut (AMReX_Cluster.cpp): status 0 hole, 1 inflection, 2 bisection,
-// 3 none; returns the cut index (cells below it stay).
-inline int br_find_cut(const int* hist, int len, int lo, int& status) {
-    status = 3;
-    if (len <= 1) return lo;
-    const int mid = len / 2;
-    int cut = -1;
-    for (int i = 0; i < len; ++i)
-        if (hist[i] == 0) {
-            status = 0;
-            if (std::abs(cut - mid) > std::abs(i - mid)) {
-                cut = i;
-                if (i > mid) break;
-            }
-        }
-    if (status == 0) return lo + cut;
-    // second differences (0 at both ends), then the largest sign change at
-    // least 2 cells from either end, nearest the middle among equals
-    int best = -1;
-    std::vector<int> d2(len, 0);
-    for (int i = 1; i < len - 1; ++i) d2[i] = hist[i + 1] - 2 * hist[i] + hist[i - 1];
-    for (int i = 2; i < len - 2; ++i) {
-        const int a = d2[i - 1], b = d2[i], dif = std::abs(a - b);
-        if (a * b < 0 && dif >= best) {
-            if (dif > best) {
-                status = 1;
-                cut = i;
-                best = dif;
-            } else if (std::abs(i - mid) < std::abs(cut - mid)) {
-                cut = i;
-            }
-        }
-    }
-    if (best <= 2) {
-        cut = mid;
-        status = 2;
-    }
-    return lo + cut;
-}
 
 // ---------------- host: the hierarchy ----------------
 template <class M, class N>
@@ -395,13 +659,18 @@ class AmrDriver {
     unsigned char* d_cover = nullptr;     // level-0 cells under level 1
     int cnx = 0, cny = 0, bf_lev = 1;
     std::vector<std::array<int, 4>> last_boxes;
-    // clustering scratch: regions and their histograms (device and host)
-    Region* d_reg = nullptr;
-    int* d_hist = nullptr;
-    Region* h_reg = nullptr;  // pinned staging of the same
-    int* h_hist = nullptr;
+    // device clustering: control block (and its pinned copy), buffers, and
+    // the generations the last clustering took (the next one's first batch)
+    BrCtl* d_brctl = nullptr;
+    BrCtl* h_brctl = nullptr;
+    int4* h_brout = nullptr;  // pinned copy of the clusters
+    BrBufs br{};
+    int br_depth = 8;
     unsigned* h_flag = nullptr;
-    size_t reg_cap = 0, hist_cap = 0;
+    // the clustering's round trips run on a stream of their own, beside the
+    // level-0 step on the main stream
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_tag = nullptr;  // recorded behind the tag flag's copy to the host
     int4* d_fboxes = nullptr;  // fine boxes in level-0 index space (cover painting)
     size_t fbox_cap = 0;
     double* spare_pool = nullptr;  // a retired fine level's pool, reused by the next
@@ -466,16 +735,50 @@ class AmrDriver {
         ZAMR_HIP(hipHostMalloc(&h_flag, sizeof(unsigned)));
         ZAMR_HIP(hipMalloc(&d_cover, ncell));
         ZAMR_HIP(hipMemset(d_cover, 0, ncell));
+        ZAMR_HIP(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+        ZAMR_HIP(hipEventCreateWithFlags(&ev_tag, hipEventDisableTiming));
+        // the device clustering's packed sort keys hold indices below 2^18
+        if (cnx >= (1 << 18) || cny >= (1 << 18) || (long long)cnx * cny >= (1ll << 29))
+            throw Error(1, "amr: the level-0 grid over blocking_factor / ref_ratio is too large for the clustering");
+        ZAMR_HIP(hipMalloc(&d_brctl, sizeof(BrCtl)));
+        ZAMR_HIP(hipHostMalloc(&h_brctl, sizeof(BrCtl)));
+        br_alloc(4096, 32 * (cnx + cny), 4096);
+    }
+
+    // Device clustering buffers at the given capacities (regions per
+    // generation, histogram entries per generation, clusters).
+    void br_free() {
+        for (void* q : {(void*)br.reg[0], (void*)br.reg[1], (void*)br.hist[0], (void*)br.hist[1], (void*)br.child,
+                        (void*)br.nchild, (void*)br.out})
+            if (q) hipFree(q);
+        if (h_brout) hipHostFree(h_brout);
+        br = BrBufs{};
+        h_brout = nullptr;
+    }
+    void br_alloc(int reg_cap, int hist_cap, int out_cap) {
+        br_free();
+        for (auto& q : br.reg) ZAMR_HIP(hipMalloc(&q, (size_t)reg_cap * sizeof(Region)));
+        for (auto& q : br.hist) ZAMR_HIP(hipMalloc(&q, (size_t)hist_cap * sizeof(int)));
+        ZAMR_HIP(hipMalloc(&br.child, 2 * (size_t)reg_cap * sizeof(Region)));
+        ZAMR_HIP(hipMalloc(&br.nchild, (size_t)reg_cap * sizeof(int)));
+        ZAMR_HIP(hipMalloc(&br.out, (size_t)out_cap * sizeof(int4)));
+        ZAMR_HIP(hipHostMalloc(&h_brout, (size_t)out_cap * sizeof(int4)));
+        br.reg_cap = reg_cap;
+        br.hist_cap = hist_cap;
+        br.out_cap = out_cap;
     }
 
     ~AmrDriver() {
         L1.reset();
         for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags[0], (void*)d_ctags[1],
-                        (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_reg, (void*)d_hist,
+                        (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_brctl,
                         (void*)d_fboxes, (void*)spare_pool, (void*)d_tmp_rects})
             if (p) hipFree(p);
-        for (void* p : {(void*)h_reg, (void*)h_hist, (void*)h_flag})
+        br_free();
+        for (void* p : {(void*)h_brctl, (void*)h_flag})
             if (p) hipHostFree(p);
+        if (cstream) hipStreamDestroy(cstream);
+        if (ev_tag) hipEventDestroy(ev_tag);
     }
 
     CoarseGrid grid() const { return CoarseGrid{L0.d_boxes, d_xbox, d_ybox, nbx, L0.nx, L0.ny}; }
@@ -528,12 +831,14 @@ class AmrDriver {
         return d_tmp_rects;
     }
 
-    // cell_cons_interp into the listed cells of level F, into buffers
-    // bufmask; with `cover`, cells under a fine box are left alone.
-    void interp(Sol& F, CellRect* d, int n, long long total, unsigned bufmask, const unsigned char* cover) {
+    // cell_cons_interp of level-0 buffer cbuf into the listed cells of level
+    // F, into buffers bufmask; with `cover`, cells under a fine box are left
+    // alone.
+    void interp(Sol& F, CellRect* d, int n, long long total, int cbuf, unsigned bufmask,
+                const unsigned char* cover) {
         if (!n || !total) return;
         hipLaunchKernelGGL((cf_interp_kernel<M>), dim3(grid1(total)), dim3(256), 0, stream(), F.d_boxes, d, n, total,
-                           grid(), L0.cur, bufmask, cover);
+                           grid(), cbuf, bufmask, cover);
         ZAMR_HIP(hipGetLastError());
     }
 
@@ -595,95 +900,48 @@ class AmrDriver {
         last_boxes.clear();
     }
 
-    // Berger-Rigoutsos on the device map d_ctags[1], one generation of
-    // clusters at a time: a cluster is the set of tags inside its region, so
-    // the region's row and column counts give its bounding box (the nonzero
-    // range), its efficiency and FindCut's input; a cluster below grid_eff
-    // is cut into two regions for the next generation.
+    // Berger-Rigoutsos on the device map d_ctags[1] (complete: tag_result()
+    // waited for it), all on the clustering stream: generation 0 is the whole
+    // map; each generation is a histogram launch and a decide launch, queued
+    // in batches (the first as deep as the last clustering went) with one
+    // copy of the control block to the host per batch.  A generation that
+    // would exceed a capacity stops the run; the buffers grow and it starts
+    // again.
     std::vector<std::array<int, 4>> gpu_cluster() {
-        std::vector<std::array<int, 4>> out;
-        std::vector<Region> cur{Region{0, 0, cnx - 1, cny - 1, 0, 0, 0}};
-        while (!cur.empty()) {
-            size_t nh = 0;
-            int ntile = 0;
-            for (Region& r : cur) {
-                const int w = r.hi0 - r.lo0 + 1, hh = r.hi1 - r.lo1 + 1;
-                r.offx = (int)nh;
-                r.offy = (int)nh + w;
-                nh += (size_t)w + hh;
-                r.tile = ntile;
-                ntile += ((w + 63) / 64) * ((hh + 63) / 64);
-            }
-            if (cur.size() > reg_cap) {
-                if (d_reg) hipFree(d_reg);
-                if (h_reg) hipHostFree(h_reg);
-                reg_cap = cur.size() * 2;
-                ZAMR_HIP(hipMalloc(&d_reg, reg_cap * sizeof(Region)));
-                ZAMR_HIP(hipHostMalloc(&h_reg, reg_cap * sizeof(Region)));
-            }
-            if (nh > hist_cap) {
-                if (d_hist) hipFree(d_hist);
-                if (h_hist) hipHostFree(h_hist);
-                hist_cap = nh * 2;
-                ZAMR_HIP(hipMalloc(&d_hist, hist_cap * sizeof(int)));
-                ZAMR_HIP(hipHostMalloc(&h_hist, hist_cap * sizeof(int)));
-            }
-            std::memcpy(h_reg, cur.data(), cur.size() * sizeof(Region));
-            ZAMR_HIP(hipMemcpyAsync(d_reg, h_reg, cur.size() * sizeof(Region), hipMemcpyHostToDevice, stream()));
-            ZAMR_HIP(hipMemsetAsync(d_hist, 0, nh * sizeof(int), stream()));
-            hipLaunchKernelGGL(cluster_hist_kernel, dim3((unsigned)ntile), dim3(256), 0, stream(), d_ctags[1], cnx,
-                               d_reg, (int)cur.size(), d_hist);
+        for (;;) {
+            hipLaunchKernelGGL(br_init_kernel, dim3(1), dim3(256), 0, cstream, d_brctl, br, cnx, cny);
             ZAMR_HIP(hipGetLastError());
-            ZAMR_HIP(hipMemcpyAsync(h_hist, d_hist, nh * sizeof(int), hipMemcpyDeviceToHost, stream()));
-            ZAMR_HIP(hipStreamSynchronize(stream()));
-            const int* h = h_hist;
-            std::vector<Region> next;
-            for (const Region& r : cur) {
-                const int* hx = h + r.offx;
-                const int* hy = h + r.offy;
-                const int w = r.hi0 - r.lo0 + 1, hh = r.hi1 - r.lo1 + 1;
-                int a0 = 0, a1 = w - 1, b0 = 0, b1 = hh - 1;
-                while (a0 < w && hx[a0] == 0) ++a0;
-                if (a0 == w) continue;  // no tags
-                while (hx[a1] == 0) --a1;
-                while (hy[b0] == 0) ++b0;
-                while (hy[b1] == 0) --b1;
-                long long count = 0;
-                for (int i = a0; i <= a1; ++i) count += hx[i];
-                const int lo0 = r.lo0 + a0, hi0 = r.lo0 + a1, lo1 = r.lo1 + b0, hi1 = r.lo1 + b1;
-                const double vol = (double)(hi0 - lo0 + 1) * (double)(hi1 - lo1 + 1);
-                if ((double)count / vol >= grid_eff) {
-                    out.push_back({lo0, lo1, hi0, hi1});
-                    continue;
+            const int peek = std::min(br.out_cap, 1024);
+            int p = 0, gens = 0;
+            for (;;) {
+                const int k = gens == 0 ? std::max(br_depth, 1) : 4;
+                for (int g = 0; g < k; ++g, p ^= 1) {
+                    hipLaunchKernelGGL(br_hist_kernel, dim3(1024), dim3(256), 0, cstream, d_ctags[1], cnx, d_brctl,
+                                       br, p);
+                    hipLaunchKernelGGL(br_decide_kernel, dim3(1), dim3(BR_DECIDE_THREADS), 0, cstream, d_brctl, br,
+                                       p, grid_eff);
                 }
-                int st[2], cut[2];
-                cut[0] = br_find_cut(hx + a0, a1 - a0 + 1, lo0, st[0]);
-                cut[1] = br_find_cut(hy + b0, b1 - b0 + 1, lo1, st[1]);
-                const int best = std::min(st[0], st[1]);
-                const int lo[2] = {lo0, lo1}, hi[2] = {hi0, hi1};
-                int dir = -1, minlen = -1;
-                for (int d = 0; d < 2; ++d) {
-                    if (st[d] != best) continue;
-                    const int l = std::min(cut[d] - lo[d], hi[d] - cut[d]);
-                    if (l >= minlen) {
-                        dir = d;
-                        minlen = l;
-                    }
-                }
-                Region a{lo0, lo1, hi0, hi1, 0, 0, 0}, b = a;
-                if (dir == 0) {
-                    a.hi0 = cut[0] - 1;
-                    b.lo0 = cut[0];
-                } else {
-                    a.hi1 = cut[1] - 1;
-                    b.lo1 = cut[1];
-                }
-                next.push_back(a);
-                next.push_back(b);
+                ZAMR_HIP(hipGetLastError());
+                gens += k;
+                ZAMR_HIP(hipMemcpyAsync(h_brctl, d_brctl, sizeof(BrCtl), hipMemcpyDeviceToHost, cstream));
+                ZAMR_HIP(hipMemcpyAsync(h_brout, br.out, (size_t)peek * sizeof(int4), hipMemcpyDeviceToHost, cstream));
+                ZAMR_HIP(hipStreamSynchronize(cstream));
+                if (h_brctl->overflow || h_brctl->nreg[p] == 0) break;
             }
-            cur.swap(next);
+            if (h_brctl->overflow) {
+                br_alloc(4 * br.reg_cap, 4 * br.hist_cap, 4 * br.out_cap);
+                continue;
+            }
+            const int n = h_brctl->nout;
+            if (n > peek) {
+                ZAMR_HIP(hipMemcpyAsync(h_brout, br.out, (size_t)n * sizeof(int4), hipMemcpyDeviceToHost, cstream));
+                ZAMR_HIP(hipStreamSynchronize(cstream));
+            }
+            br_depth = h_brctl->gens;
+            std::vector<std::array<int, 4>> out(n);
+            for (int k = 0; k < n; ++k) out[k] = {h_brout[k].x, h_brout[k].y, h_brout[k].z, h_brout[k].w};
+            return out;
         }
-        return out;
     }
 
     // Fine boxes from the clusters of the buffered, coarsened tag map:
@@ -705,10 +963,17 @@ class AmrDriver {
         return fine;
     }
 
-    // ErrorEst + buffer/coarsen on the device into d_ctags[0].  Returns true
-    // when the map differs from the one the current grids were made from
-    // (only that flag crosses to the host).
+    // ErrorEst + buffer/coarsen on the device into d_ctags[0] (tag_launch);
+    // tag_result waits for the flag that says whether the map differs from
+    // the one the current grids were made from (only that flag crosses to the
+    // host) and returns it.  Work queued on the main stream in between runs
+    // on without a host round trip.
     bool tag(int mode_last, double time_last) {
+        tag_launch(mode_last, time_last);
+        return tag_result();
+    }
+
+    void tag_launch(int mode_last, double time_last) {
         const long long n = (long long)L0.nx * L0.ny;
         hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
                            L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
@@ -718,7 +983,11 @@ class AmrDriver {
                            L0.nx, L0.ny, n_error_buf, bf_lev, d_ctags[0], cnx, cny, d_ctags[1], d_changed);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipMemcpyAsync(h_flag, d_changed, sizeof(unsigned), hipMemcpyDeviceToHost, stream()));
-        ZAMR_HIP(hipStreamSynchronize(stream()));
+        ZAMR_HIP(hipEventRecord(ev_tag, stream()));
+    }
+
+    bool tag_result() {
+        ZAMR_HIP(hipEventSynchronize(ev_tag));  // the map in d_ctags[0] is complete too
         const unsigned changed = *h_flag;
         if (have_last && !changed) return false;
         std::swap(d_ctags[0], d_ctags[1]);  // [1]: the map clustered now, compared against next time
@@ -751,8 +1020,8 @@ class AmrDriver {
     }
 
     // FillCoarsePatch: valid and in-domain ghost cells of every box of F
-    // from level 0, into F's current buffer.
-    void fill_coarse_patch(Sol& F) {
+    // from level-0 buffer cbuf, into F's current buffer.
+    void fill_coarse_patch(Sol& F, int cbuf) {
         std::vector<CellRect> all;
         for (size_t b = 0; b < F.hb.size(); ++b) {
             const HostBox& B = F.hb[b];
@@ -763,22 +1032,35 @@ class AmrDriver {
         int n = 0;
         long long total = 0;
         CellRect* d = upload_scratch(all, n, total);
-        interp(F, d, n, total, 1u << F.cur, nullptr);
+        interp(F, d, n, total, cbuf, 1u << F.cur, nullptr);
         ZAMR_HIP(hipStreamSynchronize(stream()));
     }
 
-    // AmrCore::regrid(0, time) for level 1.
+    // AmrCore::regrid(0, time) for level 1, in two parts: regrid_tags
+    // (ErrorEst on level 0 as it stands; false when the tags are those the
+    // current grids came from, which give the same grids) and regrid_grids
+    // (clustering, then MakeNewLevelFromCoarse / RemakeLevel / ClearLevel)
+    // with level 0 at `time` read from its buffer cbuf.  Work on the main
+    // stream that leaves cbuf and the fine level alone may run in between.
     struct Clock {
         double& acc;
         std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
         ~Clock() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     };
 
-    void regrid(int mode_last, double time_last, double time) {
-        if (max_level < 1) return;
+    bool regrid_tags(int mode_last, double time_last) {
+        if (max_level < 1) return false;
         Clock clk{regrid_ms};
         ++regrids;
-        if (!tag(mode_last, time_last)) return;  // the same tags give the same grids
+        return tag(mode_last, time_last);
+    }
+
+    void regrid(int mode_last, double time_last, double time) {
+        if (regrid_tags(mode_last, time_last)) regrid_grids(time, L0.cur);
+    }
+
+    void regrid_grids(double time, int cbuf) {
+        Clock clk{regrid_ms};
         auto boxes = make_grids();
         if (boxes.empty()) {
             clear_fine();
@@ -786,7 +1068,7 @@ class AmrDriver {
         }
         if (L1 && boxes == last_boxes) return;
         auto F = new_level(boxes);
-        fill_coarse_patch(*F);
+        fill_coarse_patch(*F, cbuf);
         if (L1) {  // RemakeLevel: the old level's valid cells where it covers
             std::vector<CellRect> cp;
             for (size_t b = 0; b < F->hb.size(); ++b) {
@@ -822,13 +1104,13 @@ class AmrDriver {
         L0.ensure_aux();
         have_last = false;
         for (int it = 0; it < 4; ++it) {
-            if (L1) fill_coarse_patch(*L1);
+            if (L1) fill_coarse_patch(*L1, L0.cur);
             const int old_finest = finest_level();
             regrid(-1, 0.0, 0.0);
             if (old_finest == finest_level() && it > 0) break;
         }
         if (L1) {
-            fill_coarse_patch(*L1);
+            fill_coarse_patch(*L1, L0.cur);
             fresh_aux(*L1, 0.0);
             ZAMR_HIP(hipStreamSynchronize(stream()));
             L1->reset_clock();
@@ -841,11 +1123,26 @@ class AmrDriver {
     // Average-down target, CFL reductions and clocks of one coarse step
     // (ZoomyAmr::TimeStep for lev 0 and its two level-1 substeps).
     // `fine_cb` runs after each fine substep (for the MOOD lines of Evolve).
+    // `groups_fresh`: the caller has just downloaded level 0's group state.
     template <class CB>
-    void coarse_step(int iteration, CB&& fine_cb) {
-        L0.download_groups();
+    void coarse_step(int iteration, CB&& fine_cb, bool groups_fresh = false) {
+        if (!groups_fresh) L0.download_groups();
         const double t0 = L0.h_groups[0].time;
-        if (iteration > 0 && max_level >= 1) regrid(l0_stage_ran ? L0.last_in : -1, t_last_stage, t0);
+        // The regrid at t0 comes first (ZoomyAmr.cpp:736); of it only the tags
+        // must precede the level-0 step, which refills the ghosts they read.
+        // The level-0 stages are queued behind the tag kernels before the host
+        // waits for the tag flag, and the clustering's host round trips and the
+        // new level's fill from Q^n (its buffer c0 outlives the step: order 1
+        // writes the other buffer, order 2 rotates three) overlap them.
+        const bool tagging = iteration > 0 && max_level >= 1;
+        const int c0 = L0.cur;
+        const bool overlap = L0.spatial_order == 1 || L0.nbuf() == 3;
+        if (tagging) {
+            Clock clk{regrid_ms};
+            ++regrids;
+            tag_launch(l0_stage_ran ? L0.last_in : -1, t_last_stage);
+            if (!overlap && tag_result()) regrid_grids(t0, c0);
+        }
         t_last_stage = t0;
         hipStream_t s = stream();
         if (l0_max_stale || !L0.maxinv_fresh) {
@@ -857,8 +1154,17 @@ class AmrDriver {
             ZAMR_HIP(hipGetLastError());
         }
         L0.begin_step();
-        if (L1) hipLaunchKernelGGL(sync_clock_kernel, dim3(1), dim3(1), 0, s, L0.d_groups, L1->d_groups);
         L0.advance_kernels();
+        if (tagging && overlap) {
+            bool changed;
+            {
+                Clock clk{regrid_ms};
+                changed = tag_result();
+            }
+            if (changed) regrid_grids(t0, c0);
+        }
+        // level 0's clock still reads t0 here (end_step below advances it)
+        if (L1) hipLaunchKernelGGL(sync_clock_kernel, dim3(1), dim3(1), 0, s, L0.d_groups, L1->d_groups);
         if (L1) {
             // the reference's end-of-Advance UpdateState leaves Qaux = aux(Q^{n+1});
             // ComputeDt of the next step reads it beside the averaged-down Q
@@ -873,7 +1179,7 @@ class AmrDriver {
         l0_stage_ran = true;
         l0_max_stale = false;
         if (L1) {
-            interp(*L1, d_cf, n_cf, cf_total, L1->mood() ? 7u : 3u, d_cover);
+            interp(*L1, d_cf, n_cf, cf_total, L0.cur, L1->mood() ? 7u : 3u, d_cover);
             if (l1_max_stale) {
                 hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, s, L1->d_groups, L1->ngroups);
                 auto& it = L1->items(L1->hx_of(L1->spatial_order));
@@ -919,8 +1225,7 @@ class AmrDriver {
         ZAMR_HIP(hipStreamSynchronize(stream()));
     }
 
-    bool running() {
-        L0.download_groups();
+    bool running_host() const {  // on level 0's group state as last downloaded
         const GroupState& g = L0.h_groups[0];
         return g.time < L0.time_end && (L0.max_step < 0 || g.step < L0.max_step);
     }
@@ -928,7 +1233,11 @@ class AmrDriver {
     void run_steps(int nsteps, int* done, double* time) {
         L0.download_groups();
         const int step0 = L0.h_groups[0].step;
-        for (int k = 0; k < nsteps && running(); ++k) coarse_step(L0.h_groups[0].step, [](int) {});
+        for (int k = 0; k < nsteps; ++k) {
+            if (k > 0) L0.download_groups();
+            if (!running_host()) break;
+            coarse_step(L0.h_groups[0].step, [](int) {}, true);
+        }
         L0.download_groups();
         if (done) *done = L0.h_groups[0].step - step0;
         if (time) *time = L0.h_groups[0].time;

@@ -364,7 +364,7 @@ class Solver {
             b.off_q[1] = off;
             off = align(off + cells * nq);
             b.off_q[2] = b.off_q[1];
-            if (mood() || jfnk_mode()) {  // Q^n survives the order-2 candidate for the redo / the stage-2 solve
+            if (nbuf() == 3) {  // Q^n survives the step (MOOD redo, stage-2 solve, AMR regrid overlap)
                 b.off_q[2] = off;
                 off = align(off + cells * nq);
             }
@@ -706,7 +706,11 @@ class Solver {
     bool mood() const { return positivity == "mood" && spatial_order == 2; }
     // The source is solved globally after each stage (ZoomyAmr.cpp:647-651).
     bool jfnk_mode() const { return implicit_source && implicit_global; }
-    int nbuf() const { return (mood() || jfnk_mode()) ? 3 : 2; }
+    // An order-2 level 0 under AMR keeps Q^n in its own buffer through the
+    // step too: the regrid that runs beside the level-0 step fills a new fine
+    // level from it.
+    bool amr_base() const { return max_level >= 1 && spatial_order == 2; }
+    int nbuf() const { return (mood() || jfnk_mode() || amr_base()) ? 3 : 2; }
 
     // ---------------- kernels ----------------
     int hx_of(int order) const { return order == 2 ? 2 : 1; }
@@ -882,6 +886,17 @@ class Solver {
             launch_stage_dyn(1, 1, r2);
             last_in = t1;
             cur = t2;
+        } else if (nbuf() == 3) {  // the same stages, Q^{n+1} into a third buffer
+            const int t1 = (cur + 1) % 3, t2 = (cur + 2) % 3;
+            fill_boundary(cur);
+            record(0);
+            launch_stage_dyn(2, 0, stage_args(cur, t1, cur));
+            record(1);
+            fill_boundary(t1);
+            launch_stage_dyn(2, 1, stage_args(t1, t2, cur));
+            record(2);
+            last_in = t1;
+            cur = t2;
         } else {
             fill_boundary(cur);
             record(0);
@@ -920,8 +935,9 @@ class Solver {
         prof_ms[0] = prof_ms[1] = prof_ms[2] = 0.0;
         prof_steps = 0;
         int k = 0;
-        // MOOD rotates three state buffers, so a captured chunk is a multiple of 3 steps
-        const int chunk = profiling ? std::max(nsteps, 1) : (mood() ? 15 : 16);
+        // with three state buffers the rotation repeats every 3 steps, so a
+        // captured chunk is a multiple of 3 steps
+        const int chunk = profiling ? std::max(nsteps, 1) : (nbuf() == 3 ? 15 : 16);
         if (mood()) items(1);  // the redo's work items exist before any capture
         while (k < nsteps) {
             const int n = std::min(chunk, nsteps - k);

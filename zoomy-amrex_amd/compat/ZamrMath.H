@@ -46,6 +46,32 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double div_sqrt(double a, double x) {
 #endif
 }
 
+// 1/x, for x in the normal range: on the device of the throughput build the
+// hardware estimate (v_rcp_f64) refined by two Newton steps (within an ulp
+// or so of the correctly rounded quotient); elsewhere a true division.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// True when every active lane of the wave passes `c` (device), or just `c`
+// (host): a wave-uniform test for a fast path whose result equals the
+// general one wherever it is taken.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool wave_all(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __all(c);
+#else
+    return c;
+#endif
+}
+
 // Keep the device compiler from moving work across this point (an empty asm
 // statement that "redefines" x): used to stop it from interleaving two large
 // independent evaluations, which costs registers a kernel at one wave per

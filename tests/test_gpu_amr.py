@@ -104,6 +104,23 @@ def test_subcycled_steps_with_regrids_match_oracle_bitwise():
     assert abs(z.total_mass() - o.total_mass()) <= 1e-13 * o.total_mass()
 
 
+def test_clustering_grows_its_buffers(monkeypatch):
+    """The device clustering started at one region, the minimum count
+    space and one cluster: every regrid overflows, grows its buffers and
+    repeats until they fit; grids and states stay the oracle's bit for bit."""
+    monkeypatch.setenv("ZAMR_BR_CAP", "1,1,1")
+    c, Q0 = column(steps=6)
+    print(f"[amr] 6 coarse steps column 64^2, clustering from tiny buffers", flush=True)
+    o, z = pair(c, Q0)
+    monkeypatch.delenv("ZAMR_BR_CAP")
+    dts = []
+    for k in range(6):
+        dts.append(o.time_step(sum(dts), k))
+    done, _ = z.run_steps(6)
+    assert done == 6
+    assert_same_hierarchy(o, z)
+
+
 def test_error_est_reads_the_last_fillpatch_ghosts():
     """ErrorEst after steps: neighbours across box edges read the ghost cells
     the last stage's FillPatch left (many level-0 boxes)."""

@@ -23,6 +23,8 @@
 // (FillPatchTwoLevels is handed the coarse level at t_new only), so the
 // coarse-fine ghost cells are interpolated once per coarse step into every
 // fine state buffer; stage kernels never write ghost cells, so they stay.
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -742,7 +744,15 @@ class AmrDriver {
             throw Error(1, "amr: the level-0 grid over blocking_factor / ref_ratio is too large for the clustering");
         ZAMR_HIP(hipMalloc(&d_brctl, sizeof(BrCtl)));
         ZAMR_HIP(hipHostMalloc(&h_brctl, sizeof(BrCtl)));
-        br_alloc(4096, 32 * (cnx + cny), 4096);
+        // starting capacities (regions and count entries per generation,
+        // clusters); ZAMR_BR_CAP="r,h,o" sets them, e.g. tiny, so that the
+        // grow-and-repeat path runs
+        int rcap = 4096, hcap = 32 * (cnx + cny), ocap = 4096;
+        if (const char* e = std::getenv("ZAMR_BR_CAP")) {
+            if (std::sscanf(e, "%d,%d,%d", &rcap, &hcap, &ocap) != 3)
+                throw Error(1, "ZAMR_BR_CAP must read r,h,o (three integers)");
+        }
+        br_alloc(std::max(rcap, 1), std::max(hcap, cnx + cny), std::max(ocap, 1));
     }
 
     // Device clustering buffers at the given capacities (regions per

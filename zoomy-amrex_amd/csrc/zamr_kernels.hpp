@@ -622,12 +622,19 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         VQ fyB;
         VF dyB;
         double hyB = 0.0;
+        // the first leading row and the prologue's rows are fetched together
+        // and then finished: finish() branches (domain edges), and a load is
+        // not moved across a branch, so loading row by row would wait once
+        // per row
+        const int jlead0 = (dim == 2) ? jstart + HX : jstart;
+        typename S::Raw nxt = S::fetch(B, qs, cl, jlead0);
         if constexpr (dim == 2) {
+            typename S::Raw R[2 * HX];
+#pragma unroll
+            for (int r = 0; r < 2 * HX; ++r) R[r] = S::fetch(B, qs, cl, jstart - HX + r);
             VQ P[2 * HX];
 #pragma unroll
-            for (int r = 0; r < 2 * HX; ++r) {
-                P[r] = S::load(B, qs, cl, jstart - HX + r, p, a.clamp, time);
-            }
+            for (int r = 0; r < 2 * HX; ++r) P[r] = S::finish(B, R[r], cl, jstart - HX + r, p, a.clamp, time);
             if (own) {
                 VQ qL, qR;
                 if constexpr (ORDER == 2) {
@@ -660,9 +667,6 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             W0 = P[HX];
             if constexpr (ORDER == 2) W1 = P[HX + 1];
         }
-        // first leading row
-        const int jlead0 = (dim == 2) ? jstart + HX : jstart;
-        typename S::Raw nxt = S::fetch(B, qs, cl, jlead0);
         // the window and the first leading row enter the row loop landed
         land(W0);
         land(W1);

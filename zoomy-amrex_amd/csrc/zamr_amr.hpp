@@ -675,11 +675,11 @@ class AmrDriver {
     hipEvent_t ev_tag = nullptr;  // recorded behind the tag flag's copy to the host
     int4* d_fboxes = nullptr;  // fine boxes in level-0 index space (cover painting)
     size_t fbox_cap = 0;
-    double* spare_pool = nullptr;  // a retired fine level's pool, reused by the next
-    size_t spare_cap = 0;
+    BufferCache fine_cache;  // a retired fine level's device buffers, reused by the next
 
     // level-1 launch lists (device, with their cell totals)
     CellRect* d_cf = nullptr;  // coarse-fine ghost cells
+    size_t cf_cap = 0, avg_cap = 0;
     int n_cf = 0;
     long long cf_total = 0;
     CellRect* d_avg = nullptr;  // level-0 cells under each fine box
@@ -782,7 +782,7 @@ class AmrDriver {
         L1.reset();
         for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags[0], (void*)d_ctags[1],
                         (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_brctl,
-                        (void*)d_fboxes, (void*)spare_pool, (void*)d_tmp_rects})
+                        (void*)d_fboxes, (void*)d_tmp_rects})
             if (p) hipFree(p);
         br_free();
         for (void* p : {(void*)h_brctl, (void*)h_flag})
@@ -802,21 +802,24 @@ class AmrDriver {
 
     static unsigned grid1(long long n) { return (unsigned)((n + 255) / 256); }
 
-    // Upload a rectangle list with its running cell offsets.
-    static void upload_rects(std::vector<CellRect>& r, CellRect*& d, int& n, long long& total, hipStream_t s) {
+    // Upload a rectangle list with its running cell offsets into d (capacity
+    // cap rectangles, grown to twice the need when short).
+    static void upload_rects(std::vector<CellRect>& r, CellRect*& d, size_t& cap, int& n, long long& total,
+                             hipStream_t s) {
         total = 0;
         for (auto& x : r) {
             x.start = total;
             total += (long long)x.ni * x.nj;
         }
-        if (d) hipFree(d);
-        d = nullptr;
         n = (int)r.size();
-        if (n) {
-            ZAMR_HIP(hipMalloc(&d, r.size() * sizeof(CellRect)));
-            ZAMR_HIP(hipMemcpyAsync(d, r.data(), r.size() * sizeof(CellRect), hipMemcpyHostToDevice, s));
-            ZAMR_HIP(hipStreamSynchronize(s));
+        if (!n) return;
+        if (r.size() > cap) {
+            if (d) hipFree(d);
+            cap = 2 * r.size();
+            ZAMR_HIP(hipMalloc(&d, cap * sizeof(CellRect)));
         }
+        ZAMR_HIP(hipMemcpyAsync(d, r.data(), r.size() * sizeof(CellRect), hipMemcpyHostToDevice, s));
+        ZAMR_HIP(hipStreamSynchronize(s));
     }
 
     // A rectangle list for one launch, in a scratch buffer that only grows
@@ -879,8 +882,8 @@ class AmrDriver {
                                    (B.hi[1] - B.lo[1] + 1) / 2, 0});
             fb.push_back(int4{B.lo[0] / 2, B.lo[1] / 2, B.hi[0] / 2, B.hi[1] / 2});
         }
-        upload_rects(cf, d_cf, n_cf, cf_total, stream());
-        upload_rects(avg, d_avg, n_avg, avg_total, stream());
+        upload_rects(cf, d_cf, cf_cap, n_cf, cf_total, stream());
+        upload_rects(avg, d_avg, avg_cap, n_avg, avg_total, stream());
         if (fb.size() > fbox_cap) {
             if (d_fboxes) hipFree(d_fboxes);
             fbox_cap = fb.size() * 2;
@@ -893,14 +896,10 @@ class AmrDriver {
         ZAMR_HIP(hipGetLastError());
     }
 
-    // Retire level 1, keeping its state pool for the next one.
+    // Retire level 1; its device buffers go to fine_cache for the next one.
     void retire_fine() {
         if (!L1) return;
         ZAMR_HIP(hipStreamSynchronize(stream()));
-        if (spare_pool) hipFree(spare_pool);
-        spare_pool = L1->d_pool;
-        spare_cap = L1->pool_cap;
-        L1->d_pool = nullptr;
         L1.reset();
     }
 
@@ -1018,13 +1017,10 @@ class AmrDriver {
     std::unique_ptr<Sol> new_level(const std::vector<std::array<int, 4>>& boxes) {
         auto F = std::make_unique<Sol>();
         F->hist_cap = 16;  // the step history is level 0's
-        F->given_pool = spare_pool;
-        F->given_cap = spare_cap;
+        F->cache = &fine_cache;
         // room for the next regrid's level, so that most remakes reuse the pool
         // (a new pool costs ~0.4 ms of allocation at 2M fine cells)
         F->pool_headroom = 1.25;
-        spare_pool = nullptr;
-        spare_cap = 0;
         F->configure_fine(L0, ref_ratio, boxes);
         return F;
     }

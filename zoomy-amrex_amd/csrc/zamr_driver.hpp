@@ -53,6 +53,42 @@ inline std::vector<std::pair<int, int>> chop_axis(int L, int m) {
     return r;
 }
 
+
+// Device buffers that a retired solver hands to the next one, one per role:
+// an AMR level is remade every few coarse steps, and once the buffers have
+// grown to what the level needs a remake allocates and frees nothing.  Used
+// from one host thread; the levels share a stream, so a buffer's last kernels
+// for the old level run before its first for the new one.
+struct BufferCache {
+    enum Role { POOL, BOXES, GROUPS, HIST, RECTS, ITEMS0, ITEMS1, ITEMS2, PARTIAL, NROLES };
+    void* p[NROLES] = {};
+    size_t bytes[NROLES] = {};
+    // the role's buffer if it holds at least `need` bytes (its size in got)
+    void* take(int r, size_t need, size_t& got) {
+        if (!p[r] || bytes[r] < need) return nullptr;
+        void* q = p[r];
+        got = bytes[r];
+        p[r] = nullptr;
+        bytes[r] = 0;
+        return q;
+    }
+    // keep the larger of the held and the given buffer
+    void give(int r, void* q, size_t b) {
+        if (!q) return;
+        if (p[r] && bytes[r] >= b) {
+            hipFree(q);
+            return;
+        }
+        if (p[r]) hipFree(p[r]);
+        p[r] = q;
+        bytes[r] = b;
+    }
+    ~BufferCache() {
+        for (int r = 0; r < NROLES; ++r)
+            if (p[r]) hipFree(p[r]);
+    }
+};
+
 template <class M, class N>
 class Solver {
    public:
@@ -90,9 +126,9 @@ class Solver {
     bool own_stream = false;
     double* d_pool = nullptr;
     size_t pool_elems = 0;
-    size_t pool_cap = 0;             // elements allocated (>= pool_elems when handed a larger pool)
-    double* given_pool = nullptr;    // optional pool to use instead of allocating (AMR level reuse)
-    size_t given_cap = 0;
+    size_t pool_cap = 0;             // elements allocated (>= pool_elems)
+    BufferCache* cache = nullptr;    // where device buffers come from and go back to (AMR level 1)
+    size_t cap_bytes[BufferCache::NROLES] = {};  // size of each held buffer
     double pool_headroom = 1.0;      // a new pool holds this many times what the boxes need
     BoxDesc* d_boxes = nullptr;
     GroupState* d_groups = nullptr;
@@ -134,26 +170,47 @@ class Solver {
 
     ~Solver() { release(); }
 
+    // A device buffer of at least n elements for `role` in ptr: kept if it is
+    // large enough, else taken from the cache or allocated (headroom times n;
+    // a cached level's small buffers get twice n, so that the next level's
+    // usually fit).
+    template <class T>
+    void dev_get(int role, T*& ptr, size_t n, double headroom = 1.0) {
+        if (cache && role != BufferCache::POOL) headroom = std::max(headroom, 2.0);
+        const size_t need = std::max<size_t>(n, 1) * sizeof(T);
+        if (ptr && cap_bytes[role] >= need) return;
+        dev_put(role, ptr);
+        size_t got = 0;
+        if (cache)
+            if (void* q = cache->take(role, need, got)) {
+                ptr = static_cast<T*>(q);
+                cap_bytes[role] = got;
+                return;
+            }
+        const size_t want = std::max(need, (size_t)((double)need * headroom) / sizeof(T) * sizeof(T));
+        ZAMR_HIP(hipMalloc(&ptr, want));
+        cap_bytes[role] = want;
+    }
+    template <class T>
+    void dev_put(int role, T*& ptr) {
+        if (!ptr) return;
+        if (cache) cache->give(role, ptr, cap_bytes[role]);
+        else hipFree(ptr);
+        ptr = nullptr;
+        cap_bytes[role] = 0;
+    }
+
     void release() {
         jfnk.reset();
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
-        for (auto& it : items_)
-            if (it.d) hipFree(it.d), it.d = nullptr;
-        if (d_pool) hipFree(d_pool);
-        if (d_boxes) hipFree(d_boxes);
-        if (d_groups) hipFree(d_groups);
-        if (d_rects) hipFree(d_rects);
-        if (d_hist) hipFree(d_hist);
-        if (d_partial) hipFree(d_partial);
-        if (given_pool) hipFree(given_pool);
-        given_pool = nullptr;
-        d_pool = nullptr;
-        d_boxes = nullptr;
-        d_groups = nullptr;
-        d_rects = nullptr;
-        d_hist = nullptr;
-        d_partial = nullptr;
+        for (int k = 0; k < 3; ++k) dev_put(BufferCache::ITEMS0 + k, items_[k].d);
+        dev_put(BufferCache::POOL, d_pool);
+        dev_put(BufferCache::BOXES, d_boxes);
+        dev_put(BufferCache::GROUPS, d_groups);
+        dev_put(BufferCache::RECTS, d_rects);
+        dev_put(BufferCache::HIST, d_hist);
+        dev_put(BufferCache::PARTIAL, d_partial);
         for (auto e : ev) hipEventDestroy(e);
         ev.clear();
         if (own_stream && stream) hipStreamDestroy(stream);
@@ -372,21 +429,13 @@ class Solver {
             off = align(off + cells * std::max(na, 1));
         }
         pool_elems = off;
-        if (given_pool && given_cap >= pool_elems) {
-            d_pool = given_pool;
-            pool_cap = given_cap;
-        } else {
-            if (given_pool) hipFree(given_pool);
-            const size_t want = std::max(pool_elems, (size_t)((double)pool_elems * pool_headroom));
-            ZAMR_HIP(hipMalloc(&d_pool, want * sizeof(double)));
-            pool_cap = want;
-        }
-        given_pool = nullptr;
+        dev_get(BufferCache::POOL, d_pool, pool_elems, pool_headroom);
+        pool_cap = cap_bytes[BufferCache::POOL] / sizeof(double);
         ZAMR_HIP(hipMemsetAsync(d_pool, 0, pool_elems * sizeof(double), stream));
         std::vector<BoxDesc> bd(hb.size());
         for (size_t k = 0; k < hb.size(); ++k)
             bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1], d_pool + hb[k].off_q[2]);
-        ZAMR_HIP(hipMalloc(&d_boxes, bd.size() * sizeof(BoxDesc)));
+        dev_get(BufferCache::BOXES, d_boxes, bd.size());
         ZAMR_HIP(hipMemcpyAsync(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, stream));
         h_groups.assign(ngroups, GroupState{});
         for (auto& g : h_groups) {
@@ -402,9 +451,9 @@ class Solver {
             g.max_step = max_step;
             g.active = 1;
         }
-        ZAMR_HIP(hipMalloc(&d_groups, ngroups * sizeof(GroupState)));
+        dev_get(BufferCache::GROUPS, d_groups, (size_t)ngroups);
         upload_groups();
-        ZAMR_HIP(hipMalloc(&d_hist, 2 * (size_t)hist_cap * sizeof(double)));
+        dev_get(BufferCache::HIST, d_hist, 2 * (size_t)hist_cap);
         build_rects();
         if (jfnk_mode()) {
             if (batched) throw Error(5, "solver.implicit_global = true is not part of the batched box list");
@@ -469,7 +518,7 @@ class Solver {
             }
         nrects = (int)rr.size();
         if (nrects) {
-            ZAMR_HIP(hipMalloc(&d_rects, rr.size() * sizeof(CopyRect)));
+            dev_get(BufferCache::RECTS, d_rects, rr.size());
             ZAMR_HIP(hipMemcpyAsync(d_rects, rr.data(), rr.size() * sizeof(CopyRect), hipMemcpyHostToDevice, stream));
         }
     }
@@ -537,7 +586,6 @@ class Solver {
         Items& it = items_[hx];
         const int rows = (dim == 2) ? rows_for(hx) : 1;
         if (it.d && it.hx == hx && it.rows == rows) return it;
-        if (it.d) hipFree(it.d), it.d = nullptr;
         std::vector<WorkItem> v;
         it.box_begin.clear();
         const int txc = WV - 2 * hx;  // one wave64 per item, HX halo columns each side
@@ -552,13 +600,10 @@ class Solver {
         it.hx = hx;
         it.rows = rows;
         it.n = (int)v.size();
-        ZAMR_HIP(hipMalloc(&it.d, v.size() * sizeof(WorkItem)));
+        dev_get(BufferCache::ITEMS0 + hx, it.d, v.size());
         ZAMR_HIP(hipMemcpyAsync(it.d, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
-        if (partial_cap < it.n) {
-            if (d_partial) hipFree(d_partial);
-            partial_cap = it.n;
-            ZAMR_HIP(hipMalloc(&d_partial, (size_t)partial_cap * sizeof(double)));
-        }
+        dev_get(BufferCache::PARTIAL, d_partial, (size_t)it.n);
+        partial_cap = (int)(cap_bytes[BufferCache::PARTIAL] / sizeof(double));
         invalidate_graph();
         return it;
     }

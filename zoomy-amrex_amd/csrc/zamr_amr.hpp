@@ -676,6 +676,7 @@ class AmrDriver {
     int4* d_fboxes = nullptr;  // fine boxes in level-0 index space (cover painting)
     size_t fbox_cap = 0;
     BufferCache fine_cache;  // a retired fine level's device buffers, reused by the next
+    std::unique_ptr<HostStage> stage_ = std::make_unique<HostStage>(4u << 20);  // a regrid's uploads
 
     // level-1 launch lists (device, with their cell totals)
     CellRect* d_cf = nullptr;  // coarse-fine ghost cells
@@ -803,9 +804,8 @@ class AmrDriver {
     static unsigned grid1(long long n) { return (unsigned)((n + 255) / 256); }
 
     // Upload a rectangle list with its running cell offsets into d (capacity
-    // cap rectangles, grown to twice the need when short).
-    static void upload_rects(std::vector<CellRect>& r, CellRect*& d, size_t& cap, int& n, long long& total,
-                             hipStream_t s) {
+    // cap rectangles, grown to twice the need when short), through the stage.
+    void upload_rects(std::vector<CellRect>& r, CellRect*& d, size_t& cap, int& n, long long& total) {
         total = 0;
         for (auto& x : r) {
             x.start = total;
@@ -814,16 +814,18 @@ class AmrDriver {
         n = (int)r.size();
         if (!n) return;
         if (r.size() > cap) {
+            ZAMR_HIP(hipStreamSynchronize(stream()));  // queued launches may still read d
             if (d) hipFree(d);
             cap = 2 * r.size();
             ZAMR_HIP(hipMalloc(&d, cap * sizeof(CellRect)));
         }
-        ZAMR_HIP(hipMemcpyAsync(d, r.data(), r.size() * sizeof(CellRect), hipMemcpyHostToDevice, s));
-        ZAMR_HIP(hipStreamSynchronize(s));
+        stage_->upload(d, r.data(), r.size() * sizeof(CellRect), stream());
     }
 
     // A rectangle list for one launch, in a scratch buffer that only grows
-    // (no allocation or free per regrid); synchronous like upload_rects.
+    // (no allocation or free per regrid), through the stage.  The next call
+    // overwrites the buffer only after the launch that reads it (stream
+    // order).
     CellRect* d_tmp_rects = nullptr;
     size_t tmp_rects_cap = 0;
     CellRect* upload_scratch(std::vector<CellRect>& r, int& n, long long& total) {
@@ -835,12 +837,12 @@ class AmrDriver {
         n = (int)r.size();
         if (!n) return nullptr;
         if (r.size() > tmp_rects_cap) {
+            ZAMR_HIP(hipStreamSynchronize(stream()));  // the last launch may still read it
             if (d_tmp_rects) hipFree(d_tmp_rects);
             tmp_rects_cap = r.size() * 2;
             ZAMR_HIP(hipMalloc(&d_tmp_rects, tmp_rects_cap * sizeof(CellRect)));
         }
-        ZAMR_HIP(hipMemcpyAsync(d_tmp_rects, r.data(), r.size() * sizeof(CellRect), hipMemcpyHostToDevice, stream()));
-        ZAMR_HIP(hipStreamSynchronize(stream()));
+        stage_->upload(d_tmp_rects, r.data(), r.size() * sizeof(CellRect), stream());
         return d_tmp_rects;
     }
 
@@ -882,14 +884,14 @@ class AmrDriver {
                                    (B.hi[1] - B.lo[1] + 1) / 2, 0});
             fb.push_back(int4{B.lo[0] / 2, B.lo[1] / 2, B.hi[0] / 2, B.hi[1] / 2});
         }
-        upload_rects(cf, d_cf, cf_cap, n_cf, cf_total, stream());
-        upload_rects(avg, d_avg, avg_cap, n_avg, avg_total, stream());
+        upload_rects(cf, d_cf, cf_cap, n_cf, cf_total);
+        upload_rects(avg, d_avg, avg_cap, n_avg, avg_total);
         if (fb.size() > fbox_cap) {
             if (d_fboxes) hipFree(d_fboxes);
             fbox_cap = fb.size() * 2;
             ZAMR_HIP(hipMalloc(&d_fboxes, fbox_cap * sizeof(int4)));
         }
-        ZAMR_HIP(hipMemcpyAsync(d_fboxes, fb.data(), fb.size() * sizeof(int4), hipMemcpyHostToDevice, stream()));
+        stage_->upload(d_fboxes, fb.data(), fb.size() * sizeof(int4), stream());
         ZAMR_HIP(hipMemsetAsync(d_cover, 0, (size_t)L0.nx * L0.ny, stream()));
         hipLaunchKernelGGL(cover_paint_kernel, dim3((unsigned)fb.size()), dim3(256), 0, stream(), d_fboxes, d_cover,
                            L0.nx);
@@ -899,7 +901,6 @@ class AmrDriver {
     // Retire level 1; its device buffers go to fine_cache for the next one.
     void retire_fine() {
         if (!L1) return;
-        ZAMR_HIP(hipStreamSynchronize(stream()));
         L1.reset();
     }
 
@@ -1018,6 +1019,7 @@ class AmrDriver {
         auto F = std::make_unique<Sol>();
         F->hist_cap = 16;  // the step history is level 0's
         F->cache = &fine_cache;
+        F->stage = stage_.get();
         // room for the next regrid's level, so that most remakes reuse the pool
         // (a new pool costs ~0.4 ms of allocation at 2M fine cells)
         F->pool_headroom = 1.25;
@@ -1039,7 +1041,6 @@ class AmrDriver {
         long long total = 0;
         CellRect* d = upload_scratch(all, n, total);
         interp(F, d, n, total, cbuf, 1u << F.cur, nullptr);
-        ZAMR_HIP(hipStreamSynchronize(stream()));
     }
 
     // AmrCore::regrid(0, time) for level 1, in two parts: regrid_tags
@@ -1067,6 +1068,7 @@ class AmrDriver {
 
     void regrid_grids(double time, int cbuf) {
         Clock clk{regrid_ms};
+        stage_->reset();  // tag_result() waited on an event behind every earlier upload
         auto boxes = make_grids();
         if (boxes.empty()) {
             clear_fine();

@@ -59,6 +59,39 @@ inline std::vector<std::pair<int, int>> chop_axis(int L, int m) {
 // grown to what the level needs a remake allocates and frees nothing.  Used
 // from one host thread; the levels share a stream, so a buffer's last kernels
 // for the old level run before its first for the new one.
+// Pinned host memory that a regrid's small uploads are copied through, so
+// that each is queued on the stream without waiting (a copy from pageable
+// memory returns only once the stream has reached it, draining the device
+// first).  Reset at the start of each regrid, when every copy queued from it
+// has completed; a request that does not fit is copied from the caller's
+// memory instead.
+struct HostStage {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    explicit HostStage(size_t bytes) {
+        ZAMR_HIP(hipHostMalloc(&base, bytes));
+        cap = bytes;
+    }
+    ~HostStage() {
+        if (base) hipHostFree(base);
+    }
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    void reset() { used = 0; }
+    void upload(void* dst, const void* src, size_t n, hipStream_t s) {
+        if (!n) return;
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off + n <= cap) {
+            std::memcpy(base + off, src, n);
+            used = off + n;
+            ZAMR_HIP(hipMemcpyAsync(dst, base + off, n, hipMemcpyHostToDevice, s));
+        } else {
+            ZAMR_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+            ZAMR_HIP(hipStreamSynchronize(s));
+        }
+    }
+};
+
 struct BufferCache {
     enum Role { POOL, BOXES, GROUPS, HIST, RECTS, ITEMS0, ITEMS1, ITEMS2, PARTIAL, NROLES };
     void* p[NROLES] = {};
@@ -72,14 +105,19 @@ struct BufferCache {
         bytes[r] = 0;
         return q;
     }
-    // keep the larger of the held and the given buffer
+    // keep the larger of the held and the given buffer (the other is freed
+    // once the device is idle: launches already queued may still use it)
     void give(int r, void* q, size_t b) {
         if (!q) return;
         if (p[r] && bytes[r] >= b) {
+            hipDeviceSynchronize();
             hipFree(q);
             return;
         }
-        if (p[r]) hipFree(p[r]);
+        if (p[r]) {
+            hipDeviceSynchronize();
+            hipFree(p[r]);
+        }
         p[r] = q;
         bytes[r] = b;
     }
@@ -128,6 +166,7 @@ class Solver {
     size_t pool_elems = 0;
     size_t pool_cap = 0;             // elements allocated (>= pool_elems)
     BufferCache* cache = nullptr;    // where device buffers come from and go back to (AMR level 1)
+    HostStage* stage = nullptr;      // pinned staging of uploads (AMR level 1); direct copies without
     size_t cap_bytes[BufferCache::NROLES] = {};  // size of each held buffer
     double pool_headroom = 1.0;      // a new pool holds this many times what the boxes need
     BoxDesc* d_boxes = nullptr;
@@ -170,6 +209,12 @@ class Solver {
 
     ~Solver() { release(); }
 
+    // Host-to-device copy on the stream, through the pinned stage when set.
+    void upload(void* dst, const void* src, size_t n) {
+        if (stage) stage->upload(dst, src, n, stream);
+        else ZAMR_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream));
+    }
+
     // A device buffer of at least n elements for `role` in ptr: kept if it is
     // large enough, else taken from the cache or allocated (headroom times n;
     // a cached level's small buffers get twice n, so that the next level's
@@ -179,6 +224,7 @@ class Solver {
         if (cache && role != BufferCache::POOL) headroom = std::max(headroom, 2.0);
         const size_t need = std::max<size_t>(n, 1) * sizeof(T);
         if (ptr && cap_bytes[role] >= need) return;
+        if (ptr && !cache) ZAMR_HIP(hipStreamSynchronize(stream));  // queued launches may still use it
         dev_put(role, ptr);
         size_t got = 0;
         if (cache)
@@ -436,7 +482,7 @@ class Solver {
         for (size_t k = 0; k < hb.size(); ++k)
             bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1], d_pool + hb[k].off_q[2]);
         dev_get(BufferCache::BOXES, d_boxes, bd.size());
-        ZAMR_HIP(hipMemcpyAsync(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, stream));
+        upload(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc));
         h_groups.assign(ngroups, GroupState{});
         for (auto& g : h_groups) {
             g.dt = 0.0;
@@ -462,7 +508,7 @@ class Solver {
             jfnk = std::make_unique<Jfnk<M>>();
             jfnk->setup(stream, d_boxes, bx);
         }
-        ZAMR_HIP(hipStreamSynchronize(stream));
+        if (!stage) ZAMR_HIP(hipStreamSynchronize(stream));
     }
 
     // Device descriptor of box b with the given state buffers.
@@ -489,6 +535,7 @@ class Solver {
     }
 
     void upload_groups() {
+        if (stage) return upload(d_groups, h_groups.data(), ngroups * sizeof(GroupState));
         ZAMR_HIP(hipMemcpyAsync(d_groups, h_groups.data(), ngroups * sizeof(GroupState), hipMemcpyHostToDevice,
                                 stream));
     }
@@ -519,7 +566,7 @@ class Solver {
         nrects = (int)rr.size();
         if (nrects) {
             dev_get(BufferCache::RECTS, d_rects, rr.size());
-            ZAMR_HIP(hipMemcpyAsync(d_rects, rr.data(), rr.size() * sizeof(CopyRect), hipMemcpyHostToDevice, stream));
+            upload(d_rects, rr.data(), rr.size() * sizeof(CopyRect));
         }
     }
 
@@ -601,7 +648,7 @@ class Solver {
         it.rows = rows;
         it.n = (int)v.size();
         dev_get(BufferCache::ITEMS0 + hx, it.d, v.size());
-        ZAMR_HIP(hipMemcpyAsync(it.d, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream));
+        upload(it.d, v.data(), v.size() * sizeof(WorkItem));
         dev_get(BufferCache::PARTIAL, d_partial, (size_t)it.n);
         partial_cap = (int)(cap_bytes[BufferCache::PARTIAL] / sizeof(double));
         invalidate_graph();

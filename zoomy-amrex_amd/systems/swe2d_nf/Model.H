@@ -86,6 +86,12 @@ struct Model {
 
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_hinv(amrex::Real h, amrex::Real eps)
     {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
+        // h >= eps gives h^4 >= eps^4 (rounding is monotone), where the
+        // formula is 1/h: a wave whose depths all lie there takes the
+        // reciprocal (one v_rcp and four FMAs instead of a v_rsq and eleven)
+        if (zamr_math::wave_all(h >= eps)) return zamr_math::recip(h);
+#endif
         const amrex::Real h4 = amrex::Math::powi<4>(h);
         const amrex::Real e4 = amrex::Math::powi<4>(eps);
         // h4 is +0, positive or NaN, so fmax is max(h4, e4) here in every case

@@ -955,7 +955,8 @@ class AmrDriver {
     }
 
     // Fine boxes from the clusters of the buffered, coarsened tag map:
-    // refined by bf_lev * ref_ratio, cut to amr.max_grid_size in cluster units.
+    // refined by bf_lev * ref_ratio, cut to amr.max_grid_size in cluster units
+    // (in x to a whole number of stage strips within it).
     std::vector<std::array<int, 4>> make_grids() {
         auto cl = gpu_cluster();
         std::sort(cl.begin(), cl.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
@@ -963,11 +964,18 @@ class AmrDriver {
         });
         const int r = ref_ratio * bf_lev;
         const int m = std::max(1, fine_max_grid_size / r);
+        // x pieces as wide as the most whole stage-kernel strips (60 cells at
+        // order 2) within max_grid_size allow, when that is a whole number of
+        // cluster units: only the union of the boxes enters the results, and
+        // a strip-aligned width keeps every lane of a strip on a cell
+        const int strip = WV - 2 * L0.hx_of(L0.spatial_order);
+        const int wx = (fine_max_grid_size / strip) * strip;
+        const int mx = (wx >= r && wx % r == 0) ? wx / r : m;
         std::vector<std::array<int, 4>> fine;
         for (auto& c : cl)
             for (int y0 = c[1]; y0 <= c[3]; y0 += m)
-                for (int x0 = c[0]; x0 <= c[2]; x0 += m) {
-                    const int x1 = std::min(x0 + m - 1, c[2]), y1 = std::min(y0 + m - 1, c[3]);
+                for (int x0 = c[0]; x0 <= c[2]; x0 += mx) {
+                    const int x1 = std::min(x0 + mx - 1, c[2]), y1 = std::min(y0 + m - 1, c[3]);
                     fine.push_back({x0 * r, y0 * r, (x1 + 1) * r - 1, (y1 + 1) * r - 1});
                 }
         return fine;

@@ -780,6 +780,7 @@ class AmrDriver {
     }
 
     ~AmrDriver() {
+        hipDeviceSynchronize();  // queued copies from the pinned stage and launches on the buffers below
         L1.reset();
         for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags[0], (void*)d_ctags[1],
                         (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_brctl,

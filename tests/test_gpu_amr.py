@@ -121,6 +121,29 @@ def test_clustering_grows_its_buffers(monkeypatch):
     assert_same_hierarchy(o, z)
 
 
+def test_regrid_of_a_dry_map_clears_the_fine_level():
+    """A regrid of a map without tags (every cell dry): the device clustering
+    returns no cluster and level 1 is cleared, as in the oracle; a regrid of
+    the wet column makes it again (from the retired level's buffers), with
+    the oracle's grids and fill."""
+    c, Q0 = column()
+    print(f"[amr] regrid: dry map clears level 1, wet map remakes it", flush=True)
+    o, z = pair(c, Q0)
+    assert z.finest_level() == o.finest_level() == 1
+    dry = np.zeros_like(Q0)
+    o.set_state(dry)
+    z.set_state(dry)
+    o.regrid(0.0)
+    z.regrid(0.0)
+    assert z.finest_level() == o.finest_level() == 0
+    o.set_state(Q0)
+    z.set_state(Q0)
+    o.regrid(0.0)
+    z.regrid(0.0)
+    assert z.finest_level() == 1
+    assert_same_hierarchy(o, z)
+
+
 def test_error_est_reads_the_last_fillpatch_ghosts():
     """ErrorEst after steps: neighbours across box edges read the ghost cells
     the last stage's FillPatch left (many level-0 boxes)."""

@@ -61,17 +61,6 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double recip(double x) {
 #endif
 }
 
-// True when every active lane of the wave passes `c` (device), or just `c`
-// (host): a wave-uniform test for a fast path whose result equals the
-// general one wherever it is taken.
-AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool wave_all(bool c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __all(c);
-#else
-    return c;
-#endif
-}
-
 // Keep the device compiler from moving work across this point (an empty asm
 // statement that "redefines" x): used to stop it from interleaving two large
 // independent evaluations, which costs registers a kernel at one wave per

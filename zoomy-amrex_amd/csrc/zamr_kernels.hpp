@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace zamr {
 
@@ -195,6 +196,13 @@ __device__ __forceinline__ unsigned minmod_code(double a, double b) {
     return (fabs(a) < fabs(b)) ? 1u : 2u;
 }
 
+// Models whose update_aux_variables(Q, 0) is zero but for hinv of the depth
+// (M::zamr_aux_only_hinv, with the hinv row and the wet-dry epsilon's index).
+template <class M, class = void>
+struct AuxOnlyHinv : std::false_type {};
+template <class M>
+struct AuxOnlyHinv<M, std::void_t<decltype(M::zamr_aux_only_hinv)>> : std::bool_constant<M::zamr_aux_only_hinv> {};
+
 template <class M>
 struct Sys {
     static constexpr int nq = M::n_dof_q;
@@ -349,6 +357,36 @@ struct Sys {
             const double bf = dev_max(bL, bR);
             hLs = dev_max(0.0, qL(1, 0) + bL - bf);
             hRs = dev_max(0.0, qR(1, 0) + bR - bf);
+            if constexpr (!kBitwise && AuxOnlyHinv<M>::value) {
+                // A face whose depths before and after the reconstruction all
+                // reach eps: hinv of the reconstructed depth is 1/h* there, and
+                // one reciprocal r = 1/(h h*) per side gives both the scaling
+                // h*/h = h* h* r and 1/h* = h r.  Decided per face (a face's
+                // value never depends on the faces sharing its wave).
+                const double eps = p(M::zamr_idx_eps, 0);
+                const double hL = qL(1, 0), hR = qR(1, 0);
+                if (hL >= eps && hR >= eps && hLs >= eps && hRs >= eps) {
+                    const double rL = recip(hL * hLs), rR = recip(hR * hRs);
+                    const double sL = hLs * hLs * rL, sR = hRs * hRs * rR;
+#pragma unroll
+                    for (int d = 0; d < dim; ++d) {
+                        qL(2 + d, 0) *= sL;
+                        qR(2 + d, 0) *= sR;
+                    }
+                    qL(1, 0) = hLs;
+                    qR(1, 0) = hRs;
+                    qL(0, 0) = bf;
+                    qR(0, 0) = bf;
+                    VA aL{}, aR{};
+                    aL(M::zamr_idx_hinv, 0) = hL * rL;
+                    aR(M::zamr_idx_hinv, 0) = hR * rR;
+                    VN nh{};
+                    nh(DIR, 0) = 1.0;
+                    flux = N::numerical_flux(qL, qR, aL, aR, p, nh);
+                    fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
+                    return;
+                }
+            }
             double sL, sR;
             if constexpr (kBitwise) {
                 sL = (qL(1, 0) > 0.0) ? hLs / qL(1, 0) : 0.0;

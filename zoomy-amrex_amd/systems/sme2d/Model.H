@@ -86,9 +86,11 @@ struct Model {
     {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
         // h >= eps gives h^4 >= eps^4 (rounding is monotone), where the
-        // formula is 1/h: a wave whose depths all lie there takes the
-        // reciprocal (one v_rcp and four FMAs instead of a v_rsq and eleven)
-        if (zamr_math::wave_all(h >= eps)) return zamr_math::recip(h);
+        // formula is 1/h: such a cell takes the reciprocal (one v_rcp and four
+        // FMAs instead of a v_rsq and eleven).  Decided per cell, so a cell's
+        // value never depends on the cells sharing its wave; a wave with no
+        // dry cell skips the formula.
+        if (h >= eps) return zamr_math::recip(h);
 #endif
         const amrex::Real h4 = amrex::Math::powi<4>(h);
         const amrex::Real e4 = amrex::Math::powi<4>(eps);

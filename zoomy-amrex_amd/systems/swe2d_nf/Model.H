@@ -54,6 +54,10 @@ struct Model {
     static constexpr int zamr_n_aux_in = 1;          // aux rows read from storage (per-cell inputs)
     static constexpr int zamr_aux_in[1] = {1};
     static constexpr int zamr_idx_hinv = 6;
+    // update_aux_variables(Q, 0) is zero but for hinv(Q(1), p(zamr_idx_eps)) in
+    // row zamr_idx_hinv (a face's reconstructed states take their aux so)
+    static constexpr bool zamr_aux_only_hinv = true;
+    static constexpr int zamr_idx_eps = 5;
     static const std::vector<std::string> get_boundary_tags() { return { "East", "North", "South", "West", "wall" }; }
     static const std::vector<std::string> parameter_names() { return { "g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s" }; }
     static const std::vector<T> default_parameters() { return { 9.81, 0.0, 0.0, 1.0, 1.0, 0.0001, 0.0, 0.0, 0.0 }; }
@@ -88,9 +92,11 @@ struct Model {
     {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(ZAMR_BITWISE)
         // h >= eps gives h^4 >= eps^4 (rounding is monotone), where the
-        // formula is 1/h: a wave whose depths all lie there takes the
-        // reciprocal (one v_rcp and four FMAs instead of a v_rsq and eleven)
-        if (zamr_math::wave_all(h >= eps)) return zamr_math::recip(h);
+        // formula is 1/h: such a cell takes the reciprocal (one v_rcp and four
+        // FMAs instead of a v_rsq and eleven).  Decided per cell, so a cell's
+        // value never depends on the cells sharing its wave; a wave with no
+        // dry cell skips the formula.
+        if (h >= eps) return zamr_math::recip(h);
 #endif
         const amrex::Real h4 = amrex::Math::powi<4>(h);
         const amrex::Real e4 = amrex::Math::powi<4>(eps);

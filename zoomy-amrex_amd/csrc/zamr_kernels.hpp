@@ -352,48 +352,42 @@ struct Sys {
                                                        double& hRs) {
         hLs = qL(1, 0);
         hRs = qR(1, 0);
+        VA aL, aR;
+        bool aux_set = false;
         if (wb) {  // Audusse hydrostatic reconstruction, make_rhs.H:84-98
             const double bL = qL(0, 0), bR = qR(0, 0);
             const double bf = dev_max(bL, bR);
-            hLs = dev_max(0.0, qL(1, 0) + bL - bf);
-            hRs = dev_max(0.0, qR(1, 0) + bR - bf);
-            if constexpr (!kBitwise && AuxOnlyHinv<M>::value) {
+            const double hL = qL(1, 0), hR = qR(1, 0);
+            hLs = dev_max(0.0, hL + bL - bf);
+            hRs = dev_max(0.0, hR + bR - bf);
+            double sL, sR;
+            if constexpr (kBitwise) {
+                sL = (hL > 0.0) ? hLs / hL : 0.0;
+                sR = (hR > 0.0) ? hRs / hR : 0.0;
+            } else if constexpr (AuxOnlyHinv<M>::value) {
                 // A face whose depths before and after the reconstruction all
                 // reach eps: hinv of the reconstructed depth is 1/h* there, and
                 // one reciprocal r = 1/(h h*) per side gives both the scaling
                 // h*/h = h* h* r and 1/h* = h r.  Decided per face (a face's
-                // value never depends on the faces sharing its wave).
+                // value never depends on the faces sharing its wave); only the
+                // two factors and the two aux rows differ between the paths.
                 const double eps = p(M::zamr_idx_eps, 0);
-                const double hL = qL(1, 0), hR = qR(1, 0);
                 if (hL >= eps && hR >= eps && hLs >= eps && hRs >= eps) {
                     const double rL = recip(hL * hLs), rR = recip(hR * hRs);
-                    const double sL = hLs * hLs * rL, sR = hRs * hRs * rR;
-#pragma unroll
-                    for (int d = 0; d < dim; ++d) {
-                        qL(2 + d, 0) *= sL;
-                        qR(2 + d, 0) *= sR;
-                    }
-                    qL(1, 0) = hLs;
-                    qR(1, 0) = hRs;
-                    qL(0, 0) = bf;
-                    qR(0, 0) = bf;
-                    VA aL{}, aR{};
+                    sL = hLs * hLs * rL;
+                    sR = hRs * hRs * rR;
+                    aL = VA{};
+                    aR = VA{};
                     aL(M::zamr_idx_hinv, 0) = hL * rL;
                     aR(M::zamr_idx_hinv, 0) = hR * rR;
-                    VN nh{};
-                    nh(DIR, 0) = 1.0;
-                    flux = N::numerical_flux(qL, qR, aL, aR, p, nh);
-                    fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
-                    return;
+                    aux_set = true;
+                } else {
+                    sL = (hL > 0.0) ? hLs * recip(hL) : 0.0;
+                    sR = (hR > 0.0) ? hRs * recip(hR) : 0.0;
                 }
-            }
-            double sL, sR;
-            if constexpr (kBitwise) {
-                sL = (qL(1, 0) > 0.0) ? hLs / qL(1, 0) : 0.0;
-                sR = (qR(1, 0) > 0.0) ? hRs / qR(1, 0) : 0.0;
             } else {
-                sL = (qL(1, 0) > 0.0) ? hLs * recip(qL(1, 0)) : 0.0;
-                sR = (qR(1, 0) > 0.0) ? hRs * recip(qR(1, 0)) : 0.0;
+                sL = (hL > 0.0) ? hLs * recip(hL) : 0.0;
+                sR = (hR > 0.0) ? hRs * recip(hR) : 0.0;
             }
 #pragma unroll
             for (int d = 0; d < dim; ++d) {
@@ -405,8 +399,10 @@ struct Sys {
             qL(0, 0) = bf;
             qR(0, 0) = bf;
         }
-        const VA aL = aux_of(qL, p, time, XL);
-        const VA aR = aux_of(qR, p, time, XR);
+        if (!aux_set) {
+            aL = aux_of(qL, p, time, XL);
+            aR = aux_of(qR, p, time, XR);
+        }
         VN nh{};
         nh(DIR, 0) = 1.0;
         flux = N::numerical_flux(qL, qR, aL, aR, p, nh);

@@ -54,11 +54,6 @@ inline std::vector<std::pair<int, int>> chop_axis(int L, int m) {
 }
 
 
-// Device buffers that a retired solver hands to the next one, one per role:
-// an AMR level is remade every few coarse steps, and once the buffers have
-// grown to what the level needs a remake allocates and frees nothing.  Used
-// from one host thread; the levels share a stream, so a buffer's last kernels
-// for the old level run before its first for the new one.
 // Pinned host memory that a regrid's small uploads are copied through, so
 // that each is queued on the stream without waiting (a copy from pageable
 // memory returns only once the stream has reached it, draining the device
@@ -92,6 +87,11 @@ struct HostStage {
     }
 };
 
+// Device buffers that a retired solver hands to the next one, one per role:
+// an AMR level is remade every few coarse steps, and once the buffers have
+// grown to what the level needs a remake allocates and frees nothing.  Used
+// from one host thread; the levels share a stream, so a buffer's last kernels
+// for the old level run before its first for the new one.
 struct BufferCache {
     enum Role { POOL, BOXES, GROUPS, HIST, RECTS, ITEMS0, ITEMS1, ITEMS2, PARTIAL, NROLES };
     void* p[NROLES] = {};
@@ -166,7 +166,7 @@ class Solver {
     size_t pool_elems = 0;
     size_t pool_cap = 0;             // elements allocated (>= pool_elems)
     BufferCache* cache = nullptr;    // where device buffers come from and go back to (AMR level 1)
-    HostStage* stage = nullptr;      // pinned staging of uploads (AMR level 1); direct copies without
+    HostStage* stage = nullptr;      // pinned staging of uploads (AMR level 1; else direct copies)
     size_t cap_bytes[BufferCache::NROLES] = {};  // size of each held buffer
     double pool_headroom = 1.0;      // a new pool holds this many times what the boxes need
     BoxDesc* d_boxes = nullptr;

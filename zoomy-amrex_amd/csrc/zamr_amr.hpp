@@ -204,33 +204,41 @@ __global__ void __launch_bounds__(256) error_est_kernel(CoarseGrid g, int cur, i
                                                         unsigned char* __restrict__ tags) {
 #pragma clang fp contract(off)
     using S = Sys<M>;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // a level's cells fit in 32 bits
-    if (k >= g.nx * g.ny) return;
-    const int i = k % g.nx, j = k / g.nx;
+    // one row per blockIdx.y; a wave whose cells lie in one box (the usual
+    // case) reads that box's descriptor once, wave-uniformly
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nx) return;
     typename S::VP p;
     for (int q = 0; q < S::np; ++q) p(q, 0) = pp.v[q];
-    const BoxView B = view(g.boxes, coarse_box_index(g, i, j));
-    auto h_at = [&](int c, int r) -> double {
-        if (c >= B.lo0 && c <= B.hi0 && r >= B.lo1 && r <= B.hi1) {
-            const long long idx = (long long)(c - B.gl0) + (long long)(r - B.gl1) * B.sy;
-            return qsel(B, cur)[idx + 1 * B.sc];
+    const int k = j * g.nx + i;  // a level's cells fit in 32 bits
+    auto cell = [&](const BoxView& B) {
+        auto h_at = [&](int c, int r) -> double {
+            if (c >= B.lo0 && c <= B.hi0 && r >= B.lo1 && r <= B.hi1) {
+                const long long idx = (long long)(c - B.gl0) + (long long)(r - B.gl1) * B.sy;
+                return qsel(B, cur)[idx + 1 * B.sc];
+            }
+            if (last < 0) return 0.0;
+            return S::load(B, qsel(B, last), c, r, p, clamp, time_last)(1, 0);
+        };
+        const long long idx = (long long)(i - B.gl0) + (long long)(j - B.gl1) * B.sy;
+        unsigned char t = 0;
+        if (qsel(B, cur)[idx] < b_max) {
+            double grad = 0.0;
+            const double dhdx = fabs(h_at(i + 1, j) - h_at(i - 1, j));
+            grad += dhdx * dhdx;
+            if constexpr (S::dim >= 2) {
+                const double dhdy = fabs(h_at(i, j + 1) - h_at(i, j - 1));
+                grad += dhdy * dhdy;
+            }
+            if (sqrt(grad) > threshold || qsel(B, cur)[idx + B.sc] > 1.0e-2) t = 1;
         }
-        if (last < 0) return 0.0;
-        return S::load(B, qsel(B, last), c, r, p, clamp, time_last)(1, 0);
+        tags[k] = t;
     };
-    const long long idx = (long long)(i - B.gl0) + (long long)(j - B.gl1) * B.sy;
-    unsigned char t = 0;
-    if (qsel(B, cur)[idx] < b_max) {
-        double grad = 0.0;
-        const double dhdx = fabs(h_at(i + 1, j) - h_at(i - 1, j));
-        grad += dhdx * dhdx;
-        if constexpr (S::dim >= 2) {
-            const double dhdy = fabs(h_at(i, j + 1) - h_at(i, j - 1));
-            grad += dhdy * dhdy;
-        }
-        if (sqrt(grad) > threshold || qsel(B, cur)[idx + B.sc] > 1.0e-2) t = 1;
-    }
-    tags[k] = t;
+    const int bi = coarse_box_index(g, i, j);
+    const int b0 = __builtin_amdgcn_readfirstlane(bi);
+    if (__all(bi == b0)) cell(view(g.boxes, b0));
+    else cell(view(g.boxes, bi));
 }
 
 // TagBoxArray buffer + coarsen: coarsened cell (I, J) of factor bf is tagged
@@ -994,7 +1002,8 @@ class AmrDriver {
 
     void tag_launch(int mode_last, double time_last) {
         const long long n = (long long)L0.nx * L0.ny;
-        hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
+        hipLaunchKernelGGL((error_est_kernel<M>), dim3((unsigned)((L0.nx + 255) / 256), (unsigned)L0.ny), dim3(256), 0,
+                           stream(), grid(), L0.cur, mode_last,
                            L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipMemsetAsync(d_changed, 0, sizeof(unsigned), stream()));
@@ -1235,7 +1244,8 @@ class AmrDriver {
     // or 0 before any), one byte per cell in raster order.
     void error_est_raw(int mode_last, double time_last, unsigned char* host) {
         const long long n = (long long)L0.nx * L0.ny;
-        hipLaunchKernelGGL((error_est_kernel<M>), dim3(grid1(n)), dim3(256), 0, stream(), grid(), L0.cur, mode_last,
+        hipLaunchKernelGGL((error_est_kernel<M>), dim3((unsigned)((L0.nx + 255) / 256), (unsigned)L0.ny), dim3(256), 0,
+                           stream(), grid(), L0.cur, mode_last,
                            L0.prm, L0.clamp_flag(), time_last, tag_threshold, tag_b_max, d_tags);
         ZAMR_HIP(hipGetLastError());
         ZAMR_HIP(hipMemcpyAsync(host, d_tags, (size_t)n, hipMemcpyDeviceToHost, stream()));

@@ -918,6 +918,9 @@ __global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__
         const int c = w.i0 + t;
         double lam = 0.0;
         if (t < txw && c <= B.hi0) {
+            // four rows per trip: their loads go out together instead of one
+            // row's latency per row
+#pragma unroll 4
             for (int j = w.j0; j < w.j0 + w.nj; ++j) {
                 const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
                 typename S::VQ q;

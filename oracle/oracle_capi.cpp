@@ -69,6 +69,17 @@ int orc_tag_name(int k, char* buf, int cap) {
     return n;
 }
 
+// Copies parameter k's name (Model::parameter_names) into buf; returns its length.
+int orc_param_name(int k, char* buf, int cap) {
+    auto t = Model::parameter_names();
+    if (k < 0 || k >= (int)t.size()) return -1;
+    int n = (int)t[k].size();
+    if (n >= cap) n = cap - 1;
+    std::memcpy(buf, t[k].data(), n);
+    buf[n] = 0;
+    return n;
+}
+
 void orc_default_params(double* out) {
     auto v = Model::default_parameters();
     for (int i = 0; i < Model::n_parameters; ++i) out[i] = v[i];

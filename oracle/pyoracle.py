@@ -45,6 +45,7 @@ def load(system):
     for name in ("orc_nq", "orc_na", "orc_np", "orc_dim", "orc_ntags"):
         getattr(lib, name).restype = ctypes.c_int
     lib.orc_tag_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.orc_param_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     lib.orc_default_params.argtypes = [_D]
     lib.orc_create.restype = ctypes.c_void_p
     lib.orc_create.argtypes = [_I, _D, _D]
@@ -131,7 +132,10 @@ class Oracle:
             self.tags.append(buf.value.decode())
         p = np.zeros(self.np_)
         lib.orc_default_params(_dp(p))
-        self.param_names = None
+        self.param_names = []
+        for k in range(self.np_):
+            lib.orc_param_name(k, buf, 64)
+            self.param_names.append(buf.value.decode())
         for name, v in (c["params"] or {}).items():
             p[self.param_index(name)] = v
         self.params = p
@@ -165,11 +169,8 @@ class Oracle:
             ad = np.array([a["grid_eff"], a["threshold"], a["b_max"]], dtype=np.float64)
             self.h = lib.orc_amr_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p), ai.ctypes.data_as(_I), _dp(ad))
 
-    _NAMES = ("g", "ex", "ey", "ez", "rho", "wet_dry_eps", "nu", "n_m", "lambda_s")
-
     def param_index(self, name):
-        names = self.case.get("param_names") or self._NAMES
-        return list(names).index(name)
+        return self.param_names.index(name)
 
     def close(self):
         if getattr(self, "h", None):

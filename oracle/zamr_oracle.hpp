@@ -551,6 +551,36 @@ struct Solver {
         if constexpr (dim == 2) R(i, j, idx_h + 2) += 0.5 * p(0, 0) * (hL_yr * hL_yr - hR_yl * hR_yl) / dx[1];
     }
 
+    // make_rhs.H:166-229 (compile-time gated by Model::has_diffusion): at each
+    // face the model's diffusive flux of the face-averaged state and aux with
+    // the face-normal central difference as the gradient (the other entries
+    // zero), projected on the normal; RHS += (Fd_r - Fd_l) / dx per direction.
+    void add_diffusive_flux(const Fab& fq, const Fab& fa, Fab& R, int i, int j) const {
+        if constexpr (Model::has_diffusion) {
+            constexpr int ng_q = Model::n_dof_gradQ;
+            auto face = [&](int iL, int jL, int iR, int jR, int dir, Real dxf) {
+                VQ qf;
+                VA af;
+                for (int v = 0; v < nq; ++v) qf(v, 0) = 0.5 * (fq(iL, jL, v) + fq(iR, jR, v));
+                for (int v = 0; v < na; ++v) af(v, 0) = 0.5 * (fa(iL, jL, v) + fa(iR, jR, v));
+                amrex::SmallMatrix<Real, ng_q, 1> g{};
+                for (int v = 0; v < nq; ++v) g(v * dim + dir, 0) = (fq(iR, jR, v) - fq(iL, jL, v)) / dxf;
+                const auto Fd = Model::diffusive_flux(qf, af, g, p);
+                VQ out;
+                for (int v = 0; v < nq; ++v) out(v, 0) = Fd(v * dim + dir, 0);
+                return out;
+            };
+            const VQ xr = face(i, j, i + 1, j, 0, dx[0]);
+            const VQ xl = face(i - 1, j, i, j, 0, dx[0]);
+            for (int v = 0; v < nq; ++v) R(i, j, v) += (xr(v, 0) - xl(v, 0)) / dx[0];
+            if constexpr (dim == 2) {
+                const VQ yr = face(i, j, i, j + 1, 1, dx[1]);
+                const VQ yl = face(i, j - 1, i, j, 1, dx[1]);
+                for (int v = 0; v < nq; ++v) R(i, j, v) += (yr(v, 0) - yl(v, 0)) / dx[1];
+            }
+        }
+    }
+
     // make_rhs.H:234-290 (cell-centric fused kernel).
     void compute_cell_rhs(int b, int i, int j, int order, Real time) {
         const Fab& fq = Q.fabs[b];
@@ -580,6 +610,7 @@ struct Solver {
             if (!s.implicit_source) R(i, j, n) += src(n, 0);
         }
         if (s.well_balanced) add_wb_bed_source(R, i, j, hL_xr, hR_xl, hL_yr, hR_yl);
+        add_diffusive_flux(fq, fa, R, i, j);
     }
 
     // make_rhs.H:320-342 (face pass) and :347-399 (cell pass).
@@ -630,6 +661,7 @@ struct Solver {
             }
             add_wb_bed_source(R, i, j, hL_xr, hR_xl, hL_yr, hR_yl);
         }
+        add_diffusive_flux(fq, fa, R, i, j);
     }
 
     // make_rhs.H:429-509: ONE point-implicit step with Gaussian elimination.

@@ -95,3 +95,74 @@ def smooth_state(nx, ny, seed=1, bed=True, amp=0.05):
     hu = h * 0.3 * np.sin(2 * np.pi * Y)
     hv = h * 0.2 * np.cos(2 * np.pi * X)
     return np.array([b, h, hu, hv])
+
+
+def bed_step(nx=8, ny=4, axis=0, u=0.3, v=-0.2, h_lo=1.0, b_hi=0.5, k=4):
+    """Moving water over a bed step: b = 0 below index k along `axis` and
+    b_hi from k on, a flat free surface eta = h_lo and one velocity (u, v)
+    everywhere (a 1-D state when ny == 1: rows b, h, hu).
+
+    The hydrostatic reconstruction maps the two sides of the step face to
+    the same state (b*, h*, h* u, h* v) with h* = h_lo - b_hi, because the
+    momentum is rescaled by h*/h; every other face separates equal states.
+    So a Rusanov stage has a closed form: the flux through the step face is
+    the physical flux of that common state, with no dissipation, and only
+    the cell below the step changes:
+        d(h)/dt   = (h_lo - h*) u_n / dx
+        d(hu_t)/dt = (h_lo - h*) u_n u_t / dx   for the momentum along the face,
+        d(hu_n)/dt = (h_lo - h*) u_n u_n / dx   (the pressure terms cancel
+                     against the well-balanced bed source).
+    Returns (Q, expected d/dt of the cell below the step per row)."""
+    hs = h_lo - b_hi
+    if ny == 1:
+        b = np.where(np.arange(nx) < k, 0.0, b_hi)
+        h = h_lo - b
+        Q = np.array([b, h, h * u])[:, None, :]
+        rate = np.array([0.0, (h_lo - hs) * u, (h_lo - hs) * u * u])
+        return Q, rate
+    idx = np.arange(nx if axis == 0 else ny)
+    line = np.where(idx < k, 0.0, b_hi)
+    b = np.tile(line, (ny, 1)) if axis == 0 else np.tile(line[:, None], (1, nx))
+    h = h_lo - b
+    Q = np.array([b, h, h * u, h * v])
+    un = u if axis == 0 else v
+    rate = np.array([0.0, (h_lo - hs) * un, (h_lo - hs) * un * u, (h_lo - hs) * un * v])
+    return Q, rate
+
+
+def bed_step_case(nx, ny, order, dim):
+    side = ("West", "East", "South", "North")
+    return dict(nx=nx, ny=ny, prob_lo=(0.0, 0.0), prob_hi=(float(nx), float(max(ny, 1))), max_grid_size=64,
+                spatial_order=order, implicit_source=False, well_balanced=True, clamp_positivity=False, cfl=0.45,
+                dtmin=1e-9, dtmax=1.0, time_end=1e30, max_step=-1, bc=side if dim == 2 else side[:2] + ("", ""))
+
+
+def bed_step_expected(Q, rate, dt, axis, k):
+    """The state after one stage of length dt: only the cells just below the
+    step (index k-1 along `axis`) change, by dt * rate."""
+    E = Q.copy()
+    if axis == 0:
+        E[:, :, k - 1] = Q[:, :, k - 1] + dt * rate[:, None]
+    else:
+        E[:, k - 1, :] = Q[:, k - 1, :] + dt * rate[:, None]
+    return E
+
+
+def fallback_state(nx=60, ny=8, u=0.3, v=-0.2):
+    """Three bands along x for the implicit-source fallback test system
+    (tests/systems/swe2d_fallback, S_m = -kappa m / h): h = 1 (cols < nx/3),
+    h = 4 (next third), dry (last third), one velocity in the wet bands."""
+    h = np.zeros((ny, nx))
+    h[:, : nx // 3] = 1.0
+    h[:, nx // 3: 2 * nx // 3] = 4.0
+    return np.array([0 * h, h, h * u, h * v])
+
+
+FALLBACK_DT = 1.0 / 256
+FALLBACK_KAPPA = -256.0  # -1/dt: I - dt J has zero momentum rows where h = 1
+
+
+def fallback_case(order, mgs=64, nx=60, ny=8):
+    return dict(nx=nx, ny=ny, prob_lo=(0.0, 0.0), prob_hi=(1.0, ny / nx), max_grid_size=mgs, spatial_order=order,
+                implicit_source=True, well_balanced=False, clamp_positivity=False, cfl=0.45, dtmin=1e-12, dtmax=1.0,
+                time_end=1e30, max_step=-1, bc=("wall", "wall", "wall", "wall"), params={"kappa": FALLBACK_KAPPA})

@@ -115,3 +115,70 @@ def test_mood_per_group_in_box_lists():
     assert z.mood_stats(1)[1] == 0
     assert np.array_equal(z.get_state(lev=0), o.get_state())
     assert np.array_equal(z.get_state(lev=1), quiet)
+
+
+def test_mood_run_steps_twice_matches_oracle():
+    """Two run_steps calls whose first ends with eager steps (20 = one captured
+    chunk of 15 + 5 eager): the buffer rotation then starts the second call
+    from another buffer than the captured chunk did, and the chunk is captured
+    again from there."""
+    c, Q0 = column(steps=35)
+    n, t, redos, Qo = oracle_run(c, Q0)
+    assert n == 35 and redos > 0
+    z = ZoomyAmr(deck_from_case(c), system="swe2d", dim=2, order=2, contract=False)
+    z.init_data()
+    z.set_state(Q0)
+    assert z.run_steps(20)[0] == 20
+    done, tg = z.run_steps(15)
+    assert done == 15 and tg == t
+    assert z.mood_stats()[1] == redos
+    assert np.array_equal(z.get_state(), Qo)
+
+
+def test_stage_between_graph_runs_matches_oracle():
+    """run_steps (a captured chunk of 16), a single stage (which moves the
+    current buffer), run_steps again: the second run starts from the stage's
+    output, as the oracle's sequence of the same calls does."""
+    c, Q0 = column(steps=-1, positivity="none")
+    c = dict(c, cfl=0.45, clamp_positivity=True)
+    dt = 1.0e-3
+    o = Oracle("swe2d", c)
+    o.init_solution()
+    o.set_state(Q0)
+    o.evolve(step_limit=16)
+    o.stage(2, 0.0, dt)
+    o.evolve(step_limit=16)
+    z = ZoomyAmr(deck_from_case(c), system="swe2d", dim=2, order=2, contract=False)
+    z.init_data()
+    z.set_state(Q0)
+    assert z.run_steps(16)[0] == 16
+    z.stage(2, 0.0, dt)
+    assert z.run_steps(16)[0] == 16
+    assert np.array_equal(z.get_state(), o.get_state())
+
+
+def test_mood_box_list_groups_finish_apart():
+    """MOOD box list with a finite time_end: the two members take different
+    dt and finish on different steps; a finished member's state is carried
+    through the buffer rotation, so each equals its own oracle run."""
+    c, Q0 = column(n=64, steps=10 ** 6)
+    c = dict(c, time_end=0.02)
+    quiet = np.stack([np.zeros((64, 64)), np.full((64, 64), 1.0), np.zeros((64, 64)), np.zeros((64, 64))])
+    quiet[1, :, :32] = 1.5
+    boxes = [dict(lo=(0, 0), hi=(63, 63), dx=(1 / 64, 1 / 64), prob_lo=(0.0, 0.0), group=0),
+             dict(lo=(0, 0), hi=(63, 63), dx=(1 / 64, 1 / 64), prob_lo=(0.0, 0.0), group=1)]
+    keys = {k: c[k] for k in ("spatial_order", "implicit_source", "well_balanced", "clamp_positivity", "cfl",
+                              "dtmin", "dtmax", "time_end", "max_step")}
+    keys["positivity"] = "mood"
+    z = from_boxes(boxes, [Q0, quiet], keys, contract=False)
+    z.run_steps(10 ** 5)  # returns once no group is active
+    steps = []
+    for k, Qk in enumerate((Q0, quiet)):
+        o = Oracle("swe2d", dict(c, max_grid_size=4096))
+        o.init_solution()
+        o.set_state(Qk)
+        n, t, _, _ = o.evolve()
+        steps.append(n)
+        assert z.get_time(k)[1] == n and z.get_time(k)[0] == t
+        assert np.array_equal(z.get_state(lev=k), o.get_state()), f"member {k}"
+    assert steps[0] != steps[1]

@@ -9,7 +9,8 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import FIXTURES_1D, case_1d, case_2d_pulse, fixture, rel_linf, rel_linf_rows, smooth_state
+from helpers import (FIXTURES_1D, bed_step, bed_step_case, bed_step_expected, case_1d, case_2d_pulse, fixture,
+                     rel_linf, rel_linf_rows, smooth_state)
 from oracle.pyoracle import Oracle
 from zamr import cases
 from zamr.deck import deck_from_case
@@ -289,3 +290,31 @@ def test_bad_tag_is_an_error():
     with pytest.raises(ZamrError) as e:
         ZoomyAmr(deck_from_case(c), system="swe2d", dim=2, order=2)
     assert "Nowhere" in str(e.value)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_wb_bed_step_moving_water(order, axis):
+    """Well-balanced stage over a bed step with moving water (flat surface,
+    one velocity): the momentum rescale of the hydrostatic reconstruction
+    makes the step face's two states equal, so only the cells below the step
+    change, by a closed form (tests/helpers.py: bed_step); the bitwise build
+    equals the oracle, the FMA build the closed form to 1e-14."""
+    say("bed step", order, axis)
+    nx, ny = (70, 12) if axis == 0 else (12, 70)
+    k = 33
+    Q, rate = bed_step(nx=nx, ny=ny, axis=axis, k=k)
+    c = dict(bed_step_case(nx, ny, order, 2), max_grid_size=24)
+    dt = 0.01
+    E = bed_step_expected(Q, rate, dt, axis, k)
+    o = orc("swe2d", c, Q)
+    o.update_state(0.0)
+    o.stage(order, 0.0, dt)
+    ref = o.get_state()
+    assert np.array_equal(ref[:2], E[:2]) and np.abs(ref - E).max() <= 1e-14
+    zb = gpu("swe2d", c, Q, contract=False)
+    zb.stage(order, 0.0, dt)
+    assert np.array_equal(zb.get_state(), ref)
+    z = gpu("swe2d", c, Q)
+    z.stage(order, 0.0, dt)
+    assert np.abs(z.get_state() - E).max() <= 1e-14

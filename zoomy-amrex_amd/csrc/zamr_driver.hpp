@@ -196,8 +196,13 @@ class Solver {
     };
     Items items_[3];
 
+    // A captured chunk of steps names its state buffers by index, so it is
+    // valid only while a replay starts from the buffer the capture started
+    // from (graph_cur); replaying it leaves cur there and last_in as the
+    // capture left it (graph_last_in).
     hipGraphExec_t graph_exec = nullptr;
     int graph_key = -1;
+    int graph_cur = -1, graph_last_in = -1;
 
     bool profiling = false;
     std::vector<hipEvent_t> ev;
@@ -659,6 +664,7 @@ class Solver {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         graph_exec = nullptr;
         graph_key = -1;
+        graph_cur = graph_last_in = -1;
     }
 
     // ---------------- state I/O ----------------
@@ -1034,9 +1040,12 @@ class Solver {
         while (k < nsteps) {
             const int n = std::min(chunk, nsteps - k);
             if (graph_ok() && n == chunk) {
-                if (!graph_exec || graph_key != chunk) {
+                // eager steps of a three-buffer rotation, a single stage or an
+                // Advance move cur; a graph captured from another buffer is stale
+                if (!graph_exec || graph_key != chunk || graph_cur != cur) {
                     invalidate_graph();
                     items(hx_of(spatial_order));
+                    const int cur0 = cur;
                     hipGraph_t g;
                     ZAMR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                     for (int s = 0; s < chunk; ++s) {
@@ -1047,9 +1056,14 @@ class Solver {
                     ZAMR_HIP(hipStreamEndCapture(stream, &g));
                     ZAMR_HIP(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
                     hipGraphDestroy(g);
+                    if (cur != cur0) throw Error(4, "internal: a captured chunk must return to its first buffer");
                     graph_key = chunk;
+                    graph_cur = cur0;
+                    graph_last_in = last_in;
                 }
                 ZAMR_HIP(hipGraphLaunch(graph_exec, stream));
+                cur = graph_cur;
+                last_in = graph_last_in;
             } else {
                 for (int s = 0; s < n; ++s) {
                     if (profiling) prof_slot = k + s;

@@ -218,6 +218,10 @@ struct Sys {
     static_assert(M::zamr_aux_is_local,
                   "this build evaluates Qaux in registers; the model must declare zamr_aux_is_local");
     static_assert(np <= kMaxParams, "too many model parameters");
+    static_assert(!M::has_diffusion || M::zamr_n_aux_in == 0,
+                  "a diffusive model averages both cells' aux at a face; stored aux input rows are not "
+                  "read there, so such a model is not supported");
+    static_assert(!M::has_diffusion || M::n_dof_gradQ == nq * dim, "gradQ holds n_dof_q x dimension entries");
 
     __device__ static __forceinline__ VX center(const BoxView& B, int i, int j) {
         VX X{};
@@ -407,6 +411,36 @@ struct Sys {
         nh(DIR, 0) = 1.0;
         flux = N::numerical_flux(qL, qR, aL, aR, p, nh);
         fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
+    }
+
+    // The face-normal diffusive flux of add_diffusive_flux (make_rhs.H:184-208)
+    // through the face between cells qL and qR along DIR: Model::diffusive_flux
+    // at the face average of the two cells' states and aux, with the central
+    // difference (qR - qL) / dx as the gradient's only entry, projected on the
+    // face normal.  Cell values, not the reconstructed face states.
+    template <int DIR>
+    __device__ static __forceinline__ VQ diffusive_face(const VQ& qL, const VQ& qR, const VX& XL, const VX& XR,
+                                                       double dxf, double rdxf, const VP& p, double time) {
+        VQ out;
+        if constexpr (M::has_diffusion) {
+            const VA aL = aux_of(qL, p, time, XL), aR = aux_of(qR, p, time, XR);
+            VQ qf;
+            VA af;
+#pragma unroll
+            for (int v = 0; v < nq; ++v) qf(v, 0) = 0.5 * (qL(v, 0) + qR(v, 0));
+#pragma unroll
+            for (int v = 0; v < na; ++v) af(v, 0) = 0.5 * (aL(v, 0) + aR(v, 0));
+            amrex::SmallMatrix<double, M::n_dof_gradQ, 1> g{};
+#pragma unroll
+            for (int v = 0; v < nq; ++v)
+                g(v * dim + DIR, 0) = kBitwise ? (qR(v, 0) - qL(v, 0)) / dxf : (qR(v, 0) - qL(v, 0)) * rdxf;
+            const auto Fd = M::diffusive_flux(qf, af, g, p);
+#pragma unroll
+            for (int v = 0; v < nq; ++v) out(v, 0) = Fd(v * dim + DIR, 0);
+        } else {
+            vq_zero(out);
+        }
+        return out;
     }
 
     // apply_implicit_source (make_rhs.H:429-509): one linearised backward-Euler
@@ -610,17 +644,23 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         // cache lines)
         const int cl = min(c, B.hi0 + HX);
         if (!G->active) {
-            // An order-1 step ping-pongs buffers; a finished group carries its
-            // state across so that every group's current buffer stays in step.
-            if constexpr (EPI == 2) {
-                if (own) {
-                    const double* __restrict__ qs = qsel(B, a.src);
-                    double* __restrict__ qd = qsel(B, a.dst);
-                    for (int j = w.j0; j < w.j0 + w.nj; ++j) {
-                        const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+            // A step moves the current state to another buffer (order 1:
+            // src -> dst; order 2 with three buffers: Q^n in `old` -> dst); a
+            // finished group carries its state across so that every group's
+            // current buffer stays in step.  (Two-buffer order 2 writes back
+            // into `old` itself; the MOOD redo writes the buffer stage 2 just
+            // carried into.)
+            int from = -1;
+            if constexpr (EPI == 2) from = a.src;
+            if constexpr (EPI == 1)
+                if (a.mood != 2 && !a.dstp && a.dst != a.old) from = a.old;
+            if (from >= 0 && own) {
+                const double* __restrict__ qs = qsel(B, from);
+                double* __restrict__ qd = qsel(B, a.dst);
+                for (int j = w.j0; j < w.j0 + w.nj; ++j) {
+                    const long long idx = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
 #pragma unroll
-                        for (int n = 0; n < nq; ++n) qd[idx + n * B.sc] = qs[idx + n * B.sc];
-                    }
+                    for (int n = 0; n < nq; ++n) qd[idx + n * B.sc] = qs[idx + n * B.sc];
                 }
             }
             continue;
@@ -656,6 +696,8 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         VQ fyB;
         VF dyB;
         double hyB = 0.0;
+        VQ dfyB;  // diffusive flux through the bottom face (diffusive models)
+        vq_zero(dfyB);
         // the first leading row and the prologue's rows are fetched together
         // and then finished: finish() branches (domain edges), and a load is
         // not moved across a branch, so loading row by row would wait once
@@ -697,6 +739,9 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 double hl;
                 S::template face_states<1, N>(qL, qR, S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
                                               a.wb, fyB, dyB, hl, hyB);
+                if constexpr (M::has_diffusion)
+                    dfyB = S::template diffusive_face<1>(P[HX - 1], P[HX], S::center(B, c, jstart - 1),
+                                                         S::center(B, c, jstart), B.dx1, rdy, p, time);
             }
             W0 = P[HX];
             if constexpr (ORDER == 2) W1 = P[HX + 1];
@@ -774,11 +819,27 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 Fx[n] = -(fR - fxL(n, 0)) - dmR - dxL(n, 0);  // face_divergence, make_rhs.H:142-147
             }
             const double hL_xr = from_right(hl);
+            // add_diffusive_flux's x part, (Fd_xr - Fd_xl) per row: the left
+            // face by this lane, the right one by the lane to the right
+            double Dx[nq];
+            if constexpr (M::has_diffusion) {
+                VQ qml;
+#pragma unroll
+                for (int n = 0; n < nq; ++n) qml(n, 0) = from_left(qc(n, 0));
+                VQ dfx;
+                if (xf) dfx = S::template diffusive_face<0>(qml, qc, S::center(B, c - 1, j), S::center(B, c, j), B.dx0,
+                                                            rdx, p, time);
+                else vq_zero(dfx);
+#pragma unroll
+                for (int n = 0; n < nq; ++n) Dx[n] = from_right(dfx(n, 0)) - dfx(n, 0);
+            }
             // top y-face j+1/2
             VQ fyT, sy1;
             VF dyT;
             double hyT = 0.0, hyTnext = 0.0;
             vq_zero(sy1);
+            VQ dfyT;
+            vq_zero(dfyT);
             if constexpr (dim == 2) {
                 if (own) {
                     VQ qL, qR;
@@ -802,6 +863,14 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                     S::template face_states<1, N>(qL, qR, S::center(B, c, j), S::center(B, c, j + 1), p, time, a.wb,
                                                   fyT, dyT, hyT, hyTnext);
+                    if constexpr (M::has_diffusion) {
+                        if constexpr (ORDER == 2)
+                            dfyT = S::template diffusive_face<1>(W0, W1, S::center(B, c, j), S::center(B, c, j + 1),
+                                                                 B.dx1, rdy, p, time);
+                        else
+                            dfyT = S::template diffusive_face<1>(W0, lead, S::center(B, c, j), S::center(B, c, j + 1),
+                                                                 B.dx1, rdy, p, time);
+                    }
                 }
             }
             // The prefetch is landed before this row's stores: stage 1 lands it
@@ -830,6 +899,11 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                             if constexpr (dim == 2)
                                 if (n == 3) rhs += (hyT * hyT - hyB * hyB) * hgy;
                         }
+                    }
+                    if constexpr (M::has_diffusion) {  // make_rhs.H:214-227, after the bed source
+                        rhs += kBitwise ? Dx[n] / B.dx0 : Dx[n] * rdx;
+                        if constexpr (dim == 2)
+                            rhs += kBitwise ? (dfyT(n, 0) - dfyB(n, 0)) / B.dx1 : (dfyT(n, 0) - dfyB(n, 0)) * rdy;
                     }
                     qn(n, 0) = qc(n, 0) + dt * rhs;
                 }
@@ -870,6 +944,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 fyB = fyT;
                 dyB = dyT;
                 hyB = hyTnext;
+                if constexpr (M::has_diffusion) dfyB = dfyT;
                 if constexpr (ORDER == 2) {
                     sy0 = sy1;
                     W0 = W1;

@@ -28,10 +28,12 @@ inline bool make_dir(const std::string& d) {
     return errno == EEXIST;
 }
 
-// Concatenate(Concatenate("plt_", identifier), step, 5).
+// Concatenate(Concatenate("plt_", identifier), step, 5) (ZoomyAmr.cpp:824):
+// Concatenate pads its number to at least 5 digits, so both numbers are
+// padded (identifier 0, step 12 -> plt_0000000012).
 inline std::string plotfile_name(int identifier, int step) {
     std::ostringstream s;
-    s << "plt_" << identifier << std::setw(5) << std::setfill('0') << step;
+    s << "plt_" << std::setw(5) << std::setfill('0') << identifier << std::setw(5) << std::setfill('0') << step;
     return s.str();
 }
 

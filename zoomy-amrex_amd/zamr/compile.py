@@ -19,6 +19,7 @@ REPO = PKG.parent
 CSRC = PKG / "csrc"
 COMPAT = PKG / "compat"
 SYSTEMS = PKG / "systems"
+TEST_SYSTEMS = REPO / "tests" / "systems"  # test-only headers (pathological sources for error paths)
 INCLUDE = REPO / "include"
 LIBDIR = PKG / "lib"
 OBJDIR = PKG / "_build"
@@ -27,7 +28,11 @@ ARCH = os.environ.get("ZAMR_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # Shipped variants: (system, mesh dimension, spatial order).
-VARIANTS = [("swe1d", 1, 1), ("swe2d", 2, 1), ("swe2d", 2, 2), ("swe2d_nf", 2, 2), ("sme2d", 2, 2)]
+VARIANTS = [("swe1d", 1, 1), ("swe2d", 2, 1), ("swe2d", 2, 2), ("swe2d_nf", 2, 2), ("sme2d", 2, 2),
+            ("swe2d_visc", 2, 1), ("swe2d_visc", 2, 2)]
+# Test systems (tests/systems/<name>): built beside the shipped ones, with
+# their FMA-off twins, for the tests that drive a model's error paths.
+TEST_VARIANTS = [("swe2d_fallback", 2, 2)]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -97,19 +102,27 @@ def main_path(system, dim, order, contract=True):
     return LIBDIR / f"main_{system}_d{dim}o{order}{'' if contract else '_nofma'}.ex"
 
 
-def build_all(jobs=4, with_nofma=True):
-    """Build every shipped variant (and the FMA-off twin of the order-2 2-D
-    system used by the bitwise parity tests)."""
+def build_all(jobs=6, with_nofma=True):
+    """Build every shipped variant (and the FMA-off twins used by the bitwise
+    parity tests), then the test systems."""
     by_sys = {}
     for s, d, o in VARIANTS:
         by_sys.setdefault(s, []).append((d, o))
-    tasks = [(s, v, True) for s, v in by_sys.items()]
+    tasks = [(SYSTEMS / s, s, v, True) for s, v in by_sys.items()]
     if with_nofma:
-        tasks.append(("swe2d", [(2, 2), (2, 1)], False))
-        tasks.append(("swe2d_nf", [(2, 2)], False))
-        tasks.append(("sme2d", [(2, 2)], False))
+        tasks.append((SYSTEMS / "swe2d", "swe2d", [(2, 2), (2, 1)], False))
+        tasks.append((SYSTEMS / "swe2d_nf", "swe2d_nf", [(2, 2)], False))
+        tasks.append((SYSTEMS / "sme2d", "sme2d", [(2, 2)], False))
+        tasks.append((SYSTEMS / "swe2d_visc", "swe2d_visc", [(2, 2), (2, 1)], False))
+    tests = {}
+    for s, d, o in TEST_VARIANTS:
+        tests.setdefault(s, []).append((d, o))
+    for s, v in tests.items():
+        tasks.append((TEST_SYSTEMS / s, s, v, True))
+        if with_nofma:
+            tasks.append((TEST_SYSTEMS / s, s, v, False))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(compile_system, SYSTEMS / s, s, v, c) for s, v, c in tasks]
+        futs = [ex.submit(compile_system, d, s, v, c) for d, s, v, c in tasks]
         return [p for f in futs for p in f.result()]
 
 

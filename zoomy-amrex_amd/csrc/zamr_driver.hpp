@@ -583,7 +583,7 @@ class Solver {
     int stage_waves_per_simd(int hx) const {
         if (!occ_cache[hx]) {
             int blocks = 0;
-            const void* k = hx == 2 ? (const void*)stage_kernel<M, N, 2, 1> : (const void*)stage_kernel<M, N, 1, 1>;
+            const void* k = hx == 2 ? (const void*)stage_kernel<M, N, 2, 1, 0, 1> : (const void*)stage_kernel<M, N, 1, 1, 0, 1>;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, NT, 0) != hipSuccess || blocks < 1) blocks = 2;
             occ_cache[hx] = blocks;  // a block is one wave on each of a CU's 4 SIMDs
         }
@@ -844,7 +844,7 @@ class Solver {
     template <int ORDER, int EPI>
     void launch_stage(const StageArgs& a) {
         Items& it = items(hx_of(ORDER));
-        auto kern = stage_kernel<M, N, ORDER, EPI>;
+        auto kern = a.wb ? stage_kernel<M, N, ORDER, EPI, 0, 1> : stage_kernel<M, N, ORDER, EPI, 0, 0>;
         if (per_box_launch) {
             for (size_t b = 0; b + 1 < it.box_begin.size(); ++b) {
                 const int n0 = it.box_begin[b], n = it.box_begin[b + 1] - n0;

@@ -190,11 +190,30 @@ __device__ __forceinline__ double minmod(double a, double b) {
     return (fabs(a) < fabs(b)) ? a : b;
 }
 
+// minmod for the throughput build: max(min(a, b), 0) + min(max(a, b), 0)
+// picks the argument of smaller magnitude when both have one sign (either,
+// when they are equal) and zero otherwise: minmod's value for finite
+// arguments (up to the sign of a zero result), in five FP64 operations and
+// no selects.
+__device__ __forceinline__ double minmod_minmax(double a, double b) {
+    return fmax(fmin(a, b), 0.0) + fmin(fmax(a, b), 0.0);
+}
+
 // Which argument minmod returns: 0 none (zero), 1 the first, 2 the second.
 __device__ __forceinline__ unsigned minmod_code(double a, double b) {
     if (a * b <= 0.0) return 0u;
     return (fabs(a) < fabs(b)) ? 1u : 2u;
 }
+
+// Riemann solvers whose numerical_fluctuations are proportional to the bed
+// jump qR(0) - qL(0) (N::zamr_fluct_from_bed_jump): after the hydrostatic
+// reconstruction both sides carry the face bed, so a well-balanced face has
+// none (exactly zero for finite states).
+template <class N, class = void>
+struct FluctFromBedJump : std::false_type {};
+template <class N>
+struct FluctFromBedJump<N, std::void_t<decltype(N::zamr_fluct_from_bed_jump)>>
+    : std::bool_constant<N::zamr_fluct_from_bed_jump> {};
 
 // Models whose update_aux_variables(Q, 0) is zero but for hinv of the depth
 // (M::zamr_aux_only_hinv, with the hinv row and the wet-dry epsilon's index).
@@ -343,6 +362,7 @@ struct Sys {
             return (k == 1u) ? a : ((k == 2u) ? b : 0.0);
         } else {
             if constexpr (SW == 1) *code |= minmod_code(a, b) << (2 * n);
+            if constexpr (SW == 0 && !kBitwise) return minmod_minmax(a, b);
             return minmod(a, b);
         }
     }
@@ -410,7 +430,8 @@ struct Sys {
         VN nh{};
         nh(DIR, 0) = 1.0;
         flux = N::numerical_flux(qL, qR, aL, aR, p, nh);
-        fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
+        if (FluctFromBedJump<N>::value && wb) vq_zero(fluct);
+        else fluct = N::numerical_fluctuations(qL, qR, aL, aR, p, nh);
     }
 
     // The face-normal diffusive flux of add_diffusive_flux (make_rhs.H:184-208)
@@ -443,6 +464,106 @@ struct Sys {
         return out;
     }
 
+    // The throughput build's solve of apply_implicit_source's system
+    // (I - dt J) dq = dt S.  A row of I - dt J whose entries are all known at
+    // compile time to be those of the identity, with a right-hand side known
+    // to be zero, has dq = 0 (the printed models' source-free rows: bed and
+    // depth for friction); the remaining rows solve D dq_D = rhs_D over the
+    // remaining columns, D being the matrix without the identity rows and
+    // their columns (the identity columns multiply dq = 0).  D is eliminated
+    // as make_rhs.H:467-499 eliminates the whole matrix (partial pivoting,
+    // strict '>' search, failure when the pivot is not > 0 or not finite);
+    // each pivot's reciprocal is kept for the back substitution.  The whole
+    // matrix is singular exactly when D is, and a non-finite entry anywhere
+    // fails the full elimination, so the outcome is the reference's up to
+    // rounding: on failure the explicit update q + dt S, or q_old where that
+    // is not finite (make_rhs.H:501-508).
+    __device__ static __forceinline__ VQ solve_blocks(const VQ& q_old, const VQ& S, double dt, double (&A)[nq][nq],
+                                                      double (&rhs)[nq], const bool (&lit)[nq][nq]) {
+        bool triv[nq];
+#pragma unroll
+        for (int r = 0; r < nq; ++r) {
+            bool t = zero_literal(rhs[r]);
+#pragma unroll
+            for (int c = 0; c < nq; ++c) t = t && lit[r][c];
+            triv[r] = t;
+        }
+        bool ok = true;
+#pragma unroll
+        for (int r = 0; r < nq; ++r)
+#pragma unroll
+            for (int c = 0; c < nq; ++c)
+                if (!triv[r] && triv[c] && !lit[r][c]) ok = ok && isfinite(A[r][c]);
+        double inv[nq];
+#pragma unroll
+        for (int k = 0; k < nq; ++k) {
+            if (triv[k]) continue;
+            int piv = k;
+            double best = fabs(A[k][k]);
+#pragma unroll
+            for (int r = k + 1; r < nq; ++r)
+                if (!triv[r] && fabs(A[r][k]) > best) {
+                    best = fabs(A[r][k]);
+                    piv = r;
+                }
+            ok = ok && (best > 0.0) && isfinite(best);
+            // row swap by predication (registers cannot be indexed), skipped as
+            // a whole when no lane of the wave pivots
+            if (__any(piv != k)) {
+#pragma unroll
+                for (int r = k + 1; r < nq; ++r) {
+                    if (!triv[r] && r == piv) {
+#pragma unroll
+                        for (int c = k; c < nq; ++c) {
+                            if (triv[c]) continue;
+                            const double t = A[k][c];
+                            A[k][c] = A[r][c];
+                            A[r][c] = t;
+                        }
+                        const double t = rhs[k];
+                        rhs[k] = rhs[r];
+                        rhs[r] = t;
+                    }
+                }
+            }
+            inv[k] = recip(A[k][k]);
+#pragma unroll
+            for (int r = k + 1; r < nq; ++r) {
+                if (triv[r]) continue;
+                const double f = A[r][k] * inv[k];
+#pragma unroll
+                for (int c = k + 1; c < nq; ++c)
+                    if (!triv[c]) A[r][c] -= f * A[k][c];
+                rhs[r] -= f * rhs[k];
+            }
+        }
+        double dq[nq];
+#pragma unroll
+        for (int r = nq - 1; r >= 0; --r) {
+            if (triv[r]) {
+                dq[r] = 0.0;
+                continue;
+            }
+            double sum = 0.0;
+#pragma unroll
+            for (int c = r + 1; c < nq; ++c)
+                if (!triv[c]) sum += A[r][c] * dq[c];
+            dq[r] = (rhs[r] - sum) * inv[r];
+            ok = ok && isfinite(dq[r]);
+        }
+        VQ out;
+#pragma unroll
+        for (int n = 0; n < nq; ++n) {
+            if (ok) {
+                out(n, 0) = triv[n] ? q_old(n, 0) : q_old(n, 0) + dq[n];
+            } else {
+                const double v = q_old(n, 0) + dt * S(n, 0);
+                out(n, 0) = isfinite(v) ? v : q_old(n, 0);
+            }
+        }
+        return out;
+    }
+
     // apply_implicit_source (make_rhs.H:429-509): one linearised backward-Euler
     // step with Gaussian elimination and partial pivoting; explicit fallback.
     __device__ static __forceinline__ VQ implicit_source(const VQ& q_old, const VA& a_in, double dt, const VP& p,
@@ -453,6 +574,7 @@ struct Sys {
         const auto dA_dQ = M::update_aux_variables_jacobian_wrt_variables(q_old, a_in, p, time, X);
         double A[nq][nq];
         double rhs[nq];
+        bool lit[nq][nq];  // entry of I - dt J known at compile time (a literal-zero Jacobian entry)
 #pragma unroll
         for (int r = 0; r < nq; ++r) {
             rhs[r] = zero_literal(S(r, 0)) ? 0.0 : dt * S(r, 0);
@@ -470,9 +592,11 @@ struct Sys {
                 for (int k = 0; k < na; ++k)
                     if (!zero_literal(dA_dQ(k * nq + c, 0))) chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
                 const double jc = dS_dQ(r * nq + c, 0) + chain;
-                A[r][c] = zero_literal(jc) ? (r == c ? 1.0 : 0.0) : (r == c ? 1.0 : 0.0) - dt * jc;
+                lit[r][c] = zero_literal(jc);
+                A[r][c] = lit[r][c] ? (r == c ? 1.0 : 0.0) : (r == c ? 1.0 : 0.0) - dt * jc;
             }
         }
+        if constexpr (!kBitwise) return solve_blocks(q_old, S, dt, A, rhs, lit);
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < nq; ++k) {
@@ -604,7 +728,9 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 // as bytes: the four 2-bit codes of the cell's x slope at byte idx and of its
 // y slope at byte sc + idx (idx = the cell's index in the box layout, sc the
 // component stride); 2 = replay them (single box, for derivative checks).
-template <class M, class N, int ORDER, int EPI, int SW = 0>
+// WB: 0 / 1 = well-balancing compiled off / on (the driver picks the variant
+// from solver.well_balanced); 2 = decided at run time by a.wb.
+template <class M, class N, int ORDER, int EPI, int SW = 0, int WB = 2>
 __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc* __restrict__ boxes,
                                                    const WorkItem* __restrict__ items, int nitems,
                                                    StageArgs a, GroupState* __restrict__ gs, int xcd_remap) {
@@ -623,6 +749,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
     VP p;
 #pragma unroll
     for (int k = 0; k < S::np; ++k) p(k, 0) = a.p.v[k];
+    const int wb = (WB == 2) ? a.wb : WB;
 
     const int t = threadIdx.x & (WV - 1);
     const int wv = wave_in_block();
@@ -738,7 +865,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                 }
                 double hl;
                 S::template face_states<1, N>(qL, qR, S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
-                                              a.wb, fyB, dyB, hl, hyB);
+                                              wb, fyB, dyB, hl, hyB);
                 if constexpr (M::has_diffusion)
                     dfyB = S::template diffusive_face<1>(P[HX - 1], P[HX], S::center(B, c, jstart - 1),
                                                          S::center(B, c, jstart), B.dx1, rdy, p, time);
@@ -802,7 +929,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             VF dxL;
             double hxL = 0.0, hl = 0.0;
             if (xf) {
-                S::template face_states<0, N>(qLx, eL, S::center(B, c - 1, j), S::center(B, c, j), p, time, a.wb,
+                S::template face_states<0, N>(qLx, eL, S::center(B, c - 1, j), S::center(B, c, j), p, time, wb,
                                               fxL, dxL, hl, hxL);
             } else {
                 vq_zero(fxL);
@@ -861,7 +988,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                         qL = W0;
                         qR = lead;
                     }
-                    S::template face_states<1, N>(qL, qR, S::center(B, c, j), S::center(B, c, j + 1), p, time, a.wb,
+                    S::template face_states<1, N>(qL, qR, S::center(B, c, j), S::center(B, c, j + 1), p, time, wb,
                                                   fyT, dyT, hyT, hyTnext);
                     if constexpr (M::has_diffusion) {
                         if constexpr (ORDER == 2)
@@ -889,7 +1016,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
                     double rhs = kBitwise ? Fx[n] / B.dx0 + Fy / B.dx1 : Fx[n] * rdx + Fy * rdy;
                     if (!a.impl) rhs += src(n, 0);
-                    if (a.wb) {
+                    if (wb) {
                         if constexpr (kBitwise) {
                             if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
                             if constexpr (dim == 2)

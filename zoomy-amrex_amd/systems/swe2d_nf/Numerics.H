@@ -17,6 +17,9 @@
 
 struct Numerics {
     static constexpr bool fluctuations_are_zero = false;
+    // the fluctuations are proportional to the bed jump qR(0) - qL(0), so a
+    // well-balanced face (equal reconstructed beds) has none
+    static constexpr bool zamr_fluct_from_bed_jump = true;
 
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::SmallMatrix<amrex::Real, 1, 1> local_max_abs_eigenvalue(
         Model::VQ const& Q, Model::VA const& Qaux, Model::VP const& p, Model::VN const& n)

@@ -61,6 +61,20 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double recip(double x) {
 #endif
 }
 
+// Projections on a face or boundary normal.  Every normal the driver hands
+// the model (make_rhs.H's n_hat, the boundary normals of FillPhysicalBC) is a
+// unit axis vector, so a component that is zero contributes nothing: it is
+// skipped rather than multiplied, and where the normal is known at compile
+// time (the stage kernels) no multiplication by zero is left.  For finite
+// operands the value is a * nx + b * ny (up to the sign of a zero).
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double axis_dot(double a, double b, double nx, double ny) {
+    if (ny == 0.0) return a * nx;
+    if (nx == 0.0) return b * ny;
+    return a * nx + b * ny;
+}
+// x * n_d, zero where the component n_d is zero.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double axis_scale(double x, double nd) { return nd == 0.0 ? 0.0 : x * nd; }
+
 // Keep the device compiler from moving work across this point (an empty asm
 // statement that "redefines" x): used to stop it from interleaving two large
 // independent evaluations, which costs registers a kernel at one wave per

@@ -713,16 +713,23 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 // The fused stage.  EPI: 0 = plain stage (writes dst), 1 = stage + Heun with
 // buffer `old` + end-of-step UpdateState + CFL max (order 2, second stage),
 // 2 = stage + end-of-step UpdateState + CFL max (order 1 step).
-// Occupancy request for the stage kernel (waves per SIMD); 0 leaves the
-// register budget to the compiler.
+// Occupancy request for the stage kernel (waves per SIMD).  A wave needs
+// its VGPRs for three waves to share a SIMD's 512: at most 168.  The 4-row
+// order-2 stage without Heun fits (154); the one with Heun, the end-of-step
+// update and the CFL max asks for ~200 and runs at two waves unless asked
+// for three, when the compiler spills ~20 registers to scratch (in the
+// per-item prologue, not the row loop) -- measured at 4096^2 the third wave
+// wins (0.51 -> 0.46 ms), so three are requested for it.  Rows beyond four (shallow moments) and the diffusive or recording
+// variants keep what the compiler chooses.  ZAMR_STAGE_WAVES > 0 overrides
+// the rule for every stage kernel.
 #ifndef ZAMR_STAGE_WAVES
 #define ZAMR_STAGE_WAVES 0
 #endif
-#if ZAMR_STAGE_WAVES > 0
-#define ZAMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(ZAMR_STAGE_WAVES, ZAMR_STAGE_WAVES)))
-#else
-#define ZAMR_STAGE_ATTR
-#endif
+template <class M, int ORDER, int EPI, int SW>
+constexpr int stage_min_waves() {
+    if (ZAMR_STAGE_WAVES > 0) return ZAMR_STAGE_WAVES;
+    return (M::n_dof_q <= 4 && M::dimension == 2 && !M::has_diffusion && ORDER == 2 && SW == 0) ? 3 : 1;
+}
 
 // SW: 0 = plain; 1 = record every cell's limiter choices into a.sw, read
 // as bytes: the four 2-bit codes of the cell's x slope at byte idx and of its
@@ -731,7 +738,7 @@ __host__ __device__ __forceinline__ int slots_for(int nitems) { return (((nitems
 // WB: 0 / 1 = well-balancing compiled off / on (the driver picks the variant
 // from solver.well_balanced); 2 = decided at run time by a.wb.
 template <class M, class N, int ORDER, int EPI, int SW = 0, int WB = 2>
-__global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc* __restrict__ boxes,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_min_waves<M, ORDER, EPI, SW>()))) stage_kernel(const BoxDesc* __restrict__ boxes,
                                                    const WorkItem* __restrict__ items, int nitems,
                                                    StageArgs a, GroupState* __restrict__ gs, int xcd_remap) {
     using S = Sys<M>;
@@ -814,64 +821,56 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         const int jstart = w.j0;
         const int jend = w.j0 + w.nj;
         // y state: W0 = row j, W1 = row j+1 (order 2), sy0 = limited y slope
-        // of row j, and the bottom face j-1/2 of row j (evaluated one row
-        // earlier, or here for the first row from rows jstart-HX .. jstart+HX-1)
+        // of row j, and the bottom face j-1/2 of row j, evaluated one row
+        // earlier.  In two dimensions the loop starts one row below the item
+        // (j0 = jstart - 1) and that row only evaluates its top face, the
+        // first row's bottom face: every face is evaluated by the one code
+        // path of the loop body, so its value cannot depend on where item
+        // boundaries fall (with contracted FMAs, a second copy of the face
+        // code could round differently).
         VQ W0, W1, sy0;
         vq_zero(W0);
         vq_zero(W1);
         vq_zero(sy0);
         VQ fyB;
         VF dyB;
+        vq_zero(fyB);
+#pragma unroll
+        for (int n = 0; n < 2 * nq; ++n) dyB(n, 0) = 0.0;
         double hyB = 0.0;
         VQ dfyB;  // diffusive flux through the bottom face (diffusive models)
         vq_zero(dfyB);
+        const int j0 = (dim == 2) ? jstart - 1 : jstart;
         // the first leading row and the prologue's rows are fetched together
         // and then finished: finish() branches (domain edges), and a load is
         // not moved across a branch, so loading row by row would wait once
         // per row
-        const int jlead0 = (dim == 2) ? jstart + HX : jstart;
+        const int jlead0 = (dim == 2) ? j0 + HX : jstart;
         typename S::Raw nxt = S::fetch(B, qs, cl, jlead0);
         if constexpr (dim == 2) {
-            typename S::Raw R[2 * HX];
+            constexpr int NP = 2 * HX - 1;  // rows j0 - HX + 1 .. j0 + HX - 1
+            typename S::Raw R[NP];
 #pragma unroll
-            for (int r = 0; r < 2 * HX; ++r) R[r] = S::fetch(B, qs, cl, jstart - HX + r);
-            VQ P[2 * HX];
+            for (int r = 0; r < NP; ++r) R[r] = S::fetch(B, qs, cl, j0 - HX + 1 + r);
+            VQ P[NP];
 #pragma unroll
-            for (int r = 0; r < 2 * HX; ++r) P[r] = S::finish(B, R[r], cl, jstart - HX + r, p, a.clamp, time);
-            if (own) {
-                VQ qL, qR;
-                if constexpr (ORDER == 2) {
-                    unsigned cm = 0u, c0 = 0u;
-                    const long long im = (long long)(c - B.gl0) + (long long)(jstart - 1 - B.gl1) * B.sy;
-                    const long long i0 = im + B.sy;
-                    if constexpr (SW == 2) {
-                        cm = swy[im];
-                        c0 = swy[i0];
-                    }
+            for (int r = 0; r < NP; ++r) P[r] = S::finish(B, R[r], cl, j0 - HX + 1 + r, p, a.clamp, time);
+            if constexpr (ORDER == 2) {
+                if (own) {  // the limited slope of row j0
+                    unsigned cm = 0u;
+                    const long long im = (long long)(c - B.gl0) + (long long)(j0 - B.gl1) * B.sy;
+                    if constexpr (SW == 2) cm = swy[im];
 #pragma unroll
-                    for (int n = 0; n < nq; ++n) {
-                        const double sm = S::template slope<SW>(P[0](n, 0), P[1](n, 0), P[2](n, 0), &cm, n);
-                        sy0(n, 0) = S::template slope<SW>(P[1](n, 0), P[2](n, 0), P[3](n, 0), &c0, n);
-                        qL(n, 0) = P[1](n, 0) + 0.5 * sm;
-                        qR(n, 0) = P[2](n, 0) - 0.5 * sy0(n, 0);
-                    }
-                    if constexpr (SW == 1) {
+                    for (int n = 0; n < nq; ++n)
+                        sy0(n, 0) = S::template slope<SW>(P[0](n, 0), P[1](n, 0), P[2](n, 0), &cm, n);
+                    if constexpr (SW == 1)
                         if (jstart == B.lo1) swy[im] = (unsigned char)cm;  // the ghost row below the box
-                        swy[i0] = (unsigned char)c0;
-                    }
-                } else {
-                    qL = P[0];
-                    qR = P[1];
                 }
-                double hl;
-                S::template face_states<1, N>(qL, qR, S::center(B, c, jstart - 1), S::center(B, c, jstart), p, time,
-                                              wb, fyB, dyB, hl, hyB);
-                if constexpr (M::has_diffusion)
-                    dfyB = S::template diffusive_face<1>(P[HX - 1], P[HX], S::center(B, c, jstart - 1),
-                                                         S::center(B, c, jstart), B.dx1, rdy, p, time);
+                W0 = P[1];
+                W1 = P[2];
+            } else {
+                W0 = P[0];
             }
-            W0 = P[HX];
-            if constexpr (ORDER == 2) W1 = P[HX + 1];
         }
         // the window and the first leading row enter the row loop landed
         land(W0);
@@ -879,23 +878,14 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
         land(nxt.v);
         double lam = 0.0;
         int nbad = 0;  // MOOD: troubled cells among this lane's candidate cells
-        for (int j = jstart; j < jend; ++j) {
+        for (int j = j0; j < jend; ++j) {
+            const bool cell = own && j >= jstart;  // this lane updates a cell of this row
             // The leading row, fetched one row ahead.  The fetch is
             // unconditional (past the grown box the last row is read again):
             // a conditional load keeps the old value alive beside the new one.
             const VQ lead = S::finish(B, nxt, cl, (dim == 2) ? j + HX : j, p, a.clamp, time);
             nxt = S::fetch(B, qs, cl, (dim == 2) ? min(j + HX + 1, B.hi1 + HX) : j);
-            // stage 2: Q^n of the row for the Heun combine, loaded here and used
-            // at the row's end (a load carried across rows would be copied at
-            // the loop edge, and that copy waits for the row's stores)
-            VQ qn_row;
-            if constexpr (EPI == 1) {
-                if (own) {
-                    const long long i1 = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
-#pragma unroll
-                    for (int n = 0; n < nq; ++n) qn_row(n, 0) = qo[i1 + n * B.sc];
-                }
-            }
+            VQ qn_row;  // stage 2: Q^n of the row for the Heun combine
             if constexpr (dim == 1) W0 = lead;
             const VQ& qc = W0;
             const long long isw = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
@@ -920,7 +910,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                 }
                 if constexpr (SW == 1)
-                    if (sx_rec) swx[isw] = (unsigned char)cx;
+                    if (sx_rec && j >= jstart) swx[isw] = (unsigned char)cx;
             }
             VQ qLx;
 #pragma unroll
@@ -928,7 +918,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
             VQ fxL;
             VF dxL;
             double hxL = 0.0, hl = 0.0;
-            if (xf) {
+            if (xf && j >= jstart) {
                 S::template face_states<0, N>(qLx, eL, S::center(B, c - 1, j), S::center(B, c, j), p, time, wb,
                                               fxL, dxL, hl, hxL);
             } else {
@@ -1000,12 +990,22 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                 }
             }
-            // The prefetch is landed before this row's stores: stage 1 lands it
-            // before the cell update (fewer live registers), the later stages
-            // just before the stores (more of the row's work covers the load).
-            if constexpr (EPI == 0) land(nxt.v);
+            // The prefetch is landed before the cell update, ahead of this row's
+            // stores (fewer live registers than landing it at the stores).
+            // Heun's Q^n is loaded here, its latency covered by the cell update
+            // and the source solve, and used at the row's end (a load carried
+            // across rows would be copied at the loop edge, and that copy waits
+            // for the row's stores).
+            land(nxt.v);
+            if constexpr (EPI == 1) {
+                if (cell) {
+                    const long long i1 = (long long)(c - B.gl0) + (long long)(j - B.gl1) * B.sy;
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) qn_row(n, 0) = qo[i1 + n * B.sc];
+                }
+            }
             VQ qn;
-            if (own) {
+            if (cell) {
                 const typename S::VX X = S::center(B, c, j);
                 const VA ac = S::aux_at(B, c, j, qc, p, time, X);
                 VQ src{};
@@ -1060,8 +1060,7 @@ __global__ void __launch_bounds__(NT) ZAMR_STAGE_ATTR stage_kernel(const BoxDesc
                     }
                 }
             }
-            if constexpr (EPI != 0) land(nxt.v);
-            if (own) {
+            if (cell) {
                 const long long idx = isw;
 #pragma unroll
                 for (int n = 0; n < nq; ++n) __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);

@@ -214,7 +214,9 @@ struct Model {
         const amrex::Real v = Q(3, 0) * Qaux(6, 0);
         const amrex::Real speed = zamr_math::sqrt(u * u + v * v);
         const amrex::Real r = zamr_rcbrt(amrex::max(Q(1, 0), p(5, 0)));
-        const amrex::Real k = p(0, 0) * Qaux(1, 0) * Qaux(1, 0) * speed * amrex::Math::powi<4>(r);
+        // k = c |u| with c = g n^2 r^4, as the Jacobians form it (one evaluation)
+        const amrex::Real c = p(0, 0) * Qaux(1, 0) * Qaux(1, 0) * amrex::Math::powi<4>(r);
+        const amrex::Real k = c * speed;
         res(0, 0) = 0.0;
         res(1, 0) = 0.0;
         res(2, 0) = -k * Q(2, 0);

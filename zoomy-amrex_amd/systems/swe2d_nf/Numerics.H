@@ -25,7 +25,7 @@ struct Numerics {
         Model::VQ const& Q, Model::VA const& Qaux, Model::VP const& p, Model::VN const& n)
     {
         amrex::SmallMatrix<amrex::Real, 1, 1> res{};
-        const amrex::Real un = (Q(2, 0) * n(0, 0) + Q(3, 0) * n(1, 0)) * Qaux(6, 0);
+        const amrex::Real un = zamr_math::axis_dot(Q(2, 0), Q(3, 0), n(0, 0), n(1, 0)) * Qaux(6, 0);
         res(0, 0) = std::fabs(un) + zamr_math::sqrt(p(0, 0) * amrex::max(Q(1, 0), amrex::Real(0.0)));
         return res;
     }
@@ -37,8 +37,8 @@ struct Numerics {
         Model::VQ res{};
         const amrex::Real g = p(0, 0);
         const amrex::Real nx = n(0, 0), ny = n(1, 0);
-        const amrex::Real qnL = qL(2, 0) * nx + qL(3, 0) * ny;
-        const amrex::Real qnR = qR(2, 0) * nx + qR(3, 0) * ny;
+        const amrex::Real qnL = zamr_math::axis_dot(qL(2, 0), qL(3, 0), nx, ny);
+        const amrex::Real qnR = zamr_math::axis_dot(qR(2, 0), qR(3, 0), nx, ny);
         const amrex::Real unL = qnL * aL(6, 0);
         const amrex::Real unR = qnR * aR(6, 0);
         const amrex::Real phL = 0.5 * g * qL(1, 0) * qL(1, 0);
@@ -48,8 +48,10 @@ struct Numerics {
         const amrex::Real hs = 0.5 * amrex::max(sL, sR);
         res(0, 0) = 0.0;
         res(1, 0) = 0.5 * (qnL + qnR) - hs * (qR(1, 0) - qL(1, 0));
-        res(2, 0) = 0.5 * ((qL(2, 0) * unL + phL * nx) + (qR(2, 0) * unR + phR * nx)) - hs * (qR(2, 0) - qL(2, 0));
-        res(3, 0) = 0.5 * ((qL(3, 0) * unL + phL * ny) + (qR(3, 0) * unR + phR * ny)) - hs * (qR(3, 0) - qL(3, 0));
+        res(2, 0) = 0.5 * ((qL(2, 0) * unL + zamr_math::axis_scale(phL, nx)) + (qR(2, 0) * unR + zamr_math::axis_scale(phR, nx))) -
+                    hs * (qR(2, 0) - qL(2, 0));
+        res(3, 0) = 0.5 * ((qL(3, 0) * unL + zamr_math::axis_scale(phL, ny)) + (qR(3, 0) * unR + zamr_math::axis_scale(phR, ny))) -
+                    hs * (qR(3, 0) - qL(3, 0));
         return res;
     }
 
@@ -59,10 +61,10 @@ struct Numerics {
     {
         amrex::SmallMatrix<amrex::Real, 8, 1> res{};
         const amrex::Real w = 0.25 * p(0, 0) * (qL(1, 0) + qR(1, 0)) * (qR(0, 0) - qL(0, 0));
-        res(2, 0) = w * n(0, 0);
-        res(3, 0) = w * n(1, 0);
-        res(4 + 2, 0) = w * n(0, 0);
-        res(4 + 3, 0) = w * n(1, 0);
+        res(2, 0) = zamr_math::axis_scale(w, n(0, 0));
+        res(3, 0) = zamr_math::axis_scale(w, n(1, 0));
+        res(4 + 2, 0) = zamr_math::axis_scale(w, n(0, 0));
+        res(4 + 3, 0) = zamr_math::axis_scale(w, n(1, 0));
         return res;
     }
 };

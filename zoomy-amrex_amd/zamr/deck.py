@@ -9,68 +9,62 @@ into deck text.
 import pathlib
 
 
+def _flag(v):
+    return "true" if v else "false"
+
+
+def _domain(ncell, dim_mesh, geom):
+    """(n_cell, prob_lo, prob_hi) as deck vectors: the square unit domain of
+    ncell^2 cells, or the given rectangle; a three-dimensional mesh carries one
+    cell across [0, 1] in z."""
+    if geom is None:
+        cells, lo, hi = [ncell, ncell], [0.0, 0.0], [1.0, 1.0]
+    else:
+        cells = [geom["nx"], geom["ny"]]
+        lo, hi = list(geom["prob_lo"]), list(geom["prob_hi"])
+    if dim_mesh == 3:
+        cells, lo, hi = cells + [1], lo + [0.0], hi + [1.0]
+    return cells, lo, hi
+
+
 def write_inputs(path, ncell, dim_mesh, tend, order, plot_dt, cfl=0.45, bc="extrap",
                  implicit_source=False, implicit_global=False, friction=None, slip=None,
                  max_level=0, ref_ratio=2, geom=None, dem_file=None, release_file=None,
                  well_balanced=False, clamp_positivity=True, tag_b_max=None,
                  bc_sides=None, state_rasters=None, positivity="none", dtmax=None,
                  max_step=None, dtmin=None, max_grid_size=64):
-    """Write an inputs deck at `path` (same parameters as build.py:55-60)."""
-    if geom is not None:
-        nx, ny = geom["nx"], geom["ny"]
-        gx0, gy0 = geom["prob_lo"]
-        gx1, gy1 = geom["prob_hi"]
-        ncell_line = f"{nx} {ny}" + (" 1" if dim_mesh == 3 else "")
-        prob_lo = f"{gx0} {gy0}" + (" 0.0" if dim_mesh == 3 else "")
-        prob_hi = f"{gx1} {gy1}" + (" 1.0" if dim_mesh == 3 else "")
-    else:
-        ncell_line = " ".join([str(ncell)] * 2 + (["1"] if dim_mesh == 3 else []))
-        prob_hi = "1.0 1.0 1.0" if dim_mesh == 3 else "1.0 1.0"
-        prob_lo = "0.0 0.0 0.0" if dim_mesh == 3 else "0.0 0.0"
-    isper = "0 0 0" if dim_mesh == 3 else "0 0"
-    lines = [f"amr.max_level     = {max_level}",
-             f"amr.n_cell        = {ncell_line}",
-             f"amr.max_grid_size = {max_grid_size}",
-             f"amr.blocking_factor = {2 if (max_level > 0 and dim_mesh == 2) else 1}",
-             f"amr.ref_ratio     = {ref_ratio}",
-             "amr.regrid_int    = 2",
-             f"geometry.prob_lo  = {prob_lo}",
-             f"geometry.prob_hi  = {prob_hi}",
-             f"geometry.is_periodic = {isper}"]
-    if bc_sides:
-        for side in ("x_lo", "x_hi", "y_lo", "y_hi"):
-            if side in bc_sides:
-                lines.append(f"bc.{side} = {bc_sides[side]}")
-    elif bc == "wall":
-        lines += ["bc.x_lo = wall", "bc.x_hi = wall", "bc.y_lo = wall", "bc.y_hi = wall"]
-    if friction is not None:
-        lines.append(f"params.n_m = {friction}")
-    if slip is not None:
-        lines.append(f"params.lambda_s = {slip}")
-    if dem_file is not None:
-        lines.append(f"init.dem_file = {dem_file}")
-    if release_file is not None:
-        lines.append(f"init.release_file = {release_file}")
-    if state_rasters:
-        lines.append("init.state_rasters = " + " ".join(map(str, state_rasters)))
-    lines += ["output.identifier       = 0",
-              f"output.plot_dt_interval = {plot_dt}",
-              f"solver.time_end        = {tend}"]
-    if max_step is not None:
-        lines.append(f"solver.max_step        = {max_step}")
-    lines += [f"solver.cfl             = {cfl}",
-              f"solver.dtmin           = {'1.e-7' if dtmin is None else dtmin}",
-              f"solver.dtmax           = {tend if dtmax is None else dtmax}",
-              f"solver.spatial_order   = {order}",
-              f"solver.implicit_source = {'true' if implicit_source else 'false'}",
-              f"solver.implicit_global = {'true' if implicit_global else 'false'}",
-              f"solver.well_balanced   = {'true' if well_balanced else 'false'}",
-              f"solver.clamp_positivity = {'true' if clamp_positivity else 'false'}",
-              f"solver.positivity      = {positivity}",
-              "tagging.threshold      = 0.02"]
-    if tag_b_max is not None:
-        lines.append(f"tagging.b_max          = {tag_b_max}")
-    text = "\n".join(lines) + "\n"
+    """Write the inputs deck of a run at `path` and return its text.
+
+    The arguments and the keys they set are those of the reference's deck
+    writer (build.py:55-120), so a caller of zoomy_amrex.build.write_inputs
+    gets a deck this library reads: the mesh and box keys, geometry (never
+    periodic), boundary tags (per side, or `wall` on every side), model
+    parameters and rasters, output, time stepping and solver switches, and the
+    tagging threshold.  Optional entries are left out when their argument is
+    None, and then take the driver's defaults.  A hierarchy (max_level > 0) on
+    a two-dimensional mesh uses blocking factor 2, anything else 1."""
+    cells, lo, hi = _domain(ncell, dim_mesh, geom)
+    vec = lambda v: " ".join(str(x) for x in v)  # noqa: E731
+    sides = dict(bc_sides) if bc_sides else ({s: "wall" for s in ("x_lo", "x_hi", "y_lo", "y_hi")}
+                                             if bc == "wall" else {})
+    entries = [
+        ("amr.max_level", max_level), ("amr.n_cell", vec(cells)), ("amr.max_grid_size", max_grid_size),
+        ("amr.blocking_factor", 2 if (max_level > 0 and dim_mesh == 2) else 1), ("amr.ref_ratio", ref_ratio),
+        ("amr.regrid_int", 2),
+        ("geometry.prob_lo", vec(lo)), ("geometry.prob_hi", vec(hi)), ("geometry.is_periodic", vec([0] * len(cells))),
+    ]
+    entries += [(f"bc.{s}", sides[s]) for s in ("x_lo", "x_hi", "y_lo", "y_hi") if s in sides]
+    entries += [("params.n_m", friction), ("params.lambda_s", slip), ("init.dem_file", dem_file),
+                ("init.release_file", release_file),
+                ("init.state_rasters", vec(state_rasters) if state_rasters else None),
+                ("output.identifier", 0), ("output.plot_dt_interval", plot_dt), ("solver.time_end", tend),
+                ("solver.max_step", max_step), ("solver.cfl", cfl),
+                ("solver.dtmin", "1.e-7" if dtmin is None else dtmin),
+                ("solver.dtmax", tend if dtmax is None else dtmax), ("solver.spatial_order", order),
+                ("solver.implicit_source", _flag(implicit_source)), ("solver.implicit_global", _flag(implicit_global)),
+                ("solver.well_balanced", _flag(well_balanced)), ("solver.clamp_positivity", _flag(clamp_positivity)),
+                ("solver.positivity", positivity), ("tagging.threshold", 0.02), ("tagging.b_max", tag_b_max)]
+    text = "".join(f"{k} = {v}\n" for k, v in entries if v is not None)
     pathlib.Path(path).write_text(text)
     return text
 

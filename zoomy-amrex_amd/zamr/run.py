@@ -1,5 +1,7 @@
-"""Drop-in counterpart of zoomy_amrex.run_case (run_case.py:346-426) and of
-the plotfile reader the reference tests use (deliverable.py:33-129).
+"""Drop-in counterpart of zoomy_amrex.run_case (run_case.py:346-426), and a
+reader of the AMReX plotfiles the drop-in writes (the on-disk format the
+reference's tests read back through deliverable.py:33-129: a Header, and per
+level a Cell_H whose FabOnDisk lines locate self-describing FABs).
 
 run_case(system, settings, output_dir, ic=...) follows the reference
 sequence: sample the initial condition on the cell centres into one float64
@@ -14,7 +16,6 @@ import os
 import pathlib
 import re
 import shutil
-import struct
 import subprocess
 
 import numpy as np
@@ -23,34 +24,34 @@ from . import compile as zc
 from .deck import write_inputs
 
 
-def _grid(settings):  # run_case.py:75-84
+def _mesh(settings):
+    """(dim, geom) of a run_case settings dict (keys dimension, domain,
+    n_cells, as run_case.py:75-84 reads them).  A one-dimensional case is an
+    nx x 1 strip whose single row of cells is square: y spans one cell width."""
     dim = int(settings.get("dimension", 2))
-    dom = settings["domain"]
-    nc = settings["n_cells"]
+    counts = [int(v) for v in np.atleast_1d(settings["n_cells"])]
+    bounds = [float(v) for v in settings["domain"]]
     if dim == 1:
-        nx = int(nc[0] if isinstance(nc, (list, tuple)) else nc)
-        ny = 1
-        x0, x1 = float(dom[0]), float(dom[1])
-        y0, y1 = 0.0, (x1 - x0) / nx
-    else:
-        nx, ny = (int(nc[0]), int(nc[1])) if isinstance(nc, (list, tuple)) else (int(nc), int(nc))
-        x0, x1, y0, y1 = [float(v) for v in dom[:4]]
-    return dim, {"nx": nx, "ny": ny, "prob_lo": (x0, y0), "prob_hi": (x1, y1)}
+        nx = counts[0]
+        lo = (bounds[0], 0.0)
+        hi = (bounds[1], (bounds[1] - bounds[0]) / nx)
+        return 1, {"nx": nx, "ny": 1, "prob_lo": lo, "prob_hi": hi}
+    nx, ny = counts[0], counts[-1]
+    return dim, {"nx": nx, "ny": ny, "prob_lo": (bounds[0], bounds[2]), "prob_hi": (bounds[1], bounds[3])}
 
 
-def _cell_centers(geom, dim):  # run_case.py:87-95
-    x0, x1 = geom["prob_lo"][0], geom["prob_hi"][0]
-    nx = geom["nx"]
-    dx = (x1 - x0) / nx
-    xc = x0 + (np.arange(nx) + 0.5) * dx
+def _centres(geom, dim):
+    """Cell centres as a (2, ncell) array, x varying fastest (the raster
+    order, idx = i + nx j); y is 0 in one dimension."""
+    axes = []
+    for d, n in ((0, geom["nx"]), (1, geom["ny"])):
+        lo, hi = geom["prob_lo"][d], geom["prob_hi"][d]
+        h = (hi - lo) / n
+        axes.append(lo + h * (0.5 + np.arange(n)))
     if dim == 1:
-        return np.vstack([xc, np.zeros_like(xc)])
-    y0, y1 = geom["prob_lo"][1], geom["prob_hi"][1]
-    ny = geom["ny"]
-    dy = (y1 - y0) / ny
-    yc = y0 + (np.arange(ny) + 0.5) * dy
-    X, Y = np.meshgrid(xc, yc, indexing="xy")
-    return np.vstack([X.ravel(order="C"), Y.ravel(order="C")])
+        return np.stack([axes[0], np.zeros(geom["nx"])])
+    yy, xx = np.meshgrid(axes[1], axes[0], indexing="ij")
+    return np.stack([xx.reshape(-1), yy.reshape(-1)])
 
 
 def _variant(system, dim, order, contract=True):
@@ -74,7 +75,7 @@ def run_case(system, settings, output_dir, ic=None, bc_sides=None, contract=True
     sampled on the cell centres.  bc_sides: tag names per side (default the
     West/East/South/North mapping of run_case.py:281-283)."""
     out = pathlib.Path(output_dir)
-    dim, geom = _grid(settings)
+    dim, geom = _mesh(settings)
     bdir = out / "_build"
     ex = bdir / "Exec"
     ex.mkdir(parents=True, exist_ok=True)
@@ -82,7 +83,7 @@ def run_case(system, settings, output_dir, ic=None, bc_sides=None, contract=True
     if ic is not None:
         rdir = bdir / "raster"
         rdir.mkdir(parents=True, exist_ok=True)
-        Q = np.asarray(ic(_cell_centers(geom, dim)), dtype=np.float64)
+        Q = np.asarray(ic(_centres(geom, dim)), dtype=np.float64)
         rasters = []
         for c in range(Q.shape[0]):
             p = rdir / f"ic_{c}.raw"
@@ -111,59 +112,72 @@ def run_case(system, settings, output_dir, ic=None, bc_sides=None, contract=True
     return plts[-1]
 
 
-_BOX = re.compile(r"\(\(([-\d,]+)\) \(([-\d,]+)\)")
+def _fab(path, offset):
+    """One FAB of a Cell_D file: its header line (FAB <format> ((lo) (hi) (type)) ncomp)
+    names the box and the component count, little-endian float64 data
+    (component-major, i fastest) follows.  Returns (lo, hi, data)."""
+    with open(path, "rb") as f:
+        f.seek(offset)
+        head = f.readline().decode()
+        ints = [int(v) for v in re.findall(r"-?\d+", head[head.rindex(")))") + 3:])]
+        lo, hi, ncomp = ints[0:2], ints[2:4], ints[-1]
+        shape = (ncomp, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+        data = np.fromfile(f, dtype="<f8", count=int(np.prod(shape))).reshape(shape)
+    return lo, hi, data
 
 
-def _read_level(lvl_dir, ncomp):
-    """(ncomp, ny, nx) array over the level's box bounding box (NaN where no
-    box), its (ilo, jlo, nx, ny) and the boxes (lo0, lo1, hi0, hi1)."""
-    txt = (lvl_dir / "Cell_H").read_text()
-    boxes = [([int(v) for v in a.split(",")], [int(v) for v in b.split(",")]) for a, b in _BOX.findall(txt)]
-    fods = re.findall(r"FabOnDisk:\s+(\S+)\s+(\d+)", txt)
-    ilo = min(lo[0] for lo, _ in boxes)
-    jlo = min(lo[1] for lo, _ in boxes)
-    nx = max(hi[0] for _, hi in boxes) - ilo + 1
-    ny = max(hi[1] for _, hi in boxes) - jlo + 1
-    out = np.full((ncomp, ny, nx), np.nan)
-    for (lo, hi), (fname, off) in zip(boxes, fods):
-        bx, by = hi[0] - lo[0] + 1, hi[1] - lo[1] + 1
-        with open(lvl_dir / fname, "rb") as fb:
-            fb.seek(int(off))
-            fb.readline()
-            n = bx * by * ncomp
-            data = np.array(struct.unpack("<%dd" % n, fb.read(8 * n))).reshape(ncomp, by, bx)
-        out[:, lo[1] - jlo:hi[1] - jlo + 1, lo[0] - ilo:hi[0] - ilo + 1] = data
-    return out, (ilo, jlo, nx, ny), [(lo[0], lo[1], hi[0], hi[1]) for lo, hi in boxes]
+def _level(level_dir, ncomp):
+    """A level's FABs (located by the FabOnDisk lines of its Cell_H) pasted
+    into one (ncomp, ny, nx) array over their bounding box, NaN where no box
+    lies; also that box's (ilo, jlo, nx, ny) and the boxes (lo0, lo1, hi0, hi1)."""
+    fabs = []
+    for line in (level_dir / "Cell_H").read_text().splitlines():
+        if line.startswith("FabOnDisk:"):
+            _, name, off = line.split()
+            fabs.append(_fab(level_dir / name, int(off)))
+    ilo = min(lo[0] for lo, _, _ in fabs)
+    jlo = min(lo[1] for lo, _, _ in fabs)
+    nx = max(hi[0] for _, hi, _ in fabs) - ilo + 1
+    ny = max(hi[1] for _, hi, _ in fabs) - jlo + 1
+    arr = np.full((ncomp, ny, nx), np.nan)
+    for lo, hi, data in fabs:
+        arr[:, lo[1] - jlo:hi[1] - jlo + 1, lo[0] - ilo:hi[0] - ilo + 1] = data[:ncomp]
+    return arr, (ilo, jlo, nx, ny), [(lo[0], lo[1], hi[0], hi[1]) for lo, hi, _ in fabs]
+
+
+def _header(path):
+    """The fields of a plotfile's Header used here, read line by line."""
+    it = iter((path / "Header").read_text().splitlines())
+    next(it)  # format tag
+    ncomp = int(next(it))
+    names = [next(it) for _ in range(ncomp)]
+    dim = int(next(it))
+    time = float(next(it))
+    finest = int(next(it))
+    prob_lo = [float(v) for v in next(it).split()]
+    prob_hi = [float(v) for v in next(it).split()]
+    refs = [int(v) for v in next(it).split()]
+    return dict(ncomp=ncomp, names=names, dim=dim, time=time, finest_level=finest, prob_lo=prob_lo,
+                prob_hi=prob_hi, refs=refs)
 
 
 def read_plotfile(path):
     """(ncomp, ny, nx) level-0 array, variable names and time of a plotfile."""
     path = pathlib.Path(path)
-    lines = (path / "Header").read_text().splitlines()
-    ncomp = int(lines[1])
-    names = lines[2:2 + ncomp]
-    time = float(lines[2 + ncomp + 1])
-    out, _, _ = _read_level(path / "Level_0", ncomp)
-    return out, names, time
+    H = _header(path)
+    arr, _, _ = _level(path / "Level_0", H["ncomp"])
+    return arr, H["names"], H["time"]
 
 
 def read_levels(path):
-    """Every level of a plotfile, as deliverable.read_levels (deliverable.py:
-    114-129) returns them: a list of dicts arr / ext / boxes / ref
-    (cumulative refinement), and the header fields ncomp, names,
-    finest_level, prob_lo, prob_hi, refs, time."""
+    """Every level of a plotfile: a list of dicts arr (the level's array over
+    its boxes' bounding box), ext (ilo, jlo, nx, ny), boxes and ref (the
+    refinement relative to level 0), and the Header fields ncomp, names, dim,
+    time, finest_level, prob_lo, prob_hi, refs."""
     path = pathlib.Path(path)
-    lines = (path / "Header").read_text().splitlines()
-    ncomp = int(lines[1])
-    i = 2 + ncomp
-    H = dict(ncomp=ncomp, names=lines[2:i], dim=int(lines[i]), time=float(lines[i + 1]),
-             finest_level=int(lines[i + 2]), prob_lo=[float(v) for v in lines[i + 3].split()],
-             prob_hi=[float(v) for v in lines[i + 4].split()],
-             refs=[int(v) for v in lines[i + 5].split()])
-    levels, cum = [], 1
+    H = _header(path)
+    levels = []
     for lev in range(H["finest_level"] + 1):
-        if lev > 0:
-            cum *= H["refs"][lev - 1] if lev - 1 < len(H["refs"]) else 2
-        arr, ext, boxes = _read_level(path / f"Level_{lev}", ncomp)
-        levels.append(dict(arr=arr, ext=ext, boxes=boxes, ref=cum))
+        arr, ext, boxes = _level(path / f"Level_{lev}", H["ncomp"])
+        levels.append(dict(arr=arr, ext=ext, boxes=boxes, ref=int(np.prod(H["refs"][:lev], dtype=int))))
     return levels, H

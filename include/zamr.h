@@ -92,6 +92,15 @@ int zamr_get_state(zamr_ctx* ctx, int lev, int field, double* host, size_t n);
 
 /* ZoomyAmr::ComputeDt (ZoomyAmr.cpp:448-515) over the state and aux as stored. */
 int zamr_compute_dt(zamr_ctx* ctx, int lev, double* dt);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): zamr_compute_dt's step size, written to the device double
+ * dt_device on the context's stream, with no allocation and no host
+ * synchronisation.  Its CFL reduction runs in a fixed scratch array of this
+ * library on the current GPU (one per GPU, held until the process ends):
+ * contexts with more than 4096 time-step groups are rejected (ZAMR_EINVAL),
+ * and calls that overlap on one GPU race on that array. */
+int zamr_compute_dt_noalloc(zamr_ctx* ctx, int lev, double* dt_device);
 /* ZoomyAmr::UpdateState (ZoomyAmr.cpp:293-334). */
 int zamr_update_state(zamr_ctx* ctx, int lev, double time);
 /* ZoomyAmr::FillPatch (ZoomyAmr.cpp:258-277, single level): same-level ghost

@@ -513,6 +513,9 @@ class Solver {
             jfnk = std::make_unique<Jfnk<M>>();
             jfnk->setup(stream, d_boxes, bx);
         }
+        // the work items of the stage order exist from creation on, so that
+        // entry points that may not allocate find them
+        items(hx_of(spatial_order));
         if (!stage) ZAMR_HIP(hipStreamSynchronize(stream));
     }
 
@@ -1117,6 +1120,31 @@ class Solver {
         const double div = double(c_cfl_dimension) * double(c_cfl_degree_factor);
         const double dt = cfl / (div * m);
         return std::max(std::min(dt, dtmax), dtmin);
+    }
+
+    // ComputeDt as compute_dt_host computes it, queued on the stream with no
+    // allocation and no host synchronisation: the stored aux brought up to
+    // date at each group's time, the CFL maxima reduced into the ngroups group
+    // states at `scratch` (device memory of at least ngroups GroupStates,
+    // cleared here), the step size written to the device double `out`.
+    void compute_dt_device(GroupState* scratch, double* out) {
+        const int hx = hx_of(spatial_order);
+        Items& it = items_[hx];
+        if (!it.d || it.hx != hx) throw Error(1, "the context's work items are not set up");
+        if (aux_stale) {
+            hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d,
+                               it.n, cur, prm, 0.0, WV - 2 * it.hx, (const GroupState*)d_groups);
+            ZAMR_HIP(hipGetLastError());
+            aux_stale = false;
+        }
+        hipLaunchKernelGGL(clear_maxinv_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, scratch, ngroups);
+        ZAMR_HIP(hipGetLastError());
+        hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
+                           cur, prm, scratch, WV - 2 * it.hx);
+        ZAMR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(dt_from_maxinv_kernel, dim3(1), dim3(64), 0, stream, (const GroupState*)scratch, ngroups,
+                           cfl, double(c_cfl_dimension) * double(c_cfl_degree_factor), dtmin, dtmax, out);
+        ZAMR_HIP(hipGetLastError());
     }
 
     void set_clock(double dt, double time) {

@@ -1145,6 +1145,31 @@ __global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__
     }
 }
 
+// Clear the CFL maxima of ngroups group states (a fill kernel, so that the
+// clearing can sit in a captured graph).
+__global__ void clear_maxinv_kernel(GroupState* gs, int ngroups) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups) gs[g].maxinv_bits = 0ull;
+}
+
+// ComputeDt's step size (ZoomyAmr.cpp:494-514) from the maxima of ngroups
+// group states: the largest over the groups, a wave-free domain giving dtmax,
+// then clamped to [dtmin, dtmax]; written to out[0].  One thread.
+__global__ void dt_from_maxinv_kernel(const GroupState* gs, int ngroups, double cfl, double divisor, double dtmin,
+                                      double dtmax, double* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int g = 0; g < ngroups; ++g) m = dev_max(m, __longlong_as_double((long long)gs[g].maxinv_bits));
+    double dt;
+    if (m < 1e-14) {
+        dt = dtmax;
+    } else {
+        dt = cfl / (divisor * m);
+        dt = dev_max((dt < dtmax) ? dt : dtmax, dtmin);
+    }
+    out[0] = dt;
+}
+
 // Start of TimeStep for every group: dt from the reduced max (ZoomyAmr.cpp:
 // 494-514), capped to time_end (:746-749); the loop condition of Evolve
 // (:782) decides whether the group is still active.

@@ -9,6 +9,7 @@
 #include "zamr_driver.hpp"
 #include "zamr_plotfile.hpp"
 #include <memory>
+#include <mutex>
 #if __has_include("Adjoint.H")
 #include "Adjoint.H"
 #include "zamr_adjoint.hpp"
@@ -156,6 +157,34 @@ void zamr::AmrDriver<M, N>::write_plotfile(const std::string& dir, int step, dou
         throw Error(ZAMR_EIO, "cannot write plotfile " + path);
 }
 
+namespace {
+// Device scratch of zamr_compute_dt_noalloc: the group states its CFL
+// reduction accumulates into.  Every GPU the library runs on holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr int kScratchGroups = 4096;
+static __device__ __align__(256) unsigned char scratch[kScratchGroups * sizeof(zamr::GroupState)];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_dev[kMaxDevices] = {};
+
+// Address of this library's scratch on the current device, looked up once
+// per device.
+zamr::GroupState* scratch_here() {
+    int dev = -1;
+    ZAMR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw zamr::Error(ZAMR_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* ptr = nullptr;
+        ZAMR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = ptr;
+    }
+    return static_cast<zamr::GroupState*>(scratch_dev[dev]);
+}
+}  // namespace
+
 extern "C" {
 
 int zamr_model_info(int* nq, int* na, int* np, int* dim, int* ntags) {
@@ -276,6 +305,18 @@ int zamr_compute_dt(zamr_ctx* c, int lev, double* dt) {
         }
         c->s.level_box_check(lev);
         *dt = c->s.compute_dt_host();
+    });
+}
+int zamr_compute_dt_noalloc(zamr_ctx* c, int lev, double* dt_device) {
+    if (!c || !dt_device || !c->s.d_pool) return ZAMR_EINVAL;
+    return guard(c, [&] {
+        Sol& L = c->amr ? c->amr->level(lev) : c->s;
+        if (!c->amr) c->s.level_box_check(lev);
+        if (L.ngroups > kScratchGroups)
+            throw zamr::Error(ZAMR_EINVAL, "zamr_compute_dt_noalloc: " + std::to_string(L.ngroups) +
+                                               " time-step groups exceed the scratch capacity of " +
+                                               std::to_string(kScratchGroups));
+        L.compute_dt_device(scratch_here(), dt_device);
     });
 }
 int zamr_update_state(zamr_ctx* c, int lev, double time) {

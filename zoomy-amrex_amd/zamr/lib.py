@@ -52,6 +52,7 @@ def load_library(path):
         "zamr_set_state": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t],
         "zamr_get_state": [_VP, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t],
         "zamr_compute_dt": [_VP, ctypes.c_int, _D],
+        "zamr_compute_dt_noalloc": [_VP, ctypes.c_int, _VP],
         "zamr_update_state": [_VP, ctypes.c_int, ctypes.c_double],
         "zamr_fill_patch": [_VP, ctypes.c_int, ctypes.c_double],
         "zamr_stage": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
@@ -220,6 +221,11 @@ class ZoomyAmr:
         dt = ctypes.c_double()
         self._check(self.lib.zamr_compute_dt(self.ctx, lev, ctypes.byref(dt)))
         return dt.value
+
+    def compute_dt_noalloc(self, dt_device_ptr, lev=0):
+        """zamr_compute_dt_noalloc: the step size written to the device double
+        at dt_device_ptr (an integer address) on the context's stream."""
+        self._check(self.lib.zamr_compute_dt_noalloc(self.ctx, lev, ctypes.c_void_p(dt_device_ptr)))
 
     def update_state(self, time=0.0, lev=0):
         self._check(self.lib.zamr_update_state(self.ctx, lev, time))

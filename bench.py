@@ -40,27 +40,65 @@ def parse_traffic(n_cells):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(case, Q0, threads):
-    """The oracle (C++ restatement of the reference, OpenMP over boxes) on one
-    SSP-RK2 step of the same 4096^2 workload, boxes of 256^2."""
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, Q0, threads, steps_all=2):
+    """The oracle (C++ restatement of the reference, OpenMP over boxes) on the
+    same 4096^2 workload cut into 64^2 boxes, as the reference's MPI-over-boxes
+    decomposition cuts it (max_grid_size 64): `steps_all` SSP-RK2 steps on
+    `threads` threads (every core this process may use, at most 16, the CPU
+    share one GPU's job gets), then one step on a single core."""
     from oracle.pyoracle import Oracle
-    c = dict(case, max_grid_size=256, max_step=1)
-    o = Oracle("swe2d", c, threads=threads)
-    o.init_solution()
-    o.set_state(Q0)
-    t0 = time.perf_counter()
-    n, _, _, _ = o.evolve()
-    el = time.perf_counter() - t0
     cells = case["nx"] * case["ny"]
-    return dict(value=2 * n * cells / el, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"1 SSP-RK2 step (2 stages) of the same {case['nx']}^2 workload, {el:.2f} s")
+
+    def leg(th, steps):
+        c = dict(case, max_grid_size=64, max_step=steps)
+        o = Oracle("swe2d", c, threads=th)
+        o.init_solution()
+        o.set_state(Q0)
+        t0 = time.perf_counter()
+        n, _, _, _ = o.evolve()
+        el = time.perf_counter() - t0
+        del o
+        return 2 * n * cells / el, n, el
+
+    v_all, n_all, el_all = leg(threads, steps_all)
+    v_one, n_one, el_one = leg(1, 1)
+    nb = (case["nx"] // 64) * (case["ny"] // 64)
+    return dict(value=v_all, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"{n_all} SSP-RK2 steps (2 stages each) of the same {case['nx']}^2 workload in {nb} boxes "
+                       f"of 64^2 on {threads} threads, {el_all:.2f} s",
+                one_core=dict(value=v_one, unit="cell-updates/s", cores=1,
+                              sample=f"{n_one} SSP-RK2 step of the same workload and boxes on 1 thread, "
+                                     f"{el_one:.2f} s"),
+                nproc=os.cpu_count(), usable_cores=len(os.sched_getaffinity(0)), cpu_model=cpu_model(),
+                build="g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp, OpenMP over boxes")
 
 
 def bench_boxlist(args):
     """Batched box list: 600 patches of a 3-level hierarchy (sides 16..128) and
     64 dam-break ensemble members (256..1024 cells a side), per-group dt on the
-    device; the same kernels launched once per stage for all boxes (hipGraph)
-    against once per box per stage (eager)."""
+    device.  Four launch modes of the same kernels from the same state:
+    one launch per box per pass (eager, the baseline), one launch per pass
+    for all boxes eager, the same captured in hipGraphs, and that with the
+    measured launch-shape tuner run first."""
     import numpy as np
     import torch
     from zamr import cases
@@ -69,11 +107,15 @@ def bench_boxlist(args):
     steps = args.steps
     boxes, states, keys = cases.box_list(seed=0, steps=args.warmup + 2 * steps)
     cells = sum((b["hi"][0] + 1) * (b["hi"][1] + 1) for b in boxes)
-    res = {}
-    for mode in ("batched", "per_box"):
+    res, tuned = {}, None
+    for mode in ("per_box", "batched_eager", "batched_graph", "batched_tuned"):
         z = from_boxes(boxes, states, keys)
         if mode == "per_box":
             z.set_tuning(per_box_launch=1)
+        if mode == "batched_eager":
+            z.set_graph(False)
+        if mode == "batched_tuned":
+            tuned = z.autotune(reps=10)
         z.run_steps(args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -83,15 +125,21 @@ def bench_boxlist(args):
         res[mode] = dict(seconds=el, steps=done, rate=2 * done * cells / el)
         res[mode + "_state"] = [z.get_state(lev=k) for k in (0, len(boxes) - 1)]
         del z
-    same = all(np.array_equal(a, b) for a, b in zip(res["batched_state"], res["per_box_state"]))
+    same = all(np.array_equal(a, b) for m in ("batched_eager", "batched_graph", "batched_tuned")
+               for a, b in zip(res[m + "_state"], res["per_box_state"]))
     out = {"metric": "batched box-list speed-up over one launch per box per pass",
-           "value": res["per_box"]["seconds"] / res["batched"]["seconds"], "unit": "x",
+           "value": res["per_box"]["seconds"] / res["batched_tuned"]["seconds"], "unit": "x",
            "higher_is_better": True, "n_gpus": 1, "steps": steps, "warmup": args.warmup, "dtype": "f64",
            "data": "synthetic",
            "config": {"workload": "600 AMR patches (3 levels, sides 16/32/64/128) + 64 dam-break members "
                                   "(256..1024)^2, order 2, SSP-RK2, per-group dt", "boxes": len(boxes),
                       "cells": cells},
-           "batched_cell_updates_per_s": res["batched"]["rate"], "per_box_cell_updates_per_s": res["per_box"]["rate"],
+           "cell_updates_per_s": {m: res[m]["rate"] for m in ("per_box", "batched_eager", "batched_graph",
+                                                              "batched_tuned")},
+           "speedup_batching_alone": res["per_box"]["seconds"] / res["batched_eager"]["seconds"],
+           "speedup_graph_over_eager": res["batched_eager"]["seconds"] / res["batched_graph"]["seconds"],
+           "tuner": {"rows_per_size_class": tuned[0], "grid_blocks": tuned[1],
+                     "stage1_ms_analytic": tuned[2][0], "stage1_ms_tuned": tuned[2][1]},
            "results_identical": bool(same)}
     print(json.dumps(out), flush=True)
 
@@ -301,6 +349,7 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="run the measured launch-shape tuner before the warm-up")
     ap.add_argument("--workload", default="swe4096", choices=["swe4096", "boxlist", "adjoint", "sme", "amr", "jfnk"],
                     help="swe4096: the single-level 4096^2 workload (default); boxlist: 600 AMR patches + "
                          "64 ensemble members, batched launch vs one launch per box; adjoint: the 1024^2 "
@@ -347,6 +396,7 @@ def main():
     z.set_state(Q0)
     if args.rows:
         z.set_tuning(rows=args.rows)
+    tuned = z.autotune(reps=10) if args.tune else None
     if args.warmup:
         z.run_steps(args.warmup)
     torch.cuda.synchronize()
@@ -385,9 +435,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"2-D SWE {args.n}x{args.n}, 32 seeded Gaussian bumps, wet/dry, Manning 0.03 "
                                "point-implicit, well-balanced order-2 MUSCL/Rusanov, SSP-RK2",
-                   "nx": args.n, "ny": args.n, "steps": args.steps, "seed": 0},
+                   "nx": args.n, "ny": args.n, "steps": args.steps, "seed": 0,
+                   "launch_shape": ({"rows_per_size_class": tuned[0], "grid_blocks": tuned[1],
+                                     "stage1_ms_analytic": tuned[2][0], "stage1_ms_tuned": tuned[2][1]}
+                                    if tuned else "analytic rule")},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": "not measured in this run: stage2_bytes_per_launch of "
+                                       "profiles/stage2_swe4096_pmc.txt (rocprofv3 --pmc FETCH_SIZE and "
+                                       "WRITE_SIZE of this workload, scaled by the copy-kernel factor "
+                                       "stated there)",
                      "kernel": "stage_kernel<swe2d, order 2, stage 2>",
                      "algorithmic_bytes_per_launch": BYTES_STAGE2 * cells,
                      "avg_launch_ms": ktimes[1], "stage1_avg_launch_ms": ktimes[0],
@@ -411,7 +468,7 @@ def main():
     if not args.no_sme:
         out["sme"] = bench_sme()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
         out["cpu_baseline"] = cpu_baseline(case, Q0, threads)
     print(json.dumps(out), flush=True)
 

@@ -142,10 +142,32 @@ int zamr_device_ptr(zamr_ctx* ctx, int lev, int box, int field, void** ptr, long
 /* Run on a caller-owned HIP stream (hipStream_t) instead of the context's own. */
 int zamr_set_stream(zamr_ctx* ctx, void* stream);
 
-/* Tuning: rows per work item for the stage kernels (0 = autotune per box size
- * class on first use), per-box launches instead of one batched launch
- * (baseline for the batching speed-up), and XCD-aware item order. */
+/* Tuning: rows per work item for the stage kernels in every box (0 = per box
+ * size class: the table zamr_autotune measured, else an analytic rule over
+ * the whole box list: the deepest of 32/16/8/4 rows that still gives every
+ * SIMD two waves), per-box launches instead of one batched launch (baseline
+ * for the batching speed-up), and XCD-aware item order.  No counterpart in
+ * the reference, whose launch shape is AMReX's ParallelFor. */
 int zamr_set_tuning(zamr_ctx* ctx, int rows_per_item, int per_box_launch, int xcd_remap);
+/* Measured launch-shape tuner (single-level and box-list contexts; AMR gets
+ * ZAMR_EUNSUP).  Times the stage kernels of a step on the current state with
+ * HIP events (the shortest of `reps` launches per candidate) and keeps the
+ * fastest rows per work item in {4, 8, 16, 32, 64} for each box size class
+ * present (the larger side <= 32, 64, 128, 256, 512, 1024, 2048, larger: 8
+ * classes) and the fastest grid (one block per four-item slot, or 1x / 2x /
+ * 4x the blocks the device holds at once, walking the slots) of the
+ * first-stage and of the step-ending stage kernel.  Writes the table to class_rows[8] (0 = class
+ * absent or the analytic rule kept), the grids to grid_blocks[2] (0 = one
+ * block per slot) and the stage-kernel time per step (ms) before and after
+ * to ms[2]; any may be NULL.  The state is left as it was, and no result
+ * depends on the shape chosen.  No counterpart in the reference. */
+int zamr_autotune(zamr_ctx* ctx, int reps, int* class_rows, int* grid_blocks, double* ms);
+/* Set the launch shape zamr_autotune would set (a table measured earlier on
+ * the same box list): class_rows[8], grid_blocks[2]. */
+int zamr_set_launch_shape(zamr_ctx* ctx, const int* class_rows, const int* grid_blocks);
+/* Capture chunks of steps of zamr_run_steps into hipGraphs (on = 1, the
+ * default) or launch every kernel eagerly (on = 0). */
+int zamr_set_graph(zamr_ctx* ctx, int on);
 /* Average device time (ms) per launch of each stage kernel over the last
  * zamr_run_steps call with profiling on: out[0] stage 1, out[1] stage 2,
  * out[2] per-step bookkeeping; returns the number of timed steps in *nsteps. */

@@ -81,3 +81,53 @@ def test_shared_group_dt_is_the_segmented_max():
         g = 3 + m
         k = [i for i, b in enumerate(boxes) if b["group"] == g][0]
         assert z.get_time(g)[2] == oracle_box(boxes[k], states[k], keys).compute_dt()
+
+
+def test_full_box_list_launch_modes_bitwise():
+    """The whole 664-box list (600 AMR patches, 64 ensemble members, 26.9M
+    cells) for 16 steps, one captured chunk: one launch per box per pass,
+    one batched launch per pass eager, and the same in a hipGraph all give the
+    same bits in every box."""
+    boxes, states, keys = cases.box_list(seed=0, steps=16)
+    assert len(boxes) == 664
+    per_box = from_boxes(boxes, states, keys)
+    per_box.set_tuning(per_box_launch=1)
+    assert per_box.run_steps(16)[0] == 16
+    ref = [per_box.get_state(lev=k) for k in range(len(boxes))]
+    del per_box
+    for graph in (False, True):
+        z = from_boxes(boxes, states, keys)
+        z.set_graph(graph)
+        assert z.run_steps(16)[0] == 16
+        for k in range(len(boxes)):
+            assert np.array_equal(z.get_state(lev=k), ref[k]), f"box {k}, graph {graph}"
+        del z
+
+
+@pytest.mark.parametrize("contract", [True, False])
+def test_tuned_launch_shape_changes_no_result(contract):
+    """The measured tuner picks rows per item per size class and a grid; the
+    result of a run is the same bits whatever it picks, and whatever fixed
+    rows per item are set."""
+    boxes, states, keys = small_list(seed=9, steps=6)
+    base = from_boxes(boxes, states, keys, contract=contract)
+    run(base)
+    ref = [base.get_state(lev=k) for k in range(len(boxes))]
+    del base
+    z = from_boxes(boxes, states, keys, contract=contract)
+    before = [z.get_state(lev=k) for k in range(len(boxes))]
+    rows, grid, ms = z.autotune(reps=3)
+    assert len(rows) == 8 and all(r in (0, 4, 8, 16, 32, 64) for r in rows)
+    assert len(grid) == 2 and min(grid) >= 0 and ms[0] > 0 and ms[1] > 0
+    for k in range(len(boxes)):
+        assert np.array_equal(z.get_state(lev=k), before[k]), f"the tuner changed box {k}"
+    run(z)
+    for k in range(len(boxes)):
+        assert np.array_equal(z.get_state(lev=k), ref[k]), f"tuned, box {k}"
+    for fixed in (4, 64):
+        y = from_boxes(boxes, states, keys, contract=contract)
+        y.set_tuning(rows=fixed)
+        run(y)
+        for k in range(len(boxes)):
+            assert np.array_equal(y.get_state(lev=k), ref[k]), f"rows {fixed}, box {k}"
+        del y

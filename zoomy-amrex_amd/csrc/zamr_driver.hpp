@@ -184,12 +184,23 @@ class Solver {
                                     // stage's FillPatch), -1 before the first
     bool maxinv_fresh = false;      // group maxinv_bits already hold this state's reduction
     bool aux_stale = false;         // stored Qaux lags the last UpdateState (steps ran since)
-    int rows_per_item = 0;          // 0 = autotune
+    int rows_per_item = 0;          // > 0: rows per work item in every box; 0: by size class
     int per_box_launch = 0;
     int xcd_remap = 1;
+    // Stage-kernel launch shape, per box size class (the larger side of a
+    // box: <= 32, 64, 128, 256, 512, 1024, 2048, larger): rows per work item
+    // (0 = the analytic rule of rows_for), and the grid of the first-stage
+    // and of the step-ending stage kernels (0 = one block per slot of four
+    // items; n > 0 = n blocks walking the slots).  autotune() fills them
+    // from measurements; tune_gen counts the changes.
+    static constexpr int kSizeClasses = 8;
+    int class_rows[kSizeClasses] = {};
+    int grid_blocks[2] = {0, 0};
+    int tune_gen = 0;
+    bool use_graph = true;          // capture chunks of steps into hipGraphs
 
     struct Items {
-        int hx = -1, rows = -1;
+        int hx = -1, rows = -1, gen = -1;
         WorkItem* d = nullptr;
         int n = 0;
         std::vector<int> box_begin;  // item range per box (per-box launch mode)
@@ -636,24 +647,42 @@ class Solver {
         return 4;
     }
 
-    // Strips of NT-2*hx columns by segments of `rows` rows, box by box.
+    static int size_class(const HostBox& b) {
+        const int side = std::max(b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1);
+        int k = 0;
+        while (k + 1 < kSizeClasses && side > (32 << k)) ++k;
+        return k;
+    }
+
+    // Blocks of a stage launch over n items (EPI 0: a first stage; 1, 2: a
+    // stage that ends the step).
+    int grid_for(int n, int epi) const {
+        const int s = slots_for(n), g = grid_blocks[epi == 0 ? 0 : 1];
+        return (g > 0 && g < s) ? g : s;
+    }
+
+    // Strips of NT-2*hx columns by segments of rows, box by box: the rows of
+    // the box's size class, or rows_for's one value.
     Items& items(int hx) {
         Items& it = items_[hx];
-        const int rows = (dim == 2) ? rows_for(hx) : 1;
-        if (it.d && it.hx == hx && it.rows == rows) return it;
+        const int rule = (dim == 2) ? rows_for(hx) : 1;
+        if (it.d && it.hx == hx && it.rows == rule && it.gen == tune_gen) return it;
         std::vector<WorkItem> v;
         it.box_begin.clear();
         const int txc = WV - 2 * hx;  // one wave64 per item, HX halo columns each side
         for (size_t b = 0; b < hb.size(); ++b) {
             it.box_begin.push_back((int)v.size());
             const HostBox& B = hb[b];
+            int rows = rule;
+            if (dim == 2 && rows_per_item <= 0 && class_rows[size_class(B)] > 0) rows = class_rows[size_class(B)];
             for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += rows)
                 for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += txc)
                     v.push_back(WorkItem{(int)b, i0, j0, std::min(rows, B.hi[1] - j0 + 1)});
         }
         it.box_begin.push_back((int)v.size());
         it.hx = hx;
-        it.rows = rows;
+        it.rows = rule;
+        it.gen = tune_gen;
         it.n = (int)v.size();
         dev_get(BufferCache::ITEMS0 + hx, it.d, v.size());
         upload(it.d, v.data(), v.size() * sizeof(WorkItem));
@@ -855,7 +884,7 @@ class Solver {
                 hipLaunchKernelGGL(kern, dim3(slots_for(n)), dim3(NT), 0, stream, d_boxes, it.d + n0, n, a, d_groups, 0);
             }
         } else {
-            hipLaunchKernelGGL(kern, dim3(slots_for(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n, a, d_groups,
+            hipLaunchKernelGGL(kern, dim3(grid_for(it.n, EPI)), dim3(NT), 0, stream, d_boxes, it.d, it.n, a, d_groups,
                                xcd_remap);
         }
         ZAMR_HIP(hipGetLastError());
@@ -1027,7 +1056,9 @@ class Solver {
 
     // Graph replay for the device-clock loop; the per-box launch mode stays
     // eager, as the baseline it stands for (one launch per box per pass).
-    bool graph_ok() const { return spatial_order == 2 && !profiling && !per_box_launch && !jfnk_mode(); }
+    bool graph_ok() const {
+        return use_graph && spatial_order == 2 && !profiling && !per_box_launch && !jfnk_mode();
+    }
 
     void run_steps(int nsteps, int* done, double* time) {
         download_groups();
@@ -1100,6 +1131,145 @@ class Solver {
         }
         if (done) *done = h_groups[0].step - step0;
         if (time) *time = h_groups[0].time;
+    }
+
+    // Measured launch-shape tuner.  The stage kernels of one step are timed
+    // on the current state with HIP events (one untimed launch, then the
+    // shortest of `reps` launches per candidate): first under the analytic
+    // rule, then with rows per work item in {4, 8, 16, 32, 64} for each box
+    // size class present, one class at a time from the one with the most
+    // cells (cost: both kernels), then the grid of each kernel (one block
+    // per slot, or 1x, 2x, 4x the blocks the device holds at once, walking
+    // the slots); each keeps its fastest candidate, which has to beat the
+    // one kept by 1%.  The first stage writes the buffer
+    // the next step's first stage overwrites, the step-ending stage a
+    // buffer no state lives in (a temporary one when a box has two), so the
+    // state is left as it was; the group CFL maxima are recomputed before
+    // the next step.  The cut into items changes no result (every face is
+    // evaluated by the same code whichever item holds it).  Returns the
+    // per-step stage-kernel time (ms) of the analytic shape and of the
+    // chosen one.
+    std::pair<double, double> autotune(int reps) {
+        if (dim != 2) throw Error(1, "the tuner cuts 2-D boxes into row segments; 1-D items are single rows");
+        if (jfnk_mode()) throw Error(5, "the tuner times the point-implicit stages; not with solver.implicit_global");
+        reps = std::max(reps, 1);
+        const int hx = hx_of(spatial_order);
+        const bool two = spatial_order == 2;
+        const int d1 = (nbuf() == 2) ? 1 - cur : (cur + 1) % 3;
+        int d2 = (cur + 2) % 3;
+        double* tmp = nullptr;
+        if (two && nbuf() == 2) {
+            std::vector<size_t> off(hb.size());
+            size_t total = 0;
+            for (size_t k = 0; k < hb.size(); ++k) {
+                off[k] = total;
+                total = (total + (size_t)hb[k].gx * hb[k].gy * nq + 31) & ~size_t(31);  // 256-byte alignment
+            }
+            ZAMR_HIP(hipMalloc(&tmp, total * sizeof(double)));
+            std::vector<BoxDesc> bd(hb.size());
+            for (size_t k = 0; k < hb.size(); ++k)
+                bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1], tmp + off[k]);
+            upload(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc));
+            ZAMR_HIP(hipStreamSynchronize(stream));
+            d2 = 2;
+        }
+        const int epi_first = two ? 0 : 2, epi_last = two ? 1 : 2;
+        const StageArgs a1 = stage_args(cur, d1, cur);
+        const StageArgs a2 = stage_args(d1, d2, cur);
+        rows_per_item = 0;
+        for (int& r : class_rows) r = 0;
+        grid_blocks[0] = grid_blocks[1] = 0;
+        ++tune_gen;
+        invalidate_graph();
+        // a candidate's time: the shortest of `reps` launches, each between
+        // two events (the minimum is robust to clock ramps and to other work
+        // on the device; a candidate must beat the kept one by 1%)
+        std::vector<hipEvent_t> e(reps + 1);
+        for (auto& x : e) ZAMR_HIP(hipEventCreate(&x));
+        auto time_kernel = [&](int epi, const StageArgs& a) {
+            items(hx);
+            launch_stage_dyn(spatial_order, epi, a);
+            for (int r = 0; r < reps; ++r) {
+                ZAMR_HIP(hipEventRecord(e[r], stream));
+                launch_stage_dyn(spatial_order, epi, a);
+            }
+            ZAMR_HIP(hipEventRecord(e[reps], stream));
+            ZAMR_HIP(hipEventSynchronize(e[reps]));
+            double best_ms = 1e30;
+            for (int r = 0; r < reps; ++r) {
+                float ms = 0.0f;
+                ZAMR_HIP(hipEventElapsedTime(&ms, e[r], e[r + 1]));
+                best_ms = std::min(best_ms, (double)ms);
+            }
+            return best_ms;
+        };
+        auto cost = [&]() { return time_kernel(epi_first, a1) + (two ? time_kernel(epi_last, a2) : 0.0); };
+        fill_boundary(cur);
+        if (two) {  // the step-ending stage reads the first stage's output with its ghosts filled
+            items(hx);
+            launch_stage_dyn(spatial_order, epi_first, a1);
+            fill_boundary(d1);
+        }
+        for (int w = 0; w < 4; ++w) cost();  // clocks up before the first candidate
+        const double before = cost();
+        double best = before;
+        long long cells[kSizeClasses] = {};
+        int tallest[kSizeClasses] = {};
+        for (const HostBox& b : hb) {
+            const int k = size_class(b), h = b.hi[1] - b.lo[1] + 1;
+            cells[k] += (long long)(b.hi[0] - b.lo[0] + 1) * h;
+            tallest[k] = std::max(tallest[k], h);
+        }
+        int order[kSizeClasses];
+        for (int k = 0; k < kSizeClasses; ++k) order[k] = k;
+        std::stable_sort(order, order + kSizeClasses, [&](int x, int y) { return cells[x] > cells[y]; });
+        for (int k : order) {
+            if (!cells[k]) continue;
+            int keep = 0;
+            for (int rows : {4, 8, 16, 32, 64}) {
+                class_rows[k] = rows;
+                ++tune_gen;
+                const double t = cost();
+                if (t < 0.99 * best) {
+                    best = t;
+                    keep = rows;
+                }
+                if (rows >= tallest[k]) break;  // deeper items cut this class no differently
+            }
+            class_rows[k] = keep;
+            ++tune_gen;
+        }
+        const int resident = stage_waves_per_simd(hx) * cu_count;
+        for (int kind = 0; kind < (two ? 2 : 1); ++kind) {
+            const int epi = kind == 0 ? epi_first : epi_last;
+            const StageArgs& a = kind == 0 ? a1 : a2;
+            const int slot = two ? kind : 1;
+            double tb = time_kernel(epi, a);
+            int keep = 0;
+            for (int g : {resident, 2 * resident, 4 * resident}) {
+                grid_blocks[slot] = g;
+                const double t = time_kernel(epi, a);
+                if (t < 0.99 * tb) {
+                    tb = t;
+                    keep = g;
+                }
+            }
+            grid_blocks[slot] = keep;
+        }
+        if (!two) grid_blocks[0] = grid_blocks[1];
+        const double after = cost();
+        for (auto& x : e) hipEventDestroy(x);
+        if (tmp) {
+            std::vector<BoxDesc> bd(hb.size());
+            for (size_t k = 0; k < hb.size(); ++k)
+                bd[k] = desc_of(hb[k], d_pool + hb[k].off_q[0], d_pool + hb[k].off_q[1], d_pool + hb[k].off_q[2]);
+            upload(d_boxes, bd.data(), bd.size() * sizeof(BoxDesc));
+            ZAMR_HIP(hipStreamSynchronize(stream));
+            ZAMR_HIP(hipFree(tmp));
+        }
+        maxinv_fresh = false;
+        invalidate_graph();
+        return {before, after};
     }
 
     // ---------------- reference-shaped entry points ----------------

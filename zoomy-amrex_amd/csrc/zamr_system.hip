@@ -459,6 +459,50 @@ int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
     }
     return ZAMR_OK;
 }
+int zamr_set_graph(zamr_ctx* c, int on) {
+    if (!c) return ZAMR_EINVAL;
+    c->s.use_graph = on != 0;
+    c->s.invalidate_graph();
+    if (c->amr && c->amr->L1) c->amr->L1->use_graph = on != 0;
+    return ZAMR_OK;
+}
+int zamr_set_launch_shape(zamr_ctx* c, const int* class_rows, const int* grid_blocks) {
+    if (!c || !class_rows || !grid_blocks) return ZAMR_EINVAL;
+    for (int k = 0; k < Sol::kSizeClasses; ++k)
+        if (class_rows[k] < 0 || class_rows[k] > 4096) return ZAMR_EINVAL;
+    if (grid_blocks[0] < 0 || grid_blocks[1] < 0) return ZAMR_EINVAL;
+    if (c->amr) {
+        c->err = "launch shapes are set on single-level and box-list contexts; AMR levels keep the analytic rule";
+        return ZAMR_EUNSUP;
+    }
+    for (int k = 0; k < Sol::kSizeClasses; ++k) c->s.class_rows[k] = class_rows[k];
+    c->s.grid_blocks[0] = grid_blocks[0];
+    c->s.grid_blocks[1] = grid_blocks[1];
+    c->s.rows_per_item = 0;
+    ++c->s.tune_gen;
+    c->s.invalidate_graph();
+    return ZAMR_OK;
+}
+int zamr_autotune(zamr_ctx* c, int reps, int* class_rows, int* grid_blocks, double* ms) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (c->amr) {
+        c->err = "the tuner runs on single-level and box-list contexts; AMR levels keep the analytic rule";
+        return ZAMR_EUNSUP;
+    }
+    return guard(c, [&] {
+        const auto t = c->s.autotune(reps);
+        if (class_rows)
+            for (int k = 0; k < Sol::kSizeClasses; ++k) class_rows[k] = c->s.class_rows[k];
+        if (grid_blocks) {
+            grid_blocks[0] = c->s.grid_blocks[0];
+            grid_blocks[1] = c->s.grid_blocks[1];
+        }
+        if (ms) {
+            ms[0] = t.first;
+            ms[1] = t.second;
+        }
+    });
+}
 int zamr_set_profiling(zamr_ctx* c, int on) {
     if (!c) return ZAMR_EINVAL;
     c->s.profiling = on != 0;

@@ -69,6 +69,9 @@ def load_library(path):
                             ctypes.POINTER(_VP)],
         "zamr_set_stream": [_VP, _VP],
         "zamr_set_tuning": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "zamr_set_graph": [_VP, ctypes.c_int],
+        "zamr_autotune": [_VP, ctypes.c_int, _VP, _VP, _VP],
+        "zamr_set_launch_shape": [_VP, _VP, _VP],
         "zamr_set_profiling": [_VP, ctypes.c_int],
         "zamr_kernel_times": [_VP, _D, _I],
         "zamr_dt_history": [_VP, ctypes.c_int, _D, _D, _I],
@@ -276,6 +279,26 @@ class ZoomyAmr:
 
     def set_tuning(self, rows=0, per_box_launch=0, xcd_remap=1):
         self._check(self.lib.zamr_set_tuning(self.ctx, rows, per_box_launch, xcd_remap))
+
+    def set_graph(self, on=True):
+        self._check(self.lib.zamr_set_graph(self.ctx, 1 if on else 0))
+
+    def autotune(self, reps=10):
+        """Measure the stage kernel's launch shape (include/zamr.h
+        zamr_autotune); returns (rows per size class, grid blocks,
+        (ms before, ms after))."""
+        rows = np.zeros(8, dtype=np.int32)
+        g = np.zeros(2, dtype=np.int32)
+        ms = np.zeros(2)
+        self._check(self.lib.zamr_autotune(self.ctx, reps, rows.ctypes.data, g.ctypes.data, _dp(ms)))
+        return rows.tolist(), g.tolist(), (float(ms[0]), float(ms[1]))
+
+    def set_launch_shape(self, class_rows, grid_blocks=(0, 0)):
+        rows = np.ascontiguousarray(class_rows, dtype=np.int32)
+        g = np.ascontiguousarray(grid_blocks, dtype=np.int32)
+        if rows.shape != (8,) or g.shape != (2,):
+            raise ValueError("class_rows has 8 entries, grid_blocks 2")
+        self._check(self.lib.zamr_set_launch_shape(self.ctx, rows.ctypes.data, g.ctypes.data))
 
     def set_profiling(self, on=True):
         self._check(self.lib.zamr_set_profiling(self.ctx, 1 if on else 0))

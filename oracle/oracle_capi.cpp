@@ -193,19 +193,26 @@ void* orc_amr_create(const int* is, const double* ds, const double* params, cons
     a.blocking_factor = ai[2];
     a.n_error_buf = ai[3];
     a.fine_max_grid_size = ai[4];
+    a.n_proper = ai[5];
+    a.strip = ai[6];
     a.grid_eff = ad[0];
     a.tag_threshold = ad[1];
     a.tag_b_max = ad[2];
-    auto* o = new OAmr(settings_from(is, ds), params, a);
+    OAmr* o = nullptr;
+    try {
+        o = new OAmr(settings_from(is, ds), params, a);
+    } catch (const std::exception&) {
+        return nullptr;  // amr.max_level or amr.ref_ratio out of range
+    }
     o->nthreads = is[16] > 0 ? is[16] : 1;
-    o->L0->nthreads = o->nthreads;
+    o->L[0]->nthreads = o->nthreads;
     return o;
 }
 void orc_amr_destroy(void* h) { delete static_cast<OAmr*>(h); }
 void orc_amr_init_solution(void* h) {
     auto* o = static_cast<OAmr*>(h);
-    o->L0->init_solution(o->L0->Q);
-    o->L0->init_solution(o->L0->Qaux);
+    o->L[0]->init_solution(o->L[0]->Q);
+    o->L[0]->init_solution(o->L[0]->Qaux);
 }
 void orc_amr_init_levels(void* h) { static_cast<OAmr*>(h)->init_levels(); }
 int orc_amr_finest(void* h) { return static_cast<OAmr*>(h)->finest_level(); }
@@ -248,17 +255,18 @@ void orc_amr_time_step(void* h, double time, int iteration, double* dt) {
     if (dt) *dt = o->dt_level[0];
 }
 double orc_amr_total_mass(void* h) { return static_cast<OAmr*>(h)->total_mass(); }
-void orc_amr_regrid(void* h, double time) { static_cast<OAmr*>(h)->regrid(time); }
-void orc_amr_average_down(void* h) { static_cast<OAmr*>(h)->average_down(); }
+void orc_amr_regrid(void* h, int lbase, double time) { static_cast<OAmr*>(h)->regrid(lbase, time); }
+void orc_amr_average_down(void* h, int lev) { static_cast<OAmr*>(h)->average_down(lev); }
 long orc_amr_counts(void* h, int which) {
     auto* o = static_cast<OAmr*>(h);
     if (which == 0) return o->regrids;
     if (which == 1) return o->remakes;
-    return o->L1 ? o->L1->mood_redos : 0;
+    const int lev = which - 1;  // 2: level 1, 3: level 2, ...
+    return lev <= o->finest_level() ? o->level(lev).mood_redos : 0;
 }
-// ErrorEst of level 0 as it stands, one byte per cell (raster order).
-void orc_amr_error_est(void* h, unsigned char* tags) {
-    auto t = static_cast<OAmr*>(h)->error_est();
+// ErrorEst of level lev as it stands, one byte per cell (raster order).
+void orc_amr_error_est(void* h, int lev, unsigned char* tags) {
+    auto t = static_cast<OAmr*>(h)->error_est(lev);
     std::memcpy(tags, t.data(), t.size());
 }
 // Level-1 boxes MakeNewGrids derives from a level-0 tag map.
@@ -274,8 +282,8 @@ int orc_amr_make_grids(void* h, const unsigned char* tags, int* out, int cap) {
     }
     return (int)bx.size();
 }
-// cell_cons_interp of level 0 at level-1 cell (i, j).
-void orc_amr_interp(void* h, int i, int j, double* out) { static_cast<OAmr*>(h)->interp(i, j, out); }
+// cell_cons_interp of level lev-1 at level-lev cell (i, j).
+void orc_amr_interp(void* h, int lev, int i, int j, double* out) { static_cast<OAmr*>(h)->interp(lev, i, j, out); }
 // Berger-Rigoutsos clusters of n points (x, y pairs) at efficiency eff.
 int orc_cluster(const int* pts, int n, double eff, int* out, int cap) {
     std::vector<zorc::IV> v(n);

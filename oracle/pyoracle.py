@@ -78,9 +78,11 @@ def load(system):
     V = ctypes.c_void_p
     lib.orc_amr_create.restype = V
     lib.orc_amr_create.argtypes = [_I, _D, _D, _I, _D]
-    for name in ("orc_amr_destroy", "orc_amr_init_solution", "orc_amr_init_levels", "orc_amr_average_down"):
+    for name in ("orc_amr_destroy", "orc_amr_init_solution", "orc_amr_init_levels"):
         getattr(lib, name).restype = None
         getattr(lib, name).argtypes = [V]
+    lib.orc_amr_average_down.restype = None
+    lib.orc_amr_average_down.argtypes = [V, ctypes.c_int]
     lib.orc_amr_finest.restype = ctypes.c_int
     lib.orc_amr_finest.argtypes = [V]
     lib.orc_amr_boxes.restype = ctypes.c_int
@@ -94,15 +96,15 @@ def load(system):
     lib.orc_amr_total_mass.restype = ctypes.c_double
     lib.orc_amr_total_mass.argtypes = [V]
     lib.orc_amr_regrid.restype = None
-    lib.orc_amr_regrid.argtypes = [V, ctypes.c_double]
+    lib.orc_amr_regrid.argtypes = [V, ctypes.c_int, ctypes.c_double]
     lib.orc_amr_counts.restype = ctypes.c_long
     lib.orc_amr_counts.argtypes = [V, ctypes.c_int]
     lib.orc_amr_error_est.restype = None
-    lib.orc_amr_error_est.argtypes = [V, _U8]
+    lib.orc_amr_error_est.argtypes = [V, ctypes.c_int, _U8]
     lib.orc_amr_make_grids.restype = ctypes.c_int
     lib.orc_amr_make_grids.argtypes = [V, _U8, _I, ctypes.c_int]
     lib.orc_amr_interp.restype = None
-    lib.orc_amr_interp.argtypes = [V, ctypes.c_int, ctypes.c_int, _D]
+    lib.orc_amr_interp.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
     lib.orc_cluster.restype = ctypes.c_int
     lib.orc_cluster.argtypes = [_I, ctypes.c_int, ctypes.c_double, _I, ctypes.c_int]
     lib.orc_amr_evolve.restype = ctypes.c_int
@@ -164,10 +166,13 @@ class Oracle:
             a = dict(AMR_DEFAULTS)
             a.update(amr)
             self.amr = a
+            strip = a["strip"] if a["strip"] is not None else 64 - 2 * (2 if order == 2 else 1)
             ai = np.array([a["max_level"], a["ref_ratio"], a["blocking_factor"], a["n_error_buf"],
-                           a["fine_max_grid_size"]], dtype=np.int32)
+                           a["fine_max_grid_size"], a["n_proper"], strip], dtype=np.int32)
             ad = np.array([a["grid_eff"], a["threshold"], a["b_max"]], dtype=np.float64)
             self.h = lib.orc_amr_create(iset.ctypes.data_as(_I), _dp(dset), _dp(p), ai.ctypes.data_as(_I), _dp(ad))
+            if not self.h:
+                raise ValueError("amr.max_level must be 0..3 and amr.ref_ratio 2 or 4")
 
     def param_index(self, name):
         return self.param_names.index(name)
@@ -252,13 +257,15 @@ class Oracle:
 
 
 AMR_DEFAULTS = dict(max_level=1, ref_ratio=2, blocking_factor=2, n_error_buf=1, fine_max_grid_size=64,
-                    grid_eff=0.7, threshold=0.01, b_max=1.0e30)
+                    grid_eff=0.7, threshold=0.01, b_max=1.0e30, n_proper=1, strip=None)
 
 
 class AmrOracle(Oracle):
-    """A two-level run (levels 0 and 1) of the AMR TimeStep; amr keys:
-    max_level, ref_ratio, blocking_factor, n_error_buf, fine_max_grid_size,
-    grid_eff, threshold (tagging.threshold), b_max (tagging.b_max)."""
+    """A run of the AMR TimeStep over levels 0 .. max_level (at most 3); amr
+    keys: max_level, ref_ratio (2 or 4), blocking_factor, n_error_buf,
+    n_proper, fine_max_grid_size, strip (x cut of fine boxes into whole
+    stage strips; default the device kernel's strip width), grid_eff,
+    threshold (tagging.threshold), b_max (tagging.b_max)."""
 
     def __init__(self, system, case, threads=1, amr=None):
         super().__init__(system, case, threads=threads, amr=dict(amr or {}))
@@ -305,19 +312,22 @@ class AmrOracle(Oracle):
     def total_mass(self):
         return self.lib.orc_amr_total_mass(self.h)
 
-    def regrid(self, time=0.0):
-        self.lib.orc_amr_regrid(self.h, time)
+    def regrid(self, time=0.0, lbase=0):
+        self.lib.orc_amr_regrid(self.h, lbase, time)
 
-    def average_down(self):
-        self.lib.orc_amr_average_down(self.h)
+    def average_down(self, lev=0):
+        self.lib.orc_amr_average_down(self.h, lev)
 
     def counts(self):
         """(regrids, remakes, level-1 MOOD redos)."""
         return tuple(self.lib.orc_amr_counts(self.h, k) for k in range(3))
 
-    def error_est(self):
-        t = np.zeros((self.ny, self.nx), dtype=np.uint8)
-        self.lib.orc_amr_error_est(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    def mood_redos(self, lev):
+        return self.lib.orc_amr_counts(self.h, lev + 1)
+
+    def error_est(self, lev=0):
+        t = np.zeros(self.level_shape(lev), dtype=np.uint8)
+        self.lib.orc_amr_error_est(self.h, lev, t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
         return t
 
     def make_grids(self, tags):
@@ -327,9 +337,9 @@ class AmrOracle(Oracle):
                                         out.ctypes.data_as(_I), 65536)
         return [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(n)]
 
-    def interp(self, i, j):
+    def interp(self, i, j, lev=1):
         v = np.zeros(self.nq)
-        self.lib.orc_amr_interp(self.h, i, j, _dp(v))
+        self.lib.orc_amr_interp(self.h, lev, i, j, _dp(v))
         return v
 
     def evolve(self, step_limit=-1, cap=100000):

@@ -34,7 +34,7 @@ enum {
     ZAMR_ETAG = 2,      /* boundary tag not declared by the model (ZoomyAmr.cpp:361-372) */
     ZAMR_ERASTER = 3,   /* raster missing or too small (init_solution.cpp:20-40) */
     ZAMR_EHIP = 4,      /* HIP runtime failure */
-    ZAMR_EUNSUP = 5,    /* a deck option outside this build (e.g. amr.max_level > 1) */
+    ZAMR_EUNSUP = 5,    /* a deck option outside this build (e.g. amr.max_level > 3) */
     ZAMR_EIO = 6        /* plotfile / file write failure */
 };
 
@@ -60,12 +60,12 @@ int zamr_default_param(int k, double* value);
 int zamr_march_constants(int* c_cfl_dimension, int* c_cfl_degree_factor);
 
 /* ZoomyAmr::ZoomyAmr() (ZoomyAmr.cpp:27-107): reads the ParmParse deck text
- * written by build.py:write_inputs (build.py:55-120).  amr.max_level = 1 sets
- * up the two-level hierarchy (AmrCore with ref_ratio 2, a 2-D model; the
- * amr.blocking_factor / n_error_buf / grid_eff / max_grid_size and tagging.*
- * keys apply).  solver.implicit_global = true solves the source globally
+ * written by build.py:write_inputs (build.py:55-120).  amr.max_level = 1..3
+ * sets up the hierarchy (AmrCore with amr.ref_ratio 2 or 4, a 2-D model; the
+ * amr.blocking_factor / n_error_buf / n_proper / grid_eff / max_grid_size and
+ * tagging.* keys apply).  solver.implicit_global = true solves the source globally
  * after each stage (Source/imex_solver.H: Newton + GMRES, single level).
- * Keys outside this build (amr.max_level > 1, ref_ratio != 2, implicit_global
+ * Keys outside this build (amr.max_level > 3, ref_ratio not 2 or 4, implicit_global
  * with MOOD, AMR or a box list) fail with ZAMR_EUNSUP. */
 int zamr_create(const char* inputs_text, zamr_ctx** out);
 /* Batched mode: nbox independent boxes sharing one launch per RK stage.
@@ -112,8 +112,8 @@ int zamr_stage(zamr_ctx* ctx, int lev, int order, double time, double dt);
 /* ZoomyAmr::Advance (ZoomyAmr.cpp:538-731) with a caller-given dt. */
 int zamr_advance(zamr_ctx* ctx, int lev, double time, double dt);
 /* ZoomyAmr::TimeStep (ZoomyAmr.cpp:733-762): (AMR: regrid when iteration > 0)
- * ComputeDt, time_end cap, Advance (AMR: then two level-1 substeps and
- * average_down).  On return *dt_taken holds the level-0 step size. */
+ * ComputeDt, time_end cap, Advance (AMR: then ref_ratio substeps of the next
+ * level, each the same TimeStep one level up, and average_down).  On return *dt_taken holds the level-0 step size. */
 int zamr_time_step(zamr_ctx* ctx, int lev, double time, int iteration, double* dt_taken);
 /* Up to nsteps iterations of the Evolve loop body (ZoomyAmr.cpp:782-810) with
  * dt computed and applied on the device; steps are captured in a hipGraph.
@@ -180,8 +180,8 @@ int zamr_kernel_times(zamr_ctx* ctx, double* out_ms, int* nsteps);
  * order 1 since creation. */
 int zamr_mood_stats(zamr_ctx* ctx, int group, int* troubled, long long* redos);
 
-/* ---- AMR hierarchy (amr.max_level = 1) ---------------------------------- */
-/* finest_level (AmrCore): 1 while level 1 has boxes, else 0. */
+/* ---- AMR hierarchy (amr.max_level = 1..3) ------------------------------- */
+/* finest_level (AmrCore): the highest level that has boxes. */
 int zamr_finest_level(zamr_ctx* ctx, int* finest);
 /* BoxArray of level lev (boxArray(lev)) as (lo0, lo1, hi0, hi1) quadruples in
  * the level's index space, up to cap boxes; *nboxes receives the count. */
@@ -189,8 +189,9 @@ int zamr_level_boxes(zamr_ctx* ctx, int lev, int* lohi, int cap, int* nboxes);
 /* The level loop of InitData (ZoomyAmr.cpp:137-155) on the current level-0
  * state, for a state set with zamr_set_state after zamr_init_data. */
 int zamr_init_levels(zamr_ctx* ctx);
-/* AmrCore::regrid(0, time) (ZoomyAmr.cpp:736): ErrorEst on level 0, new
- * level-1 grids, RemakeLevel / MakeNewLevelFromCoarse / ClearLevel. */
+/* AmrCore::regrid(0, time) (ZoomyAmr.cpp:736): ErrorEst from the finest
+ * level that may tag down to level 0, new grids for every level above 0
+ * (proper nesting), RemakeLevel / MakeNewLevelFromCoarse / ClearLevel. */
 int zamr_regrid(zamr_ctx* ctx, double time);
 /* ZoomyAmr::ErrorEst (ZoomyAmr.cpp:225-253) on level 0 as it stands: one byte
  * per cell (1 = tagged), raster order, n = nx * ny. */

@@ -1,28 +1,37 @@
 #pragma once
-// Two-level AMR advance of Source/ZoomyAmr.cpp on the device: ErrorEst
-// tagging (:225-253), regrid with RemakeLevel / MakeNewLevelFromCoarse /
-// ClearLevel (:161-219, AmrCore::regrid), the coarse-fine part of FillPatch
-// and FillCoarsePatch with cell_cons_interp (:258-288), the subcycled
-// TimeStep with average_down (:733-762) and the composite ComputeTotalMass
-// (:517-536).  Level 0 is the single-level Solver; level 1 is a second
-// Solver over the refined domain whose boxes follow the tags.
+// The AMR advance of Source/ZoomyAmr.cpp on the device, levels 0 ..
+// amr.max_level <= 3 at one amr.ref_ratio of 2 or 4: ErrorEst tagging
+// (:225-253), regrid with RemakeLevel / MakeNewLevelFromCoarse / ClearLevel
+// (:161-219, AmrCore::regrid), the coarse-fine part of FillPatch and
+// FillCoarsePatch with cell_cons_interp (:258-288), nsubsteps = ref_ratio
+// (:96-106), the recursive subcycled TimeStep with average_down (:733-762)
+// and the composite ComputeTotalMass (:517-536).  Level 0 is the single-level
+// Solver; every refined level is a further Solver over its refined domain
+// whose boxes follow the tags.  Two levels at ratio 2 take a path of their
+// own that overlaps the regrid with level 0's step; deeper hierarchies and
+// ratio 4 run the levels one at a time in the reference's order.
 //
 // The AMR machinery is AMReX's (external to the reference, not vendored);
 // its published algorithms are restated here in device code and host code:
 // CellConservativeLinear with linear limiting, average_down, TagBoxArray
-// buffering and coarsening, Berger-Rigoutsos clustering (ClusterList::chop).
+// buffering and coarsening, MakeNewGrids' proper nesting (the coarsened
+// lbase grids shrunk by n_proper, projected finer grids tagged below),
+// Berger-Rigoutsos clustering (ClusterList::chop).
 //
 // Where the reference reads memory it never wrote, this build fixes a value:
 // coarse cells beyond the domain in the interpolation stencil repeat the edge
-// cell (the foextrap its BCRec declares; its fill functor is NullFill);
-// ErrorEst before any stage reads ghost cells as 0; a new or remade fine
-// level's Qaux is update_aux_variables(Q, 0) (the reference leaves it unset
-// and ComputeDt reads it next).
+// cell (the foextrap its BCRec declares; its fill functor is NullFill), and
+// a stencil neighbour the coarse level does not cover repeats the centre;
+// ErrorEst before any stage reads ghost cells as 0, and on a level made or
+// remade since its last stage reads the ghosts that fill left (0 beyond the
+// domain); a new or remade fine level's Qaux is update_aux_variables(Q, 0)
+// (the reference leaves it unset and ComputeDt reads it next).
 //
-// Data movement: the coarse state is fixed while the fine level subcycles
-// (FillPatchTwoLevels is handed the coarse level at t_new only), so the
-// coarse-fine ghost cells are interpolated once per coarse step into every
-// fine state buffer; stage kernels never write ghost cells, so they stay.
+// Data movement: the coarse state is fixed while a finer level subcycles
+// (FillPatchTwoLevels is handed the coarse level at t_new only), so its
+// coarse-fine ghost cells are interpolated into every fine state buffer once
+// per coarse step (two levels) or once per fine step after the regrid that
+// reads the old ones (deeper); stage kernels never write ghost cells.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -272,6 +281,192 @@ __global__ void __launch_bounds__(256) tag_buffer_kernel(const unsigned char* __
     if (__any(in && t != prev[k]) && (threadIdx.x & (WV - 1)) == 0 &&
         __hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
         atomicOr(changed, 1u);
+}
+
+// ---------------- levels above 1, and ref_ratio 4 ----------------
+// A refined level as the coarse side of the next one: its boxes and an owner
+// map (the box of every cell of the level, -1 where no box is).
+struct OwnerGrid {
+    const BoxDesc* boxes;
+    const int* owner;
+    int nx, ny;
+};
+
+__device__ __forceinline__ int level_box(const CoarseGrid& g, int ic, int jc) { return coarse_box_index(g, ic, jc); }
+__device__ __forceinline__ int level_box(const OwnerGrid& g, int ic, int jc) {
+    return g.owner[(long long)jc * g.nx + ic];
+}
+
+// All nq components of cell (ic, jc) of a level (clamped into the domain);
+// false where the level has no box.
+template <int NQ, class G>
+__device__ __forceinline__ bool level_cell(const G& g, int buf, int ic, int jc, double* v) {
+    ic = ic < 0 ? 0 : (ic >= g.nx ? g.nx - 1 : ic);
+    jc = jc < 0 ? 0 : (jc >= g.ny ? g.ny - 1 : jc);
+    const int b = level_box(g, ic, jc);
+    if (b < 0) return false;
+    const BoxView C = view(g.boxes, b);
+    const double* __restrict__ q = qsel(C, buf);
+    const long long idx = (long long)(ic - C.gl0) + (long long)(jc - C.gl1) * C.sy;
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) v[n] = q[idx + n * C.sc];
+    return true;
+}
+
+// cell_cons_interp at ratio r (cf_interp_kernel's rule) of coarse level g
+// into the listed cells of a fine level, written to every fine buffer in
+// `bufmask`.  With `fine_owner` (the fine level's owner map, fnx cells a
+// row) cells a fine box covers are skipped.  A stencil neighbour the coarse
+// level does not cover (proper nesting leaves at most one such cell beyond
+// it) repeats the stencil's centre, which is always covered; a centre that
+// is not leaves the cell alone.
+template <class M, class G>
+__global__ void __launch_bounds__(256) interp_r_kernel(const BoxDesc* __restrict__ fine,
+                                                       const CellRect* __restrict__ rects, int nrects, long long total,
+                                                       G g, int r, int cbuf, unsigned bufmask,
+                                                       const int* __restrict__ fine_owner, int fnx) {
+#pragma clang fp contract(off)
+    constexpr int nq = M::n_dof_q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const int cell = (int)(k - R.start);
+    const int i = R.ilo + cell % R.ni, j = R.jlo + cell / R.ni;
+    if (fine_owner && fine_owner[(long long)j * fnx + i] >= 0) return;
+    const int ic = i / r, jc = j / r;  // i, j >= 0: inside the domain
+    double u0[nq], sx[nq], sy[nq], xm[nq], xp[nq], ym[nq], yp[nq];
+    if (!level_cell<nq>(g, cbuf, ic, jc, u0)) return;
+    if (!level_cell<nq>(g, cbuf, ic - 1, jc, xm))
+        for (int n = 0; n < nq; ++n) xm[n] = u0[n];
+    if (!level_cell<nq>(g, cbuf, ic + 1, jc, xp))
+        for (int n = 0; n < nq; ++n) xp[n] = u0[n];
+    if (!level_cell<nq>(g, cbuf, ic, jc - 1, ym))
+        for (int n = 0; n < nq; ++n) ym[n] = u0[n];
+    if (!level_cell<nq>(g, cbuf, ic, jc + 1, yp))
+        for (int n = 0; n < nq; ++n) yp[n] = u0[n];
+    double fx = 1.0, fy = 1.0;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+        const double u = u0[n];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double um = d == 0 ? xm[n] : ym[n];
+            const double up = d == 0 ? xp[n] : yp[n];
+            const double dc = 0.5 * (up - um);
+            const double df = 2.0 * (up - u);
+            const double db = 2.0 * (u - um);
+            double sl = (df * db >= 0.0) ? dmin(fabs(df), fabs(db)) : 0.0;
+            sl = copysign(1.0, dc) * dmin(sl, fabs(dc));
+            if (dc != 0.0) {
+                if (d == 0) fx = dmin(fx, sl / dc);
+                else fy = dmin(fy, sl / dc);
+            }
+            if (d == 0) sx[n] = dc;
+            else sy[n] = dc;
+        }
+    }
+    const double rr = (double)r;
+    const double xoff = ((i - r * ic) + 0.5) / rr - 0.5;
+    const double yoff = ((j - r * jc) + 0.5) / rr - 0.5;
+    const BoxView F = view(fine, R.box);
+    const long long idx = (long long)(i - F.gl0) + (long long)(j - F.gl1) * F.sy;
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+        const double v = u0[n] + xoff * (sx[n] * fx) + yoff * (sy[n] * fy);
+        if (bufmask & 1u) F.q0[idx + n * F.sc] = v;
+        if (bufmask & 2u) F.q1[idx + n * F.sc] = v;
+        if (bufmask & 4u) F.q2[idx + n * F.sc] = v;
+    }
+}
+
+// average_down at ratio r into coarse level g: the mean of the r x r
+// children, summed j-outer, i-inner, times 1/r^2.  Rectangles in the coarse
+// index space, one per fine box.
+template <class M, class G>
+__global__ void __launch_bounds__(256) average_down_r_kernel(const BoxDesc* __restrict__ fine,
+                                                             const CellRect* __restrict__ rects, int nrects,
+                                                             long long total, G g, int r, int fbuf, int cbuf) {
+#pragma clang fp contract(off)
+    constexpr int nq = M::n_dof_q;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const int cell = (int)(k - R.start);
+    const int ic = R.ilo + cell % R.ni, jc = R.jlo + cell / R.ni;
+    const int b = level_box(g, ic, jc);
+    if (b < 0) return;  // proper nesting puts every fine box over coarse boxes
+    const BoxView F = view(fine, R.box);
+    const BoxView C = view(g.boxes, b);
+    const double* __restrict__ qf = qsel(F, fbuf);
+    double* __restrict__ qc = qsel(C, cbuf);
+    const long long f00 = (long long)(r * ic - F.gl0) + (long long)(r * jc - F.gl1) * F.sy;
+    const long long ci = (long long)(ic - C.gl0) + (long long)(jc - C.gl1) * C.sy;
+    const double vol = 1.0 / (double)(r * r);
+    for (int n = 0; n < nq; ++n) {
+        double sum = 0.0;
+        for (int jj = 0; jj < r; ++jj)
+            for (int ii = 0; ii < r; ++ii) sum += qf[f00 + n * F.sc + (long long)jj * F.sy + ii];
+        qc[ci + n * C.sc] = vol * sum;
+    }
+}
+
+// ErrorEst on a refined level, over the listed valid cells of its boxes,
+// into a tag map of nx cells a row.  Neighbours outside the cell's box:
+// after a stage (last >= 0) the box's ghost cells as that stage's FillPatch
+// stored them, pushed through UpdateState as the end of Advance leaves them
+// (the physical-BC ghosts as S::load rebuilds them); on a level made or
+// remade since its last stage (last < 0) the ghost cells inside the domain
+// as the fill that made it stored them, and 0 beyond the domain.
+template <class M>
+__global__ void __launch_bounds__(256) error_est_boxes_kernel(const BoxDesc* __restrict__ boxes,
+                                                              const CellRect* __restrict__ rects, int nrects,
+                                                              long long total, int cur, int last, Params pp, int clamp,
+                                                              double time_last, double threshold, double b_max,
+                                                              unsigned char* __restrict__ tags, int nx) {
+#pragma clang fp contract(off)
+    using S = Sys<M>;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const int cell = (int)(k - R.start);
+    const int i = R.ilo + cell % R.ni, j = R.jlo + cell / R.ni;
+    typename S::VP p;
+    for (int q = 0; q < S::np; ++q) p(q, 0) = pp.v[q];
+    const BoxView B = view(boxes, R.box);
+    auto h_at = [&](int c, int rw) -> double {
+        const long long idx = (long long)(c - B.gl0) + (long long)(rw - B.gl1) * B.sy;
+        if (c >= B.lo0 && c <= B.hi0 && rw >= B.lo1 && rw <= B.hi1) return qsel(B, cur)[idx + B.sc];
+        if (last < 0) {
+            const bool in = c >= B.dlo0 && c <= B.dhi0 && (S::dim < 2 || (rw >= B.dlo1 && rw <= B.dhi1));
+            return in ? qsel(B, cur)[idx + B.sc] : 0.0;
+        }
+        return S::update_state(S::load(B, qsel(B, last), c, rw, p, clamp, time_last), p, clamp)(1, 0);
+    };
+    const long long idx = (long long)(i - B.gl0) + (long long)(j - B.gl1) * B.sy;
+    unsigned char t = 0;
+    if (qsel(B, cur)[idx] < b_max) {
+        double grad = 0.0;
+        const double dhdx = fabs(h_at(i + 1, j) - h_at(i - 1, j));
+        grad += dhdx * dhdx;
+        if constexpr (S::dim >= 2) {
+            const double dhdy = fabs(h_at(i, j + 1) - h_at(i, j - 1));
+            grad += dhdy * dhdy;
+        }
+        if (sqrt(grad) > threshold || qsel(B, cur)[idx + B.sc] > 1.0e-2) t = 1;
+    }
+    tags[(long long)j * nx + i] = t;
+}
+
+// Owner map of a level: every cell of rectangle k (a box's valid cells)
+// takes the rectangle's box index.
+__global__ void __launch_bounds__(256) paint_owner_kernel(const CellRect* __restrict__ rects, int nrects,
+                                                          long long total, int* __restrict__ owner, int nx) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const CellRect R = rects[find_rect(rects, nrects, k)];
+    const int cell = (int)(k - R.start);
+    const int i = R.ilo + cell % R.ni, j = R.jlo + cell / R.ni;
+    owner[(long long)j * nx + i] = R.box;
 }
 
 // ---------------- Berger-Rigoutsos on the device ----------------
@@ -654,7 +849,7 @@ class AmrDriver {
 
     Sol& L0;
     std::unique_ptr<Sol> L1;
-    int max_level = 1, ref_ratio = 2, blocking_factor = 8, n_error_buf = 1;
+    int max_level = 1, ref_ratio = 2, blocking_factor = 8, n_error_buf = 1, n_proper = 1;
     double grid_eff = 0.7, tag_threshold = 0.01, tag_b_max = 1.0e30;
     int fine_max_grid_size = 64;
 
@@ -695,6 +890,29 @@ class AmrDriver {
     int n_avg = 0;
     long long avg_total = 0;
 
+    // Hierarchies deeper than two levels or at ref_ratio 4 (the general
+    // path): levels 2 .. finest, and per level 0 .. max_level its tag maps,
+    // owner map, coverage by the next level, launch lists and clock.
+    bool general = false;
+    std::vector<std::unique_ptr<Sol>> up;
+    struct GLev {
+        int nx = 0, ny = 0, cnx = 0, cny = 0;
+        unsigned char* d_tag = nullptr;   // ErrorEst (tagging levels)
+        unsigned char* d_ctag = nullptr;  // buffered and coarsened
+        int* d_owner = nullptr;           // box of every cell (levels >= 1), -1 where none
+        unsigned char* d_cov = nullptr;   // cells under the next finer level
+        CellRect* d_valid = nullptr;      // valid cells of every box
+        CellRect* d_ghost = nullptr;      // in-domain ghost strips of every box
+        CellRect* d_avg = nullptr;        // cells of the level below under every box
+        int4* d_fb = nullptr;             // the boxes coarsened into the level below
+        size_t valid_cap = 0, ghost_cap = 0, avg_cap = 0, fb_cap = 0;
+        int n_valid = 0, n_ghost = 0, n_avg = 0;
+        long long valid_total = 0, ghost_total = 0, avg_total = 0;
+        bool max_stale = true;            // CFL maxima must be reduced from the stored state
+        double t_last = 0.0;              // start time of the level's last step
+    };
+    std::vector<GLev> gl;
+
     bool l0_stage_ran = false;  // level-0 ghosts hold a FillPatch
     double t_last_stage = 0.0;  // the time that FillPatch ran at (its step's start)
     bool l0_max_stale = true;   // level-0 CFL max must be reduced from the stored state
@@ -710,14 +928,17 @@ class AmrDriver {
         pp.query("amr.ref_ratio", ref_ratio);
         pp.query("amr.blocking_factor", blocking_factor);
         pp.query("amr.n_error_buf", n_error_buf);
+        pp.query("amr.n_proper", n_proper);
         pp.query("amr.grid_eff", grid_eff);
         pp.query("amr.max_grid_size", fine_max_grid_size);
         pp.query("tagging.threshold", tag_threshold);
         pp.query("tagging.b_max", tag_b_max);
         if (M::dimension != 2) throw Error(5, "AMR (amr.max_level > 0) is built for two-dimensional models only");
         if (L0.jfnk_mode()) throw Error(5, "AMR with solver.implicit_global = true is not part of this build");
-        if (ref_ratio != 2) throw Error(5, "amr.ref_ratio must be 2 in this build");
-        if (blocking_factor < 1 || n_error_buf < 0 || !(grid_eff > 0.0) || grid_eff > 1.0)
+        if (ref_ratio != 2 && ref_ratio != 4) throw Error(5, "amr.ref_ratio must be 2 or 4 in this build");
+        if (max_level < 1 || max_level > 3) throw Error(5, "amr.max_level must be 1..3 in this build");
+        general = max_level > 1 || ref_ratio != 2;
+        if (blocking_factor < 1 || n_error_buf < 0 || n_proper < 0 || !(grid_eff > 0.0) || grid_eff > 1.0)
             throw Error(1, "bad amr.blocking_factor / amr.n_error_buf / amr.grid_eff");
         if (L0.nx % blocking_factor != 0 || L0.ny % blocking_factor != 0)
             throw Error(1, "amr.n_cell must be divisible by amr.blocking_factor");
@@ -762,6 +983,7 @@ class AmrDriver {
                 throw Error(1, "ZAMR_BR_CAP must read r,h,o (three integers)");
         }
         br_alloc(std::max(rcap, 1), std::max(hcap, cnx + cny), std::max(ocap, 1));
+        if (general) g_setup();
     }
 
     // Device clustering buffers at the given capacities (regions per
@@ -789,7 +1011,12 @@ class AmrDriver {
 
     ~AmrDriver() {
         hipDeviceSynchronize();  // queued copies from the pinned stage and launches on the buffers below
+        up.clear();
         L1.reset();
+        for (GLev& g : gl)
+            for (void* p : {(void*)g.d_tag, (void*)g.d_ctag, (void*)g.d_owner, (void*)g.d_cov, (void*)g.d_valid,
+                            (void*)g.d_ghost, (void*)g.d_avg, (void*)g.d_fb})
+                if (p) hipFree(p);
         for (void* p : {(void*)d_xbox, (void*)d_ybox, (void*)d_tags, (void*)d_ctags[0], (void*)d_ctags[1],
                         (void*)d_changed, (void*)d_cover, (void*)d_cf, (void*)d_avg, (void*)d_brctl,
                         (void*)d_fboxes, (void*)d_tmp_rects})
@@ -802,11 +1029,18 @@ class AmrDriver {
     }
 
     CoarseGrid grid() const { return CoarseGrid{L0.d_boxes, d_xbox, d_ybox, nbx, L0.nx, L0.ny}; }
-    int finest_level() const { return L1 ? 1 : 0; }
+    int finest_level() const { return L1 ? 1 + (int)up.size() : 0; }
     Sol& level(int l) {
         if (l == 0) return L0;
         if (l == 1 && L1) return *L1;
+        if (l >= 2 && l - 2 < (int)up.size()) return *up[l - 2];
         throw Error(1, "level " + std::to_string(l) + " does not exist");
+    }
+    std::unique_ptr<Sol>& level_ptr(int l) { return l == 1 ? L1 : up.at(l - 2); }
+    template <class F>
+    void for_fine(F&& f) {
+        if (L1) f(*L1);
+        for (auto& u : up) f(*u);
     }
     hipStream_t stream() const { return L0.stream; }
 
@@ -926,17 +1160,18 @@ class AmrDriver {
     // copy of the control block to the host per batch.  A generation that
     // would exceed a capacity stops the run; the buffers grow and it starts
     // again.
-    std::vector<std::array<int, 4>> gpu_cluster() {
+    std::vector<std::array<int, 4>> gpu_cluster() { return gpu_cluster(d_ctags[1], cnx, cny); }
+    std::vector<std::array<int, 4>> gpu_cluster(const unsigned char* map, int mx, int my) {
+        if (br.hist_cap < mx + my) br_alloc(br.reg_cap, 32 * (mx + my), br.out_cap);
         for (;;) {
-            hipLaunchKernelGGL(br_init_kernel, dim3(1), dim3(256), 0, cstream, d_brctl, br, cnx, cny);
+            hipLaunchKernelGGL(br_init_kernel, dim3(1), dim3(256), 0, cstream, d_brctl, br, mx, my);
             ZAMR_HIP(hipGetLastError());
             const int peek = std::min(br.out_cap, 1024);
             int p = 0, gens = 0;
             for (;;) {
                 const int k = gens == 0 ? std::max(br_depth, 1) : 4;
                 for (int g = 0; g < k; ++g, p ^= 1) {
-                    hipLaunchKernelGGL(br_hist_kernel, dim3(1024), dim3(256), 0, cstream, d_ctags[1], cnx, d_brctl,
-                                       br, p);
+                    hipLaunchKernelGGL(br_hist_kernel, dim3(1024), dim3(256), 0, cstream, map, mx, d_brctl, br, p);
                     hipLaunchKernelGGL(br_decide_kernel, dim3(1), dim3(BR_DECIDE_THREADS), 0, cstream, d_brctl, br,
                                        p, grid_eff);
                 }
@@ -1081,6 +1316,7 @@ class AmrDriver {
     }
 
     void regrid(int mode_last, double time_last, double time) {
+        if (general) return g_regrid(0, time);
         if (regrid_tags(mode_last, time_last)) regrid_grids(time, L0.cur);
     }
 
@@ -1127,6 +1363,7 @@ class AmrDriver {
     // The level loop of InitData (ZoomyAmr.cpp:137-155) on the current
     // level-0 state; its ghosts read as 0 there.
     void init_levels() {
+        if (general) return g_init_levels();
         L0.ensure_aux();
         have_last = false;
         for (int it = 0; it < 4; ++it) {
@@ -1147,11 +1384,12 @@ class AmrDriver {
     }
 
     // Average-down target, CFL reductions and clocks of one coarse step
-    // (ZoomyAmr::TimeStep for lev 0 and its two level-1 substeps).
+    // (ZoomyAmr::TimeStep for lev 0 and its level-1 substeps).
     // `fine_cb` runs after each fine substep (for the MOOD lines of Evolve).
     // `groups_fresh`: the caller has just downloaded level 0's group state.
     template <class CB>
     void coarse_step(int iteration, CB&& fine_cb, bool groups_fresh = false) {
+        if (general) return g_time_step(0, iteration, false);
         if (!groups_fresh) L0.download_groups();
         const double t0 = L0.h_groups[0].time;
         // The regrid at t0 comes first (ZoomyAmr.cpp:736); of it only the tags
@@ -1228,6 +1466,484 @@ class AmrDriver {
         }
     }
 
+    // ================= the general path: levels 2..3, ref_ratio 4 =================
+    // The same steps as the two-level path above, one level at a time in the
+    // reference's order (no regrid beside a step): TimeStep(lev) regrids the
+    // levels above lev (AmrCore::regrid(lev): MakeNewGrids with proper
+    // nesting, then Remake / MakeNewLevelFromCoarse / Clear), steps lev,
+    // subcycles lev + 1 ref_ratio times and averages it down.  A refined
+    // level's coarse-fine ghosts are interpolated at the start of each of its
+    // steps, after the regrid whose ErrorEst reads them as the last step left
+    // them.
+
+    int lev_nx(int l) const {
+        int v = L0.nx;
+        for (int k = 0; k < l; ++k) v *= ref_ratio;
+        return v;
+    }
+    int lev_ny(int l) const {
+        int v = L0.ny;
+        for (int k = 0; k < l; ++k) v *= ref_ratio;
+        return v;
+    }
+
+    void g_setup() {
+        gl.assign(max_level + 1, GLev{});
+        for (int l = 0; l <= max_level; ++l) {
+            GLev& g = gl[l];
+            g.nx = lev_nx(l);
+            g.ny = lev_ny(l);
+            const long long cells = (long long)g.nx * g.ny;
+            if (cells >= (1ll << 31)) throw Error(1, "amr: a refined level's index space exceeds 2^31 cells");
+            g.cnx = (g.nx + bf_lev - 1) / bf_lev;
+            g.cny = (g.ny + bf_lev - 1) / bf_lev;
+            if (l < max_level) {
+                if (g.cnx >= (1 << 18) || g.cny >= (1 << 18) || (long long)g.cnx * g.cny >= (1ll << 29))
+                    throw Error(1, "amr: a level's grid over blocking_factor / ref_ratio is too large for the clustering");
+                ZAMR_HIP(hipMalloc(&g.d_tag, (size_t)cells));
+                ZAMR_HIP(hipMalloc(&g.d_ctag, (size_t)g.cnx * g.cny));
+                ZAMR_HIP(hipMalloc(&g.d_cov, (size_t)cells));
+                ZAMR_HIP(hipMemset(g.d_cov, 0, (size_t)cells));
+            }
+            if (l >= 1) {
+                ZAMR_HIP(hipMalloc(&g.d_owner, (size_t)cells * sizeof(int)));
+                ZAMR_HIP(hipMemset(g.d_owner, 0xff, (size_t)cells * sizeof(int)));
+            }
+        }
+    }
+
+    // Upload a rectangle list into a GLev buffer (grown when short).
+    void g_upload(std::vector<CellRect>& r, CellRect*& d, size_t& cap, int& n, long long& total) {
+        upload_rects(r, d, cap, n, total);
+    }
+
+    // Launch lists, owner map and the coverage map below, of level l >= 1.
+    void g_build_lists(int l) {
+        Sol& F = level(l);
+        GLev& g = gl[l];
+        const int r = ref_ratio;
+        std::vector<CellRect> valid, ghost, avg;
+        std::vector<int4> fb;
+        auto strip = [&](int b, int i0, int j0, int i1, int j1) {
+            i0 = std::max(i0, 0);
+            j0 = std::max(j0, 0);
+            i1 = std::min(i1, g.nx - 1);
+            j1 = std::min(j1, g.ny - 1);
+            if (i0 <= i1 && j0 <= j1) ghost.push_back(CellRect{b, b, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+        };
+        for (size_t b = 0; b < F.hb.size(); ++b) {
+            const HostBox& B = F.hb[b];
+            const int n = B.ng;
+            valid.push_back(CellRect{(int)b, (int)b, B.lo[0], B.lo[1], B.hi[0] - B.lo[0] + 1, B.hi[1] - B.lo[1] + 1, 0});
+            strip((int)b, B.lo[0] - n, B.lo[1] - n, B.hi[0] + n, B.lo[1] - 1);
+            strip((int)b, B.lo[0] - n, B.hi[1] + 1, B.hi[0] + n, B.hi[1] + n);
+            strip((int)b, B.lo[0] - n, B.lo[1], B.lo[0] - 1, B.hi[1]);
+            strip((int)b, B.hi[0] + 1, B.lo[1], B.hi[0] + n, B.hi[1]);
+            avg.push_back(CellRect{(int)b, (int)b, B.lo[0] / r, B.lo[1] / r, (B.hi[0] - B.lo[0] + 1) / r,
+                                   (B.hi[1] - B.lo[1] + 1) / r, 0});
+            fb.push_back(int4{B.lo[0] / r, B.lo[1] / r, B.hi[0] / r, B.hi[1] / r});
+        }
+        g_upload(valid, g.d_valid, g.valid_cap, g.n_valid, g.valid_total);
+        g_upload(ghost, g.d_ghost, g.ghost_cap, g.n_ghost, g.ghost_total);
+        g_upload(avg, g.d_avg, g.avg_cap, g.n_avg, g.avg_total);
+        if (fb.size() > g.fb_cap) {
+            ZAMR_HIP(hipStreamSynchronize(stream()));
+            if (g.d_fb) hipFree(g.d_fb);
+            g.fb_cap = fb.size() * 2;
+            ZAMR_HIP(hipMalloc(&g.d_fb, g.fb_cap * sizeof(int4)));
+        }
+        stage_->upload(g.d_fb, fb.data(), fb.size() * sizeof(int4), stream());
+        ZAMR_HIP(hipMemsetAsync(g.d_owner, 0xff, (size_t)g.nx * g.ny * sizeof(int), stream()));
+        if (g.n_valid)
+            hipLaunchKernelGGL(paint_owner_kernel, dim3(grid1(g.valid_total)), dim3(256), 0, stream(), g.d_valid,
+                               g.n_valid, g.valid_total, g.d_owner, g.nx);
+        ZAMR_HIP(hipMemsetAsync(gl[l - 1].d_cov, 0, (size_t)gl[l - 1].nx * gl[l - 1].ny, stream()));
+        if (!fb.empty())
+            hipLaunchKernelGGL(cover_paint_kernel, dim3((unsigned)fb.size()), dim3(256), 0, stream(), g.d_fb,
+                               gl[l - 1].d_cov, gl[l - 1].nx);
+        ZAMR_HIP(hipGetLastError());
+        g.max_stale = true;
+    }
+
+    // cell_cons_interp from level l - 1's current buffer into the listed
+    // cells of level l's box set F (buffers in mask).
+    void g_interp(int l, Sol& F, CellRect* d, int n, long long total, unsigned mask, const int* fine_owner) {
+        if (!n || !total) return;
+        const Sol& C = level(l - 1);
+        if (l == 1) {
+            hipLaunchKernelGGL((interp_r_kernel<M, CoarseGrid>), dim3(grid1(total)), dim3(256), 0, stream(),
+                               F.d_boxes, d, n, total, grid(), ref_ratio, C.cur, mask, fine_owner, gl[l].nx);
+        } else {
+            const OwnerGrid og{C.d_boxes, gl[l - 1].d_owner, gl[l - 1].nx, gl[l - 1].ny};
+            hipLaunchKernelGGL((interp_r_kernel<M, OwnerGrid>), dim3(grid1(total)), dim3(256), 0, stream(),
+                               F.d_boxes, d, n, total, og, ref_ratio, C.cur, mask, fine_owner, gl[l].nx);
+        }
+        ZAMR_HIP(hipGetLastError());
+    }
+
+    // FillCoarsePatch of a level-l box set: valid and in-domain ghost cells.
+    void g_coarse_patch(int l, Sol& F) {
+        std::vector<CellRect> all;
+        for (size_t b = 0; b < F.hb.size(); ++b) {
+            const HostBox& B = F.hb[b];
+            const int i0 = std::max(0, B.lo[0] - B.ng), i1 = std::min(F.nx - 1, B.hi[0] + B.ng);
+            const int j0 = std::max(0, B.lo[1] - B.ng), j1 = std::min(F.ny - 1, B.hi[1] + B.ng);
+            all.push_back(CellRect{(int)b, (int)b, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+        }
+        int n = 0;
+        long long total = 0;
+        CellRect* d = upload_scratch(all, n, total);
+        g_interp(l, F, d, n, total, 1u << F.cur, nullptr);
+    }
+
+    // The coarse-fine ghost cells of level l before one of its steps.
+    void g_cf_ghosts(int l) {
+        Sol& F = level(l);
+        GLev& g = gl[l];
+        g_interp(l, F, g.d_ghost, g.n_ghost, g.ghost_total, F.mood() ? 7u : 3u, g.d_owner);
+    }
+
+    // average_down of level l + 1 into level l.
+    void g_average_down(int l) {
+        Sol& F = level(l + 1);
+        Sol& C = level(l);
+        const GLev& g = gl[l + 1];
+        if (!g.n_avg || !g.avg_total) return;
+        if (l == 0) {
+            hipLaunchKernelGGL((average_down_r_kernel<M, CoarseGrid>), dim3(grid1(g.avg_total)), dim3(256), 0,
+                               stream(), F.d_boxes, g.d_avg, g.n_avg, g.avg_total, grid(), ref_ratio, F.cur, C.cur);
+        } else {
+            const OwnerGrid og{C.d_boxes, gl[l].d_owner, gl[l].nx, gl[l].ny};
+            hipLaunchKernelGGL((average_down_r_kernel<M, OwnerGrid>), dim3(grid1(g.avg_total)), dim3(256), 0,
+                               stream(), F.d_boxes, g.d_avg, g.n_avg, g.avg_total, og, ref_ratio, F.cur, C.cur);
+        }
+        ZAMR_HIP(hipGetLastError());
+        gl[l].max_stale = true;
+    }
+
+    // ErrorEst of level l, buffered and coarsened, on the host.
+    std::vector<unsigned char> g_tags(int l) {
+        GLev& g = gl[l];
+        if (l == 0) {
+            hipLaunchKernelGGL((error_est_kernel<M>), dim3((unsigned)((L0.nx + 255) / 256), (unsigned)L0.ny),
+                               dim3(256), 0, stream(), grid(), L0.cur, l0_stage_ran ? L0.last_in : -1, L0.prm,
+                               L0.clamp_flag(), g.t_last, tag_threshold, tag_b_max, g.d_tag);
+        } else {
+            Sol& F = level(l);
+            ZAMR_HIP(hipMemsetAsync(g.d_tag, 0, (size_t)g.nx * g.ny, stream()));
+            if (g.n_valid)
+                hipLaunchKernelGGL((error_est_boxes_kernel<M>), dim3(grid1(g.valid_total)), dim3(256), 0, stream(),
+                                   F.d_boxes, g.d_valid, g.n_valid, g.valid_total, F.cur, F.last_in, F.prm,
+                                   F.clamp_flag(), g.t_last, tag_threshold, tag_b_max, g.d_tag, g.nx);
+        }
+        ZAMR_HIP(hipGetLastError());
+        // the tag map itself stands in for the comparison map (only read)
+        hipLaunchKernelGGL(tag_buffer_kernel, dim3(grid1((long long)g.cnx * g.cny)), dim3(256), 0, stream(), g.d_tag,
+                           g.nx, g.ny, n_error_buf, bf_lev, g.d_ctag, g.cnx, g.cny, g.d_tag, d_changed);
+        ZAMR_HIP(hipGetLastError());
+        std::vector<unsigned char> ct((size_t)g.cnx * g.cny);
+        ZAMR_HIP(hipMemcpyAsync(ct.data(), g.d_ctag, ct.size(), hipMemcpyDeviceToHost, stream()));
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+        return ct;
+    }
+
+    // Proper-nesting domains (1 = allowed) of levels lbase .. max_crse in
+    // cluster units: at lbase the coarsened grids shrunk by n_proper inside
+    // the domain; above, the complement refined and grown by n_proper again.
+    std::vector<std::vector<unsigned char>> g_nesting(int lbase, int max_crse) {
+        const int bf = bf_lev, np = n_proper;
+        std::vector<std::vector<unsigned char>> comp(max_crse + 1), pn(max_crse + 1);
+        for (int l = lbase; l <= max_crse; ++l) {
+            const int cnx_l = gl[l].cnx, cny_l = gl[l].cny;
+            std::vector<unsigned char> base((size_t)cnx_l * cny_l, 0);
+            if (l == lbase) {
+                std::vector<unsigned char> cov(base.size(), 0);
+                for (const HostBox& b : level(l).hb)
+                    for (int J = b.lo[1] / bf; J <= b.hi[1] / bf; ++J)
+                        for (int I = b.lo[0] / bf; I <= b.hi[0] / bf; ++I) cov[(size_t)J * cnx_l + I] = 1;
+                for (size_t k = 0; k < base.size(); ++k) base[k] = cov[k] ? 0 : 1;
+            } else {
+                const int pcnx = gl[l - 1].cnx, rr = ref_ratio;
+                for (int J = 0; J < cny_l; ++J)
+                    for (int I = 0; I < cnx_l; ++I)
+                        base[(size_t)J * cnx_l + I] = comp[l - 1][(size_t)(J / rr) * pcnx + I / rr];
+            }
+            comp[l].assign(base.size(), 0);
+            for (int J = 0; J < cny_l; ++J)
+                for (int I = 0; I < cnx_l; ++I) {
+                    if (!base[(size_t)J * cnx_l + I]) continue;
+                    for (int jj = std::max(0, J - np); jj <= std::min(cny_l - 1, J + np); ++jj)
+                        for (int ii = std::max(0, I - np); ii <= std::min(cnx_l - 1, I + np); ++ii)
+                            comp[l][(size_t)jj * cnx_l + ii] = 1;
+                }
+            pn[l].resize(base.size());
+            for (size_t k = 0; k < pn[l].size(); ++k) pn[l][k] = comp[l][k] ? 0 : 1;
+        }
+        return pn;
+    }
+
+    // A cluster box cut by a mask into rectangles: each row's runs of set
+    // cells, consecutive rows' runs that start and end alike merged.
+    static std::vector<std::array<int, 4>> mask_rects(const std::vector<unsigned char>& mask, int w,
+                                                     const std::array<int, 4>& c) {
+        std::vector<std::array<int, 4>> done, open;
+        for (int j = c[1]; j <= c[3] + 1; ++j) {
+            std::vector<std::pair<int, int>> runs;
+            if (j <= c[3])
+                for (int i = c[0]; i <= c[2];) {
+                    if (!mask[(size_t)j * w + i]) {
+                        ++i;
+                        continue;
+                    }
+                    int e = i;
+                    while (e + 1 <= c[2] && mask[(size_t)j * w + e + 1]) ++e;
+                    runs.push_back({i, e});
+                    i = e + 1;
+                }
+            std::vector<std::array<int, 4>> next;
+            for (const auto& o : open) {
+                bool cont = false;
+                for (auto& r : runs) cont = cont || (r.first == o[0] && r.second == o[2]);
+                if (cont) next.push_back({o[0], o[1], o[2], j});
+                else done.push_back(o);
+            }
+            for (auto& r : runs) {
+                bool cont = false;
+                for (const auto& o : open) cont = cont || (r.first == o[0] && r.second == o[2]);
+                if (!cont) next.push_back({r.first, j, r.second, j});
+            }
+            open.swap(next);
+        }
+        std::sort(done.begin(), done.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
+            return a[1] != b[1] ? a[1] < b[1] : a[0] < b[0];
+        });
+        return done;
+    }
+
+    // Boxes of level l + 1 from a coarsened tag map of level l: clusters,
+    // cut by the nesting mask (empty: none), cut to amr.max_grid_size (x: whole
+    // stage strips), refined.
+    std::vector<std::array<int, 4>> g_grids(int l, const std::vector<unsigned char>& ct,
+                                            const std::vector<unsigned char>& pn) {
+        GLev& g = gl[l];
+        stage_->reset();
+        stage_->upload(g.d_ctag, ct.data(), ct.size(), stream());
+        ZAMR_HIP(hipStreamSynchronize(stream()));  // the clustering runs on its own stream
+        auto cl = gpu_cluster(g.d_ctag, g.cnx, g.cny);
+        std::sort(cl.begin(), cl.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
+            return a[1] != b[1] ? a[1] < b[1] : a[0] < b[0];
+        });
+        std::vector<std::array<int, 4>> parts;
+        for (auto& c : cl) {
+            if (pn.empty()) {
+                parts.push_back(c);
+                continue;
+            }
+            for (auto& q : mask_rects(pn, g.cnx, c)) parts.push_back(q);
+        }
+        const int r = ref_ratio * bf_lev;
+        const int m = std::max(1, fine_max_grid_size / r);
+        const int strip = WV - 2 * L0.hx_of(L0.spatial_order);
+        const int wx = (fine_max_grid_size / strip) * strip;
+        const int mx = (wx >= r && wx % r == 0) ? wx / r : m;
+        std::vector<std::array<int, 4>> fine;
+        for (auto& c : parts)
+            for (int y0 = c[1]; y0 <= c[3]; y0 += m)
+                for (int x0 = c[0]; x0 <= c[2]; x0 += mx) {
+                    const int x1 = std::min(x0 + mx - 1, c[2]), y1 = std::min(y0 + m - 1, c[3]);
+                    fine.push_back({x0 * r, y0 * r, (x1 + 1) * r - 1, (y1 + 1) * r - 1});
+                }
+        return fine;
+    }
+
+    std::vector<std::array<int, 4>> boxes_of(int l) {
+        std::vector<std::array<int, 4>> v;
+        for (const HostBox& b : level(l).hb) v.push_back({b.lo[0], b.lo[1], b.hi[0], b.hi[1]});
+        return v;
+    }
+
+    double level_time(int l) {
+        Sol& S = level(l);
+        S.download_groups();
+        return S.h_groups[0].time;
+    }
+
+    // A new box set of level l, filled from level l - 1 (valid and in-domain
+    // ghost cells) and, where `old` covers, from the old level (RemakeLevel's
+    // FillPatch), with fresh Qaux.
+    std::unique_ptr<Sol> g_make(int l, const std::vector<std::array<int, 4>>& boxes, Sol* old, double time) {
+        auto F = std::make_unique<Sol>();
+        F->hist_cap = 16;
+        F->cache = &fine_cache;
+        F->stage = stage_.get();
+        F->pool_headroom = 1.25;
+        F->configure_fine(level(l - 1), ref_ratio, boxes);
+        F->use_graph = L0.use_graph;
+        g_coarse_patch(l, *F);
+        if (old) {
+            std::vector<CellRect> cp;
+            for (size_t b = 0; b < F->hb.size(); ++b) {
+                const HostBox& B = F->hb[b];
+                for (size_t o = 0; o < old->hb.size(); ++o) {
+                    const HostBox& O = old->hb[o];
+                    const int i0 = std::max(B.lo[0] - B.ng, O.lo[0]), i1 = std::min(B.hi[0] + B.ng, O.hi[0]);
+                    const int j0 = std::max(B.lo[1] - B.ng, O.lo[1]), j1 = std::min(B.hi[1] + B.ng, O.hi[1]);
+                    if (i0 > i1 || j0 > j1) continue;
+                    cp.push_back(CellRect{(int)b, (int)o, i0, j0, i1 - i0 + 1, j1 - j0 + 1, 0});
+                }
+            }
+            int n = 0;
+            long long total = 0;
+            CellRect* d = upload_scratch(cp, n, total);
+            if (n)
+                hipLaunchKernelGGL((copy_levels_kernel<M>), dim3(grid1(total)), dim3(256), 0, stream(), F->d_boxes,
+                                   F->cur, old->d_boxes, old->cur, d, n, total);
+            ZAMR_HIP(hipGetLastError());
+            ++remakes;
+        }
+        fresh_aux(*F, time);
+        return F;
+    }
+
+    void g_set_level(int l, std::unique_ptr<Sol> F) {
+        if (l == 1) {
+            L1 = std::move(F);
+        } else {
+            if ((int)up.size() < l - 2) throw Error(4, "internal: a level made above a missing one");
+            if ((int)up.size() == l - 2) up.push_back(std::move(F));
+            else up[l - 2] = std::move(F);
+        }
+        g_build_lists(l);
+    }
+
+    void g_clear_above(int l) {
+        while (finest_level() > l) {
+            const int f = finest_level();
+            if (f >= 2) up.pop_back();
+            else L1.reset();
+            ZAMR_HIP(hipMemsetAsync(gl[f - 1].d_cov, 0, (size_t)gl[f - 1].nx * gl[f - 1].ny, stream()));
+        }
+    }
+
+    // AmrMesh::MakeNewGrids(lbase) + AmrCore::regrid(lbase, time).
+    void g_regrid(int lbase, double time) {
+        if (lbase >= max_level) return;
+        Clock clk{regrid_ms};
+        ++regrids;
+        const int max_crse = std::min(finest_level(), max_level - 1);
+        const auto pn = g_nesting(lbase, max_crse);
+        const int bf = bf_lev, r = ref_ratio, np = n_proper;
+        std::vector<std::vector<std::array<int, 4>>> ng(max_crse + 2);
+        int new_finest = lbase;
+        for (int levc = max_crse; levc >= lbase; --levc) {
+            const int levf = levc + 1;
+            const GLev& g = gl[levc];
+            auto ct = g_tags(levc);
+            if (levf < new_finest)
+                for (auto& b : ng[levf + 1]) {
+                    int lo[2], hi[2];
+                    for (int d = 0; d < 2; ++d) {
+                        lo[d] = std::max(b[d] / r - np, 0) / r / bf;
+                        hi[d] = (b[d + 2] / r + np) / r / bf;
+                    }
+                    for (int J = lo[1]; J <= std::min(hi[1], g.cny - 1); ++J)
+                        for (int I = lo[0]; I <= std::min(hi[0], g.cnx - 1); ++I) ct[(size_t)J * g.cnx + I] = 1;
+                }
+            bool full = true;
+            for (unsigned char v : pn[levc]) full = full && v;
+            for (size_t k = 0; k < ct.size(); ++k)
+                if (!pn[levc][k]) ct[k] = 0;
+            bool any = false;
+            for (unsigned char v : ct) any = any || v;
+            ng[levf] = any ? g_grids(levc, ct, full ? std::vector<unsigned char>{} : pn[levc])
+                           : std::vector<std::array<int, 4>>{};
+            if (!ng[levf].empty()) new_finest = std::max(new_finest, levf);
+        }
+        bool coarse_changed = false;
+        for (int lev = lbase + 1; lev <= new_finest; ++lev) {
+            if (lev <= finest_level()) {
+                const bool changed = ng[lev] != boxes_of(lev);
+                if (changed || coarse_changed) {
+                    auto& slot = level_ptr(lev);
+                    auto F = g_make(lev, changed ? ng[lev] : boxes_of(lev), slot.get(), time);
+                    g_set_level(lev, std::move(F));
+                }
+                coarse_changed = changed;
+            } else {
+                g_set_level(lev, g_make(lev, ng[lev], nullptr, time));
+            }
+        }
+        g_clear_above(new_finest);
+    }
+
+    // The level loop of InitData on the current level-0 state.
+    void g_init_levels() {
+        L0.ensure_aux();
+        l0_stage_ran = false;
+        for (int it = 0; it < 4; ++it) {
+            for (int l = 1; l <= finest_level(); ++l) g_coarse_patch(l, level(l));
+            const int old_finest = finest_level();
+            g_regrid(0, 0.0);
+            if (old_finest == finest_level() && it > 0) break;
+        }
+        for (int l = 1; l <= finest_level(); ++l) {
+            g_coarse_patch(l, level(l));
+            fresh_aux(level(l), 0.0);
+        }
+        ZAMR_HIP(hipStreamSynchronize(stream()));
+        for (int l = 1; l <= finest_level(); ++l) level(l).reset_clock();
+        for (GLev& g : gl) g.max_stale = true;
+    }
+
+    // ZoomyAmr::TimeStep(lev, time, iteration), the time on the level's clock.
+    // `lines`: print Evolve's MOOD line after each level's Advance.
+    void g_time_step(int lev, int iteration, bool lines) {
+        if (lev < max_level && iteration > 0) g_regrid(lev, level_time(lev));
+        Sol& S = level(lev);
+        GLev& g = gl[lev];
+        hipStream_t st = stream();
+        if (g.max_stale || !S.maxinv_fresh) {
+            if (lev == 0) S.ensure_aux();
+            hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, st, S.d_groups, S.ngroups);
+            auto& it = S.items(S.hx_of(S.spatial_order));
+            hipLaunchKernelGGL((max_inv_kernel<M, N>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, st, S.d_boxes, it.d,
+                               it.n, S.cur, S.prm, S.d_groups, WV - 2 * it.hx);
+            ZAMR_HIP(hipGetLastError());
+        }
+        if (lev > 0) g_cf_ghosts(lev);
+        g.t_last = level_time(lev);
+        S.begin_step();
+        S.advance_kernels();
+        const bool finer = lev < finest_level();
+        if (finer) {
+            hipLaunchKernelGGL(sync_clock_kernel, dim3(1), dim3(1), 0, st, S.d_groups, level(lev + 1).d_groups);
+            // Qaux = aux(Q^{n+1}) as Advance's last UpdateState leaves it; the
+            // next ComputeDt reads it beside the averaged-down Q
+            auto& it = S.items(S.hx_of(S.spatial_order));
+            hipLaunchKernelGGL((materialize_aux_kernel<M>), dim3(Sol::aux_grid(it.n)), dim3(NT), 0, st, S.d_boxes,
+                               it.d, it.n, S.cur, S.prm, 0.0, WV - 2 * it.hx, (const GroupState*)S.d_groups);
+            ZAMR_HIP(hipGetLastError());
+        }
+        S.end_step();
+        S.aux_stale = !finer;
+        S.maxinv_fresh = true;
+        g.max_stale = false;
+        if (lev == 0) {
+            l0_stage_ran = true;
+            t_last_stage = g.t_last;
+        }
+        if (lines && S.mood()) {
+            S.download_groups();
+            if (S.h_groups[0].troubled > 0)
+                std::cout << "  [MOOD] lev " << lev << " troubled cells: " << (long)S.h_groups[0].troubled
+                          << " -> whole-step order-1 redo\n";
+        }
+        if (finer) {
+            for (int i = 1; i <= ref_ratio; ++i) g_time_step(lev + 1, i, lines);
+            g_average_down(lev);
+        }
+    }
+
     // ZoomyAmr::TimeStep(0, time, iteration) at a caller-given time; returns dt.
     double time_step(double time, int iteration) {
         L0.download_groups();
@@ -1271,6 +1987,11 @@ class AmrDriver {
     }
 
     double total_mass() {
+        if (general) {
+            double m = 0.0;
+            for (int l = 0; l <= finest_level(); ++l) m += level(l).total_mass(l < finest_level() ? gl[l].d_cov : nullptr);
+            return m;
+        }
         double m = L0.total_mass(L1 ? d_cover : nullptr);
         if (L1) m += L1->total_mass();
         return m;
@@ -1284,6 +2005,7 @@ class AmrDriver {
             if (to_device) {
                 l0_max_stale = true;
                 L0.maxinv_fresh = false;
+                if (general) gl[0].max_stale = true;
             }
             return;
         }
@@ -1316,12 +2038,14 @@ class AmrDriver {
         if (to_device) {
             l1_max_stale = true;
             F.maxinv_fresh = false;
+            if (general) gl[lev].max_stale = true;
         }
     }
 
     void reset_clock() {
         L0.reset_clock();
         if (L1) L1->reset_clock();
+        for (auto& u : up) u->reset_clock();
         plot_step = 0;
         next_plot_time = 0.0;
     }
@@ -1356,7 +2080,12 @@ class AmrDriver {
                     std::cout << "  [MOOD] lev 1 troubled cells: " << (long)L1->h_groups[0].troubled
                               << " -> whole-step order-1 redo\n";
             };
-            coarse_step(step, mood_lines);
+            if (general) {
+                g_time_step(0, step, true);
+                l0_printed = true;
+            } else {
+                coarse_step(step, mood_lines);
+            }
             L0.download_groups();
             if (!l0_printed && L0.mood() && L0.h_groups[0].troubled > 0)
                 std::cout << "  [MOOD] lev 0 troubled cells: " << (long)L0.h_groups[0].troubled

@@ -324,8 +324,8 @@ class Solver {
         Deck pp(text);
         read_solver_keys(pp);
         pp.query("amr.max_level", max_level);
-        if (max_level < 0 || max_level > 1)
-            throw Error(5, "amr.max_level > 1: this build's hierarchy has at most two levels (0 and 1)");
+        if (max_level < 0 || max_level > 3)
+            throw Error(5, "amr.max_level > 3: this build's hierarchy has at most four levels (0 to 3)");
         std::vector<double> nc;
         if (!pp.queryarr("amr.n_cell", nc) || nc.size() < 2) throw Error(1, "amr.n_cell needs at least 2 values");
         nx = (int)nc[0];

@@ -261,7 +261,8 @@ int zamr_level_shape(const zamr_ctx* cc, int lev, int* nx, int* ny, int* nboxes)
     if (!c) return ZAMR_EINVAL;
     if (c->amr) {
         if (lev < 0 || lev > c->amr->max_level) return ZAMR_EINVAL;
-        const int r = lev == 0 ? 1 : c->amr->ref_ratio;
+        int r = 1;
+        for (int k = 0; k < lev; ++k) r *= c->amr->ref_ratio;
         if (nx) *nx = c->s.nx * r;
         if (ny) *ny = c->s.ny * r;
         if (nboxes) *nboxes = lev <= c->amr->finest_level() ? (int)c->amr->level(lev).hb.size() : 0;
@@ -443,7 +444,7 @@ int zamr_set_stream(zamr_ctx* c, void* stream) {
         c->s.own_stream = false;
         c->s.stream = (hipStream_t)stream;
         c->s.invalidate_graph();
-        if (c->amr && c->amr->L1) c->amr->L1->stream = (hipStream_t)stream;
+        if (c->amr) c->amr->for_fine([&](Sol& F) { F.stream = (hipStream_t)stream; });
     });
 }
 int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
@@ -452,18 +453,19 @@ int zamr_set_tuning(zamr_ctx* c, int rows, int per_box, int xcd) {
     c->s.per_box_launch = per_box;
     c->s.xcd_remap = xcd;
     c->s.invalidate_graph();
-    if (c->amr && c->amr->L1) {
-        c->amr->L1->rows_per_item = rows;
-        c->amr->L1->per_box_launch = per_box;
-        c->amr->L1->xcd_remap = xcd;
-    }
+    if (c->amr)
+        c->amr->for_fine([&](Sol& F) {
+            F.rows_per_item = rows;
+            F.per_box_launch = per_box;
+            F.xcd_remap = xcd;
+        });
     return ZAMR_OK;
 }
 int zamr_set_graph(zamr_ctx* c, int on) {
     if (!c) return ZAMR_EINVAL;
     c->s.use_graph = on != 0;
     c->s.invalidate_graph();
-    if (c->amr && c->amr->L1) c->amr->L1->use_graph = on != 0;
+    if (c->amr) c->amr->for_fine([&](Sol& F) { F.use_graph = on != 0; });
     return ZAMR_OK;
 }
 int zamr_set_launch_shape(zamr_ctx* c, const int* class_rows, const int* grid_blocks) {

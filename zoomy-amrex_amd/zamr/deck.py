@@ -78,8 +78,8 @@ def deck_from_case(case, amr=None):
     nx, ny, prob_lo, prob_hi, max_grid_size, spatial_order, implicit_source,
     well_balanced, clamp_positivity, positivity, cfl, dtmin, dtmax, time_end,
     max_step, bc, params).  `amr` (a dict over AMR_KEYS) adds the hierarchy:
-    amr.max_level, ref_ratio, blocking_factor, n_error_buf, grid_eff and
-    tagging.threshold / tagging.b_max."""
+    amr.max_level, ref_ratio, blocking_factor, n_error_buf, n_proper, grid_eff
+    and tagging.threshold / tagging.b_max."""
     c = dict(case)
     bc = c.get("bc", ("", "", "", ""))
     a = None
@@ -110,5 +110,6 @@ def deck_from_case(case, amr=None):
     if a:
         lines += [f"amr.ref_ratio = {a['ref_ratio']}", f"amr.blocking_factor = {a['blocking_factor']}",
                   f"amr.n_error_buf = {a['n_error_buf']}", f"amr.grid_eff = {a['grid_eff']!r}",
+                  f"amr.n_proper = {a.get('n_proper', 1)}",
                   f"tagging.threshold = {a['threshold']!r}", f"tagging.b_max = {a['b_max']!r}"]
     return "\n".join(lines) + "\n"

@@ -64,9 +64,9 @@ int zamr_march_constants(int* c_cfl_dimension, int* c_cfl_degree_factor);
  * sets up the hierarchy (AmrCore with amr.ref_ratio 2 or 4, a 2-D model; the
  * amr.blocking_factor / n_error_buf / n_proper / grid_eff / max_grid_size and
  * tagging.* keys apply).  solver.implicit_global = true solves the source globally
- * after each stage (Source/imex_solver.H: Newton + GMRES, single level).
- * Keys outside this build (amr.max_level > 3, ref_ratio not 2 or 4, implicit_global
- * with MOOD, AMR or a box list) fail with ZAMR_EUNSUP. */
+ * after each stage (Source/imex_solver.H: Newton + GMRES; with MOOD and on every
+ * AMR level too).  Keys outside this build (amr.max_level > 3, ref_ratio not 2
+ * or 4, implicit_global in a box list) fail with ZAMR_EUNSUP. */
 int zamr_create(const char* inputs_text, zamr_ctx** out);
 /* Batched mode: nbox independent boxes sharing one launch per RK stage.
  * inputs_text supplies the solver.* / params.* keys (geometry keys ignored). */

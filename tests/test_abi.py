@@ -60,7 +60,7 @@ def test_model_info_and_march_constants(system, dim, order, nq, na):
 @pytest.mark.parametrize("bad,code", [
     ("bc.x_lo = Nowhere\n", 2),
     ("amr.max_level = 4\n", 5),
-    ("solver.implicit_global = true\nsolver.positivity = mood\n", 5),
+    ("amr.max_level = 1\namr.ref_ratio = 3\n", 5),
     ("solver.spatial_order = 3\n", 1),
     ("solver.cfl = abc\n", 1),
 ])

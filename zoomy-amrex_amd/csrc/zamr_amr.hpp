@@ -934,7 +934,6 @@ class AmrDriver {
         pp.query("tagging.threshold", tag_threshold);
         pp.query("tagging.b_max", tag_b_max);
         if (M::dimension != 2) throw Error(5, "AMR (amr.max_level > 0) is built for two-dimensional models only");
-        if (L0.jfnk_mode()) throw Error(5, "AMR with solver.implicit_global = true is not part of this build");
         if (ref_ratio != 2 && ref_ratio != 4) throw Error(5, "amr.ref_ratio must be 2 or 4 in this build");
         if (max_level < 1 || max_level > 3) throw Error(5, "amr.max_level must be 1..3 in this build");
         general = max_level > 1 || ref_ratio != 2;
@@ -1443,7 +1442,7 @@ class AmrDriver {
         l0_stage_ran = true;
         l0_max_stale = false;
         if (L1) {
-            interp(*L1, d_cf, n_cf, cf_total, L0.cur, L1->mood() ? 7u : 3u, d_cover);
+            interp(*L1, d_cf, n_cf, cf_total, L0.cur, L1->nbuf() == 3 ? 7u : 3u, d_cover);
             if (l1_max_stale) {
                 hipLaunchKernelGGL(clear_max_kernel, dim3(1), dim3(64), 0, s, L1->d_groups, L1->ngroups);
                 auto& it = L1->items(L1->hx_of(L1->spatial_order));
@@ -1600,7 +1599,7 @@ class AmrDriver {
     void g_cf_ghosts(int l) {
         Sol& F = level(l);
         GLev& g = gl[l];
-        g_interp(l, F, g.d_ghost, g.n_ghost, g.ghost_total, F.mood() ? 7u : 3u, g.d_owner);
+        g_interp(l, F, g.d_ghost, g.n_ghost, g.ghost_total, F.nbuf() == 3 ? 7u : 3u, g.d_owner);
     }
 
     // average_down of level l + 1 into level l.

@@ -308,8 +308,6 @@ class Solver {
         pp.query("solver.face_based_rhs", face_based_rhs);
         pp.query("solver.profile_split", face_based_rhs);
         if (spatial_order != 1 && spatial_order != 2) throw Error(1, "solver.spatial_order must be 1 or 2");
-        if (implicit_global && positivity == "mood" && spatial_order == 2)
-            throw Error(5, "solver.implicit_global = true with solver.positivity = mood is not part of this build");
         if (positivity != "none" && positivity != "mood") throw Error(1, "solver.positivity must be none or mood");
         auto defs = M::default_parameters();
         for (int i = 0; i < np; ++i) prm.v[i] = defs[i];
@@ -326,6 +324,11 @@ class Solver {
         pp.query("amr.max_level", max_level);
         if (max_level < 0 || max_level > 3)
             throw Error(5, "amr.max_level > 3: this build's hierarchy has at most four levels (0 to 3)");
+        if (max_level >= 1) {
+            int rr = 2;
+            pp.query("amr.ref_ratio", rr);
+            if (rr != 2 && rr != 4) throw Error(5, "amr.ref_ratio must be 2 or 4 in this build");
+        }
         std::vector<double> nc;
         if (!pp.queryarr("amr.n_cell", nc) || nc.size() < 2) throw Error(1, "amr.n_cell needs at least 2 values");
         nx = (int)nc[0];
@@ -943,10 +946,10 @@ class Solver {
     // clock (dt and time read from the group state).
     // End of a step whose stages solved the source globally (Heun with
     // buffer old, or none at order 1; UpdateState; CFL max) in place on dst.
-    void finish_step(int dst, int old) {
+    void finish_step(int dst, int old, int heun_only = 0) {
         Items& it = items(hx_of(spatial_order));
         hipLaunchKernelGGL((finish_step_kernel<M, N>), dim3(aux_grid(it.n)), dim3(NT), 0, stream, d_boxes, it.d, it.n,
-                           dst, old, prm, clamp_flag(), d_groups, WV - 2 * it.hx);
+                           dst, old, prm, clamp_flag(), d_groups, WV - 2 * it.hx, heun_only);
         ZAMR_HIP(hipGetLastError());
     }
 
@@ -974,7 +977,28 @@ class Solver {
             fill_boundary(t1);
             launch_stage_dyn(2, 0, stage_args(t1, t2, cur));
             jfnk->solve(t2, dt, time, prm);
-            finish_step(t2, cur);
+            if (mood()) {
+                // the order-2 candidate (Heun) and its troubled cells; with any,
+                // the whole step again at order 1 from Q^n (ZoomyAmr.cpp:699-720),
+                // each stage with its global solve
+                finish_step(t2, cur, 1);
+                hipLaunchKernelGGL(mood_reset_kernel, dim3((ngroups + 63) / 64), dim3(64), 0, stream, d_groups,
+                                   ngroups);
+                ZAMR_HIP(hipGetLastError());
+                download_groups();
+                if (h_groups[0].troubled > 0) {
+                    launch_stage_dyn(1, 0, stage_args(cur, t1, cur));
+                    jfnk->solve(t1, dt, time, prm);
+                    fill_boundary(t1);
+                    launch_stage_dyn(1, 0, stage_args(t1, t2, cur));
+                    jfnk->solve(t2, dt, time, prm);
+                    finish_step(t2, cur);
+                } else {
+                    finish_step(t2, -1);
+                }
+            } else {
+                finish_step(t2, cur);
+            }
             last_in = t1;
             cur = t2;
         }

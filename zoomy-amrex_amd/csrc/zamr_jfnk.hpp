@@ -8,17 +8,24 @@
 // J x = x - (dt/eps) (S(Q + eps x) - S(Q)), eps = 1e-7 (1 + |Q|) / |x|.
 // S is ComputeSourceField's: aux = update_aux_variables(q, 0), then
 // Model::source, cell by cell (every source here is local, so its ghost
-// refresh does not enter).  Gram-Schmidt is classical, applied twice (one
-// fused pass over the basis per sweep).
+// refresh does not enter).  Gram-Schmidt is classical, applied twice.
 //
 // Vectors hold the valid cells of every box of the level, box by box, row
-// by row, component-major over the level: v[n * N + c].  Vector work is
-// HBM-streaming (a dot or axpy over k basis vectors reads k + 1 vectors);
-// the control flow (Newton, Arnoldi, Givens) runs on the host from the dot
-// products, which are reduced deterministically (fixed partials, summed in
-// order).
+// by row, component-major over the level: v[n * N + c].  Everything runs on
+// the device, the control flow included: an Arnoldi step is three passes
+// over the vectors (operator + first Gram-Schmidt dots; first subtraction +
+// second dots; second subtraction + norm), each ending in the block that
+// finishes last, which sums the per-block partials in a fixed order
+// (deterministic) and, after the third, applies the Givens rotations,
+// updates the residual estimate and decides whether the cycle goes on.  A
+// state word (`phase`) in device memory steers which of the queued kernels
+// act, so the host queues several Arnoldi steps, the cycle's end (least
+// squares, update, restart residual) and reads the state back once per
+// Newton iteration.
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "zamr_error.hpp"
@@ -26,7 +33,29 @@
 
 namespace zamr {
 
-constexpr int kJfnkBlocks = 1024;  // partial sums per reduction
+// Rows of Model::source that can be nonzero: M::zamr_source_rows when the
+// model declares it (the shallow-water friction acts on the momentum rows
+// only), every row otherwise.  The solve's vectors hold those rows only: in
+// the others F = Q - Qexp - dt S is 0, the operator is the identity on a
+// zero component, and every Krylov vector, dot product and update is the
+// same with them left out (they only add exact zeros).  The residual pass
+// checks S there and stops the solve if a declared-zero row is not.
+template <class M, class = void>
+struct SourceRows {
+    static constexpr int n = M::n_dof_q;
+    __host__ __device__ static constexpr int row(int a) { return a; }
+    __host__ __device__ static constexpr bool listed(int r) { return true; }
+};
+template <class M>
+struct SourceRows<M, std::void_t<decltype(M::zamr_source_rows)>> {
+    static constexpr int n = (int)(sizeof(M::zamr_source_rows) / sizeof(int));
+    __host__ __device__ static constexpr int row(int a) { return M::zamr_source_rows[a]; }
+    __host__ __device__ static constexpr bool listed(int r) {
+        for (int a = 0; a < n; ++a)
+            if (M::zamr_source_rows[a] == r) return true;
+        return false;
+    }
+};
 
 // Box of compact cell c: bstart[b] <= c < bstart[b + 1].
 __device__ __forceinline__ int jf_box(const long long* __restrict__ bstart, int nb, long long c) {
@@ -54,168 +83,546 @@ __device__ __forceinline__ JfCell jf_cell(const BoxDesc* __restrict__ boxes, con
     return r;
 }
 
-// Valid cells of buffer `buf` -> compact (gather), or compact -> buffer (scatter).
+// Valid cells of buffer `buf` -> compact q (all rows) and Qexp (the source
+// rows), or compact q -> buffer (scatter).
 template <class M>
 __global__ void __launch_bounds__(256) jf_move_kernel(const BoxDesc* __restrict__ boxes,
                                                       const long long* __restrict__ bstart, int nb, long long N,
-                                                      int buf, double* __restrict__ v, int to_box) {
+                                                      int buf, double* __restrict__ v, int to_box,
+                                                      double* __restrict__ qexp) {
     constexpr int nq = M::n_dof_q;
+    using R = SourceRows<M>;
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const JfCell e = jf_cell(boxes, bstart, nb, c);
     const BoxDesc& B = boxes[e.b];
     double* q = buf == 0 ? B.q[0] : (buf == 1 ? B.q[1] : B.q[2]);
     const long long idx = (long long)(e.i - B.gl[0]) + (long long)(e.j - B.gl[1]) * B.sy;
+    if (to_box) {
 #pragma unroll
-    for (int n = 0; n < nq; ++n) {
-        if (to_box) q[idx + n * B.sc] = v[n * N + c];
-        else v[n * N + c] = q[idx + n * B.sc];
+        for (int a = 0; a < R::n; ++a) q[idx + R::row(a) * B.sc] = v[R::row(a) * N + c];
+        return;
     }
+#pragma unroll
+    for (int n = 0; n < nq; ++n) v[n * N + c] = q[idx + n * B.sc];
+#pragma unroll
+    for (int a = 0; a < R::n; ++a) qexp[a * N + c] = q[idx + R::row(a) * B.sc];
 }
 
-// S(q) of every cell (mode x == nullptr: out = S(q)), or the Jacobian
-// product out = (x + (-dt/eps) S(q + eps x)) + (dt/eps) Sbase with
-// eps = eps_num / |x|, |x|^2 read from the device (*x2); |x| = 0 gives 0.
-template <class M>
-__global__ void __launch_bounds__(256) jf_source_kernel(const BoxDesc* __restrict__ boxes,
-                                                        const long long* __restrict__ bstart, int nb, long long N,
-                                                        const double* __restrict__ q, const double* __restrict__ x,
-                                                        double eps_num, const double* __restrict__ x2, double dt,
-                                                        const double* __restrict__ Sbase, Params pp, double time,
-                                                        double* __restrict__ out) {
-    using S = Sys<M>;
-    constexpr int nq = S::nq;
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    double eps = 0.0;
-    if (x) {
-        const double nx = sqrt(*x2);
-        if (nx == 0.0) {
+// Device state of one solve (GMRES(30) cycle + Newton bookkeeping).
+constexpr int kJfRestart = 30;
+struct JfState {
+    double H[kJfRestart + 1][kJfRestart];
+    double cs[kJfRestart], sn[kJfRestart], g[kJfRestart + 1], y[kJfRestart];
+    double h1[kJfRestart + 1], h2[kJfRestart + 1];
+    double vnorm[kJfRestart + 1];
+    double r0, eps_num, xnorm, resid, resid0;
+    int phase;  // 0 Arnoldi, 2 update, 3 restart residual, 6 Q update, 5 finished
+    int k;      // Arnoldi column of the current cycle
+    int kd;     // columns of the cycle done
+    int done;   // the cycle reached the tolerance
+    int src0;   // the cycle's first vector before normalising: 0 rhs, 1 r
+    int first;  // the first cycle of this Newton iteration (x starts at 0)
+    int nd;     // Newton converged
+    int newton_its;
+    long long its;          // Arnoldi steps of this Newton iteration's GMRES
+    long long gmres_total;  // over the solve
+    int bad_rows;           // a row declared source-free had a nonzero source
+    int pad_;
+};
+
+// What the host reads after a round, written by the kernels that change it
+// into host-visible memory (no copy to queue).
+struct JfStatus {
+    int phase, nd, bad_rows, pad_;
+    long long its, newton_its, gmres_total;
+};
+
+enum : int { kJfArnoldi = 0, kJfUpdate = 2, kJfRestartRes = 3, kJfFinished = 5, kJfQUpdate = 6 };
+
+constexpr int kJfCellsPerThread = 4;
+constexpr long kJfMaxIter = 2000;
+constexpr int kJfNewtonMax = 12;
+
+__device__ __forceinline__ void jf_status(const JfState* st, JfStatus* hs) {
+    hs->phase = st->phase;
+    hs->nd = st->nd;
+    hs->bad_rows = st->bad_rows;
+    hs->its = st->its;
+    hs->newton_its = st->newton_its;
+    hs->gmres_total = st->gmres_total;
+}
+
+// Sum of a per-thread value over the block (256 threads), in every thread.
+// A fixed order: wave sums by xor butterflies, then the four wave sums in
+// order.
+__device__ __forceinline__ double jf_block_sum(double v, double* lds) {
 #pragma unroll
-            for (int n = 0; n < nq; ++n) out[n * N + c] = 0.0;
-            return;
-        }
-        eps = eps_num / nx;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[wv] = v;
+    __syncthreads();
+    return ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+// out[j] = sum over the nblk blocks of part[j * nblk + b], by a fixed tree
+// (thread t sums blocks t, t + 256, ... in order, then jf_block_sum); j < m.
+__device__ __forceinline__ void jf_sums(const double* __restrict__ part, int nblk, int m, double* out, double* lds) {
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long long)j * nblk + b];
+        const double tot = jf_block_sum(s, lds);
+        if (threadIdx.x == 0) out[j] = tot;
     }
-    typename S::VP p;
-    for (int k = 0; k < S::np; ++k) p(k, 0) = pp.v[k];
+    __syncthreads();
+}
+
+// One cell's operator on the source rows: out = (u + (-dt/eps) S(q + eps u))
+// + (dt/eps) Sbase, eps = eps_num / |u| (|u| = unorm; 0 gives 0); q has all
+// rows, u, Sbase and out the source rows.
+template <class M>
+__device__ __forceinline__ void jf_apply_cell(const BoxDesc* __restrict__ boxes, const long long* __restrict__ bstart,
+                                              int nb, long long c, const double* qc, const double* uc,
+                                              const double* sb, double eps_num, double unorm, double dt,
+                                              const typename Sys<M>::VP& p, double time, double* out) {
+    using S = Sys<M>;
+    using R = SourceRows<M>;
+    constexpr int nq = S::nq;
+    if (unorm == 0.0) {
+#pragma unroll
+        for (int a = 0; a < R::n; ++a) out[a] = 0.0;
+        return;
+    }
+    const double eps = eps_num / unorm;
     const JfCell e = jf_cell(boxes, bstart, nb, c);
     const BoxView B = view(boxes, e.b);
     typename S::VQ qq;
 #pragma unroll
-    for (int n = 0; n < nq; ++n) qq(n, 0) = x ? 1.0 * q[n * N + c] + eps * x[n * N + c] : q[n * N + c];
+    for (int n = 0; n < nq; ++n) qq(n, 0) = qc[n];
+#pragma unroll
+    for (int a = 0; a < R::n; ++a) qq(R::row(a), 0) = 1.0 * qc[R::row(a)] + eps * uc[a];
     const typename S::VX X = S::center(B, e.i, e.j);
-    typename S::VA a{};
-    a = M::update_aux_variables(qq, a, p, time, X);
-    const typename S::VQ s = M::source(qq, a, p, time, X);
-    if (x) {
-        const double r = dt / eps;
+    typename S::VA av{};
+    av = M::update_aux_variables(qq, av, p, time, X);
+    const typename S::VQ sv = M::source(qq, av, p, time, X);
+    const double r = dt / eps;
 #pragma unroll
-        for (int n = 0; n < nq; ++n) out[n * N + c] = (1.0 * x[n * N + c] + (-r) * s(n, 0)) + r * Sbase[n * N + c];
-    } else {
+    for (int a = 0; a < R::n; ++a) out[a] = (1.0 * uc[a] + (-r) * sv(R::row(a), 0)) + r * sb[a];
+}
+
+#define JF_CELLS_LOOP                                                                              \
+    for (int pc = 0; pc < kJfCellsPerThread; ++pc)                                                 \
+        if (const long long c = ((long long)blockIdx.x * kJfCellsPerThread + pc) * blockDim.x + threadIdx.x; \
+            c < N)
+
+// Zero the state (all threads, 8-byte words) and publish it.
+__global__ void __launch_bounds__(256) jf_init_kernel(JfState* st, JfStatus* hs) {
+    static_assert(sizeof(JfState) % 8 == 0, "the state is a whole number of 8-byte words");
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(st);
+    for (int i = threadIdx.x; i < (int)(sizeof(JfState) / 8); i += blockDim.x) w[i] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) jf_status(st, hs);
+}
+
+// A new cycle's Hessenberg matrix and residual vector, zeroed by all threads.
+__device__ __forceinline__ void jf_clear_cycle(JfState* st) {
+    for (int i = threadIdx.x; i < (kJfRestart + 1) * kJfRestart; i += blockDim.x) (&st->H[0][0])[i] = 0.0;
+    for (int j = threadIdx.x; j <= kJfRestart; j += blockDim.x) st->g[j] = 0.0;
+    __syncthreads();
+}
+
+// Newton residual: S = S(q), F = (q - Qexp) + (-dt) S, rhs = -F on the source
+// rows; partials of |F|^2 (row 0) and of |q|^2 over every row (row 1).
+template <class M>
+__global__ void __launch_bounds__(256) jf_newton_kernel(const BoxDesc* __restrict__ boxes,
+                                                        const long long* __restrict__ bstart, int nb, long long N,
+                                                        const double* __restrict__ q, const double* __restrict__ qexp,
+                                                        double* __restrict__ S, double* __restrict__ rhs, double dt,
+                                                        Params pp, double time, JfState* st, double* part, int it) {
+    using Sy = Sys<M>;
+    using R = SourceRows<M>;
+    constexpr int nq = Sy::nq;
+    __shared__ double lds[4];
+    // iteration `it` runs once the previous one is finished
+    if (st->nd || st->newton_its != it || (it > 0 && st->phase != kJfFinished)) return;
+    typename Sy::VP p;
+    for (int k = 0; k < Sy::np; ++k) p(k, 0) = pp.v[k];
+    double f2 = 0.0, q2 = 0.0;
+    bool bad = false;
+    JF_CELLS_LOOP {
+        const JfCell e = jf_cell(boxes, bstart, nb, c);
+        const BoxView B = view(boxes, e.b);
+        typename Sy::VQ qq;
 #pragma unroll
-        for (int n = 0; n < nq; ++n) out[n * N + c] = s(n, 0);
-    }
-}
-
-// out = a x + b y (elementwise over len).
-__global__ void __launch_bounds__(256) jf_lincomb_kernel(long long len, double a, const double* __restrict__ x,
-                                                         double b, const double* __restrict__ y,
-                                                         double* __restrict__ out) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < len) out[k] = a * x[k] + b * y[k];
-}
-
-// w += sign * sum_j coef[j] V[j]  (coef on the device, m <= 31).
-__global__ void __launch_bounds__(256) jf_multi_axpy_kernel(long long len, int m, double* const* __restrict__ V,
-                                                            const double* __restrict__ coef, double sign,
-                                                            double* __restrict__ w) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= len) return;
-    double acc = w[k];
-    for (int j = 0; j < m; ++j) acc += (sign * coef[j]) * V[j][k];
-    w[k] = acc;
-}
-
-// out = x / sqrt(*x2) when *x2 > 0.
-__global__ void __launch_bounds__(256) jf_normalize_kernel(long long len, const double* __restrict__ x,
-                                                           const double* __restrict__ x2, double* __restrict__ out) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const double n2 = *x2;
-    if (k < len && n2 > 0.0) out[k] = x[k] / sqrt(n2);
-}
-
-// partial[j * kJfnkBlocks + blk] = this block's share of <V[j], w>.
-__global__ void __launch_bounds__(256) jf_dots_kernel(long long len, int m, double* const* __restrict__ V,
-                                                      const double* __restrict__ w, double* __restrict__ partial) {
-    __shared__ double red[256];
-    for (int j = 0; j < m; ++j) {
-        const double* __restrict__ v = V[j];
-        double s = 0.0;
-        for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < len;
-             k += (long long)gridDim.x * blockDim.x)
-            s += v[k] * w[k];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
+        for (int n = 0; n < nq; ++n) qq(n, 0) = q[n * N + c];
+        const typename Sy::VX X = Sy::center(B, e.i, e.j);
+        typename Sy::VA av{};
+        av = M::update_aux_variables(qq, av, p, time, X);
+        const typename Sy::VQ sv = M::source(qq, av, p, time, X);
+#pragma unroll
+        for (int n = 0; n < nq; ++n) {
+            q2 += qq(n, 0) * qq(n, 0);
+            if (!R::listed(n) && sv(n, 0) != 0.0) bad = true;
         }
-        if (threadIdx.x == 0) partial[j * kJfnkBlocks + blockIdx.x] = red[0];
-        __syncthreads();
-    }
-}
-
-// partial[blk] = this block's share of <a, b> (pointers passed directly).
-__global__ void __launch_bounds__(256) jf_dot_kernel(long long len, const double* __restrict__ a,
-                                                     const double* __restrict__ b, double* __restrict__ partial) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < len;
-         k += (long long)gridDim.x * blockDim.x)
-        s += a[k] * b[k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-// out[j] = the sum of partial[j * kJfnkBlocks + 0 .. kJfnkBlocks), one block
-// per j: each thread adds its run of kJfnkBlocks / 256 partials in order, then
-// a fixed tree over the threads (deterministic, and ~10x faster than one
-// thread walking all the partials).
-__global__ void __launch_bounds__(256) jf_sum_partials_kernel(int m, const double* __restrict__ partial,
-                                                              double* __restrict__ out) {
-    static_assert(kJfnkBlocks % 256 == 0, "partials per thread");
-    __shared__ double red[256];
-    const int j = blockIdx.x;
-    if (j >= m) return;
-    constexpr int per = kJfnkBlocks / 256;
-    const double* __restrict__ p = partial + (long long)j * kJfnkBlocks + threadIdx.x * per;
-    double s = 0.0;
 #pragma unroll
-    for (int b = 0; b < per; ++b) s += p[b];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+        for (int a = 0; a < R::n; ++a) {
+            const double F = (qq(R::row(a), 0) - qexp[a * N + c]) + (-dt) * sv(R::row(a), 0);
+            S[a * N + c] = sv(R::row(a), 0);
+            rhs[a * N + c] = -F;
+            f2 += F * F;
+        }
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+    if (bad) st->bad_rows = 1;
+    double v = jf_block_sum(f2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    v = jf_block_sum(q2, lds);
+    if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = v;
+}
+
+// Newton's convergence test and the start of the GMRES cycle.
+__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
+                                                      int it) {
+    __shared__ double lds[4];
+    __shared__ double tot[2];
+    if (st->nd || st->newton_its != it || (it > 0 && st->phase != kJfFinished)) return;
+    jf_sums(part, nblk, 2, tot, lds);
+    const double resid = sqrt(tot[0]);
+    const double r0 = it == 0 ? resid : st->resid0;
+    const bool conv = resid < 1e-12 || (r0 > 0.0 && resid < 1e-9 * r0) || st->bad_rows;
+    if (!conv) jf_clear_cycle(st);
+    if (threadIdx.x != 0) return;
+    if (it == 0) st->resid0 = resid;
+    st->resid = resid;
+    if (conv) {
+        st->nd = 1;
+        jf_status(st, hs);
+        return;
+    }
+    st->eps_num = 1e-7 * (1.0 + sqrt(tot[1]));
+    st->r0 = resid;  // |rhs| = |F|
+    st->vnorm[0] = resid;
+    st->src0 = 0;
+    st->k = 0;
+    st->kd = 0;
+    st->done = 0;
+    st->first = 1;
+    st->its = 0;
+    st->g[0] = resid;
+    st->phase = kJfArnoldi;
+    jf_status(st, hs);
+}
+
+// Arnoldi pass 1: V_k = src / vnorm_k (stored; src = rhs or r at k = 0, the
+// orthogonalised w after), w = J V_k, partials of <V_j, w>, j <= k.  The
+// operator's |V_k| is taken as 1: V_k is a normalised vector (the reference
+// recomputes its norm, 1 to rounding).
+template <class M>
+__global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restrict__ boxes,
+                                                          const long long* __restrict__ bstart, int nb, long long N,
+                                                          const double* __restrict__ q, const double* __restrict__ S,
+                                                          const double* __restrict__ rhs, const double* __restrict__ r,
+                                                          double* __restrict__ w, double* const* __restrict__ V,
+                                                          double dt, Params pp, double time, const JfState* st,
+                                                          double* part) {
+    using Sy = Sys<M>;
+    using R = SourceRows<M>;
+    constexpr int nq = Sy::nq, na = R::n;
+    __shared__ double lds[4];
+    if (st->nd || st->phase != kJfArnoldi) return;
+    const int k = st->k;
+    const double eps_num = st->eps_num;
+    const double* __restrict__ bk = k > 0 ? w : (st->src0 == 0 ? rhs : r);
+    const double vk = st->vnorm[k];
+    double* __restrict__ vout = V[k];
+    typename Sy::VP p;
+    for (int i = 0; i < Sy::np; ++i) p(i, 0) = pp.v[i];
+    double uc[kJfCellsPerThread][na], wc[kJfCellsPerThread][na];
+    JF_CELLS_LOOP {
+        double qc[nq], sb[na];
+#pragma unroll
+        for (int n = 0; n < nq; ++n) qc[n] = q[n * N + c];
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+            uc[pc][a] = bk[a * N + c] / vk;
+            sb[a] = S[a * N + c];
+        }
+        jf_apply_cell<M>(boxes, bstart, nb, c, qc, uc[pc], sb, eps_num, 1.0, dt, p, time, wc[pc]);
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+            vout[a * N + c] = uc[pc][a];
+            w[a * N + c] = wc[pc][a];
+        }
+    }
+    for (int j = 0; j <= k; ++j) {
+        const double* __restrict__ vj = V[j];
+        double sacc = 0.0;
+        JF_CELLS_LOOP {
+#pragma unroll
+            for (int a = 0; a < na; ++a) sacc += (j == k ? uc[pc][a] : vj[a * N + c]) * wc[pc][a];
+        }
+        const double v = jf_block_sum(sacc, lds);
+        if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// The Gram-Schmidt coefficients of pass 1 (WHICH 0: h1) or 2 (WHICH 1: h2).
+template <int WHICH>
+__global__ void __launch_bounds__(256) jf_coef_tail(JfState* st, const double* part, int nblk) {
+    __shared__ double lds[4];
+    if (st->nd || st->phase != kJfArnoldi) return;
+    jf_sums(part, nblk, st->k + 1, WHICH == 0 ? st->h1 : st->h2, lds);
+}
+
+// Arnoldi passes 2 (PASS 0) and 3 (PASS 1) over the len vector elements:
+// w -= h_j V_j for j = 0..k in order, w written back, then partials of
+// <V_j, w> (pass 2) or of |w|^2 (pass 3).
+template <int PASS>
+__global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double* __restrict__ w,
+                                                           double* const* __restrict__ V, const JfState* st,
+                                                           double* part) {
+    __shared__ double lds[4];
+    __shared__ double hv[kJfRestart + 1];
+    if (st->nd || st->phase != kJfArnoldi) return;
+    const int k = st->k;
+    if ((int)threadIdx.x <= k) hv[threadIdx.x] = PASS == 0 ? st->h1[threadIdx.x] : st->h2[threadIdx.x];
+    __syncthreads();
+    constexpr int kE = kJfCellsPerThread * 4;  // elements per thread
+    double we[kE];
+    const long long base = (long long)blockIdx.x * kE * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const long long i = base + (long long)e * blockDim.x;
+        we[e] = i < len ? w[i] : 0.0;
+    }
+    for (int j = 0; j <= k; ++j) {
+        const double* __restrict__ vj = V[j];
+        const double h = hv[j];
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            const long long i = base + (long long)e * blockDim.x;
+            if (i < len) we[e] -= h * vj[i];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const long long i = base + (long long)e * blockDim.x;
+        if (i < len) w[i] = we[e];
+    }
+    const int m = PASS == 0 ? k + 1 : 1;
+    for (int j = 0; j < m; ++j) {
+        double sacc = 0.0;
+        if (PASS == 0) {
+            const double* __restrict__ vj = V[j];
+#pragma unroll
+            for (int e = 0; e < kE; ++e) {
+                const long long i = base + (long long)e * blockDim.x;
+                if (i < len) sacc += vj[i] * we[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < kE; ++e) sacc += we[e] * we[e];
+        }
+        const double v = jf_block_sum(sacc, lds);
+        if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// The end of an Arnoldi step: H[j][k] = h1 + h2, H[k+1][k] = |w|, Givens
+// rotations, the residual estimate, and the cycle's next move; at the end
+// of a cycle the least-squares coefficients y = H^-1 g.
+__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
+                                                      double rtol) {
+    __shared__ double lds[4];
+    __shared__ double nn[1];
+    if (st->nd || st->phase != kJfArnoldi) return;
+    jf_sums(part, nblk, 1, nn, lds);
+    if (threadIdx.x != 0) return;
+    const int k = st->k;
+    for (int j = 0; j <= k; ++j) st->H[j][k] = 0.0 + st->h1[j] + st->h2[j];
+    const double hn = sqrt(nn[0]);
+    st->H[k + 1][k] = hn;
+    for (int j = 0; j < k; ++j) {
+        const double t = st->cs[j] * st->H[j][k] + st->sn[j] * st->H[j + 1][k];
+        st->H[j + 1][k] = -st->sn[j] * st->H[j][k] + st->cs[j] * st->H[j + 1][k];
+        st->H[j][k] = t;
+    }
+    const double den = hypot(st->H[k][k], st->H[k + 1][k]);
+    st->cs[k] = den > 0.0 ? st->H[k][k] / den : 1.0;
+    st->sn[k] = den > 0.0 ? st->H[k + 1][k] / den : 0.0;
+    st->H[k][k] = den;
+    st->H[k + 1][k] = 0.0;
+    st->g[k + 1] = -st->sn[k] * st->g[k];
+    st->g[k] = st->cs[k] * st->g[k];
+    st->its += 1;
+    st->kd = k + 1;
+    const bool conv = fabs(st->g[k + 1]) < st->r0 * rtol;
+    if (conv || hn == 0.0 || st->its >= kJfMaxIter || k + 1 == kJfRestart) {
+        st->done = conv ? 1 : 0;
+        const int kd = k + 1;
+        for (int i = kd - 1; i >= 0; --i) {
+            double t = st->g[i];
+            for (int j = i + 1; j < kd; ++j) t -= st->H[i][j] * st->y[j];
+            st->y[i] = st->H[i][i] != 0.0 ? t / st->H[i][i] : 0.0;
+        }
+        st->phase = kJfUpdate;
+        jf_status(st, hs);
+    } else {
+        st->vnorm[k + 1] = hn;
+        st->k = k + 1;
+    }
+}
+
+// x += y_j V_j, j in order (x = 0 in the first cycle), per cell; converged:
+// Q += x on the source rows; else partials of |x|^2 for the restart residual.
+template <class M>
+__global__ void __launch_bounds__(256) jf_update_kernel(long long N, double* __restrict__ x, double* __restrict__ q,
+                                                        double* const* __restrict__ V, const JfState* st,
+                                                        double* part) {
+    using R = SourceRows<M>;
+    constexpr int na = R::n;
+    __shared__ double lds[4];
+    __shared__ double yv[kJfRestart];
+    if (st->nd || st->phase != kJfUpdate) return;
+    const int kd = st->kd, first = st->first, done = st->done;
+    if ((int)threadIdx.x < kd) yv[threadIdx.x] = st->y[threadIdx.x];
+    __syncthreads();
+    double x2 = 0.0;
+    JF_CELLS_LOOP {
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+            const long long i = a * N + c;
+            double xv = first ? 0.0 : x[i];
+            for (int j = 0; j < kd; ++j) xv += yv[j] * V[j][i];
+            if (done) {
+                q[R::row(a) * N + c] += 1.0 * xv;
+            } else {
+                x[i] = xv;
+                x2 += xv * xv;
+            }
+        }
+    }
+    if (done) return;
+    const double v = jf_block_sum(x2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(256) jf_update_tail(JfState* st, JfStatus* hs, const double* part, int nblk) {
+    __shared__ double lds[4];
+    __shared__ double tot[1];
+    if (st->nd || st->phase != kJfUpdate) return;
+    if (st->done) {
+        if (threadIdx.x == 0) {
+            st->gmres_total += st->its;
+            st->newton_its += 1;
+            st->phase = kJfFinished;
+            jf_status(st, hs);
+        }
+        return;
+    }
+    jf_sums(part, nblk, 1, tot, lds);
+    if (threadIdx.x == 0) {
+        st->xnorm = sqrt(tot[0]);
+        st->first = 0;
+        st->phase = kJfRestartRes;
+        jf_status(st, hs);
+    }
+}
+
+// Restart residual: r = rhs - J x, partials of |r|^2.
+template <class M>
+__global__ void __launch_bounds__(256) jf_restart_kernel(const BoxDesc* __restrict__ boxes,
+                                                         const long long* __restrict__ bstart, int nb, long long N,
+                                                         const double* __restrict__ q, const double* __restrict__ S,
+                                                         const double* __restrict__ rhs, const double* __restrict__ x,
+                                                         double* __restrict__ r, double dt, Params pp, double time,
+                                                         const JfState* st, double* part) {
+    using Sy = Sys<M>;
+    using R = SourceRows<M>;
+    constexpr int nq = Sy::nq, na = R::n;
+    __shared__ double lds[4];
+    if (st->nd || st->phase != kJfRestartRes) return;
+    typename Sy::VP p;
+    for (int i = 0; i < Sy::np; ++i) p(i, 0) = pp.v[i];
+    const double eps_num = st->eps_num, xn = st->xnorm;
+    double r2 = 0.0;
+    JF_CELLS_LOOP {
+        double qc[nq], xc[na], sb[na], wc[na];
+#pragma unroll
+        for (int n = 0; n < nq; ++n) qc[n] = q[n * N + c];
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+            xc[a] = x[a * N + c];
+            sb[a] = S[a * N + c];
+        }
+        jf_apply_cell<M>(boxes, bstart, nb, c, qc, xc, sb, eps_num, xn, dt, p, time, wc);
+#pragma unroll
+        for (int a = 0; a < na; ++a) {
+            const double rr = 1.0 * rhs[a * N + c] + (-1.0) * wc[a];
+            r[a * N + c] = rr;
+            r2 += rr * rr;
+        }
+    }
+    const double v = jf_block_sum(r2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// Converged: Q update; else a new cycle from r.
+__global__ void __launch_bounds__(256) jf_restart_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
+                                                       double rtol) {
+    __shared__ double lds[4];
+    __shared__ double tot[1];
+    if (st->nd || st->phase != kJfRestartRes) return;
+    jf_sums(part, nblk, 1, tot, lds);
+    const double rn = sqrt(tot[0]);
+    const bool fin = rn < st->r0 * rtol || rn == 0.0 || st->its >= kJfMaxIter;
+    if (!fin) jf_clear_cycle(st);
+    if (threadIdx.x != 0) return;
+    if (fin) {
+        st->phase = kJfQUpdate;
+        jf_status(st, hs);
+        return;
+    }
+    st->vnorm[0] = rn;
+    st->src0 = 1;
+    st->k = 0;
+    st->kd = 0;
+    st->done = 0;
+    st->g[0] = rn;
+    st->phase = kJfArnoldi;
+    jf_status(st, hs);
+}
+
+// Q += x on the source rows after a restart residual met the tolerance.
+template <class M>
+__global__ void __launch_bounds__(256) jf_qupdate_kernel(long long N, double* __restrict__ q,
+                                                         const double* __restrict__ x, const JfState* st) {
+    using R = SourceRows<M>;
+    if (st->nd || st->phase != kJfQUpdate) return;
+    JF_CELLS_LOOP {
+#pragma unroll
+        for (int a = 0; a < R::n; ++a) q[R::row(a) * N + c] += 1.0 * x[a * N + c];
+    }
+}
+
+__global__ void jf_qupdate_tail(JfState* st, JfStatus* hs) {
+    if (st->nd || st->phase != kJfQUpdate || threadIdx.x != 0) return;
+    st->gmres_total += st->its;
+    st->newton_its += 1;
+    st->phase = kJfFinished;
+    jf_status(st, hs);
 }
 
 // End of a step whose stages solved the source globally: Heun with buffer
-// `old` (order 2; old < 0 at order 1), end-of-step UpdateState and the CFL
-// max, in place on buffer `dst` — the arithmetic of the fused stage-2
-// epilogue.
+// `old` (order 2; old < 0 at order 1 or when already applied), end-of-step
+// UpdateState and the CFL max, in place on buffer `dst` — the arithmetic of
+// the fused stage-2 epilogue.  heun_only: Heun alone, counting the MOOD
+// troubled cells of the candidate (h < 0, NaN or +-inf) into the group.
 template <class M, class N>
 __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restrict__ boxes,
                                                          const WorkItem* __restrict__ items, int nitems, int dst,
                                                          int old, Params pp, int clamp,
-                                                         GroupState* __restrict__ gs, int txw) {
+                                                         GroupState* __restrict__ gs, int txw, int heun_only) {
     using S = Sys<M>;
     constexpr int nq = S::nq, dim = S::dim;
     typename S::VP p;
@@ -228,6 +635,7 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
         const double time = G->time;
         const double rdx = 1.0 / B.dx0, rdy = 1.0 / B.dx1;
         double lam = 0.0;
+        int nbad = 0;
         if (t < txw && c <= B.hi0 && G->active) {
             double* __restrict__ qd = qsel(B, dst);
             const double* __restrict__ qo = old >= 0 ? qsel(B, old) : nullptr;
@@ -239,6 +647,12 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
                 if (qo) {
 #pragma unroll
                     for (int n = 0; n < nq; ++n) qn(n, 0) = 0.5 * qo[idx + n * B.sc] + 0.5 * qn(n, 0);
+                }
+                if (heun_only) {
+                    nbad += (!(qn(1, 0) >= 0.0) || !(qn(1, 0) < 1.0e300)) ? 1 : 0;
+#pragma unroll
+                    for (int n = 0; n < nq; ++n) qd[idx + n * B.sc] = qn(n, 0);
+                    continue;
                 }
                 qn = S::update_state(qn, p, clamp);
                 const typename S::VX X = S::center(B, c, j);
@@ -255,54 +669,67 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
                 for (int n = 0; n < nq; ++n) qd[idx + n * B.sc] = qn(n, 0);
             }
         }
+        if (heun_only) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) nbad += __shfl_xor(nbad, off, 64);
+            if (t == 0 && nbad > 0) atomicAdd(&G->troubled, nbad);
+            continue;
+        }
         lam = wave_max(lam);
         if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
     }
 }
 
-// Host side: the vectors, Newton and GMRES.  Scalars that only steer device
-// work (Gram-Schmidt coefficients, norms for eps and normalisation) stay on
-// the device; one transfer per Arnoldi step brings down what the host needs
-// for the Hessenberg matrix and the convergence test.
+// Host side: the vectors and the queueing.  The host queues Newton
+// iterations ahead -- each the residual pass, a round of Arnoldi steps (as
+// many as the last solve's iterations took on average) and the cycle's
+// update -- as far as the last solve went, its converging residual test
+// included, then waits once.  Kernels act only in their phase (a Newton
+// iteration only after the previous one finished), so over-queued work
+// returns at once and under-queued work is completed by further rounds
+// (more Arnoldi steps, a restart, the next iteration) after reading the
+// state back.  The kernels write the few words the host reads into
+// host-visible memory.
 template <class M>
 class Jfnk {
    public:
     static constexpr int nq = M::n_dof_q;
-    static constexpr int kRestart = 30;
-    static constexpr long kMaxIter = 2000;
-    // device scalar slots
-    static constexpr int kH1 = 0, kH2 = 32, kHN = 64, kNX = 65, kF2 = 66, kQ2 = 67, kR2 = 68, kSlots = 72;
+    static constexpr int na = SourceRows<M>::n;
+    static constexpr int kRestart = kJfRestart;
 
     hipStream_t stream = nullptr;
     const BoxDesc* boxes = nullptr;
     int nb = 0;
-    long long N = 0, len = 0;  // cells, doubles per vector
+    long long N = 0, len = 0;  // cells, doubles per Krylov vector (source rows)
+    int nblk = 0;              // blocks of a vector pass
     long long* d_bstart = nullptr;
-    double *q = nullptr, *qexp = nullptr, *S = nullptr, *F = nullptr, *rhs = nullptr, *x = nullptr, *w = nullptr,
-           *r = nullptr;
-    std::vector<double*> V;        // basis vectors, kRestart + 1
-    double* d_partial = nullptr;   // (kRestart + 2) x kJfnkBlocks
-    double* d_sc = nullptr;        // kSlots device scalars
-    double* d_y = nullptr;         // kRestart coefficients (uploaded)
-    double** d_Vsel = nullptr;     // pointer lists, one row per use
-    double* h_pin = nullptr;       // pinned: scalars down, coefficients up
-    long newton_its = 0, gmres_its = 0;          // last solve
+    double *q = nullptr, *qexp = nullptr, *S = nullptr, *rhs = nullptr, *x = nullptr, *w = nullptr, *r = nullptr;
+    std::vector<double*> V;   // basis vectors, kRestart + 1
+    double** d_V = nullptr;   // their device pointer list
+    double* d_part = nullptr; // (kRestart + 1) x nblk partial sums
+    JfState* d_st = nullptr;
+    JfStatus* h_status = nullptr;  // host-visible, written by the kernels
+    long newton_its = 0, gmres_its = 0;  // last solve
     long long total_solves = 0, total_newton = 0, total_gmres = 0;
+    int chunk = 2;        // Arnoldi steps queued in the first round
+    int newton_pred = 0;  // Newton iterations of the last solve
 
     ~Jfnk() { release(); }
     void release() {
-        for (double* p : {q, qexp, S, F, rhs, x, w, r, d_partial, d_sc, d_y})
+        for (double* p : {q, qexp, S, rhs, x, w, r, d_part})
             if (p) hipFree(p);
         for (double* p : V)
             if (p) hipFree(p);
         V.clear();
-        if (d_Vsel) hipFree(d_Vsel);
+        if (d_V) hipFree(d_V);
+        if (d_st) hipFree(d_st);
         if (d_bstart) hipFree(d_bstart);
-        if (h_pin) hipHostFree(h_pin);
-        q = qexp = S = F = rhs = x = w = r = d_partial = d_sc = d_y = nullptr;
-        d_Vsel = nullptr;
+        if (h_status) hipHostFree(h_status);
+        q = qexp = S = rhs = x = w = r = d_part = nullptr;
+        d_V = nullptr;
+        d_st = nullptr;
+        h_status = nullptr;
         d_bstart = nullptr;
-        h_pin = nullptr;
     }
 
     // Vectors for a level with the given boxes (lo, hi per box, in order).
@@ -314,162 +741,98 @@ class Jfnk {
         std::vector<long long> st(nb + 1, 0);
         for (int b = 0; b < nb; ++b) st[b + 1] = st[b] + (long long)(bx[b][2] - bx[b][0] + 1) * (bx[b][3] - bx[b][1] + 1);
         N = st[nb];
-        len = N * nq;
+        len = N * na;
+        // a cell pass covers kJfCellsPerThread cells per thread, an element
+        // pass 4 kJfCellsPerThread elements per thread: one grid for both,
+        // the larger of the two needs
+        const long long per_blk_cells = (long long)kJfCellsPerThread * 256;
+        const long long per_blk_elems = (long long)kJfCellsPerThread * 4 * 256;
+        nblk = (int)std::max((N + per_blk_cells - 1) / per_blk_cells, (len + per_blk_elems - 1) / per_blk_elems);
+        nblk = std::max(nblk, 1);
         ZAMR_HIP(hipMalloc(&d_bstart, st.size() * sizeof(long long)));
         ZAMR_HIP(hipMemcpy(d_bstart, st.data(), st.size() * sizeof(long long), hipMemcpyHostToDevice));
-        for (double** p : {&q, &qexp, &S, &F, &rhs, &x, &w, &r}) ZAMR_HIP(hipMalloc(p, len * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&q, (size_t)N * nq * sizeof(double)));
+        for (double** p : {&qexp, &S, &rhs, &x, &w, &r}) ZAMR_HIP(hipMalloc(p, len * sizeof(double)));
         V.assign(kRestart + 1, nullptr);
         for (auto& v : V) ZAMR_HIP(hipMalloc(&v, len * sizeof(double)));
-        // pointer-list rows: row k (k <= kRestart) = V[0..k]
-        ZAMR_HIP(hipMalloc(&d_Vsel, (size_t)(kRestart + 1) * (kRestart + 1) * sizeof(double*)));
-        std::vector<double*> rows((size_t)(kRestart + 1) * (kRestart + 1), nullptr);
-        for (int k = 0; k <= kRestart; ++k)
-            for (int j = 0; j <= k; ++j) rows[(size_t)k * (kRestart + 1) + j] = V[j];
-        ZAMR_HIP(hipMemcpy(d_Vsel, rows.data(), rows.size() * sizeof(double*), hipMemcpyHostToDevice));
-        ZAMR_HIP(hipMalloc(&d_partial, (kRestart + 2) * (size_t)kJfnkBlocks * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_sc, kSlots * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_y, kRestart * sizeof(double)));
-        ZAMR_HIP(hipHostMalloc(&h_pin, (kSlots + kRestart) * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_V, V.size() * sizeof(double*)));
+        ZAMR_HIP(hipMemcpy(d_V, V.data(), V.size() * sizeof(double*), hipMemcpyHostToDevice));
+        ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nblk * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_st, sizeof(JfState)));
+        ZAMR_HIP(hipHostMalloc(&h_status, sizeof(JfStatus), hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(h_status, 0, sizeof(JfStatus));
     }
 
     static unsigned g1(long long n) { return (unsigned)((n + 255) / 256); }
-    double* const* basis(int k) const { return d_Vsel + (size_t)k * (kRestart + 1); }  // V[0..k]
-
-    // out[j] = <vs[j], y> for the m vectors of a pointer row, on the device.
-    void dots_dev(double* const* vs, int m, const double* y, double* out) {
-        if (m <= 0) return;
-        hipLaunchKernelGGL(jf_dots_kernel, dim3(kJfnkBlocks), dim3(256), 0, stream, len, m, vs, y, d_partial);
-        hipLaunchKernelGGL(jf_sum_partials_kernel, dim3((unsigned)m), dim3(256), 0, stream, m, d_partial, out);
-        ZAMR_HIP(hipGetLastError());
-    }
-    // <y, y> into a device slot.
-    void norm2_dev(const double* y, double* out) {
-        hipLaunchKernelGGL(jf_dot_kernel, dim3(kJfnkBlocks), dim3(256), 0, stream, len, y, y, d_partial);
-        hipLaunchKernelGGL(jf_sum_partials_kernel, dim3(1), dim3(256), 0, stream, 1, d_partial, out);
-        ZAMR_HIP(hipGetLastError());
-    }
-    // Device scalars [first, first + n) to the host.
-    void fetch(int first, int n) {
-        ZAMR_HIP(hipMemcpyAsync(h_pin + first, d_sc + first, n * sizeof(double), hipMemcpyDeviceToHost, stream));
-        ZAMR_HIP(hipStreamSynchronize(stream));
-    }
-    void lincomb(double* out, double a, const double* xx, double b, const double* yy) {
-        hipLaunchKernelGGL(jf_lincomb_kernel, dim3(g1(len)), dim3(256), 0, stream, len, a, xx, b, yy, out);
-        ZAMR_HIP(hipGetLastError());
-    }
-
-    // out = J v at the frozen iterate (q, S); |v|^2 is computed into slot kNX.
-    void apply(const double* v, double* out, double dt, double nrmQ, const Params& p, double time) {
-        norm2_dev(v, d_sc + kNX);
-        hipLaunchKernelGGL((jf_source_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, q, v,
-                           1e-7 * (1.0 + nrmQ), (const double*)(d_sc + kNX), dt, S, p, time, out);
-        ZAMR_HIP(hipGetLastError());
-    }
-
-    // x = GMRES(J, rhs) to rtol (zero initial guess), |rhs| = r0 known.
-    long gmres(double r0, double dt, double nrmQ, const Params& p, double time, double rtol) {
-        ZAMR_HIP(hipMemsetAsync(x, 0, len * sizeof(double), stream));
-        if (r0 == 0.0) return 0;
-        auto conv = [&](double rr) { return rr < r0 * rtol; };
-        const double* rr = rhs;  // the residual of x = 0
-        double rnorm = r0;
-        long its = 0;
-        while (its < kMaxIter) {
-            lincomb(V[0], 1.0 / rnorm, rr, 0.0, rr);
-            std::vector<double> g(kRestart + 1, 0.0), cs(kRestart, 0.0), sn(kRestart, 0.0);
-            std::vector<std::vector<double>> H(kRestart + 1, std::vector<double>(kRestart, 0.0));
-            g[0] = rnorm;
-            int kd = 0;
-            bool done = false;
-            for (int k = 0; k < kRestart; ++k) {
-                apply(V[k], w, dt, nrmQ, p, time);
-                // classical Gram-Schmidt, twice, coefficients kept on the device
-                dots_dev(basis(k), k + 1, w, d_sc + kH1);
-                hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, k + 1, basis(k),
-                                   (const double*)(d_sc + kH1), -1.0, w);
-                dots_dev(basis(k), k + 1, w, d_sc + kH2);
-                hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, k + 1, basis(k),
-                                   (const double*)(d_sc + kH2), -1.0, w);
-                norm2_dev(w, d_sc + kHN);
-                hipLaunchKernelGGL(jf_normalize_kernel, dim3(g1(len)), dim3(256), 0, stream, len, w,
-                                   (const double*)(d_sc + kHN), V[k + 1]);
-                ZAMR_HIP(hipGetLastError());
-                fetch(0, kHN + 1);
-                for (int j = 0; j <= k; ++j) H[j][k] = h_pin[kH1 + j] + h_pin[kH2 + j];
-                const double hn = std::sqrt(h_pin[kHN]);
-                H[k + 1][k] = hn;
-                for (int j = 0; j < k; ++j) {
-                    const double t = cs[j] * H[j][k] + sn[j] * H[j + 1][k];
-                    H[j + 1][k] = -sn[j] * H[j][k] + cs[j] * H[j + 1][k];
-                    H[j][k] = t;
-                }
-                const double den = std::hypot(H[k][k], H[k + 1][k]);
-                cs[k] = den > 0.0 ? H[k][k] / den : 1.0;
-                sn[k] = den > 0.0 ? H[k + 1][k] / den : 0.0;
-                H[k][k] = den;
-                H[k + 1][k] = 0.0;
-                g[k + 1] = -sn[k] * g[k];
-                g[k] = cs[k] * g[k];
-                ++its;
-                kd = k + 1;
-                if (conv(std::fabs(g[k + 1])) || hn == 0.0 || its >= kMaxIter) {
-                    done = conv(std::fabs(g[k + 1]));
-                    break;
-                }
-            }
-            std::vector<double> y(kd, 0.0);
-            for (int i = kd - 1; i >= 0; --i) {
-                double t = g[i];
-                for (int j = i + 1; j < kd; ++j) t -= H[i][j] * y[j];
-                y[i] = H[i][i] != 0.0 ? t / H[i][i] : 0.0;
-            }
-            for (int j = 0; j < kd; ++j) h_pin[kSlots + j] = y[j];
-            ZAMR_HIP(hipMemcpyAsync(d_y, h_pin + kSlots, kd * sizeof(double), hipMemcpyHostToDevice, stream));
-            hipLaunchKernelGGL(jf_multi_axpy_kernel, dim3(g1(len)), dim3(256), 0, stream, len, kd, basis(kd - 1),
-                               (const double*)d_y, 1.0, x);
-            ZAMR_HIP(hipGetLastError());
-            ZAMR_HIP(hipStreamSynchronize(stream));  // the pinned coefficients are reused
-            if (done) break;
-            apply(x, w, dt, nrmQ, p, time);
-            lincomb(r, 1.0, rhs, -1.0, w);
-            rr = r;
-            norm2_dev(r, d_sc + kR2);
-            fetch(kR2, 1);
-            rnorm = std::sqrt(h_pin[kR2]);
-            if (conv(rnorm) || rnorm == 0.0) break;
-        }
-        return its;
-    }
 
     // The solve on the valid cells of buffer `buf` (Qexp in, Q out).
     void solve(int buf, double dt, double time, const Params& p) {
-        hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 0);
+        constexpr double rtol = 1e-4;
+        const dim3 G(nblk), T(256), one(1);
+        JfStatus* hs = nullptr;
+        ZAMR_HIP(hipHostGetDevicePointer((void**)&hs, h_status, 0));
+        double* const* dV = (double* const*)d_V;
+        const double* cpart = d_part;
+        hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 0,
+                           qexp);
+        hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st, hs);
         ZAMR_HIP(hipGetLastError());
-        ZAMR_HIP(hipMemcpyAsync(qexp, q, len * sizeof(double), hipMemcpyDeviceToDevice, stream));
-        double resid0 = -1.0;
-        newton_its = 0;
-        gmres_its = 0;
-        for (int it = 0; it < 12; ++it) {
-            hipLaunchKernelGGL((jf_source_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, q,
-                               (const double*)nullptr, 0.0, (const double*)nullptr, 0.0, (const double*)nullptr, p,
-                               time, S);
+        auto arnoldi = [&](int steps) {
+            for (int k = 0; k < steps; ++k) {
+                hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
+                                   (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time,
+                                   (const JfState*)d_st, d_part);
+                hipLaunchKernelGGL(jf_coef_tail<0>, one, T, 0, stream, d_st, cpart, nblk);
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, G, T, 0, stream, len, w, dV, (const JfState*)d_st, d_part);
+                hipLaunchKernelGGL(jf_coef_tail<1>, one, T, 0, stream, d_st, cpart, nblk);
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, G, T, 0, stream, len, w, dV, (const JfState*)d_st, d_part);
+                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, hs, cpart, nblk, rtol);
+            }
+            hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);
+            hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, hs, cpart, nblk);
             ZAMR_HIP(hipGetLastError());
-            lincomb(F, 1.0, q, -1.0, qexp);
-            lincomb(F, 1.0, F, -dt, S);
-            norm2_dev(F, d_sc + kF2);
-            norm2_dev(q, d_sc + kQ2);
-            fetch(kF2, 2);
-            const double resid = std::sqrt(h_pin[kF2]);
-            if (it == 0) resid0 = resid;
-            if (resid < 1e-12 || (resid0 > 0 && resid < 1e-9 * resid0)) break;
-            lincomb(rhs, -1.0, F, 0.0, F);
-            const double nrmQ = std::sqrt(h_pin[kQ2]);
-            gmres_its += gmres(resid, dt, nrmQ, p, time, 1e-4);
-            lincomb(q, 1.0, q, 1.0, x);
-            ++newton_its;
+        };
+        // Newton iteration `it`: the residual pass, then a round of Arnoldi
+        // steps and the cycle's update; queued ahead as far as the last solve
+        // went (its converging residual test included), then read back once
+        auto iteration = [&](int it, int steps) {
+            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
+                               (const double*)qexp, S, rhs, dt, p, time, d_st, d_part, it);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, hs, cpart, nblk, it);
+            arnoldi(steps);
+        };
+        const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
+        for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2));
+        ZAMR_HIP(hipStreamSynchronize(stream));
+        long long rounds = 0;
+        for (;;) {
+            if (h_status->nd || h_status->newton_its >= kJfNewtonMax) break;
+            if (++rounds > kJfMaxIter || h_status->its >= kJfMaxIter)
+                throw Error(4, "internal: the global source solve's GMRES did not settle");
+            if (h_status->phase == kJfFinished) {  // the next iteration was not queued, or ran early
+                iteration((int)h_status->newton_its, std::max(chunk, 2));
+            } else {  // the running cycle needs more steps, a restart, or its update
+                hipLaunchKernelGGL((jf_restart_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
+                                   (const double*)S, (const double*)rhs, (const double*)x, r, dt, p, time,
+                                   (const JfState*)d_st, d_part);
+                hipLaunchKernelGGL(jf_restart_tail, one, T, 0, stream, d_st, hs, cpart, nblk, rtol);
+                hipLaunchKernelGGL((jf_qupdate_kernel<M>), G, T, 0, stream, N, q, (const double*)x,
+                                   (const JfState*)d_st);
+                hipLaunchKernelGGL(jf_qupdate_tail, one, dim3(64), 0, stream, d_st, hs);
+                arnoldi(4);
+            }
+            ZAMR_HIP(hipStreamSynchronize(stream));
         }
-        hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 1);
+        const int it = (int)h_status->newton_its;
+        if (it > 0) chunk = (int)std::min<long long>(std::max<long long>(h_status->gmres_total / it, 2), kRestart);
+        if (h_status->bad_rows)
+            throw Error(4, "the global source solve: a row the model declares source-free has a nonzero source");
+        newton_pred = it;
+        hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 1,
+                           (double*)nullptr);
         ZAMR_HIP(hipGetLastError());
+        newton_its = h_status->newton_its;
+        gmres_its = (long)h_status->gmres_total;
         ++total_solves;
         total_newton += newton_its;
         total_gmres += gmres_its;

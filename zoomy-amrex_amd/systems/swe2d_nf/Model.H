@@ -24,6 +24,8 @@
 // Extra members: `zamr_aux_is_local` (the live aux row is a per-cell function
 // of (Q, p)) and `zamr_aux_in` = {1}: aux row 1 is a stored per-cell input that
 // only the source and its Jacobians read.
+// `zamr_source_rows`: the rows `source` can make nonzero (the global
+// source solve carries only those).
 #include <AMReX_Array4.H>
 #include <AMReX_Vector.H>
 #include <AMReX_SmallMatrix.H>
@@ -51,6 +53,7 @@ struct Model {
     static constexpr bool has_diffusion = false;
     static constexpr int n_boundary_tags = 5;
     static constexpr bool zamr_aux_is_local = true;
+    static constexpr int zamr_source_rows[2] = {2, 3};  // Manning friction: momentum rows only
     static constexpr int zamr_n_aux_in = 1;          // aux rows read from storage (per-cell inputs)
     static constexpr int zamr_aux_in[1] = {1};
     static constexpr int zamr_idx_hinv = 6;

@@ -1062,8 +1062,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_m
             }
             if (cell) {
                 const long long idx = isw;
+                // A first stage's output (read by the next stage only) goes
+                // through the L2, where the halves of the cache lines that
+                // neighbouring strips share meet before write-back (streaming
+                // stores wrote them back as two partial lines: 1.11x the bytes);
+                // the step's output streams past the caches.
 #pragma unroll
-                for (int n = 0; n < nq; ++n) __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);
+                for (int n = 0; n < nq; ++n) {
+                    if constexpr (EPI == 0) qd[idx + n * B.sc] = qn(n, 0);
+                    else __builtin_nontemporal_store(qn(n, 0), qd + idx + n * B.sc);
+                }
             }
             // carry the top face and the slope down and shift the window
             if constexpr (dim == 2) {

@@ -150,11 +150,14 @@ def bench_adjoint(reps=3):
     (segment recompute + reverse stages) and of the checkpointed forward,
     over the wall time of the plain graph-captured forward of the same 200
     steps from the same state.  Host copies of the gauge arrays are inside
-    the timed calls; building the input arrays is not."""
+    the timed calls; building the input arrays is not.  The PyTorch leg times
+    the whole zero-copy GaugeSeries round on CUDA tensors: the initial state
+    built from (b, n) in torch, the forward, the misfit and loss.backward()
+    down to the gradients of b and n."""
     import numpy as np
     import torch
     from zamr import cases
-    from zamr.adjoint import GaugeRun, frozen_dts
+    from zamr.adjoint import GaugeRun, GaugeSeries, frozen_dts
     from zamr.deck import deck_from_case
     from zamr.lib import ZoomyAmr
 
@@ -168,7 +171,21 @@ def bench_adjoint(reps=3):
     plain.init_data()
     aux = plain.get_state(field=1)
     aux[1] = c["n0"]
-    tf, tc, tb = [], [], []
+    dev = torch.device("cuda")
+    b_t, n_t, eta_t = (torch.tensor(a, device=dev) for a in (c["b0"], c["n0"], c["eta0"]))
+    truth_t = torch.tensor(truth, device=dev)
+
+    def torch_round():
+        b = b_t.clone().requires_grad_(True)
+        nf = n_t.clone().requires_grad_(True)
+        h = torch.clamp(eta_t - b, min=0.0)
+        z = torch.zeros_like(h)
+        G = GaugeSeries.apply(torch.stack([b, h, z, z]), nf, run)
+        loss = 0.5 * ((G - truth_t) ** 2).sum()
+        loss.backward()
+        return b.grad, nf.grad
+
+    tf, tc, tb, tt = [], [], [], []
     for r in range(reps + 1):
         plain.set_state(Q0)
         plain.set_state(aux, field=1)
@@ -186,14 +203,22 @@ def bench_adjoint(reps=3):
         t3 = time.perf_counter()
         run.z.adj_backward(G - truth)
         t4 = time.perf_counter()
+        qb, nb = run.z.adj_gradient()
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        gb_t, gn_t = torch_round()
+        torch.cuda.synchronize()
+        t6 = time.perf_counter()
         if r:  # the first round loads kernels and builds the graph
             tf.append(t1 - t0)
             tc.append(t3 - t2)
             tb.append(t4 - t3)
-    qb, nb = run.z.adj_gradient()
+            tt.append(t6 - t5)
     assert np.isfinite(qb).all() and np.isfinite(nb).all()
-    f, ck, bw = min(tf), min(tc), min(tb)
+    assert bool(torch.isfinite(gb_t).all()) and bool(torch.isfinite(gn_t).all())
+    f, ck, bw, tr = min(tf), min(tc), min(tb), min(tt)
     return {"adjoint_over_forward": bw / f, "gradient_over_forward": (ck + bw) / f,
+            "torch_gradient_over_forward": tr / f, "torch_gradient_ms": 1e3 * tr,
             "forward_ms": 1e3 * f, "checkpointed_forward_ms": 1e3 * ck, "backward_ms": 1e3 * bw,
             "workload": "2-D SWE 1024x1024, per-cell Manning, 200 SSP-RK2 steps (frozen dt), 256 gauges, "
                         "checkpoint every 20 steps; gradients w.r.t. the initial state and the Manning field",

@@ -239,6 +239,16 @@ int zamr_adj_gradient(zamr_ctx* ctx, double* qbar0, size_t n_qbar, double* auxba
  * the adjoint takes at slope-limiter switching points.  on = 0 turns it off.
  * A backward must follow a forward that computed its limiter choices. */
 int zamr_adj_freeze_limiter(zamr_ctx* ctx, int on);
+/* Device-resident adjoint I/O, for callers that keep their tensors on the GPU
+ * (the PyTorch wrapper).  With on = 1, zamr_adj_forward's gauge_h,
+ * zamr_adj_backward's gauge_hbar and zamr_adj_gradient's qbar0 / auxbar are
+ * device pointers (same layouts), and the three calls queue their kernels
+ * and copies on the context's stream (zamr_set_stream) and return without
+ * waiting for it.  The caller writes the initial state and the friction row
+ * in place through zamr_device_ptr (fields 0 and 1, valid cells) on that
+ * stream, and reads results in stream order.  on = 0 restores host arrays and
+ * waiting calls.  No counterpart in the reference. */
+int zamr_adj_device_io(zamr_ctx* ctx, int on);
 
 #ifdef __cplusplus
 }

@@ -76,17 +76,20 @@ def misfit_diff(Gp, Gm, truth):
     return 0.5 * np.sum((Gp - Gm) * (Gp + Gm - 2.0 * truth))
 
 
-def grads(run, c, truth):
-    b = torch.tensor(c["b0"], requires_grad=True)
-    nf = torch.tensor(c["n0"], requires_grad=True)
-    eta0 = torch.tensor(c["eta0"])
-    u, v = (torch.tensor(a) for a in c["uv"])
+def grads(run, c, truth, device="cpu"):
+    """(loss, dL/db, dL/dn) through GaugeSeries with tensors on `device`
+    (CUDA tensors take the zero-copy path)."""
+    b = torch.tensor(c["b0"], device=device, requires_grad=True)
+    nf = torch.tensor(c["n0"], device=device, requires_grad=True)
+    eta0 = torch.tensor(c["eta0"], device=device)
+    u, v = (torch.tensor(a, device=device) for a in c["uv"])
     h = torch.clamp(eta0 - b, min=0.0)
     Q0 = torch.stack([b, h, h * u, h * v])
     G = GaugeSeries.apply(Q0, nf, run)
-    loss = 0.5 * ((G - torch.tensor(truth)) ** 2).sum()
+    assert G.device == Q0.device
+    loss = 0.5 * ((G - torch.tensor(truth, device=device)) ** 2).sum()
     loss.backward()
-    return float(loss.detach()), b.grad.numpy().copy(), nf.grad.numpy().copy()
+    return float(loss.detach()), b.grad.cpu().numpy().copy(), nf.grad.cpu().numpy().copy()
 
 
 def fd_noise(Gp, truth, eps):
@@ -247,6 +250,44 @@ def test_directional_derivative():
         assert abs(an - fd) <= 1e-6 * abs(fd), (freeze, an, fd)
 
 
+def test_zero_copy_path_matches_host_path():
+    """CUDA tensors go through the library's device arrays on the caller's
+    stream (zamr_device_ptr, zamr_set_stream, zamr_adj_device_io): the same
+    loss and gradients, bit for bit, as the host-memory path, on torch's
+    default stream and on a side stream, with the host entry points never
+    called, and the host path still right after it."""
+    c, run, truth, _ = setup(n=64, steps=30, seg=7, manning=0.3, freeze=False)
+    ref = grads(run, c, truth)
+    host_calls = ("set_state", "get_state", "adj_forward", "adj_backward", "adj_gradient")
+    saved = {k: getattr(run.z, k) for k in host_calls}
+
+    def refuse(*a, **k):
+        raise AssertionError("host-memory entry point called on the zero-copy path")
+
+    for k in host_calls:
+        setattr(run.z, k, refuse)
+    try:
+        got = grads(run, c, truth, device="cuda")
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            got_side = grads(run, c, truth, device="cuda")
+        torch.cuda.synchronize()
+    finally:
+        for k, f in saved.items():
+            setattr(run.z, k, f)
+    for g in (got, got_side):
+        assert g[0] == ref[0]
+        assert np.array_equal(g[1], ref[1]) and np.array_equal(g[2], ref[2])
+    again = grads(run, c, truth)
+    assert again[0] == ref[0] and np.array_equal(again[1], ref[1]) and np.array_equal(again[2], ref[2])
+    # in device I/O mode a host array is refused before anything is queued
+    host = np.zeros((len(run.dts), len(run.gauges[0])))
+    with pytest.raises(ZamrError) as e:
+        run.z.adj_forward_device(host.ctypes.data)
+    assert e.value.code == 1
+    run.close()
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     """Recomputation from checkpoints reproduces the stored states bit for bit."""
     c, run, truth, dts = setup(steps=24, seg=24, freeze=False)
@@ -260,10 +301,12 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 def test_full_size_gradient_matches_finite_differences():
     """The 1024^2, 200-step inversion case (fluid at rest at t = 0, 256
     gauges, a checkpoint every 20 steps): central FD at the 32 seeded cells,
-    eps = 1e-6 for the bathymetry.  Over these 200 short steps (t ~ 4e-3) the
-    friction barely acts: the Manning gradients are ~1e-7 of the bathymetry
-    ones and their finite differences (eps = 1e-3) resolve them only to the
-    rounding of the misfit sums."""
+    eps = 1e-6 for the bathymetry, raw.  Over these 200 short steps (t ~ 4e-3)
+    the friction barely acts: the Manning gradients are ~1e-7 of the
+    bathymetry ones, and a single-cell difference resolves them only to the
+    rounding of the misfit sums, so the per-cell Manning check takes eps =
+    1e-3 net of that rounding; the Manning gradient is checked raw along the
+    direction of its largest derivative."""
     c, run, truth, _ = setup(n=1024, steps=200, seg=20, seed=0, ngauges=256, nfd=32, current=False)
     L, gb, gn = grads(run, c, truth)
     sb, sn = np.abs(gb).max(), np.abs(gn).max()
@@ -279,6 +322,24 @@ def test_full_size_gradient_matches_finite_differences():
     print("beyond the differences' rounding: b max %.3e; n max %.3e" % (max(eb), max(en)), flush=True)
     assert max(rb) <= 1e-6, max(rb)
     assert max(en) <= 1e-6, max(en)
+    # The Manning gradient raw, along d = sign(dL/dn): of all perturbations
+    # of unit amplitude per cell this has the largest derivative, sum |dL/dn|
+    # (~6e-5 here).  Two FP64 forwards differ by rounding of ~2e-15 in the
+    # misfit whatever the step, so a central difference carries ~1e-15/eps of
+    # absolute rounding: ~4e-5 of the derivative at eps = 1e-6 (printed), and
+    # below 1e-6 of it from eps = 1e-4 on, where the truncation (the loss is
+    # close to quadratic in n) is still far smaller.
+    d = np.sign(gn)
+    an = float(np.sum(gn * d))
+    Q0 = c["state"](c["b0"])
+    for e in (1e-6, 1e-4, 1e-3):
+        Gp = run.forward(Q0, c["n0"] + e * d)
+        Gm = run.forward(Q0, c["n0"] - e * d)
+        fd = misfit_diff(Gp, Gm, truth) / (2 * e)
+        print("Manning along sign(dL/dn), eps %.0e: adjoint %.12e FD %.12e raw rel %.3e"
+              % (e, an, fd, abs(an - fd) / abs(fd)), flush=True)
+        if e >= 1e-4:
+            assert abs(an - fd) <= 1e-6 * abs(fd), (e, an, fd)
 
 
 def test_unsupported_configurations():

@@ -378,6 +378,19 @@ __global__ void gauge_inject_kernel(double* __restrict__ qb, long long sc, const
     if (g < ng) qb[gidx[g] + sc] += gbar[g];
 }
 
+// Valid cells of a grown-box array (ncomp components, component stride sc,
+// row stride gx, ngrow ghost cells a side) into a dense (ncomp, ny, nx) array.
+__global__ void valid_gather_kernel(const double* __restrict__ src, long long sc, int gx, int ngrow, int nx, int ny,
+                                    int ncomp, double* __restrict__ dst) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)nx * ny;
+    if (k >= per * ncomp) return;
+    const int c = (int)(k / per);
+    const long long r = k - c * per;
+    const int j = (int)(r / nx), i = (int)(r - (long long)j * nx);
+    dst[k] = src[(long long)(i + ngrow) + (long long)(j + ngrow) * gx + c * sc];
+}
+
 template <class M, class N, class AD>
 struct AdjointDriver {
     using Sol = Solver<M, N>;
@@ -413,11 +426,26 @@ struct AdjointDriver {
     hipEvent_t ev_go = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     bool have_forward = false;
     bool have_backward = false;
+    // device I/O: the gauge, gauge-adjoint and gradient arrays the calls are
+    // given are device pointers, and the calls do not wait for the stream
+    bool device_io = false;
     // frozen limiter: the first forward after freeze_limiter(1) records every
     // face's minmod choices, later forwards replay them
     bool freeze = false, sw_recorded = false, replayed = false;
     int fwd_sw = 0;  // stage-kernel build of the last forward (the recompute uses the same)
     unsigned* d_sw = nullptr;
+
+    // In device I/O mode every array the caller hands over must be device
+    // (or managed) memory; a host address there would be copied as if it
+    // were device memory.
+    static void require_device(const void* p, const char* what) {
+        if (!p) return;
+        hipPointerAttribute_t at{};
+        const hipError_t e = hipPointerGetAttributes(&at, p);
+        if (e != hipSuccess) (void)hipGetLastError();
+        if (e != hipSuccess || (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged))
+            throw Error(ZAMR_EINVAL, std::string("adjoint: device I/O mode, but ") + what + " is not device memory");
+    }
 
     explicit AdjointDriver(Sol& sol) : s(sol) {}
     ~AdjointDriver() { release(); }
@@ -643,7 +671,9 @@ struct AdjointDriver {
     void forward(double* gh) {
         if (!pool) throw Error(ZAMR_EINVAL, "adjoint: call zamr_adj_setup first");
         check_config();
-        s.ensure_aux();
+        if (device_io) require_device(gh, "gauge_h");
+        // (the stage kernels evaluate Qaux per cell from the state and the
+        // stored input rows, so a stale stored Qaux is not read here)
         const int sw = freeze ? (sw_recorded ? 2 : 1) : 0;
         slot[0].held = slot[1].held = -1;
         for (int m = 0; m < nseg; ++m) {
@@ -666,8 +696,9 @@ struct AdjointDriver {
         s.aux_stale = true;
         s.maxinv_fresh = false;
         if (gh && ng > 0)
-            ZAMR_HIP(hipMemcpyAsync(gh, d_gh, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        ZAMR_HIP(hipStreamSynchronize(s.stream));
+            ZAMR_HIP(hipMemcpyAsync(gh, d_gh, (size_t)nsteps * ng * sizeof(double),
+                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.stream));
+        if (!device_io) ZAMR_HIP(hipStreamSynchronize(s.stream));
         have_forward = true;
         have_backward = false;
     }
@@ -717,8 +748,9 @@ struct AdjointDriver {
                                      "linearises a forward that computed them (freeze off, or the recording forward)");
         if (ng > 0) {
             if (!gbar) throw Error(ZAMR_EINVAL, "adjoint: gauge adjoint array is null");
-            ZAMR_HIP(hipMemcpyAsync(d_gbar, gbar, (size_t)nsteps * ng * sizeof(double), hipMemcpyHostToDevice,
-                                    s.stream));
+            if (device_io) require_device(gbar, "gauge_hbar");
+            ZAMR_HIP(hipMemcpyAsync(d_gbar, gbar, (size_t)nsteps * ng * sizeof(double),
+                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.stream));
         }
         ZAMR_HIP(hipMemsetAsync(Ab, 0, bufn * sizeof(double), s.stream));
         ZAMR_HIP(hipMemsetAsync(Bb, 0, bufn * sizeof(double), s.stream));
@@ -747,7 +779,7 @@ struct AdjointDriver {
             if (m > 0) ZAMR_HIP(hipStreamWaitEvent(s.stream, ev_rec[1 - k], 0));
             k = 1 - k;
         }
-        ZAMR_HIP(hipStreamSynchronize(s.stream));
+        if (!device_io) ZAMR_HIP(hipStreamSynchronize(s.stream));
         have_backward = true;
     }
 
@@ -759,6 +791,19 @@ struct AdjointDriver {
         const int NX = b.hi[0] - b.lo[0] + 1, NY = b.hi[1] - b.lo[1] + 1;
         if ((qbar && nqb != (size_t)nq * NX * NY) || (fbar && nfb != (size_t)NX * NY))
             throw Error(ZAMR_EINVAL, "adjoint: gradient array size mismatch");
+        if (device_io) {
+            require_device(qbar, "qbar0");
+            require_device(fbar, "auxbar");
+            auto gather = [&](const double* src, int ncomp, double* dst) {
+                const long long n = (long long)ncomp * NX * NY;
+                hipLaunchKernelGGL(valid_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, src,
+                                   (long long)cells, b.gx, b.ng, NX, NY, ncomp, dst);
+                ZAMR_HIP(hipGetLastError());
+            };
+            if (qbar) gather(Ab, nq, qbar);
+            if (fbar) gather(nbar, 1, fbar);
+            return;
+        }
         std::vector<double> tmp(cells * nq), tn(cells);
         ZAMR_HIP(hipMemcpyAsync(tmp.data(), Ab, cells * nq * sizeof(double), hipMemcpyDeviceToHost, s.stream));
         ZAMR_HIP(hipMemcpyAsync(tn.data(), nbar, cells * sizeof(double), hipMemcpyDeviceToHost, s.stream));

@@ -1081,7 +1081,8 @@ class Solver {
     // Graph replay for the device-clock loop; the per-box launch mode stays
     // eager, as the baseline it stands for (one launch per box per pass).
     bool graph_ok() const {
-        return use_graph && spatial_order == 2 && !profiling && !per_box_launch && !jfnk_mode();
+        // (the legacy null stream, which a caller may hand over, cannot be captured)
+        return use_graph && stream != nullptr && spatial_order == 2 && !profiling && !per_box_launch && !jfnk_mode();
     }
 
     void run_steps(int nsteps, int* done, double* time) {

@@ -667,6 +667,17 @@ int zamr_adj_gradient(zamr_ctx* c, double* qbar0, size_t nq_cells, double* auxba
     }
     return guard(c, [&] { c->adj->gradient(qbar0, nq_cells, auxbar, ncells); });
 }
+int zamr_adj_device_io(zamr_ctx* c, int on) {
+    if (!c || !c->s.d_pool) return ZAMR_EINVAL;
+    if (!c->adj) {
+        c->err = "adjoint: call zamr_adj_setup first";
+        return ZAMR_EINVAL;
+    }
+    return guard(c, [&] {
+        ZAMR_HIP(hipStreamSynchronize(c->s.stream));
+        c->adj->device_io = on != 0;
+    });
+}
 int zamr_adj_freeze_limiter(zamr_ctx* c, int on) {
     if (!c || !c->s.d_pool) return ZAMR_EINVAL;
     if (!c->adj) {
@@ -684,6 +695,7 @@ int zamr_adj_setup(zamr_ctx* c, int, int, const double*, int, const int*, const 
 int zamr_adj_forward(zamr_ctx* c, double*) { return no_adjoint(c); }
 int zamr_adj_backward(zamr_ctx* c, const double*) { return no_adjoint(c); }
 int zamr_adj_gradient(zamr_ctx* c, double*, size_t, double*, size_t) { return no_adjoint(c); }
+int zamr_adj_device_io(zamr_ctx* c, int) { return no_adjoint(c); }
 int zamr_adj_freeze_limiter(zamr_ctx* c, int) { return no_adjoint(c); }
 #endif
 

@@ -8,8 +8,12 @@ behind a torch.autograd.Function.
     loss = 0.5 * ((G - G_truth) ** 2).sum()
     loss.backward()                               # b.grad, n_field.grad
 
-The library does the work on the GPU; torch only carries the tensors (the
-wrapper moves them through host memory)."""
+The library does the work on the GPU.  CUDA tensors stay on the device: the
+initial state and the Manning field are copied in place into the library's
+state arrays (zamr_device_ptr), the gauge series and the gradients are
+written into torch-allocated device tensors, and everything is queued on the
+caller's current torch stream (zamr_set_stream), with no host round trip and
+no host wait.  CPU tensors go through host memory."""
 import numpy as np
 import torch
 
@@ -29,6 +33,9 @@ class GaugeRun:
         self.gauges = gauges
         self.z.adj_setup(self.dts, gauges[0], gauges[1], seg)
         self.generation = 0
+        self._stream = None  # the stream handed to the library (None: its own)
+        self._device = None
+        self._aux_loc = None
 
     def set_inputs(self, Q0, nfield):
         self.z.set_state(np.asarray(Q0, dtype=np.float64))
@@ -54,8 +61,68 @@ class GaugeRun:
         later ones (finite-difference checks through minmod switch points)."""
         self.z.adj_freeze_limiter(on)
 
+    # ---- zero-copy path: CUDA tensors on the caller's current stream ----
+    def _bind_stream(self, device):
+        st = torch.cuda.current_stream(device).cuda_stream
+        if st != self._stream:
+            self.z.set_stream(st)
+            self._stream = st
+
+    def _valid_view(self, loc, row0, nrows):
+        """Strided torch view of rows [row0, row0 + nrows) of the valid cells
+        of a library array located by zamr_device_ptr, shape (nrows, ny, nx)."""
+        ptr, rs, cs, ngrow, _ = loc
+        nx, ny, _ = self.z.shape()
+        start = ptr + 8 * (row0 * cs + ngrow * rs + ngrow)
+        return _device_array(start, (nrows, ny, nx), (8 * cs, 8 * rs, 8), self._device)
+
+    def forward_device(self, Q0, nfield):
+        """Gauge depths (nsteps, ngauges) as a device tensor; Q0 (n_q, ny, nx)
+        and nfield (ny, nx) are CUDA float64 tensors."""
+        self._device = Q0.device
+        self._bind_stream(Q0.device)
+        if self._aux_loc is None:
+            # the Qaux array of the one box stays where it is for the
+            # context's life; only its stored input row (Manning) is written
+            self._aux_loc = self.z.device_ptr(field=1)
+        q = self._valid_view(self.z.device_ptr(field=0), 0, self.z.nq)
+        q.copy_(Q0)
+        self._valid_view(self._aux_loc, 1, 1)[0].copy_(nfield)
+        gh = torch.empty((len(self.dts), len(self.gauges[0])), dtype=torch.float64, device=Q0.device)
+        self.z.adj_forward_device(gh.data_ptr())
+        self.generation += 1
+        return gh
+
+    def backward_device(self, gbar):
+        """(adjoint of Q0, adjoint of the Manning field) as device tensors for
+        the device gauge adjoint gbar (nsteps, ngauges)."""
+        self._bind_stream(gbar.device)
+        gbar = gbar.to(dtype=torch.float64).contiguous()
+        self.z.adj_backward_device(gbar.data_ptr())
+        nx, ny, _ = self.z.shape()
+        qb = torch.empty((self.z.nq, ny, nx), dtype=torch.float64, device=gbar.device)
+        nb = torch.empty((ny, nx), dtype=torch.float64, device=gbar.device)
+        self.z.adj_gradient_device(qb.data_ptr(), nb.data_ptr())
+        return qb, nb
+
     def close(self):
         self.z.close()
+
+
+class _CudaArray:
+    """A device address with a shape and byte strides, exported through the
+    CUDA array interface so that torch.as_tensor views it without a copy."""
+
+    def __init__(self, ptr, shape, strides):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "strides": tuple(strides), "typestr": "<f8",
+                                         "data": (int(ptr), False), "version": 2}
+
+
+def _device_array(ptr, shape, strides, device):
+    t = torch.as_tensor(_CudaArray(ptr, shape, strides), device=device)
+    if t.data_ptr() != ptr or t.dtype != torch.float64:
+        raise RuntimeError("torch did not view the library's device array in place")
+    return t
 
 
 def frozen_dts(case, Q0, nfield, steps, contract=True, system="swe2d_nf"):
@@ -81,16 +148,24 @@ class GaugeSeries(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, Q0, nfield, run):
-        gh = run.forward(Q0.detach().cpu().numpy(), nfield.detach().cpu().numpy())
         ctx.run = run
+        if Q0.is_cuda:
+            if not nfield.is_cuda or Q0.device != nfield.device:
+                raise ValueError("Q0 and nfield must be on the same device")
+            gh = run.forward_device(Q0.detach().to(torch.float64), nfield.detach().to(torch.float64))
+            ctx.generation = run.generation
+            return gh.to(dtype=Q0.dtype)
+        gh = run.forward(Q0.detach().numpy(), nfield.detach().numpy())
         ctx.generation = run.generation
-        return torch.from_numpy(gh).to(device=Q0.device, dtype=Q0.dtype)
+        return torch.from_numpy(gh).to(dtype=Q0.dtype)
 
     @staticmethod
     def backward(ctx, gbar):
         run = ctx.run
         if run.generation != ctx.generation:
             raise RuntimeError("GaugeRun was run forward again before this backward; its checkpoints are gone")
-        qb, nb = run.backward(gbar.detach().cpu().numpy())
-        return (torch.from_numpy(qb).to(device=gbar.device, dtype=gbar.dtype),
-                torch.from_numpy(nb).to(device=gbar.device, dtype=gbar.dtype), None)
+        if gbar.is_cuda:
+            qb, nb = run.backward_device(gbar.detach())
+            return qb.to(dtype=gbar.dtype), nb.to(dtype=gbar.dtype), None
+        qb, nb = run.backward(gbar.detach().numpy())
+        return torch.from_numpy(qb).to(dtype=gbar.dtype), torch.from_numpy(nb).to(dtype=gbar.dtype), None

@@ -89,6 +89,7 @@ def load_library(path):
         "zamr_adj_backward": [_VP, _D],
         "zamr_adj_gradient": [_VP, _D, ctypes.c_size_t, _D, ctypes.c_size_t],
         "zamr_adj_freeze_limiter": [_VP, ctypes.c_int],
+        "zamr_adj_device_io": [_VP, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -330,9 +331,16 @@ class ZoomyAmr:
         self._check(self.lib.zamr_adj_setup(self.ctx, len(dts), seg_steps, _dp(dts), len(gi),
                                             gi.ctypes.data_as(_I), gj.ctypes.data_as(_I)))
 
+    def _adj_io(self, device):
+        """Put the adjoint calls in device or host I/O mode (zamr_adj_device_io)."""
+        if getattr(self, "_adj_device", False) != device:
+            self._check(self.lib.zamr_adj_device_io(self.ctx, 1 if device else 0))
+            self._adj_device = device
+
     def adj_forward(self):
         """Frozen-dt forward from the current state; returns the gauge depths
         after every step, shape (nsteps, ngauges)."""
+        self._adj_io(False)
         ns, ng = self._adj
         gh = np.zeros((ns, ng))
         self._check(self.lib.zamr_adj_forward(self.ctx, _dp(gh)))
@@ -341,13 +349,33 @@ class ZoomyAmr:
     def adj_backward(self, gbar):
         ns, ng = self._adj
         gbar = np.ascontiguousarray(gbar, dtype=np.float64).reshape(ns, ng)
+        self._adj_io(False)
         self._check(self.lib.zamr_adj_backward(self.ctx, _dp(gbar)))
 
     def adj_freeze_limiter(self, on=True):
         self._check(self.lib.zamr_adj_freeze_limiter(self.ctx, 1 if on else 0))
 
+    def adj_forward_device(self, gauge_h_ptr):
+        """Frozen-dt forward; gauge depths (nsteps, ngauges) to the device
+        address gauge_h_ptr.  The adj_*_device calls queue on the context's
+        stream and return without waiting."""
+        self._adj_io(True)
+        self._check(self.lib.zamr_adj_forward(self.ctx, ctypes.cast(_VP(gauge_h_ptr), _D)))
+
+    def adj_backward_device(self, gbar_ptr):
+        self._adj_io(True)
+        self._check(self.lib.zamr_adj_backward(self.ctx, ctypes.cast(_VP(gbar_ptr), _D)))
+
+    def adj_gradient_device(self, qbar_ptr, auxbar_ptr):
+        """The gradients into dense device arrays (n_q, ny, nx) and (ny, nx)."""
+        self._adj_io(True)
+        nx, ny, _ = self.shape()
+        self._check(self.lib.zamr_adj_gradient(self.ctx, ctypes.cast(_VP(qbar_ptr), _D), self.nq * nx * ny,
+                                               ctypes.cast(_VP(auxbar_ptr), _D), nx * ny))
+
     def adj_gradient(self):
         """(adjoint of the initial state (n_q, ny, nx), adjoint of the friction aux row (ny, nx))."""
+        self._adj_io(False)
         nx, ny, _ = self.shape()
         qb = np.zeros((self.nq, ny, nx))
         fb = np.zeros((ny, nx))

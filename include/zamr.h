@@ -212,9 +212,12 @@ int zamr_dt_history(zamr_ctx* ctx, int cap, double* dts, double* times, int* n);
 /* ---- Adjoint (gradients) -------------------------------------------------
  * No counterpart in the reference, which has no gradients.  Systems that ship
  * an Adjoint.H beside their Model.H (swe2d_nf: per-cell Manning field in aux
- * row 1) support these; other systems return ZAMR_EUNSUP.  Scope: one box
- * covering the domain, order 2, point-implicit source, no well-balanced
- * reconstruction, no positivity clamp (ZAMR_EUNSUP otherwise).
+ * row 1) support these; other systems return ZAMR_EUNSUP.  Scope: single-level
+ * decks (any number of boxes; the adjoint runs the domain as one box, whose
+ * forward is the boxes' own bit for bit), order 2, point-implicit source,
+ * with or without the well-balanced (hydrostatic) reconstruction and
+ * UpdateState's positivity clamp; MOOD, solver.implicit_global, AMR and box
+ * lists give ZAMR_EUNSUP.
  *
  * zamr_adj_setup: nsteps SSP-RK2 steps with the frozen step sizes dts[nsteps]
  * (time t_k = dts[0] + ... + dts[k-1]), a checkpoint every seg_steps steps,

@@ -61,6 +61,47 @@ void hz_face_reverse(const double* qL, const double* qR, int dir, const double* 
     }
 }
 
+// The oracle's face evaluation (make_rhs.H:31-131) with the hydrostatic
+// reconstruction, order 1 (the face states are the two cells' states), on a
+// 2x2 domain: flux (4), D+ (4), D- (4), then the reconstructed depths h*_L,
+// h*_R (the balancing bed source's inputs).
+void hz_wb_face(const double* qL, const double* qR, int dir, const double* pp, double* out) {
+    zorc::Settings st;
+    st.nx = 2;
+    st.ny = 2;
+    st.max_grid_size = 2;
+    st.spatial_order = 1;
+    st.well_balanced = true;
+    zorc::Solver<Model, Numerics> o(st, pp);
+    auto& fq = o.Q.fabs[0];
+    const auto& fa = o.Qaux.fabs[0];
+    const int iR = dir == 0 ? 1 : 0, jR = dir == 0 ? 0 : 1;
+    for (int n = 0; n < 4; ++n) {
+        fq(0, 0, n) = qL[n];
+        fq(iR, jR, n) = qR[n];
+    }
+    VQ F;
+    amrex::SmallMatrix<double, 8, 1> D;
+    double hl = 0.0, hr = 0.0;
+    o.eval_face(fq, fa, 0, 0, iR, jR, dir, 1, F, D, 0.0, true, &hl, &hr);
+    for (int n = 0; n < 4; ++n) out[n] = F(n, 0);
+    for (int n = 0; n < 8; ++n) out[4 + n] = D(n, 0);
+    out[12] = hl;
+    out[13] = hr;
+}
+
+void hz_wb_face_reverse(const double* qL, const double* qR, int dir, const double* pp, const double* Fb,
+                        const double* Dpb, const double* Dmb, double kL, double kR, double* qLb, double* qRb) {
+    const VP p = vp(pp);
+    VQ lb, rb;
+    if (dir == 0) Adjoint::wb_face_reverse<0>(vq(qL), vq(qR), p, vq(Fb), vq(Dpb), vq(Dmb), kL, kR, lb, rb);
+    else Adjoint::wb_face_reverse<1>(vq(qL), vq(qR), p, vq(Fb), vq(Dpb), vq(Dmb), kL, kR, lb, rb);
+    for (int n = 0; n < 4; ++n) {
+        qLb[n] = lb(n, 0);
+        qRb[n] = rb(n, 0);
+    }
+}
+
 // apply_implicit_source (make_rhs.H:429-509) on V with the stage-start aux
 // of U: Model source and Jacobians, I - dt J, the oracle's elimination.
 int hz_implicit(const double* Vp, const double* Up, double nn, double dt, const double* pp, double* W) {

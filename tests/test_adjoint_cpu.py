@@ -40,9 +40,11 @@ def hz(tmp_path_factory):
     lib.hz_dhinv.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.hz_implicit.restype = ctypes.c_int
     for f in ("hz_flux", "hz_face_reverse", "hz_implicit", "hz_manning", "hz_source_reverse", "hz_bc",
-              "hz_bc_reverse"):
+              "hz_bc_reverse", "hz_wb_face", "hz_wb_face_reverse"):
         if f != "hz_implicit":
             getattr(lib, f).restype = None
+    lib.hz_wb_face_reverse.argtypes = [_D, _D, ctypes.c_int, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double, _D,
+                                       _D]
     return lib
 
 
@@ -84,6 +86,58 @@ def test_face_reverse_matches_finite_differences(hz, d, dry):
                 fm = flux(hz, qm, qR, d) if side == 0 else flux(hz, qL, qm, d)
                 fd = np.dot(w, (fp - fm) / (2 * e))
                 assert abs(fd - qb[n]) <= 1e-6 * max(1.0, abs(fd)), (side, n, fd, qb[n])
+
+
+def wb_face(hz, qL, qR, d):
+    out = np.zeros(14)
+    hz.hz_wb_face(dp(qL), dp(qR), d, dp(P), dp(out))
+    return out
+
+
+@pytest.mark.parametrize("d", [0, 1])
+@pytest.mark.parametrize("kind", ["wet", "dry_side", "dry_cell"])
+def test_wb_face_reverse_matches_finite_differences(hz, d, kind):
+    """The hydrostatic-reconstruction face reverse against central
+    differences of the oracle's well-balanced face (flux, fluctuations and the
+    reconstructed depths that feed the balancing bed source), weighted by a
+    random output adjoint; `dry_side`: the lower side's water lies below the
+    face bed (h* = 0 there), `dry_cell`: one side has h = 0."""
+    rng = np.random.default_rng(31 + d + 3 * ["wet", "dry_side", "dry_cell"].index(kind))
+    checked = 0
+    for _ in range(60):
+        qL, qR = rand_face(rng)
+        if kind == "dry_side":
+            qL[0], qR[0] = 0.0, rng.uniform(0.5, 1.0)
+            qL[1] = rng.uniform(0.05, 0.3)  # hL + bL - bf < 0
+        elif kind == "dry_cell":
+            qL[1] = 0.0
+            qL[2:] = 0.0
+        w = rng.normal(size=14)  # flux, D+, D-, then the h* weights kL, kR
+        w[0] = 0.0
+        qLb, qRb = np.zeros(4), np.zeros(4)
+        hz.hz_wb_face_reverse(dp(qL), dp(qR), d, dp(P), dp(w[:4].copy()), dp(w[4:8].copy()), dp(w[8:12].copy()),
+                              w[12], w[13], dp(qLb), dp(qRb))
+        base = wb_face(hz, qL, qR, d)
+        if kind == "dry_side":
+            assert base[12] == 0.0
+        for side, q, qb in ((0, qL, qLb), (1, qR, qRb)):
+            for n in range(4):
+                if kind == "dry_cell" and side == 0 and n > 0:
+                    continue  # h = 0 sits on the wet/dry kink: one-sided there
+                e = 1e-7 * max(1.0, abs(q[n]))
+                qp, qm = q.copy(), q.copy()
+                qp[n] += e
+                qm[n] -= e
+                fp = wb_face(hz, qp, qR, d) if side == 0 else wb_face(hz, qL, qp, d)
+                fm = wb_face(hz, qm, qR, d) if side == 0 else wb_face(hz, qL, qm, d)
+
+                def J(f):
+                    return np.dot(w[:12], f[:12]) + 0.5 * w[12] * f[12] ** 2 + 0.5 * w[13] * f[13] ** 2
+
+                fd = (J(fp) - J(fm)) / (2 * e)
+                assert abs(fd - qb[n]) <= 1e-6 * max(1.0, abs(fd)), (kind, side, n, fd, qb[n])
+                checked += 1
+    assert checked > 0
 
 
 def test_hinv_derivative(hz):

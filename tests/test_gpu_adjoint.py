@@ -41,13 +41,28 @@ def vortex(n):
     return u, v
 
 
+def island(n):
+    """A bed bump that rises 0.25 above the still surface (eta = 1), off the
+    domain's centre: its shore dries and wets as the hump's wave runs up."""
+    x, y = cases.cell_centers(n, n)
+    X, Y = np.meshgrid(x, y)
+    return 1.25 * np.exp(-((X - 0.27) ** 2 + (Y - 0.7) ** 2) / (2 * 0.08 ** 2))
+
+
 def setup(n=64, steps=40, seg=7, seed=1, bc=("", "", "", ""), contract=True, ngauges=24, nfd=10, gauges=None,
-          current=True, freeze=True, manning=None):
+          current=True, freeze=True, manning=None, keys=None, bed=None):
     """The inversion case at size n; the truth run, then (freeze) the limiter
     frozen at the base forward's choices for the derivative checks.
     `manning` raises the base Manning coefficient (and shifts the truth's by
-    the same amount) so that short runs feel the friction."""
+    the same amount) so that short runs feel the friction; `keys` overrides
+    solver keys of the case (well_balanced, clamp_positivity,
+    max_grid_size); `bed` is added to the base and truth bathymetry."""
     c = cases.swe_adjoint(n=n, seed=seed, steps=steps, ngauges=ngauges, nfd=nfd, seg=seg, bc=bc)
+    if keys:
+        c["case"] = dict(c["case"], **keys)
+    if bed is not None:
+        c["b0"] = c["b0"] + bed
+        c["b_truth"] = c["b_truth"] + bed
     if gauges is not None:
         c["gauges"] = gauges
     if manning is not None:
@@ -276,7 +291,7 @@ def test_zero_copy_path_matches_host_path():
         for k, f in saved.items():
             setattr(run.z, k, f)
     for g in (got, got_side):
-        assert g[0] == ref[0]
+        assert abs(g[0] - ref[0]) <= 1e-14 * abs(ref[0])  # torch's device sum of the loss
         assert np.array_equal(g[1], ref[1]) and np.array_equal(g[2], ref[2])
     again = grads(run, c, truth)
     assert again[0] == ref[0] and np.array_equal(again[1], ref[1]) and np.array_equal(again[2], ref[2])
@@ -286,6 +301,136 @@ def test_zero_copy_path_matches_host_path():
         run.z.adj_forward_device(host.ctypes.data)
     assert e.value.code == 1
     run.close()
+
+
+WB_CLAMP = dict(well_balanced=True, clamp_positivity=True)
+# Near a shore the friction (~ n^2 / h^(4/3)) makes the loss strongly curved
+# in n, and a central difference of step 1e-3 carries truncation errors of a
+# few 1e-6 there (measured: 2.4e-6 at 1e-3, 2.5e-7 at 3e-4, 2.3e-8 at 1e-4 on
+# a shore cell, falling as eps^2); these cases take 3e-4.
+FD_EPS_SHORE = 3e-4
+
+
+def min_depth_unclamped(c, dts, contract=True):
+    """The least depth after any step of the base forward run with the
+    positivity clamp off (negative where the clamp has work to do)."""
+    z = ZoomyAmr(deck_from_case(dict(c["case"], clamp_positivity=False, max_step=10 ** 9)), system="swe2d_nf",
+                 dim=2, order=2, contract=contract)
+    z.init_data()
+    aux = z.get_state(field=1)
+    aux[1] = c["n0"]
+    z.set_state(c["state"](c["b0"]))
+    z.set_state(aux, field=1)
+    lo, t = np.inf, 0.0
+    for dt in dts:
+        z.advance(t, dt)
+        t += dt
+        lo = min(lo, float(z.get_state()[1].min()))
+    z.close()
+    return lo
+
+
+@pytest.mark.parametrize("bed,wb", [("bumps", True), ("island", True), ("island", False)])
+def test_well_balanced_clamped_gradient_matches_finite_differences(bed, wb):
+    """The C2-style forward (hydrostatic reconstruction with its balancing bed
+    source, UpdateState's positivity clamp): gradients against central
+    differences of the GPU forward, cell by cell and along a smooth
+    whole-field direction.  `island`: a bump above the surface with a dry
+    top and a shore; without the hydrostatic reconstruction the run there
+    makes negative depths that the clamp cuts (checked: the same run without
+    the clamp goes negative)."""
+    extra = island(64) if bed == "island" else None
+    c, run, truth, dts = setup(n=64, steps=40, seg=7, manning=0.3, keys=dict(WB_CLAMP, well_balanced=wb), bed=extra)
+    if not wb:
+        assert min_depth_unclamped(c, dts) < 0.0
+    L, gb, gn = grads(run, c, truth)
+    assert np.isfinite(gb).all() and np.isfinite(gn).all()
+    sb, sn = np.abs(gb).max(), np.abs(gn).max()
+    worst, where = 0.0, None
+    for i, j in zip(*c["fd_cells"]):
+        for field, g, sc in (("b", gb, sb), ("n", gn, sn)):
+            err = rel_err(g[j, i], fd_cell(run, c, truth, field, i, j, eps=FD_EPS if field == "b" else FD_EPS_SHORE),
+                          sc)
+            if err > worst:
+                worst, where = err, (field, i, j, g[j, i])
+    assert worst <= 1e-6, (worst, where)
+    x, y = cases.cell_centers(64, 64)
+    X, Y = np.meshgrid(x, y)
+    db = np.sin(3 * X + 1) * np.cos(2 * Y)
+    dn = 0.01 * np.cos(5 * X) * np.sin(4 * Y + 0.5)
+    an = np.sum(gb * db) + np.sum(gn * dn)
+    e = 1e-6
+    Gp = run.forward(c["state"](c["b0"] + e * db), c["n0"] + e * dn)
+    Gm = run.forward(c["state"](c["b0"] - e * db), c["n0"] - e * dn)
+    fd = misfit_diff(Gp, Gm, truth) / (2 * e)
+    assert abs(an - fd) <= 1e-6 * abs(fd), (an, fd)
+
+
+def test_well_balanced_clamped_gradient_matches_oracle_finite_differences():
+    """The same forward in the FMA-free build against central differences of
+    the CPU oracle's well-balanced, clamped forward (limiter frozen at its own
+    base run's choices), over the drying island."""
+    c, run, truth, dts = setup(steps=16, seg=5, manning=0.3, contract=False, keys=WB_CLAMP, bed=island(64))
+    L, gb, gn = grads(run, c, truth)
+    gi, gj = c["gauges"]
+    o = Oracle("swe2d_nf", c["case"])
+    o.init_solution()
+
+    def oracle_series(b, nf):
+        o.set_state(c["state"](b))
+        aux = run.aux0.copy()
+        aux[1] = nf
+        o.set_aux(aux)
+        t, G = 0.0, []
+        for dt in dts:
+            o.advance(t, dt)
+            t += dt
+            G.append(o.get_state()[1][gj, gi])
+        return np.array(G)
+
+    o.set_tape(1)
+    G0 = oracle_series(c["b0"], c["n0"])
+    assert np.array_equal(G0, run.forward(c["state"](c["b0"]), c["n0"]))
+    sb, sn = np.abs(gb).max(), np.abs(gn).max()
+    worst, where = 0.0, None
+    for i, j in list(zip(*c["fd_cells"]))[:4]:
+        for field, g, sc in (("b", gb, sb), ("n", gn, sn)):
+            b, nf = c["b0"].copy(), c["n0"].copy()
+            a = b if field == "b" else nf
+            a0 = a[j, i]
+            e = FD_EPS if field == "b" else FD_EPS_SHORE
+            a[j, i] = a0 + e
+            o.set_tape(2)
+            Gp = oracle_series(b, nf)
+            a[j, i] = a0 - e
+            o.set_tape(2)
+            Gm = oracle_series(b, nf)
+            err = rel_err(g[j, i], FD(misfit_diff(Gp, Gm, truth) / (2 * e), fd_noise(Gp, truth, e)), sc)
+            if err > worst:
+                worst, where = err, (field, i, j)
+    o.close()
+    assert worst <= 1e-6, (worst, where)
+
+
+def test_multi_box_forward_and_gradient_equal_single_box():
+    """A deck chopped into 16 boxes (max_grid_size 16): the gauge series, the
+    end state the forward leaves in the boxes and the gradients equal those
+    of the one-box deck bit for bit, through the host path and the zero-copy
+    path."""
+    one = setup(n=64, steps=24, seg=7, manning=0.3, keys=WB_CLAMP, bed=island(64), freeze=False)
+    many = setup(n=64, steps=24, seg=7, manning=0.3, keys=dict(WB_CLAMP, max_grid_size=16), bed=island(64),
+                 freeze=False)
+    (c1, r1, t1, d1), (c2, r2, t2, d2) = one, many
+    assert r2.z.shape()[2] == 16 and np.array_equal(d1, d2) and np.array_equal(t1, t2)
+    Q0 = c1["state"](c1["b0"])
+    assert np.array_equal(r1.forward(Q0, c1["n0"]), r2.forward(Q0, c2["n0"]))
+    assert np.array_equal(r1.z.get_state(), r2.z.get_state())
+    g1 = grads(r1, c1, t1)
+    for dev in ("cpu", "cuda"):
+        g2 = grads(r2, c2, t2, device=dev)
+        # (torch sums the loss in another order on the device)
+        assert abs(g1[0] - g2[0]) <= 1e-14 * abs(g1[0]), dev
+        assert np.array_equal(g1[1], g2[1]) and np.array_equal(g1[2], g2[2]), dev
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
@@ -349,16 +494,12 @@ def test_unsupported_configurations():
     with pytest.raises(ZamrError) as e:
         z.adj_setup([1e-3] * 4, [1], [1], 2)
     assert e.value.code == 5
-    z = ZoomyAmr(deck_from_case(dict(case, well_balanced=True)), system="swe2d_nf", dim=2, order=2)
-    z.init_data()
-    with pytest.raises(ZamrError) as e:
-        z.adj_setup([1e-3] * 4, [1], [1], 2)
-    assert e.value.code == 5
-    z = ZoomyAmr(deck_from_case(dict(case, max_grid_size=16)), system="swe2d_nf", dim=2, order=2)
-    z.init_data()
-    with pytest.raises(ZamrError) as e:
-        z.adj_setup([1e-3] * 4, [1], [1], 2)
-    assert e.value.code == 5
+    for keys in (dict(positivity="mood"), dict(implicit_source=False), dict(implicit_global=True)):
+        z = ZoomyAmr(deck_from_case(dict(case, **keys)), system="swe2d_nf", dim=2, order=2)
+        z.init_data()
+        with pytest.raises(ZamrError) as e:
+            z.adj_setup([1e-3] * 4, [1], [1], 2)
+        assert e.value.code == 5, keys
     z = ZoomyAmr(deck_from_case(case), system="swe2d_nf", dim=2, order=2)
     z.init_data()
     with pytest.raises(ZamrError):

@@ -42,7 +42,8 @@ struct AdjArgs {
     const double* heun; // optional: out += heun / 2 (read at the same cell first)
     double* nbar;       // adjoint of the friction field (one row, box layout), accumulated
     double dt, time;
-    int clamp;
+    int clamp;  // the forward's positivity clamp of the stage input (UpdateState)
+    int wb;     // well-balanced (hydrostatic) reconstruction
 };
 
 template <class M, class AD>
@@ -54,11 +55,14 @@ struct AdjSys {
 
     // Reverse of one face of the forward divergence.  The face sits between
     // L and R (Lm, Rp the outer stencil cells); vtL, vtR are dt * Vb of L and
-    // R; rd = 1/dx along DIR.  RHS_L holds (-F - D-) rd and RHS_R (F - D+) rd.
-    // Returns the adjoint contributions to the four stencil cells.
+    // R; rd = 1/dx along DIR.  RHS_L holds (-F - D-) rd and RHS_R (F - D+) rd;
+    // with well-balancing (wb) the face states are hydrostatically
+    // reconstructed first and RHS_L / RHS_R of the normal momentum also hold
+    // +g/2 h*_L^2 rd / -g/2 h*_R^2 rd (the balancing bed source).  Returns the
+    // adjoint contributions to the four stencil cells.
     template <int ORDER, int DIR>
     __device__ static __forceinline__ void face(const VQ& Lm, const VQ& L, const VQ& R, const VQ& Rp, const VQ& vtL,
-                                                const VQ& vtR, double rd, const VP& p, VQ& cLm, VQ& cL, VQ& cR,
+                                                const VQ& vtR, double rd, const VP& p, int wb, VQ& cLm, VQ& cL, VQ& cR,
                                                 VQ& cRp) {
         VQ qL, qR;
         unsigned sl[nq], sr[nq];
@@ -86,7 +90,13 @@ struct AdjSys {
             Dmb(n, 0) = -vtL(n, 0) * rd;
             Dpb(n, 0) = -vtR(n, 0) * rd;
         }
-        AD::template face_reverse<DIR>(qL, qR, p, Fb, Dpb, Dmb, qLb, qRb);
+        if (wb) {
+            const double kg = p(0, 0) * rd;
+            AD::template wb_face_reverse<DIR>(qL, qR, p, Fb, Dpb, Dmb, kg * vtL(2 + DIR, 0), -kg * vtR(2 + DIR, 0),
+                                              qLb, qRb);
+        } else {
+            AD::template face_reverse<DIR>(qL, qR, p, Fb, Dpb, Dmb, qLb, qRb);
+        }
 #pragma unroll
         for (int n = 0; n < nq; ++n) {
             double lm = 0.0, l = qLb(n, 0), r = qRb(n, 0), rp = 0.0;
@@ -229,7 +239,7 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
         vzero(cL);
         vzero(cR);
         vzero(cRp);
-        if (face_lane && (vx || vxl)) A::template face<ORDER, 0>(Lm, L, u, Rp, vl, vt, rdx, p, cLm, cL, cR, cRp);
+        if (face_lane && (vx || vxl)) A::template face<ORDER, 0>(Lm, L, u, Rp, vl, vt, rdx, p, a.wb, cLm, cL, cR, cRp);
         VQ G;
 #pragma unroll
         for (int n = 0; n < nq; ++n)
@@ -308,7 +318,7 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
         vzero(cRp);
         const bool vy0 = (j >= B.dlo1) && (j <= B.dhi1);
         const bool vy1 = (j + 1 >= B.dlo1) && (j + 1 <= B.dhi1);
-        if (own && vx && (vy0 || vy1)) A::template face<ORDER, 1>(U0, U1, U2, U3, P0, P1, rdy, p, cLm, cL, cR, cRp);
+        if (own && vx && (vy0 || vy1)) A::template face<ORDER, 1>(U0, U1, U2, U3, P0, P1, rdy, p, a.wb, cLm, cL, cR, cRp);
         if (j - 1 >= B.dlo1 && j + 2 <= B.dhi1) {
             vadd(R0, cLm);
             vadd(R1, cL);
@@ -346,9 +356,12 @@ __global__ void __launch_bounds__(NT) ZAMR_ADJ_ATTR adjoint_stage_kernel(const B
         land_point(pin);
         if (store_row) {
             const long long idx = (long long)(c - B.gl0) + (long long)(j - 1 - B.gl1) * B.sy;
+            // the stage read its input through UpdateState's clamp: no
+            // adjoint reaches a depth the clamp replaced by 0
+            const bool cut = a.clamp && a.U[idx + B.sc] < 0.0;
 #pragma unroll
             for (int n = 0; n < nq; ++n) {
-                double v = R0(n, 0);
+                double v = (n == 1 && cut) ? 0.0 : R0(n, 0);
                 if (a.heun) v += 0.5 * hn(n, 0);
                 a.out[idx + n * B.sc] = v;
             }
@@ -376,6 +389,39 @@ __global__ void gauge_inject_kernel(double* __restrict__ qb, long long sc, const
                                     const double* __restrict__ gbar) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < ng) qb[gidx[g] + sc] += gbar[g];
+}
+
+// Reverse of UpdateState's positivity clamp on a step's output, X_{k+1} =
+// clamp(Z), Z = X_k / 2 + W / 2 (W's depth is the pre-implicit V's: the
+// friction step leaves h alone): the depth adjoint of the cells the clamp
+// replaced (stored X_{k+1} depth 0 from a negative Z) is dropped, in place.
+__global__ void clamp_reverse_kernel(double* __restrict__ qb, const double* __restrict__ xn,
+                                     const double* __restrict__ xo, const double* __restrict__ v, long long sc,
+                                     long long cells) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cells) return;
+    const long long i = k + sc;  // depth row
+    if (xn[i] == 0.0 && 0.5 * xo[i] + 0.5 * v[i] < 0.0) qb[i] = 0.0;
+}
+
+// Valid cells of every box of a level (grid.y = box) to or from one box
+// covering the domain (grown lower corner (gl0, gl1), row stride sy, component
+// stride sc): the state buffer buf of the boxes (buf < 0: their stored Qaux),
+// nc components.
+__global__ void box_exchange_kernel(const BoxDesc* __restrict__ boxes, int buf, double* __restrict__ one, int gl0,
+                                    int gl1, int sy, long long sc, int nc, int to_one) {
+    const BoxView B = view(boxes, blockIdx.y);
+    const int w = B.hi0 - B.lo0 + 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= w * (B.hi1 - B.lo1 + 1)) return;
+    const int i = B.lo0 + k % w, j = B.lo1 + k / w;
+    double* q = buf < 0 ? B.aux : qsel(B, buf);
+    const long long ib = (long long)(i - B.gl0) + (long long)(j - B.gl1) * B.sy;
+    const long long io = (long long)(i - gl0) + (long long)(j - gl1) * sy;
+    for (int n = 0; n < nc; ++n) {
+        if (to_one) one[io + n * sc] = q[ib + n * B.sc];
+        else q[ib + n * B.sc] = one[io + n * sc];
+    }
 }
 
 // Valid cells of a grown-box array (ncomp components, component stride sc,
@@ -447,11 +493,26 @@ struct AdjointDriver {
             throw Error(ZAMR_EINVAL, std::string("adjoint: device I/O mode, but ") + what + " is not device memory");
     }
 
+    // The working layout: the context's solver when its one box is the
+    // domain, else a single-box solver over the same domain with the same
+    // settings (`one`), into which the forward gathers the boxes' state and
+    // from which it scatters the end state back.  The forward's values do not
+    // depend on the box decomposition, so the gradient is that of the
+    // context's own multi-box forward.
+    std::unique_ptr<Sol> one;
+    Sol& w() { return one ? *one : s; }
+    const Sol& w() const { return one ? *one : s; }
+    // the working solver runs on the context's current stream
+    void bind() {
+        if (one) one->stream = s.stream;
+    }
+
     explicit AdjointDriver(Sol& sol) : s(sol) {}
     ~AdjointDriver() { release(); }
     void release() {
+        bind();
         if (side) hipStreamSynchronize(side);
-        if (s.stream) hipStreamSynchronize(s.stream);
+        if (w().stream) hipStreamSynchronize(w().stream);
         if (pool) hipFree(pool);
         if (d_groups) hipFree(d_groups);
         if (d_gidx) hipFree(d_gidx);
@@ -475,19 +536,32 @@ struct AdjointDriver {
     }
 
     void check_config() const {
-        if (s.batched || s.hb.size() != 1)
-            throw Error(ZAMR_EUNSUP, "the adjoint runs on a single box covering the domain (max_grid_size >= n_cell)");
+        if (s.batched || s.hb.empty())
+            throw Error(ZAMR_EUNSUP, "the adjoint runs on a single-level domain (not on box lists)");
         if (s.spatial_order != ORDER) throw Error(ZAMR_EUNSUP, "the adjoint is built for solver.spatial_order = 2");
-        if (s.well_balanced) throw Error(ZAMR_EUNSUP, "the adjoint does not cover solver.well_balanced = true");
         if (!s.implicit_source) throw Error(ZAMR_EUNSUP, "the adjoint is built for solver.implicit_source = true");
         if (s.implicit_global)
             throw Error(ZAMR_EUNSUP, "the adjoint reverses the point-implicit source (solver.implicit_global = false)");
-        if (s.clamp_positivity) throw Error(ZAMR_EUNSUP, "the adjoint needs solver.clamp_positivity = false");
-        if (s.positivity != "none") throw Error(ZAMR_EUNSUP, "the adjoint needs solver.positivity = none");
+        if (s.positivity != "none")
+            throw Error(ZAMR_EUNSUP, "the adjoint covers solver.positivity = none (with or without the clamp)");
         const HostBox& b = s.hb[0];
-        if (b.lo[0] != b.dlo[0] || b.hi[0] != b.dhi[0] || b.lo[1] != b.dlo[1] || b.hi[1] != b.dhi[1])
-            throw Error(ZAMR_EUNSUP, "the adjoint box must be the whole domain");
-        if (b.hi[0] - b.lo[0] < 7 || b.hi[1] - b.lo[1] < 7) throw Error(ZAMR_EINVAL, "the adjoint needs >= 8 cells a side");
+        if (b.dhi[0] - b.dlo[0] < 7 || b.dhi[1] - b.dlo[1] < 7)
+            throw Error(ZAMR_EINVAL, "the adjoint needs >= 8 cells a side");
+    }
+
+    // Valid cells of the context's boxes (state buffer k, or the stored Qaux)
+    // to or from the working box.
+    void exchange(bool to_one, bool aux) {
+        if (!one) return;
+        const HostBox& o = one->hb[0];
+        int maxc = 0;
+        for (const HostBox& b : s.hb) maxc = std::max(maxc, (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1));
+        double* dst = aux ? one->d_pool + o.off_aux : one->d_pool + o.off_q[one->cur];
+        const int nc = aux ? (int)std::max(M::n_dof_qaux, 1) : nq;
+        hipLaunchKernelGGL(box_exchange_kernel, dim3((unsigned)((maxc + 255) / 256), (unsigned)s.hb.size()), dim3(256),
+                           0, w().stream, s.d_boxes, aux ? -1 : s.cur, dst, o.lo[0] - o.ng, o.lo[1] - o.ng, o.gx,
+                           (long long)o.gx * o.gy, nc, to_one ? 1 : 0);
+        ZAMR_HIP(hipGetLastError());
     }
 
     void setup(int nsteps_, int seg_, const double* dt, int ng_, const int* gi, const int* gj) {
@@ -495,7 +569,13 @@ struct AdjointDriver {
         if (nsteps_ < 1 || seg_ < 1 || !dt) throw Error(ZAMR_EINVAL, "adjoint: nsteps and segment length must be >= 1");
         if (ng_ < 0 || (ng_ > 0 && (!gi || !gj))) throw Error(ZAMR_EINVAL, "adjoint: bad gauge arrays");
         release();
-        const HostBox& b = s.hb[0];
+        if (s.hb.size() > 1 && !one) {
+            const HostBox& d = s.hb[0];
+            one = std::make_unique<Sol>();
+            one->configure_fine(s, 1, {{d.dlo[0], d.dlo[1], d.dhi[0], d.dhi[1]}});
+        }
+        bind();
+        const HostBox& b = w().hb[0];
         nsteps = nsteps_;
         seg = std::min(seg_, nsteps);
         nseg = (nsteps + seg - 1) / seg;
@@ -553,9 +633,9 @@ struct AdjointDriver {
             g.dt = dts[k];
             g.time = times[k];
             g.time_end = 1e300;
-            g.cfl = s.cfl;
-            g.dtmin = s.dtmin;
-            g.dtmax = s.dtmax;
+            g.cfl = w().cfl;
+            g.dtmin = w().dtmin;
+            g.dtmax = w().dtmax;
             g.divisor = 1.0;
             g.maxinv_bits = 0ull;
             g.step = k;
@@ -564,71 +644,72 @@ struct AdjointDriver {
             gs[k] = g;
         }
         ZAMR_HIP(hipMalloc(&d_groups, gs.size() * sizeof(GroupState)));
-        ZAMR_HIP(hipMemcpyAsync(d_groups, gs.data(), gs.size() * sizeof(GroupState), hipMemcpyHostToDevice, s.stream));
+        ZAMR_HIP(hipMemcpyAsync(d_groups, gs.data(), gs.size() * sizeof(GroupState), hipMemcpyHostToDevice, w().stream));
         if (ng > 0) {
             ZAMR_HIP(hipMalloc(&d_gidx, ng * sizeof(long long)));
-            ZAMR_HIP(hipMemcpyAsync(d_gidx, gidx.data(), ng * sizeof(long long), hipMemcpyHostToDevice, s.stream));
+            ZAMR_HIP(hipMemcpyAsync(d_gidx, gidx.data(), ng * sizeof(long long), hipMemcpyHostToDevice, w().stream));
         }
         for (auto& sl : slot) {
             std::vector<BoxDesc> dd(seg);
             for (int i = 0; i < seg; ++i) {
-                dd[i] = s.desc_of(b, sl.X[i], sl.Y[i]);
+                dd[i] = w().desc_of(b, sl.X[i], sl.Y[i]);
                 dd[i].group = 0;
             }
             ZAMR_HIP(hipMalloc(&sl.d_desc, dd.size() * sizeof(BoxDesc)));
             ZAMR_HIP(hipMemcpyAsync(sl.d_desc, dd.data(), dd.size() * sizeof(BoxDesc), hipMemcpyHostToDevice,
-                                    s.stream));
-            ZAMR_HIP(hipStreamSynchronize(s.stream));  // dd leaves scope
+                                    w().stream));
+            ZAMR_HIP(hipStreamSynchronize(w().stream));  // dd leaves scope
         }
         build_items();
         ZAMR_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_go, &ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1]})
             ZAMR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        s.items(HX - 1);  // the forward stage's work items, built before any second-stream use
-        ZAMR_HIP(hipStreamSynchronize(s.stream));
+        w().items(HX - 1);  // the forward stage's work items, built before any second-stream use
+        ZAMR_HIP(hipStreamSynchronize(w().stream));
     }
 
     void build_items() {
-        const HostBox& b = s.hb[0];
-        const int rows = s.rows_for(2);
+        const HostBox& b = w().hb[0];
+        const int rows = w().rows_for(2);
         const int txc = WV - 2 * HX;
         std::vector<WorkItem> v;
         for (int j0 = b.lo[1]; j0 <= b.hi[1]; j0 += rows)
             for (int i0 = b.lo[0]; i0 <= b.hi[0]; i0 += txc) v.push_back(WorkItem{0, i0, j0, std::min(rows, b.hi[1] - j0 + 1)});
         nitems = (int)v.size();
         ZAMR_HIP(hipMalloc(&d_items, v.size() * sizeof(WorkItem)));
-        ZAMR_HIP(hipMemcpyAsync(d_items, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s.stream));
-        ZAMR_HIP(hipStreamSynchronize(s.stream));
+        ZAMR_HIP(hipMemcpyAsync(d_items, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, w().stream));
+        ZAMR_HIP(hipStreamSynchronize(w().stream));
     }
 
     int seg_len(int m) const { return std::min(seg, nsteps - m * seg); }
-    double* state_buf(int k) const { return s.d_pool + s.hb[0].off_q[k]; }
+    double* state_buf(int k) const { return w().d_pool + w().hb[0].off_q[k]; }
 
     // Stage of step `step` on stream st; sw = 0 plain, 1 record, 2 replay.
     template <int EPI>
     void stage(const BoxDesc* boxes, StageArgs a, int step, int sw, hipStream_t st) {
-        auto& it = s.items(HX - 1);
+        auto& it = w().items(HX - 1);
         const dim3 grid(slots_for(it.n)), block(NT);
         if (sw == 0) {
             hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 0>), grid, block, 0, st, boxes, it.d, it.n, a,
-                               d_groups + step, s.xcd_remap);
+                               d_groups + step, w().xcd_remap);
         } else {
             a.sw = d_sw + ((size_t)2 * step + (EPI == 0 ? 0 : 1)) * cells;
             if (sw == 1)
                 hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 1>), grid, block, 0, st, boxes, it.d, it.n, a,
-                                   d_groups + step, s.xcd_remap);
+                                   d_groups + step, w().xcd_remap);
             else
                 hipLaunchKernelGGL((stage_kernel<M, N, ORDER, EPI, 2>), grid, block, 0, st, boxes, it.d, it.n, a,
-                                   d_groups + step, s.xcd_remap);
+                                   d_groups + step, w().xcd_remap);
         }
         ZAMR_HIP(hipGetLastError());
     }
 
     void freeze_limiter(bool on) {
         if (!pool) throw Error(ZAMR_EINVAL, "adjoint: call zamr_adj_setup first");
+        bind();
         if (on && !d_sw) ZAMR_HIP(hipMalloc(&d_sw, (size_t)2 * nsteps * cells * sizeof(unsigned)));
         if (!on && d_sw) {
-            ZAMR_HIP(hipStreamSynchronize(s.stream));
+            ZAMR_HIP(hipStreamSynchronize(w().stream));
             hipFree(d_sw);
             d_sw = nullptr;
         }
@@ -638,7 +719,7 @@ struct AdjointDriver {
 
     void sample(const double* q, int step) {
         if (ng == 0) return;
-        hipLaunchKernelGGL(gauge_sample_kernel, dim3((ng + 255) / 256), dim3(256), 0, s.stream, q, (long long)cells,
+        hipLaunchKernelGGL(gauge_sample_kernel, dim3((ng + 255) / 256), dim3(256), 0, w().stream, q, (long long)cells,
                            d_gidx, ng, d_gh + (size_t)step * ng);
         ZAMR_HIP(hipGetLastError());
     }
@@ -654,10 +735,10 @@ struct AdjointDriver {
         const int len = seg_len(m);
         for (int i = 0; i < len; ++i) {
             const int step = m * seg + i;
-            StageArgs a1 = s.stage_args(0, 1, 0);
+            StageArgs a1 = w().stage_args(0, 1, 0);
             a1.pre = sl.V1[i];
             stage<0>(sl.d_desc + i, a1, step, sw, st);
-            StageArgs a2 = s.stage_args(1, 0, 0);
+            StageArgs a2 = w().stage_args(1, 0, 0);
             a2.pre = sl.V2[i];
             a2.dstp = sl.X[i + 1];
             stage<1>(sl.d_desc + i, a2, step, sw, st);
@@ -672,24 +753,28 @@ struct AdjointDriver {
         if (!pool) throw Error(ZAMR_EINVAL, "adjoint: call zamr_adj_setup first");
         check_config();
         if (device_io) require_device(gh, "gauge_h");
+        bind();
         // (the stage kernels evaluate Qaux per cell from the state and the
         // stored input rows, so a stale stored Qaux is not read here)
+        exchange(true, false);
+        exchange(true, true);
         const int sw = freeze ? (sw_recorded ? 2 : 1) : 0;
         slot[0].held = slot[1].held = -1;
         for (int m = 0; m < nseg; ++m) {
-            copy(ck[m], state_buf(s.cur), s.stream);
+            copy(ck[m], state_buf(w().cur), w().stream);
             if (m + 1 < nseg) {
                 for (int i = 0; i < seg_len(m); ++i) {
                     const int k = m * seg + i;
-                    stage<0>(s.d_boxes, s.stage_args(s.cur, 1 - s.cur, s.cur), k, sw, s.stream);
-                    stage<1>(s.d_boxes, s.stage_args(1 - s.cur, s.cur, s.cur), k, sw, s.stream);
-                    sample(state_buf(s.cur), k);
+                    stage<0>(w().d_boxes, w().stage_args(w().cur, 1 - w().cur, w().cur), k, sw, w().stream);
+                    stage<1>(w().d_boxes, w().stage_args(1 - w().cur, w().cur, w().cur), k, sw, w().stream);
+                    sample(state_buf(w().cur), k);
                 }
             } else {
-                run_stored(m, 0, true, sw, s.stream);
-                copy(state_buf(s.cur), slot[0].X[seg_len(m)], s.stream);
+                run_stored(m, 0, true, sw, w().stream);
+                copy(state_buf(w().cur), slot[0].X[seg_len(m)], w().stream);
             }
         }
+        exchange(false, false);
         if (sw == 1) sw_recorded = true;
         replayed = sw == 2;
         fwd_sw = sw;
@@ -697,8 +782,8 @@ struct AdjointDriver {
         s.maxinv_fresh = false;
         if (gh && ng > 0)
             ZAMR_HIP(hipMemcpyAsync(gh, d_gh, (size_t)nsteps * ng * sizeof(double),
-                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.stream));
-        if (!device_io) ZAMR_HIP(hipStreamSynchronize(s.stream));
+                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, w().stream));
+        if (!device_io) ZAMR_HIP(hipStreamSynchronize(w().stream));
         have_forward = true;
         have_backward = false;
     }
@@ -706,7 +791,7 @@ struct AdjointDriver {
     void reverse_stage(const Slot& sl, const double* U, const double* V, const double* Wb, double wscale, double* out,
                        const double* heun, int k) {
         AdjArgs a;
-        a.p = s.prm;
+        a.p = w().prm;
         a.U = U;
         a.V = V;
         a.Wb = Wb;
@@ -716,9 +801,10 @@ struct AdjointDriver {
         a.nbar = nbar;
         a.dt = dts[k];
         a.time = times[k];
-        a.clamp = 0;
+        a.clamp = w().clamp_flag();
+        a.wb = w().well_balanced ? 1 : 0;
         hipLaunchKernelGGL((adjoint_stage_kernel<M, AD, ORDER>), dim3((nitems + NT / WV - 1) / (NT / WV)), dim3(NT), 0,
-                           s.stream, sl.d_desc, d_items, nitems, a);
+                           w().stream, sl.d_desc, d_items, nitems, a);
         ZAMR_HIP(hipGetLastError());
     }
 
@@ -728,8 +814,13 @@ struct AdjointDriver {
         for (int i = seg_len(m) - 1; i >= 0; --i) {
             const int step = m * seg + i;
             if (ng > 0) {
-                hipLaunchKernelGGL(gauge_inject_kernel, dim3((ng + 255) / 256), dim3(256), 0, s.stream, Ab,
+                hipLaunchKernelGGL(gauge_inject_kernel, dim3((ng + 255) / 256), dim3(256), 0, w().stream, Ab,
                                    (long long)cells, d_gidx, ng, d_gbar + (size_t)step * ng);
+                ZAMR_HIP(hipGetLastError());
+            }
+            if (w().clamp_flag()) {
+                hipLaunchKernelGGL(clamp_reverse_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, w().stream,
+                                   Ab, sl.X[i + 1], sl.X[i], sl.V2[i], (long long)cells, (long long)cells);
                 ZAMR_HIP(hipGetLastError());
             }
             reverse_stage(sl, sl.Y[i], sl.V2[i], Ab, 0.5, Bb, nullptr, step);
@@ -743,6 +834,7 @@ struct AdjointDriver {
     // order slot reuse both ways.
     void backward(const double* gbar) {
         if (!have_forward) throw Error(ZAMR_EINVAL, "adjoint: run zamr_adj_forward first");
+        bind();
         if (replayed)
             throw Error(ZAMR_EINVAL, "adjoint: the last forward replayed frozen limiter choices; the backward "
                                      "linearises a forward that computed them (freeze off, or the recording forward)");
@@ -750,21 +842,21 @@ struct AdjointDriver {
             if (!gbar) throw Error(ZAMR_EINVAL, "adjoint: gauge adjoint array is null");
             if (device_io) require_device(gbar, "gauge_hbar");
             ZAMR_HIP(hipMemcpyAsync(d_gbar, gbar, (size_t)nsteps * ng * sizeof(double),
-                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.stream));
+                                    device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, w().stream));
         }
-        ZAMR_HIP(hipMemsetAsync(Ab, 0, bufn * sizeof(double), s.stream));
-        ZAMR_HIP(hipMemsetAsync(Bb, 0, bufn * sizeof(double), s.stream));
-        ZAMR_HIP(hipMemsetAsync(nbar, 0, cells * sizeof(double), s.stream));
-        s.items(HX - 1);  // (re)built on the main stream, before the recompute stream reads it
+        ZAMR_HIP(hipMemsetAsync(Ab, 0, bufn * sizeof(double), w().stream));
+        ZAMR_HIP(hipMemsetAsync(Bb, 0, bufn * sizeof(double), w().stream));
+        ZAMR_HIP(hipMemsetAsync(nbar, 0, cells * sizeof(double), w().stream));
+        w().items(HX - 1);  // (re)built on the main stream, before the recompute stream reads it
         // the recompute stream starts after everything queued so far (the forward)
-        ZAMR_HIP(hipEventRecord(ev_go, s.stream));
+        ZAMR_HIP(hipEventRecord(ev_go, w().stream));
         ZAMR_HIP(hipStreamWaitEvent(side, ev_go, 0));
         int k = (slot[0].held == nseg - 1) ? 0 : ((slot[1].held == nseg - 1) ? 1 : -1);
         if (k < 0) {
             k = 0;
-            run_stored(nseg - 1, 0, false, fwd_sw, s.stream);
+            run_stored(nseg - 1, 0, false, fwd_sw, w().stream);
         }
-        ZAMR_HIP(hipEventRecord(ev_rev[1 - k], s.stream));  // the other slot is free
+        ZAMR_HIP(hipEventRecord(ev_rev[1 - k], w().stream));  // the other slot is free
         for (int m = nseg - 1; m >= 0; --m) {
             if (m > 0) {
                 // the recompute runs the build of the stage kernel the forward
@@ -775,11 +867,11 @@ struct AdjointDriver {
                 ZAMR_HIP(hipEventRecord(ev_rec[1 - k], side));
             }
             reverse_segment(m, k);
-            ZAMR_HIP(hipEventRecord(ev_rev[k], s.stream));
-            if (m > 0) ZAMR_HIP(hipStreamWaitEvent(s.stream, ev_rec[1 - k], 0));
+            ZAMR_HIP(hipEventRecord(ev_rev[k], w().stream));
+            if (m > 0) ZAMR_HIP(hipStreamWaitEvent(w().stream, ev_rec[1 - k], 0));
             k = 1 - k;
         }
-        if (!device_io) ZAMR_HIP(hipStreamSynchronize(s.stream));
+        if (!device_io) ZAMR_HIP(hipStreamSynchronize(w().stream));
         have_backward = true;
     }
 
@@ -787,7 +879,8 @@ struct AdjointDriver {
     // field (ny * nx), valid cells, raster order.
     void gradient(double* qbar, size_t nqb, double* fbar, size_t nfb) {
         if (!have_backward) throw Error(ZAMR_EINVAL, "adjoint: run zamr_adj_backward first");
-        const HostBox& b = s.hb[0];
+        bind();
+        const HostBox& b = w().hb[0];
         const int NX = b.hi[0] - b.lo[0] + 1, NY = b.hi[1] - b.lo[1] + 1;
         if ((qbar && nqb != (size_t)nq * NX * NY) || (fbar && nfb != (size_t)NX * NY))
             throw Error(ZAMR_EINVAL, "adjoint: gradient array size mismatch");
@@ -796,7 +889,7 @@ struct AdjointDriver {
             require_device(fbar, "auxbar");
             auto gather = [&](const double* src, int ncomp, double* dst) {
                 const long long n = (long long)ncomp * NX * NY;
-                hipLaunchKernelGGL(valid_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, src,
+                hipLaunchKernelGGL(valid_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, w().stream, src,
                                    (long long)cells, b.gx, b.ng, NX, NY, ncomp, dst);
                 ZAMR_HIP(hipGetLastError());
             };
@@ -805,9 +898,9 @@ struct AdjointDriver {
             return;
         }
         std::vector<double> tmp(cells * nq), tn(cells);
-        ZAMR_HIP(hipMemcpyAsync(tmp.data(), Ab, cells * nq * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        ZAMR_HIP(hipMemcpyAsync(tn.data(), nbar, cells * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        ZAMR_HIP(hipStreamSynchronize(s.stream));
+        ZAMR_HIP(hipMemcpyAsync(tmp.data(), Ab, cells * nq * sizeof(double), hipMemcpyDeviceToHost, w().stream));
+        ZAMR_HIP(hipMemcpyAsync(tn.data(), nbar, cells * sizeof(double), hipMemcpyDeviceToHost, w().stream));
+        ZAMR_HIP(hipStreamSynchronize(w().stream));
         for (int j = 0; j < NY; ++j)
             for (int i = 0; i < NX; ++i) {
                 const size_t li = (size_t)(i + b.ng) + (size_t)(j + b.ng) * b.gx;

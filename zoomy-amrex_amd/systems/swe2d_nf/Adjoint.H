@@ -166,6 +166,77 @@ struct Adjoint {
         qRb(3, 0) = mRb[1];
     }
 
+    // Reverse of the hydrostatic (Audusse) reconstruction of make_rhs.H:84-98
+    // followed by the face functions on the reconstructed states:
+    //   bf = max(bL, bR), h* = max(0, h + b - bf), momenta scaled by h*/h (by
+    //   0 on a side with h <= 0), and b* = bf on both sides;
+    // the reconstructed depths also feed the cells' balancing bed source
+    // (make_rhs.H:160-162), g/2 h*^2 / dx on the normal momentum.  kL, kR
+    // weigh that use: the adjoint of h*_L from it is kL h*_L (of h*_R,
+    // kR h*_R).  qL, qR are the face states before the reconstruction; Fb,
+    // Dpb, Dmb the adjoints of flux and fluctuations as for face_reverse.
+    template <int DIR>
+    static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE void wb_face_reverse(VQ const& qL, VQ const& qR, VP const& p,
+                                                                         VQ const& Fb, VQ const& Dpb, VQ const& Dmb,
+                                                                         amrex::Real kL, amrex::Real kR, VQ& qLb,
+                                                                         VQ& qRb)
+    {
+        const amrex::Real bL = qL(0, 0), bR = qR(0, 0);
+        const bool bf_from_R = bL < bR;  // amrex::max(bL, bR) = (bL < bR) ? bR : bL
+        const amrex::Real bf = bf_from_R ? bR : bL;
+        const amrex::Real hL = qL(1, 0), hR = qR(1, 0);
+        const amrex::Real xL = hL + bL - bf, xR = hR + bR - bf;
+        const bool posL = 0.0 < xL, posR = 0.0 < xR;  // max(0, x)
+        const amrex::Real hLs = posL ? xL : amrex::Real(0.0), hRs = posR ? xR : amrex::Real(0.0);
+        const bool wetL = hL > 0.0, wetR = hR > 0.0;
+        const amrex::Real sL = wetL ? hLs / hL : amrex::Real(0.0), sR = wetR ? hRs / hR : amrex::Real(0.0);
+        VQ L, R;
+        L(0, 0) = bf;
+        R(0, 0) = bf;
+        L(1, 0) = hLs;
+        R(1, 0) = hRs;
+        for (int d = 0; d < 2; ++d) {
+            L(2 + d, 0) = qL(2 + d, 0) * sL;
+            R(2 + d, 0) = qR(2 + d, 0) * sR;
+        }
+        VQ lb, rb;
+        face_reverse<DIR>(L, R, p, Fb, Dpb, Dmb, lb, rb);
+        amrex::Real bfb = lb(0, 0) + rb(0, 0);
+        amrex::Real hLsb = lb(1, 0) + kL * hLs, hRsb = rb(1, 0) + kR * hRs;
+        amrex::Real sLb = 0.0, sRb = 0.0;
+        for (int d = 0; d < 2; ++d) {
+            qLb(2 + d, 0) = lb(2 + d, 0) * sL;
+            qRb(2 + d, 0) = rb(2 + d, 0) * sR;
+            sLb += lb(2 + d, 0) * qL(2 + d, 0);
+            sRb += rb(2 + d, 0) * qR(2 + d, 0);
+        }
+        amrex::Real hLb = 0.0, hRb = 0.0, bLb = 0.0, bRb = 0.0;
+        if (wetL) {  // s = h*/h
+            hLsb += sLb / hL;
+            hLb -= sLb * hLs / (hL * hL);
+        }
+        if (wetR) {
+            hRsb += sRb / hR;
+            hRb -= sRb * hRs / (hR * hR);
+        }
+        if (posL) {  // h* = h + b - bf
+            hLb += hLsb;
+            bLb += hLsb;
+            bfb -= hLsb;
+        }
+        if (posR) {
+            hRb += hRsb;
+            bRb += hRsb;
+            bfb -= hRsb;
+        }
+        if (bf_from_R) bRb += bfb;
+        else bLb += bfb;
+        qLb(0, 0) = bLb;
+        qLb(1, 0) = hLb;
+        qRb(0, 0) = bRb;
+        qRb(1, 0) = hRb;
+    }
+
     // Transpose of Model::boundary_conditions on a ghost adjoint: tags 0-3
     // (extrapolation) copy, tag 4 (wall) flips the momentum normal to side
     // direction dir.

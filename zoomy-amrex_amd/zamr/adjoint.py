@@ -35,7 +35,8 @@ class GaugeRun:
         self.generation = 0
         self._stream = None  # the stream handed to the library (None: its own)
         self._device = None
-        self._aux_loc = None
+        self._boxes = None
+        self._aux_views = None
 
     def set_inputs(self, Q0, nfield):
         self.z.set_state(np.asarray(Q0, dtype=np.float64))
@@ -68,26 +69,30 @@ class GaugeRun:
             self.z.set_stream(st)
             self._stream = st
 
-    def _valid_view(self, loc, row0, nrows):
+    def _valid_view(self, loc, box, row0, nrows):
         """Strided torch view of rows [row0, row0 + nrows) of the valid cells
-        of a library array located by zamr_device_ptr, shape (nrows, ny, nx)."""
+        of a box's library array located by zamr_device_ptr, shape (nrows,
+        ny, nx) of the box (lo0, lo1, hi0, hi1)."""
         ptr, rs, cs, ngrow, _ = loc
-        nx, ny, _ = self.z.shape()
+        lo0, lo1, hi0, hi1 = box
         start = ptr + 8 * (row0 * cs + ngrow * rs + ngrow)
-        return _device_array(start, (nrows, ny, nx), (8 * cs, 8 * rs, 8), self._device)
+        return _device_array(start, (nrows, hi1 - lo1 + 1, hi0 - lo0 + 1), (8 * cs, 8 * rs, 8), self._device)
 
     def forward_device(self, Q0, nfield):
         """Gauge depths (nsteps, ngauges) as a device tensor; Q0 (n_q, ny, nx)
         and nfield (ny, nx) are CUDA float64 tensors."""
         self._device = Q0.device
         self._bind_stream(Q0.device)
-        if self._aux_loc is None:
-            # the Qaux array of the one box stays where it is for the
-            # context's life; only its stored input row (Manning) is written
-            self._aux_loc = self.z.device_ptr(field=1)
-        q = self._valid_view(self.z.device_ptr(field=0), 0, self.z.nq)
-        q.copy_(Q0)
-        self._valid_view(self._aux_loc, 1, 1)[0].copy_(nfield)
+        if self._boxes is None:
+            # the boxes' Qaux arrays stay where they are for the context's
+            # life; only their stored input row (Manning) is written
+            self._boxes = self.z.level_boxes(0)
+            self._aux_views = [self._valid_view(self.z.device_ptr(field=1, box=k), bx, 1, 1)[0]
+                               for k, bx in enumerate(self._boxes)]
+        for k, (lo0, lo1, hi0, hi1) in enumerate(self._boxes):
+            q = self._valid_view(self.z.device_ptr(field=0, box=k), self._boxes[k], 0, self.z.nq)
+            q.copy_(Q0[:, lo1:hi1 + 1, lo0:hi0 + 1])
+            self._aux_views[k].copy_(nfield[lo1:hi1 + 1, lo0:hi0 + 1])
         gh = torch.empty((len(self.dts), len(self.gauges[0])), dtype=torch.float64, device=Q0.device)
         self.z.adj_forward_device(gh.data_ptr())
         self.generation += 1

@@ -356,7 +356,12 @@ struct Sys {
     // forward with the limiter frozen).
     template <int SW>
     __device__ static __forceinline__ double slope(double qm, double q, double qp, unsigned* code, int n) {
-        const double a = q - qm, b = qp - q;
+        return slope_ab<SW>(q - qm, qp - q, code, n);
+    }
+
+    // The same from the two differences a = q - qm and b = qp - q.
+    template <int SW>
+    __device__ static __forceinline__ double slope_ab(double a, double b, unsigned* code, int n) {
         if constexpr (SW == 2) {
             const unsigned k = (*code >> (2 * n)) & 3u;
             return (k == 1u) ? a : ((k == 2u) ? b : 0.0);
@@ -391,13 +396,18 @@ struct Sys {
             } else if constexpr (AuxOnlyHinv<M>::value) {
                 // A face whose depths before and after the reconstruction all
                 // reach eps: hinv of the reconstructed depth is 1/h* there, and
-                // one reciprocal r = 1/(h h*) per side gives both the scaling
-                // h*/h = h* h* r and 1/h* = h r.  Decided per face (a face's
-                // value never depends on the faces sharing its wave); only the
-                // two factors and the two aux rows differ between the paths.
+                // r = 1/(h h*) per side gives both the scaling h*/h = h* h* r
+                // and 1/h* = h r; both sides' r come from one reciprocal of the
+                // product of the four depths (a normal number on this path).
+                // Decided per face (a face's value never depends on the faces
+                // sharing its wave); only the two factors and the two aux rows
+                // differ between the paths.
                 const double eps = p(M::zamr_idx_eps, 0);
-                if (hL >= eps && hR >= eps && hLs >= eps && hRs >= eps) {
-                    const double rL = recip(hL * hLs), rR = recip(hR * hRs);
+                const double pL = hL * hLs, pR = hR * hRs;
+                const double P = pL * pR;
+                if (hL >= eps && hR >= eps && hLs >= eps && hRs >= eps && __builtin_isnormal(P)) {
+                    const double R = recip(P);
+                    const double rL = pR * R, rR = pL * R;
                     sL = hLs * hLs * rL;
                     sR = hRs * hRs * rR;
                     aL = VA{};
@@ -493,7 +503,58 @@ struct Sys {
         for (int r = 0; r < nq; ++r)
 #pragma unroll
             for (int c = 0; c < nq; ++c)
-                if (!triv[r] && triv[c] && !lit[r][c]) ok = ok && isfinite(A[r][c]);
+                if (!triv[r] && triv[c] && !lit[r][c]) ok = ok & isfinite(A[r][c]);
+        // Two rows left (momentum friction in two dimensions): D = [a b; c d]
+        // by its determinant.  The elimination fails exactly when an entry
+        // is not finite, both entries of the pivot column are zero or the
+        // second pivot (det / first pivot) is zero, i.e. det == 0 for finite
+        // entries; the solution is Cramer's, one reciprocal for both rows.
+        int nact = 0, ia = 0, ib = 0;
+#pragma unroll
+        for (int r = 0; r < nq; ++r) {
+            if (triv[r]) continue;
+            if (nact == 0) ia = r;
+            if (nact == 1) ib = r;
+            ++nact;
+        }
+        if (nact == 2) {
+            double a = 0.0, b = 0.0, c = 0.0, d = 0.0, r0 = 0.0, r1 = 0.0;
+            bool lb = false, lc = false;
+#pragma unroll
+            for (int r = 0; r < nq; ++r) {
+                if (r == ia) {
+                    r0 = rhs[r];
+#pragma unroll
+                    for (int k = 0; k < nq; ++k) {
+                        if (k == ia) a = A[r][k];
+                        if (k == ib) { b = A[r][k]; lb = lit[r][k]; }
+                    }
+                }
+                if (r == ib) {
+                    r1 = rhs[r];
+#pragma unroll
+                    for (int k = 0; k < nq; ++k) {
+                        if (k == ia) { c = A[r][k]; lc = lit[r][k]; }
+                        if (k == ib) d = A[r][k];
+                    }
+                }
+            }
+            // Literal-zero couplings drop out (a product with one is not
+            // folded by the compiler: it could be NaN for a non-finite
+            // factor).  A non-finite entry makes det non-finite (or, for an
+            // entry whose partner coupling is a literal zero, the solution),
+            // so the checks on det and on the solution cover every entry.
+            const double bc = (lb || lc) ? 0.0 : b * c;
+            const double det = a * d - bc;
+            const double rdet = recip(det);
+            const double x0 = (lb ? d * r0 : d * r0 - b * r1) * rdet;
+            const double x1 = (lc ? a * r1 : a * r1 - c * r0) * rdet;
+            ok = ok & (det != 0.0) & isfinite(det) & isfinite(x0) & isfinite(x1);
+            double dq[nq];
+#pragma unroll
+            for (int n = 0; n < nq; ++n) dq[n] = (n == ia) ? x0 : ((n == ib) ? x1 : 0.0);
+            return finish_solve(q_old, S, dt, dq, triv, ok);
+        }
         double inv[nq];
 #pragma unroll
         for (int k = 0; k < nq; ++k) {
@@ -506,7 +567,7 @@ struct Sys {
                     best = fabs(A[r][k]);
                     piv = r;
                 }
-            ok = ok && (best > 0.0) && isfinite(best);
+            ok = ok & (best > 0.0) & isfinite(best);
             // row swap by predication (registers cannot be indexed), skipped as
             // a whole when no lane of the wave pivots
             if (__any(piv != k)) {
@@ -549,17 +610,30 @@ struct Sys {
             for (int c = r + 1; c < nq; ++c)
                 if (!triv[c]) sum += A[r][c] * dq[c];
             dq[r] = (rhs[r] - sum) * inv[r];
-            ok = ok && isfinite(dq[r]);
+            ok = ok & isfinite(dq[r]);
         }
+        return finish_solve(q_old, S, dt, dq, triv, ok);
+    }
+
+    // q_old + dq where the solve succeeded, else the explicit fallback
+    // (make_rhs.H:501-508): q_old + dt S where that is finite, else q_old.
+    // A row whose source is a literal zero falls back to q_old (q_old +
+    // dt * 0 is q_old up to the sign of a zero, or not finite exactly when
+    // q_old is not), and a trivial row (dq = 0) keeps q_old on success:
+    // such rows take no arithmetic, and a row that is both keeps q_old on
+    // either path.
+    __device__ static __forceinline__ VQ finish_solve(const VQ& q_old, const VQ& S, double dt, const double (&dq)[nq],
+                                                      const bool (&triv)[nq], bool ok) {
         VQ out;
 #pragma unroll
         for (int n = 0; n < nq; ++n) {
-            if (ok) {
-                out(n, 0) = triv[n] ? q_old(n, 0) : q_old(n, 0) + dq[n];
-            } else {
+            double fb = q_old(n, 0);
+            if (!zero_literal(S(n, 0))) {
                 const double v = q_old(n, 0) + dt * S(n, 0);
-                out(n, 0) = isfinite(v) ? v : q_old(n, 0);
+                fb = isfinite(v) ? v : q_old(n, 0);
             }
+            const double ov = triv[n] ? q_old(n, 0) : q_old(n, 0) + dq[n];
+            out(n, 0) = ok ? ov : fb;
         }
         return out;
     }
@@ -900,8 +974,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_m
 #pragma unroll
                 for (int n = 0; n < nq; ++n) {
                     if constexpr (ORDER == 2) {
-                        const double qm = from_left(qc(n, 0)), qp = from_right(qc(n, 0));
-                        const double sx = S::template slope<SW>(qm, qc(n, 0), qp, &cx, n);
+                        // the forward difference qp - q of the left lane is
+                        // this lane's backward difference q - qm, bit for bit
+                        const double b = from_right(qc(n, 0)) - qc(n, 0);
+                        const double sx = S::template slope_ab<SW>(from_left(b), b, &cx, n);
                         eR(n, 0) = qc(n, 0) + 0.5 * sx;
                         eL(n, 0) = qc(n, 0) - 0.5 * sx;
                     } else {
@@ -1008,31 +1084,43 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_m
             if (cell) {
                 const typename S::VX X = S::center(B, c, j);
                 const VA ac = S::aux_at(B, c, j, qc, p, time, X);
-                VQ src{};
-                if (!a.impl) src = M::source(qc, ac, p, time, X);
+                // The cell's RHS and forward-Euler update.  The explicit
+                // source is added in a branch of its own (a.impl is uniform
+                // over the launch): added through selects, its adds would run
+                // for every cell of an implicit-source stage too.
+                auto assemble = [&](auto with_src, const VQ& src) {
+                    VQ out;
 #pragma unroll
-                for (int n = 0; n < nq; ++n) {
-                    double Fy = 0.0;
-                    if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
-                    double rhs = kBitwise ? Fx[n] / B.dx0 + Fy / B.dx1 : Fx[n] * rdx + Fy * rdy;
-                    if (!a.impl) rhs += src(n, 0);
-                    if (wb) {
-                        if constexpr (kBitwise) {
-                            if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
-                            if constexpr (dim == 2)
-                                if (n == 3) rhs += 0.5 * p(0, 0) * (hyT * hyT - hyB * hyB) / B.dx1;
-                        } else {
-                            if (n == 2) rhs += (hL_xr * hL_xr - hxL * hxL) * hgx;
-                            if constexpr (dim == 2)
-                                if (n == 3) rhs += (hyT * hyT - hyB * hyB) * hgy;
+                    for (int n = 0; n < nq; ++n) {
+                        double Fy = 0.0;
+                        if constexpr (dim == 2) Fy = -(fyT(n, 0) - fyB(n, 0)) - dyT(nq + n, 0) - dyB(n, 0);
+                        double rhs = kBitwise ? Fx[n] / B.dx0 + Fy / B.dx1 : Fx[n] * rdx + Fy * rdy;
+                        if constexpr (decltype(with_src)::value) rhs += src(n, 0);
+                        if (wb) {
+                            if constexpr (kBitwise) {
+                                if (n == 2) rhs += 0.5 * p(0, 0) * (hL_xr * hL_xr - hxL * hxL) / B.dx0;
+                                if constexpr (dim == 2)
+                                    if (n == 3) rhs += 0.5 * p(0, 0) * (hyT * hyT - hyB * hyB) / B.dx1;
+                            } else {
+                                if (n == 2) rhs += (hL_xr * hL_xr - hxL * hxL) * hgx;
+                                if constexpr (dim == 2)
+                                    if (n == 3) rhs += (hyT * hyT - hyB * hyB) * hgy;
+                            }
                         }
+                        if constexpr (M::has_diffusion) {  // make_rhs.H:214-227, after the bed source
+                            rhs += kBitwise ? Dx[n] / B.dx0 : Dx[n] * rdx;
+                            if constexpr (dim == 2)
+                                rhs += kBitwise ? (dfyT(n, 0) - dfyB(n, 0)) / B.dx1 : (dfyT(n, 0) - dfyB(n, 0)) * rdy;
+                        }
+                        out(n, 0) = qc(n, 0) + dt * rhs;
                     }
-                    if constexpr (M::has_diffusion) {  // make_rhs.H:214-227, after the bed source
-                        rhs += kBitwise ? Dx[n] / B.dx0 : Dx[n] * rdx;
-                        if constexpr (dim == 2)
-                            rhs += kBitwise ? (dfyT(n, 0) - dfyB(n, 0)) / B.dx1 : (dfyT(n, 0) - dfyB(n, 0)) * rdy;
-                    }
-                    qn(n, 0) = qc(n, 0) + dt * rhs;
+                    return out;
+                };
+                if (a.impl) {
+                    qn = assemble(std::false_type{}, qc);
+                } else {
+                    const VQ src = M::source(qc, ac, p, time, X);
+                    qn = assemble(std::true_type{}, src);
                 }
                 const long long idx = isw;
                 if (a.pre) {

@@ -71,24 +71,11 @@ struct Model {
     using VN = amrex::SmallMatrix<amrex::Real, 2, 1>;
     using VX = amrex::SmallMatrix<amrex::Real, 3, 1>;
 
-    // x^(-1/3) for x > 0 from IEEE + - * only: exponent-halving seed, then
-    // Newton on y^-3 = x.  Deterministic, so host and device agree bitwise.
+    // x^(-1/3) for x > 0 (zamr_math::rcbrt: the same steps on host and in
+    // the bitwise device build, so the two agree bit for bit)
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_rcbrt(amrex::Real x)
     {
-        unsigned long long bits;
-        __builtin_memcpy(&bits, &x, sizeof(bits));
-        // the seed from the high word alone: a 32-bit division by 3 (the low
-        // word moves the seed by under 2^-20, far inside its ~3.5% error)
-        const unsigned int hi = static_cast<unsigned int>(bits >> 32);
-        bits = static_cast<unsigned long long>(0x553ef0ffu - hi / 3u) << 32;
-        amrex::Real y;
-        __builtin_memcpy(&y, &bits, sizeof(y));
-        const amrex::Real third = 1.0 / 3.0;
-        for (int it = 0; it < 4; ++it) {
-            const amrex::Real y3 = y * y * y;
-            y = y * (4.0 - x * y3) * third;
-        }
-        return y;
+        return zamr_math::rcbrt(x);
     }
 
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_hinv(amrex::Real h, amrex::Real eps)

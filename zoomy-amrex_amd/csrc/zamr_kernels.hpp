@@ -626,14 +626,18 @@ struct Sys {
                                                       const bool (&triv)[nq], bool ok) {
         VQ out;
 #pragma unroll
-        for (int n = 0; n < nq; ++n) {
-            double fb = q_old(n, 0);
-            if (!zero_literal(S(n, 0))) {
-                const double v = q_old(n, 0) + dt * S(n, 0);
-                fb = isfinite(v) ? v : q_old(n, 0);
+        for (int n = 0; n < nq; ++n) out(n, 0) = triv[n] ? q_old(n, 0) : q_old(n, 0) + dq[n];
+        // the fallback is evaluated only by waves with a failed solve
+        if (__any(!ok)) {
+#pragma unroll
+            for (int n = 0; n < nq; ++n) {
+                double fb = q_old(n, 0);
+                if (!zero_literal(S(n, 0))) {
+                    const double v = q_old(n, 0) + dt * S(n, 0);
+                    fb = isfinite(v) ? v : q_old(n, 0);
+                }
+                if (!ok) out(n, 0) = fb;
             }
-            const double ov = triv[n] ? q_old(n, 0) : q_old(n, 0) + dq[n];
-            out(n, 0) = ok ? ov : fb;
         }
         return out;
     }
@@ -662,10 +666,16 @@ struct Sys {
                 // make_rhs.H's arithmetic bit for bit, and the identity rows
                 // of I - dt J fold away.
                 double chain = 0.0;
+                bool has_chain = false;
 #pragma unroll
                 for (int k = 0; k < na; ++k)
-                    if (!zero_literal(dA_dQ(k * nq + c, 0))) chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
-                const double jc = dS_dQ(r * nq + c, 0) + chain;
+                    if (!zero_literal(dA_dQ(k * nq + c, 0))) {
+                        chain += dS_dA(r * na + k, 0) * dA_dQ(k * nq + c, 0);
+                        has_chain = true;
+                    }
+                // (the throughput build adds no all-literal-zero chain term:
+                // x + 0 differs from x only for x = -0)
+                const double jc = (kBitwise || has_chain) ? dS_dQ(r * nq + c, 0) + chain : dS_dQ(r * nq + c, 0);
                 lit[r][c] = zero_literal(jc);
                 A[r][c] = lit[r][c] ? (r == c ? 1.0 : 0.0) : (r == c ? 1.0 : 0.0) - dt * jc;
             }

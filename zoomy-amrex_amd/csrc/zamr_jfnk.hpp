@@ -14,9 +14,10 @@
 // by row, component-major over the level: v[n * N + c].  Everything runs on
 // the device, the control flow included: an Arnoldi step is three passes
 // over the vectors (operator + first Gram-Schmidt dots; first subtraction +
-// second dots; second subtraction + norm), each ending in the block that
-// finishes last, which sums the per-block partials in a fixed order
-// (deterministic) and, after the third, applies the Givens rotations,
+// second dots; second subtraction + norm).  Each pass leaves per-block
+// partial sums, summed in a fixed order (deterministic): the Gram-Schmidt
+// coefficients by every block of the pass that uses them, the norm by a
+// one-block kernel after the third, which applies the Givens rotations,
 // updates the residual estimate and decides whether the cycle goes on.  A
 // state word (`phase`) in device memory steers which of the queued kernels
 // act, so the host queues several Arnoldi steps, the cycle's end (least
@@ -140,7 +141,11 @@ struct JfStatus {
 
 enum : int { kJfArnoldi = 0, kJfUpdate = 2, kJfRestartRes = 3, kJfFinished = 5, kJfQUpdate = 6 };
 
-constexpr int kJfCellsPerThread = 4;
+// Cells per thread of a cell pass (elements per thread of a vector pass: 4x
+// that).  Measured at 1024^2 (ms per step): 1 -> 1.57, 2 -> 1.22, 4 -> 1.26,
+// 8 -> 1.42; fewer blocks cut the per-block partial sums every pass reads,
+// more cells per thread cut the loads a thread keeps in flight.
+constexpr int kJfCellsPerThread = 2;
 constexpr long kJfMaxIter = 2000;
 constexpr int kJfNewtonMax = 12;
 
@@ -366,27 +371,23 @@ __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restr
     }
 }
 
-// The Gram-Schmidt coefficients of pass 1 (WHICH 0: h1) or 2 (WHICH 1: h2).
-template <int WHICH>
-__global__ void __launch_bounds__(256) jf_coef_tail(JfState* st, const double* part, int nblk) {
-    __shared__ double lds[4];
-    if (st->nd || st->phase != kJfArnoldi) return;
-    jf_sums(part, nblk, st->k + 1, WHICH == 0 ? st->h1 : st->h2, lds);
-}
-
 // Arnoldi passes 2 (PASS 0) and 3 (PASS 1) over the len vector elements:
-// w -= h_j V_j for j = 0..k in order, w written back, then partials of
-// <V_j, w> (pass 2) or of |w|^2 (pass 3).
+// the Gram-Schmidt coefficients h_j of the previous pass (h1 after pass 1,
+// h2 after pass 2) summed from its per-block partials `pin` by every block
+// in the same fixed order (so all blocks hold the same values; block 0
+// keeps them in the state for the column's end), then w -= h_j V_j for
+// j = 0..k in order, w written back, then partials of <V_j, w> (pass 2) or
+// of |w|^2 (pass 3) into `pout`, a buffer other than `pin`.
 template <int PASS>
 __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double* __restrict__ w,
-                                                           double* const* __restrict__ V, const JfState* st,
-                                                           double* part) {
+                                                           double* const* __restrict__ V, JfState* st,
+                                                           const double* __restrict__ pin, double* __restrict__ pout) {
     __shared__ double lds[4];
     __shared__ double hv[kJfRestart + 1];
     if (st->nd || st->phase != kJfArnoldi) return;
     const int k = st->k;
-    if ((int)threadIdx.x <= k) hv[threadIdx.x] = PASS == 0 ? st->h1[threadIdx.x] : st->h2[threadIdx.x];
-    __syncthreads();
+    jf_sums(pin, gridDim.x, k + 1, hv, lds);
+    if (blockIdx.x == 0 && (int)threadIdx.x <= k) (PASS == 0 ? st->h1 : st->h2)[threadIdx.x] = hv[threadIdx.x];
     constexpr int kE = kJfCellsPerThread * 4;  // elements per thread
     double we[kE];
     const long long base = (long long)blockIdx.x * kE * blockDim.x + threadIdx.x;
@@ -424,7 +425,7 @@ __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double
             for (int e = 0; e < kE; ++e) sacc += we[e] * we[e];
         }
         const double v = jf_block_sum(sacc, lds);
-        if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = v;
+        if (threadIdx.x == 0) pout[(long long)j * gridDim.x + blockIdx.x] = v;
     }
 }
 
@@ -707,6 +708,7 @@ class Jfnk {
     std::vector<double*> V;   // basis vectors, kRestart + 1
     double** d_V = nullptr;   // their device pointer list
     double* d_part = nullptr; // (kRestart + 1) x nblk partial sums
+    double* d_part2 = nullptr; // the same, for the pass that reads d_part's
     JfState* d_st = nullptr;
     JfStatus* h_status = nullptr;  // host-visible, written by the kernels
     long newton_its = 0, gmres_its = 0;  // last solve
@@ -716,7 +718,7 @@ class Jfnk {
 
     ~Jfnk() { release(); }
     void release() {
-        for (double* p : {q, qexp, S, rhs, x, w, r, d_part})
+        for (double* p : {q, qexp, S, rhs, x, w, r, d_part, d_part2})
             if (p) hipFree(p);
         for (double* p : V)
             if (p) hipFree(p);
@@ -725,7 +727,7 @@ class Jfnk {
         if (d_st) hipFree(d_st);
         if (d_bstart) hipFree(d_bstart);
         if (h_status) hipHostFree(h_status);
-        q = qexp = S = rhs = x = w = r = d_part = nullptr;
+        q = qexp = S = rhs = x = w = r = d_part = d_part2 = nullptr;
         d_V = nullptr;
         d_st = nullptr;
         h_status = nullptr;
@@ -758,6 +760,7 @@ class Jfnk {
         ZAMR_HIP(hipMalloc(&d_V, V.size() * sizeof(double*)));
         ZAMR_HIP(hipMemcpy(d_V, V.data(), V.size() * sizeof(double*), hipMemcpyHostToDevice));
         ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nblk * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_part2, (size_t)(kRestart + 1) * nblk * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_st, sizeof(JfState)));
         ZAMR_HIP(hipHostMalloc(&h_status, sizeof(JfStatus), hipHostMallocCoherent | hipHostMallocMapped));
         std::memset(h_status, 0, sizeof(JfStatus));
@@ -782,10 +785,10 @@ class Jfnk {
                 hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
                                    (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time,
                                    (const JfState*)d_st, d_part);
-                hipLaunchKernelGGL(jf_coef_tail<0>, one, T, 0, stream, d_st, cpart, nblk);
-                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, G, T, 0, stream, len, w, dV, (const JfState*)d_st, d_part);
-                hipLaunchKernelGGL(jf_coef_tail<1>, one, T, 0, stream, d_st, cpart, nblk);
-                hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, G, T, 0, stream, len, w, dV, (const JfState*)d_st, d_part);
+                // pass 1 -> d_part -> pass 2 -> d_part2 -> pass 3 -> d_part
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, G, T, 0, stream, len, w, dV, d_st, cpart, d_part2);
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, G, T, 0, stream, len, w, dV, d_st,
+                                   (const double*)d_part2, d_part);
                 hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, hs, cpart, nblk, rtol);
             }
             hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);

@@ -84,6 +84,32 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double rcbrt(double x) {
     return y;
 }
 
+// Arithmetic that leaves out terms with a structurally zero operand.  A value
+// the compiler knows to be the constant zero after inlining (an entry of a
+// model matrix that is never set, a component of an axis normal) makes a
+// term that is an exact zero for finite operands, which IEEE rules forbid
+// the compiler to drop (0 x is NaN for infinite x, 0 + x is +0 for x = -0):
+// a printed model never contains such terms, and these helpers omit them.
+// For finite values the results equal the plain expressions (up to the sign
+// of a zero), on host and device alike.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool known_zero(double v) { return __builtin_constant_p(v) && v == 0.0; }
+// acc + x
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double add(double acc, double x) {
+    if (known_zero(x)) return acc;
+    if (known_zero(acc)) return x;
+    return acc + x;
+}
+// a b
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double mul(double a, double b) {
+    if (known_zero(a) || known_zero(b)) return 0.0;
+    return a * b;
+}
+// acc + a b
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double madd(double acc, double a, double b) {
+    if (known_zero(a) || known_zero(b)) return acc;
+    return add(acc, a * b);
+}
+
 // Projections on a face or boundary normal.  Every normal the driver hands
 // the model (make_rhs.H's n_hat, the boundary normals of FillPhysicalBC) is a
 // unit axis vector, so a component that is zero contributes nothing: it is

@@ -483,7 +483,11 @@ struct Sys {
     // their columns (the identity columns multiply dq = 0).  D is eliminated
     // as make_rhs.H:467-499 eliminates the whole matrix (partial pivoting,
     // strict '>' search, failure when the pivot is not > 0 or not finite);
-    // each pivot's reciprocal is kept for the back substitution.  The whole
+    // each pivot's reciprocal is kept for the back substitution.  Entries
+    // known at compile time to be zero stay out of the pivot search (|0| is
+    // never > the pivot candidate), the row swaps and the row operations
+    // (a zero multiplier or a zero pivot-row entry changes nothing), so a
+    // block-diagonal D is eliminated block by block.  The whole
     // matrix is singular exactly when D is, and a non-finite entry anywhere
     // fails the full elimination, so the outcome is the reference's up to
     // rounding: on failure the explicit update q + dt S, or q_old where that
@@ -563,7 +567,7 @@ struct Sys {
             double best = fabs(A[k][k]);
 #pragma unroll
             for (int r = k + 1; r < nq; ++r)
-                if (!triv[r] && fabs(A[r][k]) > best) {
+                if (!triv[r] && !zero_literal(A[r][k]) && fabs(A[r][k]) > best) {
                     best = fabs(A[r][k]);
                     piv = r;
                 }
@@ -573,7 +577,7 @@ struct Sys {
             if (__any(piv != k)) {
 #pragma unroll
                 for (int r = k + 1; r < nq; ++r) {
-                    if (!triv[r] && r == piv) {
+                    if (!triv[r] && !zero_literal(A[r][k]) && r == piv) {
 #pragma unroll
                         for (int c = k; c < nq; ++c) {
                             if (triv[c]) continue;
@@ -590,12 +594,12 @@ struct Sys {
             inv[k] = recip(A[k][k]);
 #pragma unroll
             for (int r = k + 1; r < nq; ++r) {
-                if (triv[r]) continue;
+                if (triv[r] || zero_literal(A[r][k])) continue;
                 const double f = A[r][k] * inv[k];
 #pragma unroll
                 for (int c = k + 1; c < nq; ++c)
-                    if (!triv[c]) A[r][c] -= f * A[k][c];
-                rhs[r] -= f * rhs[k];
+                    if (!triv[c] && !zero_literal(A[k][c])) A[r][c] -= f * A[k][c];
+                if (!zero_literal(rhs[k])) rhs[r] -= f * rhs[k];
             }
         }
         double dq[nq];
@@ -608,7 +612,7 @@ struct Sys {
             double sum = 0.0;
 #pragma unroll
             for (int c = r + 1; c < nq; ++c)
-                if (!triv[c]) sum += A[r][c] * dq[c];
+                if (!triv[c] && !zero_literal(A[r][c])) sum += A[r][c] * dq[c];
             dq[r] = (rhs[r] - sum) * inv[r];
             ok = ok & isfinite(dq[r]);
         }

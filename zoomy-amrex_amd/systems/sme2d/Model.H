@@ -102,8 +102,10 @@ struct Model {
     // sum_jk A_ijk x_j y_k for i = 1, 2 (x, y the first and second moments)
     static AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real zamr_A(int i, const amrex::Real* x, const amrex::Real* y)
     {
-        return (i == 1) ? 0.4 * (x[0] * y[1] + x[1] * y[0])
-                        : (2.0 / 3.0) * x[0] * y[0] + (2.0 / 7.0) * x[1] * y[1];
+        using zamr_math::madd;
+        using zamr_math::mul;
+        return (i == 1) ? mul(0.4, madd(mul(x[0], y[1]), x[1], y[0]))
+                        : madd(mul(mul(2.0 / 3.0, x[0]), y[0]), mul(2.0 / 7.0, x[1]), y[1]);
     }
 
     // Flux along direction d of the conservative part (rows 0..7).
@@ -168,7 +170,7 @@ struct Model {
             const amrex::Real Bm[2][2] = {{0.2 * mk[1], -0.2 * mk[0]}, {-mk[0], -mk[1] * (1.0 / 7.0)}};
             for (int i = 1; i <= 2; ++i)
                 for (int j = 1; j <= 2; ++j)
-                    B[m(c, i) * 8 + m(d, j)] = ((i == j) ? -mean : 0.0) + Bm[i - 1][j - 1];
+                    B[m(c, i) * 8 + m(d, j)] = zamr_math::add((i == j) ? -mean : 0.0, Bm[i - 1][j - 1]);
         }
     }
 
@@ -197,48 +199,48 @@ struct Model {
         const amrex::Real at[2] = {Q(m(e, 1), 0) * a, Q(m(e, 2), 0) * a};
         const amrex::Real cj[2] = {1.0 / 3.0, 1.0 / 5.0};
         // h row
-        A[1 * 8 + m(d, 0)] += 1.0;
+        A[1 * 8 + m(d, 0)] = zamr_math::add(A[1 * 8 + m(d, 0)], 1.0);
         // mean momentum along d
         {
             const int r = m(d, 0);
-            A[r * 8 + 1] += -un * un - cj[0] * an[0] * an[0] - cj[1] * an[1] * an[1] + p(0, 0) * h;
-            A[r * 8 + m(d, 0)] += 2.0 * un;
-            for (int j = 0; j < 2; ++j) A[r * 8 + m(d, j + 1)] += 2.0 * cj[j] * an[j];
+            A[r * 8 + 1] = zamr_math::add(A[r * 8 + 1], -un * un - cj[0] * an[0] * an[0] - cj[1] * an[1] * an[1] + p(0, 0) * h);
+            A[r * 8 + m(d, 0)] = zamr_math::add(A[r * 8 + m(d, 0)], 2.0 * un);
+            for (int j = 0; j < 2; ++j) A[r * 8 + m(d, j + 1)] = zamr_math::add(A[r * 8 + m(d, j + 1)], 2.0 * cj[j] * an[j]);
         }
         // mean momentum across d
         {
             const int r = m(e, 0);
-            A[r * 8 + 1] += -un * ut - cj[0] * an[0] * at[0] - cj[1] * an[1] * at[1];
-            A[r * 8 + m(d, 0)] += ut;
-            A[r * 8 + m(e, 0)] += un;
+            A[r * 8 + 1] = zamr_math::add(A[r * 8 + 1], -un * ut - cj[0] * an[0] * at[0] - cj[1] * an[1] * at[1]);
+            A[r * 8 + m(d, 0)] = zamr_math::add(A[r * 8 + m(d, 0)], ut);
+            A[r * 8 + m(e, 0)] = zamr_math::add(A[r * 8 + m(e, 0)], un);
             for (int j = 0; j < 2; ++j) {
-                A[r * 8 + m(d, j + 1)] += cj[j] * at[j];
-                A[r * 8 + m(e, j + 1)] += cj[j] * an[j];
+                A[r * 8 + m(d, j + 1)] = zamr_math::add(A[r * 8 + m(d, j + 1)], cj[j] * at[j]);
+                A[r * 8 + m(e, j + 1)] = zamr_math::add(A[r * 8 + m(e, j + 1)], cj[j] * an[j]);
             }
         }
         // moments along d: h (2 u_n a_i + A_i(a, a))
         for (int i = 1; i <= 2; ++i) {
             const int r = m(d, i);
-            A[r * 8 + 1] += -2.0 * un * an[i - 1] - zamr_A(i, an, an);
-            A[r * 8 + m(d, 0)] += 2.0 * an[i - 1];
+            A[r * 8 + 1] = zamr_math::add(A[r * 8 + 1], -2.0 * un * an[i - 1] - zamr_A(i, an, an));
+            A[r * 8 + m(d, 0)] = zamr_math::add(A[r * 8 + m(d, 0)], 2.0 * an[i - 1]);
             for (int q = 1; q <= 2; ++q) {
                 amrex::Real ex[2] = {0.0, 0.0};
                 ex[q - 1] = 1.0;
                 // d/d(h a_q) of A_i(a, a) h = A_i(e_q, a) + A_i(a, e_q)
-                A[r * 8 + m(d, q)] += ((i == q) ? 2.0 * un : 0.0) + zamr_A(i, ex, an) + zamr_A(i, an, ex);
+                A[r * 8 + m(d, q)] = zamr_math::add(A[r * 8 + m(d, q)], zamr_math::add(zamr_math::add((i == q) ? 2.0 * un : 0.0, zamr_A(i, ex, an)), zamr_A(i, an, ex)));
             }
         }
         // moments across d: h (u_n b_i + u_t a_i + A_i(b, a))
         for (int i = 1; i <= 2; ++i) {
             const int r = m(e, i);
-            A[r * 8 + 1] += -(un * at[i - 1] + ut * an[i - 1] + zamr_A(i, at, an));
-            A[r * 8 + m(d, 0)] += at[i - 1];
-            A[r * 8 + m(e, 0)] += an[i - 1];
+            A[r * 8 + 1] = zamr_math::add(A[r * 8 + 1], -(un * at[i - 1] + ut * an[i - 1] + zamr_A(i, at, an)));
+            A[r * 8 + m(d, 0)] = zamr_math::add(A[r * 8 + m(d, 0)], at[i - 1]);
+            A[r * 8 + m(e, 0)] = zamr_math::add(A[r * 8 + m(e, 0)], an[i - 1]);
             for (int q = 1; q <= 2; ++q) {
                 amrex::Real ex[2] = {0.0, 0.0};
                 ex[q - 1] = 1.0;
-                A[r * 8 + m(d, q)] += ((i == q) ? ut : 0.0) + zamr_A(i, at, ex);
-                A[r * 8 + m(e, q)] += ((i == q) ? un : 0.0) + zamr_A(i, ex, an);
+                A[r * 8 + m(d, q)] = zamr_math::add(A[r * 8 + m(d, q)], zamr_math::add((i == q) ? ut : 0.0, zamr_A(i, at, ex)));
+                A[r * 8 + m(e, q)] = zamr_math::add(A[r * 8 + m(e, q)], zamr_math::add((i == q) ? un : 0.0, zamr_A(i, ex, an)));
             }
         }
     }
@@ -314,7 +316,7 @@ struct Model {
         for (int d = 0; d < 2; ++d)
             for (int i = 0; i < 3; ++i) {
                 for (int j = 0; j < 3; ++j) res(m(d, i) * 8 + m(d, j), 0) = -w[i] * slip * a;
-                res(m(d, i) * 8 + m(d, i), 0) += -w[i] * nu * a * a * cd[i];
+                if (cd[i] != 0.0) res(m(d, i) * 8 + m(d, i), 0) += -w[i] * nu * a * a * cd[i];
             }
         return res;
     }
@@ -330,7 +332,8 @@ struct Model {
         for (int d = 0; d < 2; ++d) {
             const amrex::Real sum = Q(m(d, 0), 0) + Q(m(d, 1), 0) + Q(m(d, 2), 0);
             for (int i = 0; i < 3; ++i)
-                res(m(d, i) * 7 + 6, 0) = -w[i] * slip * sum - 2.0 * w[i] * nu * a * cd[i] * Q(m(d, i), 0);
+                res(m(d, i) * 7 + 6, 0) = (cd[i] != 0.0) ? -w[i] * slip * sum - 2.0 * w[i] * nu * a * cd[i] * Q(m(d, i), 0)
+                                                         : -w[i] * slip * sum;
         }
         return res;
     }

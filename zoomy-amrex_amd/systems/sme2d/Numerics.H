@@ -30,7 +30,7 @@ struct Numerics {
         for (int d = 0; d < 2; ++d) {
             if (n(d, 0) == 0.0) continue;
             Model::zamr_quasilinear_dir(Q, Qaux, p, d, Ad);
-            for (int k = 0; k < 64; ++k) A[k] += n(d, 0) * Ad[k];
+            for (int k = 0; k < 64; ++k) A[k] = zamr_math::madd(A[k], n(d, 0), Ad[k]);
         }
         const amrex::Real c2 = p(0, 0) * amrex::max(Q(1, 0), p(5, 0));
         const amrex::Real c = zamr_math::sqrt(c2);
@@ -40,9 +40,9 @@ struct Numerics {
             amrex::Real s = 0.0;
             for (int j = 1; j < 8; ++j) {
                 amrex::Real v = std::fabs(A[i * 8 + j]);
-                if (i == 1 && j != 1) v *= c;
-                if (i != 1 && j == 1) v *= rc;
-                s += v;
+                if (i == 1 && j != 1) v = zamr_math::mul(v, c);
+                if (i != 1 && j == 1) v = zamr_math::mul(v, rc);
+                s = zamr_math::add(s, v);
             }
             rho = amrex::max(rho, s);
         }
@@ -61,8 +61,8 @@ struct Numerics {
             const Model::VQ a = Model::zamr_flux_dir(qL, aL, p, d);
             const Model::VQ b = Model::zamr_flux_dir(qR, aR, p, d);
             for (int r = 0; r < 8; ++r) {
-                fL(r, 0) += n(d, 0) * a(r, 0);
-                fR(r, 0) += n(d, 0) * b(r, 0);
+                fL(r, 0) = zamr_math::madd(fL(r, 0), n(d, 0), a(r, 0));
+                fR(r, 0) = zamr_math::madd(fR(r, 0), n(d, 0), b(r, 0));
             }
         }
         amrex::Real sL = local_max_abs_eigenvalue(qL, aL, p, n)(0, 0);
@@ -96,15 +96,15 @@ struct Numerics {
                 Model::zamr_ncp_dir(q, a, p, d, B);
                 for (int i = 0; i < 8; ++i) {
                     amrex::Real acc = 0.0;
-                    for (int j = 0; j < 8; ++j) acc += B[i * 8 + j] * dq(j, 0);
-                    I(i, 0) += w3[k] * n(d, 0) * acc;
+                    for (int j = 0; j < 8; ++j) acc = zamr_math::madd(acc, B[i * 8 + j], dq(j, 0));
+                    I(i, 0) = zamr_math::madd(I(i, 0), zamr_math::mul(w3[k], n(d, 0)), acc);
                 }
             }
         }
         amrex::SmallMatrix<amrex::Real, 16, 1> res{};
         for (int i = 0; i < 8; ++i) {
-            res(i, 0) = 0.5 * I(i, 0);
-            res(8 + i, 0) = 0.5 * I(i, 0);
+            res(i, 0) = zamr_math::mul(0.5, I(i, 0));
+            res(8 + i, 0) = zamr_math::mul(0.5, I(i, 0));
         }
         return res;
     }

@@ -40,6 +40,38 @@ def parse_traffic(n_cells):
     return float(m.group(1)) if m else None
 
 
+def pattern_ceiling(n, t1_ms, t2_ms):
+    """The HBM rate the stage kernels' memory access pattern reaches with no
+    arithmetic (zamr_probe_pattern: the same loads and stores in the same
+    decomposition over an n^2 box of 4 rows) and a plain 1 GiB device copy,
+    from the calibration library; None when it is not built."""
+    import ctypes
+    from zamr import compile as zc
+    path = zc.LIBDIR / "libzamr_probe.so"
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    lib.zamr_probe_pattern.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
+    lib.zamr_probe_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    out = {}
+    cells = n * n
+    for key, heun, algo in (("stage1", 0, BYTES_STAGE1), ("stage2", 1, BYTES_STAGE2)):
+        if lib.zamr_probe_pattern(n, N_Q, heun, 20, ctypes.byref(ms)) != 0:
+            return None
+        out[key + "_pattern_ms"] = ms.value
+        out[key + "_pattern_GBps"] = algo * cells / (ms.value * 1e-3) / 1e9
+    if lib.zamr_probe_copy(1 << 30, 20, ctypes.byref(ms)) != 0:
+        return None
+    out["copy_GBps"] = 2 * (1 << 30) / (ms.value * 1e-3) / 1e9
+    out["stage2_of_pattern"] = out["stage2_pattern_ms"] / t2_ms if t2_ms > 0 else 0.0
+    out["stage1_of_pattern"] = out["stage1_pattern_ms"] / t1_ms if t1_ms > 0 else 0.0
+    out["note"] = ("the stage kernels' loads and stores alone (no arithmetic) and a plain copy: "
+                   "stage2_of_pattern = pattern time / stage-2 launch time")
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (lscpu's "Model name", else /proc/cpuinfo)."""
     import subprocess
@@ -478,6 +510,9 @@ def main():
                      else 0.0,
                      "timed_launches": nk},
     }
+    pc = pattern_ceiling(args.n, ktimes[0], ktimes[1])
+    if pc is not None:
+        out["roofline"]["pattern_ceiling"] = pc
     if args.calibrate_copy:
         # a plain fill (writes 1 GiB) and a plain sum (reads 1 GiB)
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")

@@ -102,6 +102,16 @@ def compile_system(system_dir, name, variants, contract=True, objdir=OBJDIR, lib
     return outs
 
 
+def compile_probe(libdir=LIBDIR):
+    """The bandwidth-calibration library bench.py times the stage kernels'
+    memory access pattern with (csrc/zamr_probe.hip)."""
+    libdir = pathlib.Path(libdir)
+    libdir.mkdir(parents=True, exist_ok=True)
+    out = libdir / "libzamr_probe.so"
+    _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "zamr_probe.hip"), "-o", str(out)])
+    return out
+
+
 def main_path(system, dim, order, contract=True):
     return LIBDIR / f"main_{system}_d{dim}o{order}{'' if contract else '_nofma'}.ex"
 

@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
             continue;
         }
         lam = wave_max(lam);
-        if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
+        if (t == 0) group_max(&G->maxinv_bits, lam);
     }
 }
 

@@ -771,6 +771,19 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// A wave's non-negative maximum v into a group's CFL word (the bit pattern
+// of a non-negative double: such doubles order as their bit patterns).  One
+// atomic per wave on one word per group contends -- a launch has thousands
+// of items -- and only a wave that raises the maximum needs it.  The word,
+// read here without ordering, is at most its true value: it only grows
+// during a launch, from the value it held when the launch started (kernel
+// boundaries order the clearing before it), so a stale read costs an atomic
+// and never skips a needed one.
+__device__ __forceinline__ void group_max(unsigned long long* word, double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (v > 0.0 && b > *reinterpret_cast<volatile unsigned long long*>(word)) atomicMax(word, b);
+}
+
 // Whole-wave lane shifts (DPP wave_shr / wave_shl): lane i receives lane
 // i-1 (from_left) or lane i+1 (from_right); the edge lane receives 0, by the
 // bound control bit, so no register has to be cleared first.
@@ -1192,7 +1205,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_m
         }
         if constexpr (EPI != 0) {
             lam = wave_max(lam);
-            if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
+            if (t == 0) group_max(&G->maxinv_bits, lam);
         }
         if constexpr (EPI == 1) {
             if (a.mood == 1) {
@@ -1251,7 +1264,7 @@ __global__ void __launch_bounds__(NT) max_inv_kernel(const BoxDesc* __restrict__
             }
         }
         lam = wave_max(lam);
-        if (t == 0 && lam > 0.0) atomicMax(&G->maxinv_bits, (unsigned long long)__double_as_longlong(lam));
+        if (t == 0) group_max(&G->maxinv_bits, lam);
     }
 }
 

@@ -137,6 +137,7 @@ struct JfState {
 struct JfStatus {
     int phase, nd, bad_rows, pad_;
     long long its, newton_its, gmres_total;
+    int its_hist[16];  // Arnoldi steps of each finished Newton iteration (the host's next queueing)
 };
 
 enum : int { kJfArnoldi = 0, kJfUpdate = 2, kJfRestartRes = 3, kJfFinished = 5, kJfQUpdate = 6 };
@@ -514,6 +515,7 @@ __global__ void __launch_bounds__(256) jf_update_tail(JfState* st, JfStatus* hs,
     if (st->nd || st->phase != kJfUpdate) return;
     if (st->done) {
         if (threadIdx.x == 0) {
+            if (st->newton_its < 16) hs->its_hist[st->newton_its] = (int)st->its;
             st->gmres_total += st->its;
             st->newton_its += 1;
             st->phase = kJfFinished;
@@ -608,6 +610,7 @@ __global__ void __launch_bounds__(256) jf_qupdate_kernel(long long N, double* __
 
 __global__ void jf_qupdate_tail(JfState* st, JfStatus* hs) {
     if (st->nd || st->phase != kJfQUpdate || threadIdx.x != 0) return;
+    if (st->newton_its < 16) hs->its_hist[st->newton_its] = (int)st->its;
     st->gmres_total += st->its;
     st->newton_its += 1;
     st->phase = kJfFinished;
@@ -715,6 +718,7 @@ class Jfnk {
     long long total_solves = 0, total_newton = 0, total_gmres = 0;
     int chunk = 2;        // Arnoldi steps queued in the first round
     int newton_pred = 0;  // Newton iterations of the last solve
+    std::vector<int> steps_pred;  // Arnoldi steps of each of those iterations
 
     ~Jfnk() { release(); }
     void release() {
@@ -804,8 +808,19 @@ class Jfnk {
             hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, hs, cpart, nblk, it);
             arnoldi(steps);
         };
-        const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
-        for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2));
+        if ((int)steps_pred.size() == newton_pred && newton_pred > 0 && newton_pred < kJfNewtonMax) {
+            // as the last solve went: its iterations with their own Arnoldi
+            // step counts, then the residual pass that found it converged
+            // (a solve that needs more is finished by the rounds below)
+            for (int it = 0; it < newton_pred; ++it) iteration(it, std::max(steps_pred[it], 1));
+            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
+                               (const double*)qexp, S, rhs, dt, p, time, d_st, d_part, newton_pred);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, hs, cpart, nblk, newton_pred);
+            ZAMR_HIP(hipGetLastError());
+        } else {
+            const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
+            for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2));
+        }
         ZAMR_HIP(hipStreamSynchronize(stream));
         long long rounds = 0;
         for (;;) {
@@ -831,6 +846,7 @@ class Jfnk {
         if (h_status->bad_rows)
             throw Error(4, "the global source solve: a row the model declares source-free has a nonzero source");
         newton_pred = it;
+        steps_pred.assign(h_status->its_hist, h_status->its_hist + std::min(it, 16));
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 1,
                            (double*)nullptr);
         ZAMR_HIP(hipGetLastError());

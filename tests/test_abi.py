@@ -90,3 +90,13 @@ def test_write_inputs_keys(tmp_path):
     assert kv["solver.clamp_positivity"] == "false" and kv["solver.positivity"] == "none"
     assert kv["params.n_m"] == "0.03"
     assert (tmp_path / "inputs").read_text() == text
+
+
+def test_probe_library_exports():
+    """The bandwidth-calibration library bench.py times the stage kernels'
+    access pattern with is built beside the system libraries."""
+    path = zc.LIBDIR / "libzamr_probe.so"
+    assert path.exists(), "run __graft_entry__.build()"
+    lib = ctypes.CDLL(str(path))
+    for name in ("zamr_probe_pattern", "zamr_probe_copy"):
+        assert hasattr(lib, name)

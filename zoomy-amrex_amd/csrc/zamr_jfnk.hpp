@@ -130,10 +130,12 @@ struct JfState {
     long long gmres_total;  // over the solve
     int bad_rows;           // a row declared source-free had a nonzero source
     int pad_;
+    int its_hist[16];       // Arnoldi steps of each finished Newton iteration
 };
 
-// What the host reads after a round, written by the kernels that change it
-// into host-visible memory (no copy to queue).
+// What the host reads after a round: the state's words it steers by, copied
+// into host-visible memory by one small kernel at the end of the round (the
+// kernels of the round write device memory only).
 struct JfStatus {
     int phase, nd, bad_rows, pad_;
     long long its, newton_its, gmres_total;
@@ -150,7 +152,11 @@ constexpr int kJfCellsPerThread = 2;
 constexpr long kJfMaxIter = 2000;
 constexpr int kJfNewtonMax = 12;
 
-__device__ __forceinline__ void jf_status(const JfState* st, JfStatus* hs) {
+// The status of the state, for the host; also each finished iteration's
+// Arnoldi step count.
+__global__ void __launch_bounds__(64) jf_publish_kernel(const JfState* st, JfStatus* hs) {
+    if (threadIdx.x < 16) hs->its_hist[threadIdx.x] = st->its_hist[threadIdx.x];
+    if (threadIdx.x != 0) return;
     hs->phase = st->phase;
     hs->nd = st->nd;
     hs->bad_rows = st->bad_rows;
@@ -172,15 +178,47 @@ __device__ __forceinline__ double jf_block_sum(double v, double* lds) {
     return ((lds[0] + lds[1]) + lds[2]) + lds[3];
 }
 
-// out[j] = sum over the nblk blocks of part[j * nblk + b], by a fixed tree
-// (thread t sums blocks t, t + 256, ... in order, then jf_block_sum); j < m.
-__device__ __forceinline__ void jf_sums(const double* __restrict__ part, int nblk, int m, double* out, double* lds) {
-    for (int j = 0; j < m; ++j) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(long long)j * nblk + b];
-        const double tot = jf_block_sum(s, lds);
-        if (threadIdx.x == 0) out[j] = tot;
+// out[j] = sum over the nblk blocks of part[j * nblk + b], j < m <= 31, by a
+// fixed tree -- thread t sums blocks t, t + 256, ... in order, a wave's sums
+// by xor butterflies, then the four wave sums in order (jf_block_sum's) --
+// so every block gets the same values.  A thread's loads for eight of its
+// blocks go out together (indices clamped into range, the extra values not
+// added), so a row costs a memory latency per 2048 blocks instead of one per
+// load; one barrier for all rows.  One row at a time: more loads in flight
+// would take registers the vector passes that call this need for their
+// occupancy.  `out` is shared memory, valid in every thread on return.
+__device__ __forceinline__ void jf_sums(const double* __restrict__ part, int nblk, int m, double* out) {
+    constexpr int kR = 1, kU = 8;
+    __shared__ double wsum[kJfRestart + 1][4];
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+    for (int j0 = 0; j0 < m; j0 += kR) {
+        double s[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) s[r] = 0.0;
+        for (int b0 = t; b0 < nblk; b0 += kU * 256) {
+            double v[kR][kU];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                const long long row = (long long)min(j0 + r, m - 1) * nblk;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) v[r][u] = part[row + min(b0 + u * 256, nblk - 1)];
+            }
+#pragma unroll
+            for (int r = 0; r < kR; ++r)
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (b0 + u * 256 < nblk) s[r] += v[r][u];
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            double v = s[r];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0 && j0 + r < m) wsum[j0 + r][wv] = v;
+        }
     }
+    __syncthreads();
+    if (t < m) out[t] = ((wsum[t][0] + wsum[t][1]) + wsum[t][2]) + wsum[t][3];
     __syncthreads();
 }
 
@@ -222,13 +260,11 @@ __device__ __forceinline__ void jf_apply_cell(const BoxDesc* __restrict__ boxes,
         if (const long long c = ((long long)blockIdx.x * kJfCellsPerThread + pc) * blockDim.x + threadIdx.x; \
             c < N)
 
-// Zero the state (all threads, 8-byte words) and publish it.
-__global__ void __launch_bounds__(256) jf_init_kernel(JfState* st, JfStatus* hs) {
+// Zero the state (all threads, 8-byte words).
+__global__ void __launch_bounds__(256) jf_init_kernel(JfState* st) {
     static_assert(sizeof(JfState) % 8 == 0, "the state is a whole number of 8-byte words");
     unsigned long long* w = reinterpret_cast<unsigned long long*>(st);
     for (int i = threadIdx.x; i < (int)(sizeof(JfState) / 8); i += blockDim.x) w[i] = 0ull;
-    __syncthreads();
-    if (threadIdx.x == 0) jf_status(st, hs);
 }
 
 // A new cycle's Hessenberg matrix and residual vector, zeroed by all threads.
@@ -238,20 +274,47 @@ __device__ __forceinline__ void jf_clear_cycle(JfState* st) {
     __syncthreads();
 }
 
-// Newton residual: S = S(q), F = (q - Qexp) + (-dt) S, rhs = -F on the source
-// rows; partials of |F|^2 (row 0) and of |q|^2 over every row (row 1).
+// A finished Newton iteration's bookkeeping (one thread).
+__device__ __forceinline__ void jf_newton_done(JfState* st) {
+    if (st->newton_its < 16) st->its_hist[st->newton_its] = (int)st->its;
+    st->gmres_total += st->its;
+    st->newton_its += 1;
+    st->phase = kJfFinished;
+}
+
+// Newton iteration `it` runs once the previous one is finished, or, fused,
+// right after the previous one's GMRES cycle converged with its correction
+// not yet applied (phase kJfUpdate, done): the Newton residual pass then
+// applies it first.  Read by the residual pass and its tail alike (the pass
+// changes none of these words).
+__device__ __forceinline__ bool jf_newton_fused(const JfState* st, int it) {
+    return it > 0 && st->phase == kJfUpdate && st->done && st->newton_its == it - 1;
+}
+__device__ __forceinline__ bool jf_newton_runs(const JfState* st, int it) {
+    return !st->nd && ((st->newton_its == it && (it == 0 || st->phase == kJfFinished)) || jf_newton_fused(st, it));
+}
+
+// Newton residual: (fused) Q += x + y_j V_j on the source rows, the converged
+// cycle's correction as jf_update_kernel applies it; then S = S(q),
+// F = (q - Qexp) + (-dt) S, rhs = -F on the source rows; partials of |F|^2
+// (row 0) and of |q|^2 over every row (row 1).
 template <class M>
 __global__ void __launch_bounds__(256) jf_newton_kernel(const BoxDesc* __restrict__ boxes,
                                                         const long long* __restrict__ bstart, int nb, long long N,
-                                                        const double* __restrict__ q, const double* __restrict__ qexp,
+                                                        double* __restrict__ q, const double* __restrict__ qexp,
+                                                        const double* __restrict__ x, double* const* __restrict__ V,
                                                         double* __restrict__ S, double* __restrict__ rhs, double dt,
                                                         Params pp, double time, JfState* st, double* part, int it) {
     using Sy = Sys<M>;
     using R = SourceRows<M>;
     constexpr int nq = Sy::nq;
     __shared__ double lds[4];
-    // iteration `it` runs once the previous one is finished
-    if (st->nd || st->newton_its != it || (it > 0 && st->phase != kJfFinished)) return;
+    __shared__ double yv[kJfRestart];
+    if (!jf_newton_runs(st, it)) return;
+    const bool fused = jf_newton_fused(st, it);
+    const int kd = fused ? st->kd : 0, first = st->first;
+    if ((int)threadIdx.x < kd) yv[threadIdx.x] = st->y[threadIdx.x];
+    __syncthreads();
     typename Sy::VP p;
     for (int k = 0; k < Sy::np; ++k) p(k, 0) = pp.v[k];
     double f2 = 0.0, q2 = 0.0;
@@ -262,6 +325,16 @@ __global__ void __launch_bounds__(256) jf_newton_kernel(const BoxDesc* __restric
         typename Sy::VQ qq;
 #pragma unroll
         for (int n = 0; n < nq; ++n) qq(n, 0) = q[n * N + c];
+        if (fused) {
+#pragma unroll
+            for (int a = 0; a < R::n; ++a) {
+                const long long i = a * N + c;
+                double xv = first ? 0.0 : x[i];
+                for (int j = 0; j < kd; ++j) xv += yv[j] * V[j][i];
+                qq(R::row(a), 0) = qq(R::row(a), 0) + 1.0 * xv;
+                q[R::row(a) * N + c] = qq(R::row(a), 0);
+            }
+        }
         const typename Sy::VX X = Sy::center(B, e.i, e.j);
         typename Sy::VA av{};
         av = M::update_aux_variables(qq, av, p, time, X);
@@ -287,22 +360,21 @@ __global__ void __launch_bounds__(256) jf_newton_kernel(const BoxDesc* __restric
 }
 
 // Newton's convergence test and the start of the GMRES cycle.
-__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
-                                                      int it) {
-    __shared__ double lds[4];
+__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double* part, int nblk, int it) {
     __shared__ double tot[2];
-    if (st->nd || st->newton_its != it || (it > 0 && st->phase != kJfFinished)) return;
-    jf_sums(part, nblk, 2, tot, lds);
+    if (!jf_newton_runs(st, it)) return;
+    const bool fused = jf_newton_fused(st, it);
+    jf_sums(part, nblk, 2, tot);
     const double resid = sqrt(tot[0]);
     const double r0 = it == 0 ? resid : st->resid0;
     const bool conv = resid < 1e-12 || (r0 > 0.0 && resid < 1e-9 * r0) || st->bad_rows;
     if (!conv) jf_clear_cycle(st);
     if (threadIdx.x != 0) return;
+    if (fused) jf_newton_done(st);  // the previous iteration's end
     if (it == 0) st->resid0 = resid;
     st->resid = resid;
     if (conv) {
         st->nd = 1;
-        jf_status(st, hs);
         return;
     }
     st->eps_num = 1e-7 * (1.0 + sqrt(tot[1]));
@@ -316,7 +388,6 @@ __global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, JfStatus* hs,
     st->its = 0;
     st->g[0] = resid;
     st->phase = kJfArnoldi;
-    jf_status(st, hs);
 }
 
 // Arnoldi pass 1: V_k = src / vnorm_k (stored; src = rhs or r at k = 0, the
@@ -387,7 +458,7 @@ __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double
     __shared__ double hv[kJfRestart + 1];
     if (st->nd || st->phase != kJfArnoldi) return;
     const int k = st->k;
-    jf_sums(pin, gridDim.x, k + 1, hv, lds);
+    jf_sums(pin, gridDim.x, k + 1, hv);
     if (blockIdx.x == 0 && (int)threadIdx.x <= k) (PASS == 0 ? st->h1 : st->h2)[threadIdx.x] = hv[threadIdx.x];
     constexpr int kE = kJfCellsPerThread * 4;  // elements per thread
     double we[kE];
@@ -433,46 +504,75 @@ __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double
 // The end of an Arnoldi step: H[j][k] = h1 + h2, H[k+1][k] = |w|, Givens
 // rotations, the residual estimate, and the cycle's next move; at the end
 // of a cycle the least-squares coefficients y = H^-1 g.
-__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
-                                                      double rtol) {
-    __shared__ double lds[4];
+__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, const double* part, int nblk, double rtol) {
+    // the column, the rotations and g in shared memory: loaded by all
+    // threads at once, worked on by one, stored back by all
     __shared__ double nn[1];
+    __shared__ double hc[kJfRestart + 1], cs[kJfRestart], sn[kJfRestart], g[kJfRestart + 1], y[kJfRestart];
+    __shared__ double Hs[kJfRestart][kJfRestart];
+    __shared__ int fin;
     if (st->nd || st->phase != kJfArnoldi) return;
-    jf_sums(part, nblk, 1, nn, lds);
-    if (threadIdx.x != 0) return;
-    const int k = st->k;
-    for (int j = 0; j <= k; ++j) st->H[j][k] = 0.0 + st->h1[j] + st->h2[j];
-    const double hn = sqrt(nn[0]);
-    st->H[k + 1][k] = hn;
-    for (int j = 0; j < k; ++j) {
-        const double t = st->cs[j] * st->H[j][k] + st->sn[j] * st->H[j + 1][k];
-        st->H[j + 1][k] = -st->sn[j] * st->H[j][k] + st->cs[j] * st->H[j + 1][k];
-        st->H[j][k] = t;
+    const int k = st->k, t = threadIdx.x;
+    if (t <= k) {
+        hc[t] = 0.0 + st->h1[t] + st->h2[t];
+        g[t] = st->g[t];
     }
-    const double den = hypot(st->H[k][k], st->H[k + 1][k]);
-    st->cs[k] = den > 0.0 ? st->H[k][k] / den : 1.0;
-    st->sn[k] = den > 0.0 ? st->H[k + 1][k] / den : 0.0;
-    st->H[k][k] = den;
-    st->H[k + 1][k] = 0.0;
-    st->g[k + 1] = -st->sn[k] * st->g[k];
-    st->g[k] = st->cs[k] * st->g[k];
-    st->its += 1;
-    st->kd = k + 1;
-    const bool conv = fabs(st->g[k + 1]) < st->r0 * rtol;
-    if (conv || hn == 0.0 || st->its >= kJfMaxIter || k + 1 == kJfRestart) {
-        st->done = conv ? 1 : 0;
-        const int kd = k + 1;
-        for (int i = kd - 1; i >= 0; --i) {
-            double t = st->g[i];
-            for (int j = i + 1; j < kd; ++j) t -= st->H[i][j] * st->y[j];
-            st->y[i] = st->H[i][i] != 0.0 ? t / st->H[i][i] : 0.0;
+    if (t < k) {
+        cs[t] = st->cs[t];
+        sn[t] = st->sn[t];
+    }
+    jf_sums(part, nblk, 1, nn);
+    if (t == 0) {
+        const double hn = sqrt(nn[0]);
+        hc[k + 1] = hn;
+        for (int j = 0; j < k; ++j) {
+            const double u = cs[j] * hc[j] + sn[j] * hc[j + 1];
+            hc[j + 1] = -sn[j] * hc[j] + cs[j] * hc[j + 1];
+            hc[j] = u;
         }
-        st->phase = kJfUpdate;
-        jf_status(st, hs);
-    } else {
-        st->vnorm[k + 1] = hn;
-        st->k = k + 1;
+        const double den = hypot(hc[k], hc[k + 1]);
+        cs[k] = den > 0.0 ? hc[k] / den : 1.0;
+        sn[k] = den > 0.0 ? hc[k + 1] / den : 0.0;
+        hc[k] = den;
+        hc[k + 1] = 0.0;
+        g[k + 1] = -sn[k] * g[k];
+        g[k] = cs[k] * g[k];
+        st->cs[k] = cs[k];
+        st->sn[k] = sn[k];
+        st->g[k] = g[k];
+        st->g[k + 1] = g[k + 1];
+        st->its += 1;
+        st->kd = k + 1;
+        const bool conv = fabs(g[k + 1]) < st->r0 * rtol;
+        fin = conv || hn == 0.0 || st->its >= kJfMaxIter || k + 1 == kJfRestart;
+        if (fin) {
+            st->done = conv ? 1 : 0;
+            st->phase = kJfUpdate;
+        } else {
+            st->vnorm[k + 1] = hn;
+            st->k = k + 1;
+        }
     }
+    __syncthreads();
+    if (t <= k + 1) st->H[t][k] = hc[t];
+    if (!fin) return;
+    // the cycle's end: y = H^-1 g over its kd = k + 1 columns (H upper
+    // triangular after the rotations; column k is the one above)
+    const int kd = k + 1;
+    for (int e = t; e < kd * kd; e += blockDim.x) {
+        const int i = e / kd, j = e % kd;
+        if (j >= i) Hs[i][j] = j == k ? hc[i] : st->H[i][j];
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int i = kd - 1; i >= 0; --i) {
+            double u = g[i];
+            for (int j = i + 1; j < kd; ++j) u -= Hs[i][j] * y[j];
+            y[i] = Hs[i][i] != 0.0 ? u / Hs[i][i] : 0.0;
+        }
+    }
+    __syncthreads();
+    if (t < kd) st->y[t] = y[t];
 }
 
 // x += y_j V_j, j in order (x = 0 in the first cycle), per cell; converged:
@@ -509,26 +609,18 @@ __global__ void __launch_bounds__(256) jf_update_kernel(long long N, double* __r
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-__global__ void __launch_bounds__(256) jf_update_tail(JfState* st, JfStatus* hs, const double* part, int nblk) {
-    __shared__ double lds[4];
+__global__ void __launch_bounds__(256) jf_update_tail(JfState* st, const double* part, int nblk) {
     __shared__ double tot[1];
     if (st->nd || st->phase != kJfUpdate) return;
     if (st->done) {
-        if (threadIdx.x == 0) {
-            if (st->newton_its < 16) hs->its_hist[st->newton_its] = (int)st->its;
-            st->gmres_total += st->its;
-            st->newton_its += 1;
-            st->phase = kJfFinished;
-            jf_status(st, hs);
-        }
+        if (threadIdx.x == 0) jf_newton_done(st);
         return;
     }
-    jf_sums(part, nblk, 1, tot, lds);
+    jf_sums(part, nblk, 1, tot);
     if (threadIdx.x == 0) {
         st->xnorm = sqrt(tot[0]);
         st->first = 0;
         st->phase = kJfRestartRes;
-        jf_status(st, hs);
     }
 }
 
@@ -571,19 +663,16 @@ __global__ void __launch_bounds__(256) jf_restart_kernel(const BoxDesc* __restri
 }
 
 // Converged: Q update; else a new cycle from r.
-__global__ void __launch_bounds__(256) jf_restart_tail(JfState* st, JfStatus* hs, const double* part, int nblk,
-                                                       double rtol) {
-    __shared__ double lds[4];
+__global__ void __launch_bounds__(256) jf_restart_tail(JfState* st, const double* part, int nblk, double rtol) {
     __shared__ double tot[1];
     if (st->nd || st->phase != kJfRestartRes) return;
-    jf_sums(part, nblk, 1, tot, lds);
+    jf_sums(part, nblk, 1, tot);
     const double rn = sqrt(tot[0]);
     const bool fin = rn < st->r0 * rtol || rn == 0.0 || st->its >= kJfMaxIter;
     if (!fin) jf_clear_cycle(st);
     if (threadIdx.x != 0) return;
     if (fin) {
         st->phase = kJfQUpdate;
-        jf_status(st, hs);
         return;
     }
     st->vnorm[0] = rn;
@@ -593,7 +682,6 @@ __global__ void __launch_bounds__(256) jf_restart_tail(JfState* st, JfStatus* hs
     st->done = 0;
     st->g[0] = rn;
     st->phase = kJfArnoldi;
-    jf_status(st, hs);
 }
 
 // Q += x on the source rows after a restart residual met the tolerance.
@@ -608,13 +696,9 @@ __global__ void __launch_bounds__(256) jf_qupdate_kernel(long long N, double* __
     }
 }
 
-__global__ void jf_qupdate_tail(JfState* st, JfStatus* hs) {
+__global__ void jf_qupdate_tail(JfState* st) {
     if (st->nd || st->phase != kJfQUpdate || threadIdx.x != 0) return;
-    if (st->newton_its < 16) hs->its_hist[st->newton_its] = (int)st->its;
-    st->gmres_total += st->its;
-    st->newton_its += 1;
-    st->phase = kJfFinished;
-    jf_status(st, hs);
+    jf_newton_done(st);
 }
 
 // End of a step whose stages solved the source globally: Heun with buffer
@@ -685,15 +769,16 @@ __global__ void __launch_bounds__(NT) finish_step_kernel(const BoxDesc* __restri
 }
 
 // Host side: the vectors and the queueing.  The host queues Newton
-// iterations ahead -- each the residual pass, a round of Arnoldi steps (as
-// many as the last solve's iterations took on average) and the cycle's
-// update -- as far as the last solve went, its converging residual test
-// included, then waits once.  Kernels act only in their phase (a Newton
-// iteration only after the previous one finished), so over-queued work
-// returns at once and under-queued work is completed by further rounds
-// (more Arnoldi steps, a restart, the next iteration) after reading the
-// state back.  The kernels write the few words the host reads into
-// host-visible memory.
+// iterations ahead -- each the residual pass (which first applies the
+// previous iteration's converged correction) and a round of Arnoldi steps
+// (as many as that iteration took in the last solve) -- as far as the last
+// solve went, its converging residual test included, then waits once.
+// Kernels act only in their phase (a Newton iteration only after the
+// previous one finished or converged), so over-queued work returns at once
+// and under-queued work is completed by further rounds (more Arnoldi steps
+// and the cycle's update, a restart, the next iteration) after reading the
+// state back, which one small kernel copies to host-visible memory at the
+// end of each round.
 template <class M>
 class Jfnk {
    public:
@@ -782,9 +867,11 @@ class Jfnk {
         const double* cpart = d_part;
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 0,
                            qexp);
-        hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st, hs);
+        hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st);
         ZAMR_HIP(hipGetLastError());
-        auto arnoldi = [&](int steps) {
+        // `steps` Arnoldi steps; with_update: then the cycle's update (a
+        // converged cycle's is otherwise applied by the next Newton pass)
+        auto arnoldi = [&](int steps, bool with_update) {
             for (int k = 0; k < steps; ++k) {
                 hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
                                    (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time,
@@ -793,35 +880,43 @@ class Jfnk {
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, G, T, 0, stream, len, w, dV, d_st, cpart, d_part2);
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, G, T, 0, stream, len, w, dV, d_st,
                                    (const double*)d_part2, d_part);
-                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, hs, cpart, nblk, rtol);
+                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, cpart, nblk, rtol);
             }
-            hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);
-            hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, hs, cpart, nblk);
+            if (with_update) {
+                hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);
+                hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, cpart, nblk);
+            }
             ZAMR_HIP(hipGetLastError());
         };
         // Newton iteration `it`: the residual pass, then a round of Arnoldi
-        // steps and the cycle's update; queued ahead as far as the last solve
-        // went (its converging residual test included), then read back once
+        // steps; queued ahead as far as the last solve went (its converging
+        // residual test included), then read back once
         auto iteration = [&](int it, int steps) {
-            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
-                               (const double*)qexp, S, rhs, dt, p, time, d_st, d_part, it);
-            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, hs, cpart, nblk, it);
-            arnoldi(steps);
+            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
+                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_part, it);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, cpart, nblk, it);
+            arnoldi(steps, false);
         };
         if ((int)steps_pred.size() == newton_pred && newton_pred > 0 && newton_pred < kJfNewtonMax) {
             // as the last solve went: its iterations with their own Arnoldi
             // step counts, then the residual pass that found it converged
             // (a solve that needs more is finished by the rounds below)
             for (int it = 0; it < newton_pred; ++it) iteration(it, std::max(steps_pred[it], 1));
-            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
-                               (const double*)qexp, S, rhs, dt, p, time, d_st, d_part, newton_pred);
-            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, hs, cpart, nblk, newton_pred);
+            hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
+                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_part, newton_pred);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, cpart, nblk, newton_pred);
             ZAMR_HIP(hipGetLastError());
         } else {
             const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
             for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2));
         }
-        ZAMR_HIP(hipStreamSynchronize(stream));
+        // the round's status for the host, then the one wait
+        auto wait = [&]() {
+            hipLaunchKernelGGL(jf_publish_kernel, one, dim3(64), 0, stream, (const JfState*)d_st, hs);
+            ZAMR_HIP(hipGetLastError());
+            ZAMR_HIP(hipStreamSynchronize(stream));
+        };
+        wait();
         long long rounds = 0;
         for (;;) {
             if (h_status->nd || h_status->newton_its >= kJfNewtonMax) break;
@@ -833,13 +928,13 @@ class Jfnk {
                 hipLaunchKernelGGL((jf_restart_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
                                    (const double*)S, (const double*)rhs, (const double*)x, r, dt, p, time,
                                    (const JfState*)d_st, d_part);
-                hipLaunchKernelGGL(jf_restart_tail, one, T, 0, stream, d_st, hs, cpart, nblk, rtol);
+                hipLaunchKernelGGL(jf_restart_tail, one, T, 0, stream, d_st, cpart, nblk, rtol);
                 hipLaunchKernelGGL((jf_qupdate_kernel<M>), G, T, 0, stream, N, q, (const double*)x,
                                    (const JfState*)d_st);
-                hipLaunchKernelGGL(jf_qupdate_tail, one, dim3(64), 0, stream, d_st, hs);
-                arnoldi(4);
+                hipLaunchKernelGGL(jf_qupdate_tail, one, dim3(64), 0, stream, d_st);
+                arnoldi(4, true);
             }
-            ZAMR_HIP(hipStreamSynchronize(stream));
+            wait();
         }
         const int it = (int)h_status->newton_its;
         if (it > 0) chunk = (int)std::min<long long>(std::max<long long>(h_status->gmres_total / it, 2), kRestart);

@@ -449,16 +449,19 @@ __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restr
 // in the same fixed order (so all blocks hold the same values; block 0
 // keeps them in the state for the column's end), then w -= h_j V_j for
 // j = 0..k in order, w written back, then partials of <V_j, w> (pass 2) or
-// of |w|^2 (pass 3) into `pout`, a buffer other than `pin`.
+// of |w|^2 (pass 3) into `pout`, a buffer other than `pin`.  The previous
+// pass left `nprev` partials per coefficient (its grid); the grid of these
+// passes covers the len elements exactly.
 template <int PASS>
 __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double* __restrict__ w,
                                                            double* const* __restrict__ V, JfState* st,
-                                                           const double* __restrict__ pin, double* __restrict__ pout) {
+                                                           const double* __restrict__ pin, int nprev,
+                                                           double* __restrict__ pout) {
     __shared__ double lds[4];
     __shared__ double hv[kJfRestart + 1];
     if (st->nd || st->phase != kJfArnoldi) return;
     const int k = st->k;
-    jf_sums(pin, gridDim.x, k + 1, hv);
+    jf_sums(pin, nprev, k + 1, hv);
     if (blockIdx.x == 0 && (int)threadIdx.x <= k) (PASS == 0 ? st->h1 : st->h2)[threadIdx.x] = hv[threadIdx.x];
     constexpr int kE = kJfCellsPerThread * 4;  // elements per thread
     double we[kE];
@@ -790,7 +793,8 @@ class Jfnk {
     const BoxDesc* boxes = nullptr;
     int nb = 0;
     long long N = 0, len = 0;  // cells, doubles per Krylov vector (source rows)
-    int nblk = 0;              // blocks of a vector pass
+    int nblk = 0;              // blocks of a cell pass
+    int nblk_e = 0;            // blocks of an element pass (Gram-Schmidt passes 2 and 3)
     long long* d_bstart = nullptr;
     double *q = nullptr, *qexp = nullptr, *S = nullptr, *rhs = nullptr, *x = nullptr, *w = nullptr, *r = nullptr;
     std::vector<double*> V;   // basis vectors, kRestart + 1
@@ -834,12 +838,13 @@ class Jfnk {
         N = st[nb];
         len = N * na;
         // a cell pass covers kJfCellsPerThread cells per thread, an element
-        // pass 4 kJfCellsPerThread elements per thread: one grid for both,
-        // the larger of the two needs
+        // pass 4 kJfCellsPerThread elements per thread, each over a grid of
+        // its own (the partial sums a pass leaves are one per block of it)
         const long long per_blk_cells = (long long)kJfCellsPerThread * 256;
         const long long per_blk_elems = (long long)kJfCellsPerThread * 4 * 256;
-        nblk = (int)std::max((N + per_blk_cells - 1) / per_blk_cells, (len + per_blk_elems - 1) / per_blk_elems);
-        nblk = std::max(nblk, 1);
+        nblk = (int)std::max<long long>((N + per_blk_cells - 1) / per_blk_cells, 1);
+        nblk_e = (int)std::max<long long>((len + per_blk_elems - 1) / per_blk_elems, 1);
+        const int nmax = std::max(nblk, nblk_e);
         ZAMR_HIP(hipMalloc(&d_bstart, st.size() * sizeof(long long)));
         ZAMR_HIP(hipMemcpy(d_bstart, st.data(), st.size() * sizeof(long long), hipMemcpyHostToDevice));
         ZAMR_HIP(hipMalloc(&q, (size_t)N * nq * sizeof(double)));
@@ -848,8 +853,8 @@ class Jfnk {
         for (auto& v : V) ZAMR_HIP(hipMalloc(&v, len * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_V, V.size() * sizeof(double*)));
         ZAMR_HIP(hipMemcpy(d_V, V.data(), V.size() * sizeof(double*), hipMemcpyHostToDevice));
-        ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nblk * sizeof(double)));
-        ZAMR_HIP(hipMalloc(&d_part2, (size_t)(kRestart + 1) * nblk * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nmax * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_part2, (size_t)(kRestart + 1) * nmax * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_st, sizeof(JfState)));
         ZAMR_HIP(hipHostMalloc(&h_status, sizeof(JfStatus), hipHostMallocCoherent | hipHostMallocMapped));
         std::memset(h_status, 0, sizeof(JfStatus));
@@ -860,7 +865,7 @@ class Jfnk {
     // The solve on the valid cells of buffer `buf` (Qexp in, Q out).
     void solve(int buf, double dt, double time, const Params& p) {
         constexpr double rtol = 1e-4;
-        const dim3 G(nblk), T(256), one(1);
+        const dim3 G(nblk), Ge(nblk_e), T(256), one(1);
         JfStatus* hs = nullptr;
         ZAMR_HIP(hipHostGetDevicePointer((void**)&hs, h_status, 0));
         double* const* dV = (double* const*)d_V;
@@ -877,10 +882,10 @@ class Jfnk {
                                    (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time,
                                    (const JfState*)d_st, d_part);
                 // pass 1 -> d_part -> pass 2 -> d_part2 -> pass 3 -> d_part
-                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, G, T, 0, stream, len, w, dV, d_st, cpart, d_part2);
-                hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, G, T, 0, stream, len, w, dV, d_st,
-                                   (const double*)d_part2, d_part);
-                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, cpart, nblk, rtol);
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, Ge, T, 0, stream, len, w, dV, d_st, cpart, nblk, d_part2);
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, Ge, T, 0, stream, len, w, dV, d_st,
+                                   (const double*)d_part2, nblk_e, d_part);
+                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, cpart, nblk_e, rtol);
             }
             if (with_update) {
                 hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);

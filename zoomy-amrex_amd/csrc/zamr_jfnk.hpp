@@ -129,7 +129,7 @@ struct JfState {
     long long its;          // Arnoldi steps of this Newton iteration's GMRES
     long long gmres_total;  // over the solve
     int bad_rows;           // a row declared source-free had a nonzero source
-    int pad_;
+    int nrun;               // 1 + the last Newton iteration whose residual pass ran
     int its_hist[16];       // Arnoldi steps of each finished Newton iteration
 };
 
@@ -357,15 +357,15 @@ __global__ void __launch_bounds__(256) jf_newton_kernel(const BoxDesc* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = v;
     v = jf_block_sum(q2, lds);
     if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->nrun = it + 1;
 }
 
-// Newton's convergence test and the start of the GMRES cycle.
-__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double* part, int nblk, int it) {
-    __shared__ double tot[2];
-    if (!jf_newton_runs(st, it)) return;
-    const bool fused = jf_newton_fused(st, it);
-    jf_sums(part, nblk, 2, tot);
-    const double resid = sqrt(tot[0]);
+// The end of Newton iteration `it`'s residual pass, from its |F|^2 and |q|^2
+// (one block, all threads; the caller read `fused` before any of it): the
+// previous iteration's bookkeeping when the pass applied its correction,
+// the convergence test, and the start of the GMRES cycle.
+__device__ __forceinline__ void jf_newton_end(JfState* st, int it, double f2, double q2, bool fused) {
+    const double resid = sqrt(f2);
     const double r0 = it == 0 ? resid : st->resid0;
     const bool conv = resid < 1e-12 || (r0 > 0.0 && resid < 1e-9 * r0) || st->bad_rows;
     if (!conv) jf_clear_cycle(st);
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double*
         st->nd = 1;
         return;
     }
-    st->eps_num = 1e-7 * (1.0 + sqrt(tot[1]));
+    st->eps_num = 1e-7 * (1.0 + sqrt(q2));
     st->r0 = resid;  // |rhs| = |F|
     st->vnorm[0] = resid;
     st->src0 = 0;
@@ -390,27 +390,64 @@ __global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double*
     st->phase = kJfArnoldi;
 }
 
+// Newton's convergence test and the start of the GMRES cycle.
+// The end of a residual pass that no Arnoldi step follows in the queue (the
+// converging test the host queues last).
+__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double* part, int nblk, int it) {
+    __shared__ double tot[2];
+    if (st->nd || st->nrun != it + 1) return;
+    const bool fused = jf_newton_fused(st, it);
+    jf_sums(part, nblk, 2, tot);
+    jf_newton_end(st, it, tot[0], tot[1], fused);
+}
+
 // Arnoldi pass 1: V_k = src / vnorm_k (stored; src = rhs or r at k = 0, the
 // orthogonalised w after), w = J V_k, partials of <V_j, w>, j <= k.  The
 // operator's |V_k| is taken as 1: V_k is a normalised vector (the reference
-// recomputes its norm, 1 to rounding).
+// recomputes its norm, 1 to rounding).  newton_it >= 0: queued right after
+// Newton iteration newton_it's residual pass; if that pass ran (`nrun`,
+// which only it writes), this is its cycle's first step and applies the
+// pass's end first: every block works it out from the pass's partials
+// `npart` (|rhs| = |F| is vnorm_0, and eps), block 0 also writes it to the
+// state, the other blocks read none of what it writes, and a converged test
+// stops them all.  Otherwise (the pass did not run: an earlier cycle is
+// still going) it is an ordinary step, as the passes after it are.
 template <class M>
 __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restrict__ boxes,
                                                           const long long* __restrict__ bstart, int nb, long long N,
                                                           const double* __restrict__ q, const double* __restrict__ S,
                                                           const double* __restrict__ rhs, const double* __restrict__ r,
                                                           double* __restrict__ w, double* const* __restrict__ V,
-                                                          double dt, Params pp, double time, const JfState* st,
-                                                          double* part) {
+                                                          double dt, Params pp, double time, JfState* st,
+                                                          double* part, const double* __restrict__ npart,
+                                                          int newton_it) {
     using Sy = Sys<M>;
     using R = SourceRows<M>;
     constexpr int nq = Sy::nq, na = R::n;
     __shared__ double lds[4];
-    if (st->nd || st->phase != kJfArnoldi) return;
-    const int k = st->k;
-    const double eps_num = st->eps_num;
-    const double* __restrict__ bk = k > 0 ? w : (st->src0 == 0 ? rhs : r);
-    const double vk = st->vnorm[k];
+    __shared__ double tot[2];
+    int k;
+    double eps_num, vk;
+    const double* __restrict__ bk;
+    if (newton_it >= 0 && !st->nd && st->nrun == newton_it + 1) {
+        const bool fused = blockIdx.x == 0 && jf_newton_fused(st, newton_it);
+        jf_sums(npart, gridDim.x, 2, tot);
+        const double resid = sqrt(tot[0]);
+        const double r0 = newton_it == 0 ? resid : st->resid0;
+        const bool conv = resid < 1e-12 || (r0 > 0.0 && resid < 1e-9 * r0) || st->bad_rows;
+        if (blockIdx.x == 0) jf_newton_end(st, newton_it, tot[0], tot[1], fused);
+        if (conv) return;
+        k = 0;
+        eps_num = 1e-7 * (1.0 + sqrt(tot[1]));
+        vk = resid;
+        bk = rhs;
+    } else {
+        if (st->nd || st->phase != kJfArnoldi) return;
+        k = st->k;
+        eps_num = st->eps_num;
+        vk = st->vnorm[k];
+        bk = k > 0 ? w : (st->src0 == 0 ? rhs : r);
+    }
     double* __restrict__ vout = V[k];
     typename Sy::VP p;
     for (int i = 0; i < Sy::np; ++i) p(i, 0) = pp.v[i];
@@ -801,6 +838,7 @@ class Jfnk {
     double** d_V = nullptr;   // their device pointer list
     double* d_part = nullptr; // (kRestart + 1) x nblk partial sums
     double* d_part2 = nullptr; // the same, for the pass that reads d_part's
+    double* d_npart = nullptr; // 2 x nblk partial sums of a Newton residual pass
     JfState* d_st = nullptr;
     JfStatus* h_status = nullptr;  // host-visible, written by the kernels
     long newton_its = 0, gmres_its = 0;  // last solve
@@ -811,7 +849,7 @@ class Jfnk {
 
     ~Jfnk() { release(); }
     void release() {
-        for (double* p : {q, qexp, S, rhs, x, w, r, d_part, d_part2})
+        for (double* p : {q, qexp, S, rhs, x, w, r, d_part, d_part2, d_npart})
             if (p) hipFree(p);
         for (double* p : V)
             if (p) hipFree(p);
@@ -820,7 +858,7 @@ class Jfnk {
         if (d_st) hipFree(d_st);
         if (d_bstart) hipFree(d_bstart);
         if (h_status) hipHostFree(h_status);
-        q = qexp = S = rhs = x = w = r = d_part = d_part2 = nullptr;
+        q = qexp = S = rhs = x = w = r = d_part = d_part2 = d_npart = nullptr;
         d_V = nullptr;
         d_st = nullptr;
         h_status = nullptr;
@@ -855,6 +893,7 @@ class Jfnk {
         ZAMR_HIP(hipMemcpy(d_V, V.data(), V.size() * sizeof(double*), hipMemcpyHostToDevice));
         ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nmax * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_part2, (size_t)(kRestart + 1) * nmax * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_npart, (size_t)2 * nblk * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_st, sizeof(JfState)));
         ZAMR_HIP(hipHostMalloc(&h_status, sizeof(JfStatus), hipHostMallocCoherent | hipHostMallocMapped));
         std::memset(h_status, 0, sizeof(JfStatus));
@@ -874,13 +913,15 @@ class Jfnk {
                            qexp);
         hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st);
         ZAMR_HIP(hipGetLastError());
-        // `steps` Arnoldi steps; with_update: then the cycle's update (a
-        // converged cycle's is otherwise applied by the next Newton pass)
-        auto arnoldi = [&](int steps, bool with_update) {
+        // `steps` Arnoldi steps, the first of them right after Newton
+        // iteration newton_it's residual pass (-1: continuing a cycle);
+        // with_update: then the cycle's update (a converged cycle's is
+        // otherwise applied by the next Newton pass)
+        auto arnoldi = [&](int steps, bool with_update, int newton_it) {
             for (int k = 0; k < steps; ++k) {
                 hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
-                                   (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time,
-                                   (const JfState*)d_st, d_part);
+                                   (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time, d_st,
+                                   d_part, (const double*)d_npart, k == 0 ? newton_it : -1);
                 // pass 1 -> d_part -> pass 2 -> d_part2 -> pass 3 -> d_part
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, Ge, T, 0, stream, len, w, dV, d_st, cpart, nblk, d_part2);
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, Ge, T, 0, stream, len, w, dV, d_st,
@@ -894,13 +935,13 @@ class Jfnk {
             ZAMR_HIP(hipGetLastError());
         };
         // Newton iteration `it`: the residual pass, then a round of Arnoldi
-        // steps; queued ahead as far as the last solve went (its converging
-        // residual test included), then read back once
+        // steps (the first applies the pass's end); queued ahead as far as
+        // the last solve went (its converging residual test included, with
+        // the pass's end on its own), then read back once
         auto iteration = [&](int it, int steps) {
             hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
-                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_part, it);
-            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, cpart, nblk, it);
-            arnoldi(steps, false);
+                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_npart, it);
+            arnoldi(steps, false, it);
         };
         if ((int)steps_pred.size() == newton_pred && newton_pred > 0 && newton_pred < kJfNewtonMax) {
             // as the last solve went: its iterations with their own Arnoldi
@@ -908,8 +949,8 @@ class Jfnk {
             // (a solve that needs more is finished by the rounds below)
             for (int it = 0; it < newton_pred; ++it) iteration(it, std::max(steps_pred[it], 1));
             hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
-                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_part, newton_pred);
-            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, cpart, nblk, newton_pred);
+                               (const double*)x, dV, S, rhs, dt, p, time, d_st, d_npart, newton_pred);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, (const double*)d_npart, nblk, newton_pred);
             ZAMR_HIP(hipGetLastError());
         } else {
             const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
@@ -937,7 +978,7 @@ class Jfnk {
                 hipLaunchKernelGGL((jf_qupdate_kernel<M>), G, T, 0, stream, N, q, (const double*)x,
                                    (const JfState*)d_st);
                 hipLaunchKernelGGL(jf_qupdate_tail, one, dim3(64), 0, stream, d_st);
-                arnoldi(4, true);
+                arnoldi(4, true, -1);
             }
             wait();
         }

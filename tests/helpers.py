@@ -166,3 +166,39 @@ def fallback_case(order, mgs=64, nx=60, ny=8):
     return dict(nx=nx, ny=ny, prob_lo=(0.0, 0.0), prob_hi=(1.0, ny / nx), max_grid_size=mgs, spatial_order=order,
                 implicit_source=True, well_balanced=False, clamp_positivity=False, cfl=0.45, dtmin=1e-12, dtmax=1.0,
                 time_end=1e30, max_step=-1, bc=("wall", "wall", "wall", "wall"), params={"kappa": FALLBACK_KAPPA})
+
+
+def manning_restart_case(n=48, nm=3.0, decades=3.0, u=0.3):
+    """A global-source stage whose GMRES needs restarts: a flat bed, depths
+    falling smoothly from 1 m to 10^-decades m across the domain, a swirl of
+    speed u and strong Manning friction.  The per-cell backward-Euler rates
+    dt c |m| then span several decades, which GMRES(30) does not resolve in
+    one cycle, while Newton still converges (depths stay well above the
+    wet-dry threshold)."""
+    from zamr import cases
+    case, Q0 = cases.swe_bumps(n=n, seed=3, nbumps=4, steps=6, manning=nm)
+    case.update(implicit_global=True, spatial_order=2, max_grid_size=24, well_balanced=False,
+                clamp_positivity=False)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)  # [row j, column i]
+    h = 10.0 ** (-decades * 0.5 * (X + Y))
+    Q = np.zeros_like(Q0)
+    Q[1] = h
+    Q[2] = u * h * np.cos(2 * np.pi * Y)
+    Q[3] = -u * h * np.sin(2 * np.pi * X)
+    return case, Q
+
+
+def manning_backward_euler(Qexp, nm, dt, g=9.81, eps=1e-4):
+    """The backward-Euler step of swe2d's Manning source from Qexp, per cell:
+    the momentum m keeps its direction and |m| (1 + dt c |m|) = |m_exp| with
+    c = g n^2 hinv max(h, eps)^(-4/3); h and b do not change."""
+    h = Qexp[1]
+    hinv = np.sqrt(2.0) * h / np.sqrt(h ** 4 + np.maximum(h ** 4, eps ** 4))
+    c = g * nm * nm * hinv * np.maximum(h, eps) ** (-4.0 / 3.0)
+    s = np.hypot(Qexp[2], Qexp[3])
+    scale = np.where(s > 0, 2.0 / (1.0 + np.sqrt(1.0 + 4.0 * dt * c * s)), 1.0)
+    be = Qexp.copy()
+    be[2] *= scale
+    be[3] *= scale
+    return be

@@ -10,7 +10,7 @@ scale, far below the O(dt^2) gap to the point-implicit step."""
 import numpy as np
 import pytest
 
-from helpers import rel_linf
+from helpers import manning_backward_euler, manning_restart_case, rel_linf
 from oracle.pyoracle import Oracle
 from zamr import cases
 from zamr.deck import deck_from_case
@@ -79,6 +79,32 @@ def test_stage_is_the_closed_form_backward_euler():
     err = max(np.abs(Q[2] - Qexp[2] * sc).max(), np.abs(Q[3] - Qexp[3] * sc).max())
     corr = np.sqrt(np.sum((Qexp[2] - Qexp[2] * sc) ** 2) + np.sum((Qexp[3] - Qexp[3] * sc) ** 2))
     assert err <= 1e-9 * corr, (err, corr)
+
+
+def test_stage_with_gmres_restarts_is_the_closed_form_backward_euler():
+    """The closed form on a state whose depths fall from 1 m to 1 mm under
+    strong friction: GMRES takes more than 30 steps per Newton iteration on
+    average, so the device's restart path (the cycle's update with partial
+    sums, the restart residual, a new cycle from it, the Q update after a
+    restarted cycle, and rounds the host queues after reading the state
+    back) is exercised, and the result is still the exact backward-Euler
+    step to Newton's tolerance."""
+    nm, dt = 3.0, 2e-3
+    case, Q0 = manning_restart_case(nm=nm)
+    for contract in (False, True):
+        print(f"[jfnk] closed-form stage with restarts contract={contract}", flush=True)
+        f = _gpu(dict(case, params={"n_m": 0.0}), Q0, contract)
+        f.stage(2, 0.0, dt)
+        Qexp = f.get_state()
+        z = _gpu(case, Q0, contract)
+        z.stage(2, 0.0, dt)
+        Q = z.get_state()
+        solves, newton, gm = z.jfnk_stats()
+        assert solves == 1 and newton < 12 and gm > 30 * newton, (solves, newton, gm)
+        be = manning_backward_euler(Qexp, nm, dt)
+        assert np.array_equal(Q[:2], Qexp[:2])
+        err = np.abs(Q[2:] - be[2:]).max()
+        assert err <= 1e-9 * np.linalg.norm(Qexp[2:] - be[2:]), (contract, err)
 
 
 def test_no_source_equals_point_implicit_bitwise():

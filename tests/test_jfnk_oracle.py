@@ -9,13 +9,15 @@ solver:
   * the Manning source of swe2d, -g n^2 |m| hinv max(h, eps)^{-4/3} m,
     changes only the momentum m, along its own direction, so one
     backward-Euler step has a closed form per cell: |m| solves
-    |m| (1 + dt c |m|) = |m_exp| with c = g n^2 hinv max(h, eps)^{-4/3};
+    |m| (1 + dt c |m|) = |m_exp| with c = g n^2 hinv max(h, eps)^{-4/3}
+    (also on a state that makes GMRES restart);
   * on the shallow-moment system the returned state satisfies the
     backward-Euler equation with the source ComputeSourceField forms.
 """
 import numpy as np
 import pytest
 
+from helpers import manning_backward_euler, manning_restart_case
 from oracle.pyoracle import Oracle, gmres_dense
 from zamr import cases
 
@@ -103,6 +105,30 @@ def test_manning_stage_is_the_closed_form_backward_euler():
     p.set_state(Q0)
     p.stage(2, 0.0, dt)
     assert np.abs(p.get_state()[2:] - be[2:]).max() > 1e3 * max(err, 1e-16)
+
+
+def test_manning_stage_with_gmres_restarts_is_the_closed_form():
+    """The same closed form on a state whose depths fall from 1 m to 1 mm
+    under strong friction: the per-cell rates spread over decades, GMRES(30)
+    takes more than 30 steps per Newton iteration on average, so restarts
+    (restart residual, a new cycle from it, the Q update after a restarted
+    cycle) are on the path."""
+    nm, dt = 3.0, 2e-3
+    case, Q0 = manning_restart_case(nm=nm)
+    f = Oracle("swe2d", dict(case, params={"n_m": 0.0}))
+    f.set_state(Q0)
+    f.stage(2, 0.0, dt)
+    Qexp = f.get_state()
+    o = Oracle("swe2d", case)
+    o.set_state(Q0)
+    o.stage(2, 0.0, dt)
+    Q = o.get_state()
+    newton, gm = o.jfnk_iters()
+    assert newton < 12 and gm > 30 * newton  # converged, and some cycle restarted
+    be = manning_backward_euler(Qexp, nm, dt)
+    assert np.array_equal(Q[:2], Qexp[:2])
+    err = np.abs(Q[2:] - be[2:]).max()
+    assert err <= 1e-9 * np.linalg.norm(Qexp[2:] - be[2:]), err
 
 
 def test_sme_stage_solves_backward_euler():

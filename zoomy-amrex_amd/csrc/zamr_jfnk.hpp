@@ -115,6 +115,7 @@ constexpr int kJfRestart = 30;
 struct JfState {
     double H[kJfRestart + 1][kJfRestart];
     double cs[kJfRestart], sn[kJfRestart], g[kJfRestart + 1], y[kJfRestart];
+    double gp[kJfRestart + 1];  // g[k] before column k's rotation (each written once a cycle)
     double h1[kJfRestart + 1], h2[kJfRestart + 1];
     double vnorm[kJfRestart + 1];
     double r0, eps_num, xnorm, resid, resid0;
@@ -130,6 +131,8 @@ struct JfState {
     long long gmres_total;  // over the solve
     int bad_rows;           // a row declared source-free had a nonzero source
     int nrun;               // 1 + the last Newton iteration whose residual pass ran
+    int umark, ustep;       // sequence number and Arnoldi step of the last step whose third pass ran
+    int its0, pad_;         // its at the cycle's start
     int its_hist[16];       // Arnoldi steps of each finished Newton iteration
 };
 
@@ -265,6 +268,8 @@ __global__ void __launch_bounds__(256) jf_init_kernel(JfState* st) {
     static_assert(sizeof(JfState) % 8 == 0, "the state is a whole number of 8-byte words");
     unsigned long long* w = reinterpret_cast<unsigned long long*>(st);
     for (int i = threadIdx.x; i < (int)(sizeof(JfState) / 8); i += blockDim.x) w[i] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) st->umark = -1;  // no step yet (sequence numbers start at 1)
 }
 
 // A new cycle's Hessenberg matrix and residual vector, zeroed by all threads.
@@ -386,7 +391,9 @@ __device__ __forceinline__ void jf_newton_end(JfState* st, int it, double f2, do
     st->done = 0;
     st->first = 1;
     st->its = 0;
+    st->its0 = 0;
     st->g[0] = resid;
+    st->gp[0] = resid;
     st->phase = kJfArnoldi;
 }
 
@@ -401,6 +408,100 @@ __global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double*
     jf_newton_end(st, it, tot[0], tot[1], fused);
 }
 
+// The end of Arnoldi step k from w2 = |w|^2 (one block, all threads):
+// H[j][k] = h1 + h2, H[k+1][k] = |w|, the Givens rotations, the residual
+// estimate, the cycle's next move and, at its end, the least-squares
+// coefficients y = H^-1 g.  It reads only words written by earlier kernels
+// and never rewritten by this one (the step's h1 and h2, the earlier
+// rotations and rotated g, the pre-rotation gp[k], r0, its0), so every
+// block of a pass can work it out while one of them (`store`) writes the
+// results.  Returns, in every thread, whether the cycle ends here.
+__device__ __forceinline__ bool jf_column_end(JfState* st, int k, double w2, double rtol, bool store) {
+    // the column, the rotations and g in shared memory: loaded by all
+    // threads at once, worked on by one
+    __shared__ double hc[kJfRestart + 1], cs[kJfRestart], sn[kJfRestart], g[kJfRestart + 1], y[kJfRestart];
+    __shared__ double Hs[kJfRestart][kJfRestart];
+    __shared__ int fin;
+    const int t = threadIdx.x;
+    if (t <= k) hc[t] = 0.0 + st->h1[t] + st->h2[t];
+    if (t < k) {
+        cs[t] = st->cs[t];
+        sn[t] = st->sn[t];
+        g[t] = st->g[t];
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double hn = sqrt(w2);
+        hc[k + 1] = hn;
+        for (int j = 0; j < k; ++j) {
+            const double u = cs[j] * hc[j] + sn[j] * hc[j + 1];
+            hc[j + 1] = -sn[j] * hc[j] + cs[j] * hc[j + 1];
+            hc[j] = u;
+        }
+        const double den = hypot(hc[k], hc[k + 1]);
+        cs[k] = den > 0.0 ? hc[k] / den : 1.0;
+        sn[k] = den > 0.0 ? hc[k + 1] / den : 0.0;
+        hc[k] = den;
+        hc[k + 1] = 0.0;
+        const double gk = st->gp[k];
+        const double gnext = -sn[k] * gk;
+        g[k] = cs[k] * gk;
+        const long long its = st->its0 + k + 1;
+        const bool conv = fabs(gnext) < st->r0 * rtol;
+        fin = conv || hn == 0.0 || its >= kJfMaxIter || k + 1 == kJfRestart;
+        if (store) {
+            st->cs[k] = cs[k];
+            st->sn[k] = sn[k];
+            st->g[k] = g[k];
+            st->gp[k + 1] = gnext;
+            st->its = its;
+            st->kd = k + 1;
+            if (fin) {
+                st->done = conv ? 1 : 0;
+                st->phase = kJfUpdate;
+            } else {
+                st->vnorm[k + 1] = hn;
+                st->k = k + 1;
+            }
+        }
+    }
+    __syncthreads();
+    const bool f = fin;
+    if (!store) return f;
+    if (t <= k + 1) st->H[t][k] = hc[t];
+    if (!f) return f;
+    // the cycle's end: y over its kd = k + 1 columns (H upper triangular
+    // after the rotations; column k is the one above)
+    const int kd = k + 1;
+    for (int e = t; e < kd * kd; e += blockDim.x) {
+        const int i = e / kd, j = e % kd;
+        if (j >= i) Hs[i][j] = j == k ? hc[i] : st->H[i][j];
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int i = kd - 1; i >= 0; --i) {
+            double u = g[i];
+            for (int j = i + 1; j < kd; ++j) u -= Hs[i][j] * y[j];
+            y[i] = Hs[i][i] != 0.0 ? u / Hs[i][i] : 0.0;
+        }
+    }
+    __syncthreads();
+    if (t < kd) st->y[t] = y[t];
+    return f;
+}
+
+// The end of the Arnoldi step a round of queued steps closes with (sequence
+// number seq - 1), when its third pass ran: the next step's first pass ends
+// the step before it itself.
+__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, const double* part3, int nblk, double rtol,
+                                                      int seq) {
+    __shared__ double nn[1];
+    if (st->nd || st->umark != seq - 1 || st->phase != kJfArnoldi) return;
+    const int k = st->ustep;
+    jf_sums(part3, nblk, 1, nn);
+    jf_column_end(st, k, nn[0], rtol, true);
+}
+
 // Arnoldi pass 1: V_k = src / vnorm_k (stored; src = rhs or r at k = 0, the
 // orthogonalised w after), w = J V_k, partials of <V_j, w>, j <= k.  The
 // operator's |V_k| is taken as 1: V_k is a normalised vector (the reference
@@ -411,7 +512,11 @@ __global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double*
 // `npart` (|rhs| = |F| is vnorm_0, and eps), block 0 also writes it to the
 // state, the other blocks read none of what it writes, and a converged test
 // stops them all.  Otherwise (the pass did not run: an earlier cycle is
-// still going) it is an ordinary step, as the passes after it are.
+// still going) it is an ordinary step, as the passes after it are.  An
+// ordinary step whose sequence number `seq` follows that of the step whose
+// third pass ran last (`umark`, written only by third passes) first ends
+// that step (jf_column_end from its |w|^2 partials `part3`: every block,
+// block 0 storing it) and then, unless the cycle ended there, runs the next.
 template <class M>
 __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restrict__ boxes,
                                                           const long long* __restrict__ bstart, int nb, long long N,
@@ -420,7 +525,8 @@ __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restr
                                                           double* __restrict__ w, double* const* __restrict__ V,
                                                           double dt, Params pp, double time, JfState* st,
                                                           double* part, const double* __restrict__ npart,
-                                                          int newton_it) {
+                                                          int newton_it, const double* __restrict__ part3,
+                                                          int nblk3, double rtol, int seq) {
     using Sy = Sys<M>;
     using R = SourceRows<M>;
     constexpr int nq = Sy::nq, na = R::n;
@@ -441,6 +547,16 @@ __global__ void __launch_bounds__(256) jf_arnoldi1_kernel(const BoxDesc* __restr
         eps_num = 1e-7 * (1.0 + sqrt(tot[1]));
         vk = resid;
         bk = rhs;
+    } else if (!st->nd && st->umark == seq - 1 && st->phase == kJfArnoldi) {
+        // the step queued just before this one ran its third pass: its end
+        // first (every block, block 0 storing it), then the next step
+        const int kp = st->ustep;
+        jf_sums(part3, nblk3, 1, tot);
+        if (jf_column_end(st, kp, tot[0], rtol, blockIdx.x == 0)) return;
+        k = kp + 1;
+        eps_num = st->eps_num;
+        vk = sqrt(tot[0]);
+        bk = w;
     } else {
         if (st->nd || st->phase != kJfArnoldi) return;
         k = st->k;
@@ -493,13 +609,17 @@ template <int PASS>
 __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double* __restrict__ w,
                                                            double* const* __restrict__ V, JfState* st,
                                                            const double* __restrict__ pin, int nprev,
-                                                           double* __restrict__ pout) {
+                                                           double* __restrict__ pout, int seq) {
     __shared__ double lds[4];
     __shared__ double hv[kJfRestart + 1];
     if (st->nd || st->phase != kJfArnoldi) return;
     const int k = st->k;
     jf_sums(pin, nprev, k + 1, hv);
     if (blockIdx.x == 0 && (int)threadIdx.x <= k) (PASS == 0 ? st->h1 : st->h2)[threadIdx.x] = hv[threadIdx.x];
+    if (PASS == 1 && blockIdx.x == 0 && threadIdx.x == 0) {  // this step's end is due (read by later kernels)
+        st->umark = seq;
+        st->ustep = k;
+    }
     constexpr int kE = kJfCellsPerThread * 4;  // elements per thread
     double we[kE];
     const long long base = (long long)blockIdx.x * kE * blockDim.x + threadIdx.x;
@@ -539,80 +659,6 @@ __global__ void __launch_bounds__(256) jf_arnoldi23_kernel(long long len, double
         const double v = jf_block_sum(sacc, lds);
         if (threadIdx.x == 0) pout[(long long)j * gridDim.x + blockIdx.x] = v;
     }
-}
-
-// The end of an Arnoldi step: H[j][k] = h1 + h2, H[k+1][k] = |w|, Givens
-// rotations, the residual estimate, and the cycle's next move; at the end
-// of a cycle the least-squares coefficients y = H^-1 g.
-__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, const double* part, int nblk, double rtol) {
-    // the column, the rotations and g in shared memory: loaded by all
-    // threads at once, worked on by one, stored back by all
-    __shared__ double nn[1];
-    __shared__ double hc[kJfRestart + 1], cs[kJfRestart], sn[kJfRestart], g[kJfRestart + 1], y[kJfRestart];
-    __shared__ double Hs[kJfRestart][kJfRestart];
-    __shared__ int fin;
-    if (st->nd || st->phase != kJfArnoldi) return;
-    const int k = st->k, t = threadIdx.x;
-    if (t <= k) {
-        hc[t] = 0.0 + st->h1[t] + st->h2[t];
-        g[t] = st->g[t];
-    }
-    if (t < k) {
-        cs[t] = st->cs[t];
-        sn[t] = st->sn[t];
-    }
-    jf_sums(part, nblk, 1, nn);
-    if (t == 0) {
-        const double hn = sqrt(nn[0]);
-        hc[k + 1] = hn;
-        for (int j = 0; j < k; ++j) {
-            const double u = cs[j] * hc[j] + sn[j] * hc[j + 1];
-            hc[j + 1] = -sn[j] * hc[j] + cs[j] * hc[j + 1];
-            hc[j] = u;
-        }
-        const double den = hypot(hc[k], hc[k + 1]);
-        cs[k] = den > 0.0 ? hc[k] / den : 1.0;
-        sn[k] = den > 0.0 ? hc[k + 1] / den : 0.0;
-        hc[k] = den;
-        hc[k + 1] = 0.0;
-        g[k + 1] = -sn[k] * g[k];
-        g[k] = cs[k] * g[k];
-        st->cs[k] = cs[k];
-        st->sn[k] = sn[k];
-        st->g[k] = g[k];
-        st->g[k + 1] = g[k + 1];
-        st->its += 1;
-        st->kd = k + 1;
-        const bool conv = fabs(g[k + 1]) < st->r0 * rtol;
-        fin = conv || hn == 0.0 || st->its >= kJfMaxIter || k + 1 == kJfRestart;
-        if (fin) {
-            st->done = conv ? 1 : 0;
-            st->phase = kJfUpdate;
-        } else {
-            st->vnorm[k + 1] = hn;
-            st->k = k + 1;
-        }
-    }
-    __syncthreads();
-    if (t <= k + 1) st->H[t][k] = hc[t];
-    if (!fin) return;
-    // the cycle's end: y = H^-1 g over its kd = k + 1 columns (H upper
-    // triangular after the rotations; column k is the one above)
-    const int kd = k + 1;
-    for (int e = t; e < kd * kd; e += blockDim.x) {
-        const int i = e / kd, j = e % kd;
-        if (j >= i) Hs[i][j] = j == k ? hc[i] : st->H[i][j];
-    }
-    __syncthreads();
-    if (t == 0) {
-        for (int i = kd - 1; i >= 0; --i) {
-            double u = g[i];
-            for (int j = i + 1; j < kd; ++j) u -= Hs[i][j] * y[j];
-            y[i] = Hs[i][i] != 0.0 ? u / Hs[i][i] : 0.0;
-        }
-    }
-    __syncthreads();
-    if (t < kd) st->y[t] = y[t];
 }
 
 // x += y_j V_j, j in order (x = 0 in the first cycle), per cell; converged:
@@ -720,7 +766,9 @@ __global__ void __launch_bounds__(256) jf_restart_tail(JfState* st, const double
     st->k = 0;
     st->kd = 0;
     st->done = 0;
+    st->its0 = st->its;
     st->g[0] = rn;
+    st->gp[0] = rn;
     st->phase = kJfArnoldi;
 }
 
@@ -839,6 +887,7 @@ class Jfnk {
     double* d_part = nullptr; // (kRestart + 1) x nblk partial sums
     double* d_part2 = nullptr; // the same, for the pass that reads d_part's
     double* d_npart = nullptr; // 2 x nblk partial sums of a Newton residual pass
+    double* d_part3 = nullptr; // nblk_e partial sums of |w|^2, the third Arnoldi pass
     JfState* d_st = nullptr;
     JfStatus* h_status = nullptr;  // host-visible, written by the kernels
     long newton_its = 0, gmres_its = 0;  // last solve
@@ -849,7 +898,7 @@ class Jfnk {
 
     ~Jfnk() { release(); }
     void release() {
-        for (double* p : {q, qexp, S, rhs, x, w, r, d_part, d_part2, d_npart})
+        for (double* p : {q, qexp, S, rhs, x, w, r, d_part, d_part2, d_npart, d_part3})
             if (p) hipFree(p);
         for (double* p : V)
             if (p) hipFree(p);
@@ -858,7 +907,7 @@ class Jfnk {
         if (d_st) hipFree(d_st);
         if (d_bstart) hipFree(d_bstart);
         if (h_status) hipHostFree(h_status);
-        q = qexp = S = rhs = x = w = r = d_part = d_part2 = d_npart = nullptr;
+        q = qexp = S = rhs = x = w = r = d_part = d_part2 = d_npart = d_part3 = nullptr;
         d_V = nullptr;
         d_st = nullptr;
         h_status = nullptr;
@@ -894,6 +943,7 @@ class Jfnk {
         ZAMR_HIP(hipMalloc(&d_part, (size_t)(kRestart + 1) * nmax * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_part2, (size_t)(kRestart + 1) * nmax * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_npart, (size_t)2 * nblk * sizeof(double)));
+        ZAMR_HIP(hipMalloc(&d_part3, (size_t)nblk_e * sizeof(double)));
         ZAMR_HIP(hipMalloc(&d_st, sizeof(JfState)));
         ZAMR_HIP(hipHostMalloc(&h_status, sizeof(JfStatus), hipHostMallocCoherent | hipHostMallocMapped));
         std::memset(h_status, 0, sizeof(JfStatus));
@@ -912,22 +962,28 @@ class Jfnk {
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 0,
                            qexp);
         hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st);
+        int seq = 1;  // sequence numbers of the queued Arnoldi steps and their round-closing kernels
         ZAMR_HIP(hipGetLastError());
         // `steps` Arnoldi steps, the first of them right after Newton
         // iteration newton_it's residual pass (-1: continuing a cycle);
         // with_update: then the cycle's update (a converged cycle's is
         // otherwise applied by the next Newton pass)
+        // Every step gets a sequence number; a step ends the one before it
+        // (when that one ran), and a one-block kernel ends the round's last.
         auto arnoldi = [&](int steps, bool with_update, int newton_it) {
             for (int k = 0; k < steps; ++k) {
+                const int u = seq++;
                 hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
                                    (const double*)S, (const double*)rhs, (const double*)r, w, dV, dt, p, time, d_st,
-                                   d_part, (const double*)d_npart, k == 0 ? newton_it : -1);
-                // pass 1 -> d_part -> pass 2 -> d_part2 -> pass 3 -> d_part
-                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, Ge, T, 0, stream, len, w, dV, d_st, cpart, nblk, d_part2);
+                                   d_part, (const double*)d_npart, k == 0 ? newton_it : -1, (const double*)d_part3,
+                                   nblk_e, rtol, u);
+                // pass 1 -> d_part -> pass 2 -> d_part2 -> pass 3 -> d_part3
+                hipLaunchKernelGGL(jf_arnoldi23_kernel<0>, Ge, T, 0, stream, len, w, dV, d_st, cpart, nblk, d_part2,
+                                   u);
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, Ge, T, 0, stream, len, w, dV, d_st,
-                                   (const double*)d_part2, nblk_e, d_part);
-                hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, cpart, nblk_e, rtol);
+                                   (const double*)d_part2, nblk_e, d_part3, u);
             }
+            hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, (const double*)d_part3, nblk_e, rtol, seq++);
             if (with_update) {
                 hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);
                 hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, cpart, nblk);

@@ -84,32 +84,6 @@ __device__ __forceinline__ JfCell jf_cell(const BoxDesc* __restrict__ boxes, con
     return r;
 }
 
-// Valid cells of buffer `buf` -> compact q (all rows) and Qexp (the source
-// rows), or compact q -> buffer (scatter).
-template <class M>
-__global__ void __launch_bounds__(256) jf_move_kernel(const BoxDesc* __restrict__ boxes,
-                                                      const long long* __restrict__ bstart, int nb, long long N,
-                                                      int buf, double* __restrict__ v, int to_box,
-                                                      double* __restrict__ qexp) {
-    constexpr int nq = M::n_dof_q;
-    using R = SourceRows<M>;
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const JfCell e = jf_cell(boxes, bstart, nb, c);
-    const BoxDesc& B = boxes[e.b];
-    double* q = buf == 0 ? B.q[0] : (buf == 1 ? B.q[1] : B.q[2]);
-    const long long idx = (long long)(e.i - B.gl[0]) + (long long)(e.j - B.gl[1]) * B.sy;
-    if (to_box) {
-#pragma unroll
-        for (int a = 0; a < R::n; ++a) q[idx + R::row(a) * B.sc] = v[R::row(a) * N + c];
-        return;
-    }
-#pragma unroll
-    for (int n = 0; n < nq; ++n) v[n * N + c] = q[idx + n * B.sc];
-#pragma unroll
-    for (int a = 0; a < R::n; ++a) qexp[a * N + c] = q[idx + R::row(a) * B.sc];
-}
-
 // Device state of one solve (GMRES(30) cycle + Newton bookkeeping).
 constexpr int kJfRestart = 30;
 struct JfState {
@@ -157,7 +131,8 @@ constexpr int kJfNewtonMax = 12;
 
 // The status of the state, for the host; also each finished iteration's
 // Arnoldi step count.
-__global__ void __launch_bounds__(64) jf_publish_kernel(const JfState* st, JfStatus* hs) {
+// (all threads of a block, after the state's last write and a barrier)
+__device__ __forceinline__ void jf_publish(const JfState* st, JfStatus* hs) {
     if (threadIdx.x < 16) hs->its_hist[threadIdx.x] = st->its_hist[threadIdx.x];
     if (threadIdx.x != 0) return;
     hs->phase = st->phase;
@@ -263,14 +238,44 @@ __device__ __forceinline__ void jf_apply_cell(const BoxDesc* __restrict__ boxes,
         if (const long long c = ((long long)blockIdx.x * kJfCellsPerThread + pc) * blockDim.x + threadIdx.x; \
             c < N)
 
-// Zero the state (all threads, 8-byte words).
-__global__ void __launch_bounds__(256) jf_init_kernel(JfState* st) {
+// Zero the state (all threads of one block, 8-byte words), before the
+// block's threads part ways.
+__device__ __forceinline__ void jf_init_state(JfState* st) {
     static_assert(sizeof(JfState) % 8 == 0, "the state is a whole number of 8-byte words");
     unsigned long long* w = reinterpret_cast<unsigned long long*>(st);
     for (int i = threadIdx.x; i < (int)(sizeof(JfState) / 8); i += blockDim.x) w[i] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) st->umark = -1;  // no step yet (sequence numbers start at 1)
 }
+
+// Valid cells of buffer `buf` -> compact q (all rows) and Qexp (the source
+// rows), or compact q -> buffer (scatter).  With `st` (the gather that
+// starts a solve) block 0 also zeroes the solver state.
+template <class M>
+__global__ void __launch_bounds__(256) jf_move_kernel(const BoxDesc* __restrict__ boxes,
+                                                      const long long* __restrict__ bstart, int nb, long long N,
+                                                      int buf, double* __restrict__ v, int to_box,
+                                                      double* __restrict__ qexp, JfState* st) {
+    constexpr int nq = M::n_dof_q;
+    using R = SourceRows<M>;
+    if (st && blockIdx.x == 0) jf_init_state(st);  // the gather starts a solve
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const JfCell e = jf_cell(boxes, bstart, nb, c);
+    const BoxDesc& B = boxes[e.b];
+    double* q = buf == 0 ? B.q[0] : (buf == 1 ? B.q[1] : B.q[2]);
+    const long long idx = (long long)(e.i - B.gl[0]) + (long long)(e.j - B.gl[1]) * B.sy;
+    if (to_box) {
+#pragma unroll
+        for (int a = 0; a < R::n; ++a) q[idx + R::row(a) * B.sc] = v[R::row(a) * N + c];
+        return;
+    }
+#pragma unroll
+    for (int n = 0; n < nq; ++n) v[n * N + c] = q[idx + n * B.sc];
+#pragma unroll
+    for (int a = 0; a < R::n; ++a) qexp[a * N + c] = q[idx + R::row(a) * B.sc];
+}
+
 
 // A new cycle's Hessenberg matrix and residual vector, zeroed by all threads.
 __device__ __forceinline__ void jf_clear_cycle(JfState* st) {
@@ -400,12 +405,21 @@ __device__ __forceinline__ void jf_newton_end(JfState* st, int it, double f2, do
 // Newton's convergence test and the start of the GMRES cycle.
 // The end of a residual pass that no Arnoldi step follows in the queue (the
 // converging test the host queues last).
-__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double* part, int nblk, int it) {
+__device__ __forceinline__ void jf_newton_tail_body(JfState* st, const double* part, int nblk, int it) {
     __shared__ double tot[2];
     if (st->nd || st->nrun != it + 1) return;
     const bool fused = jf_newton_fused(st, it);
     jf_sums(part, nblk, 2, tot);
     jf_newton_end(st, it, tot[0], tot[1], fused);
+}
+
+__global__ void __launch_bounds__(256) jf_newton_tail(JfState* st, const double* part, int nblk, int it,
+                                                      JfStatus* hs, int pub) {
+    jf_newton_tail_body(st, part, nblk, it);
+    if (pub) {  // the last kernel of a queued round: the status for the host
+        __syncthreads();
+        jf_publish(st, hs);
+    }
 }
 
 // The end of Arnoldi step k from w2 = |w|^2 (one block, all threads):
@@ -493,13 +507,21 @@ __device__ __forceinline__ bool jf_column_end(JfState* st, int k, double w2, dou
 // The end of the Arnoldi step a round of queued steps closes with (sequence
 // number seq - 1), when its third pass ran: the next step's first pass ends
 // the step before it itself.
-__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, const double* part3, int nblk, double rtol,
-                                                      int seq) {
+__device__ __forceinline__ void jf_column_tail_body(JfState* st, const double* part3, int nblk, double rtol, int seq) {
     __shared__ double nn[1];
     if (st->nd || st->umark != seq - 1 || st->phase != kJfArnoldi) return;
     const int k = st->ustep;
     jf_sums(part3, nblk, 1, nn);
     jf_column_end(st, k, nn[0], rtol, true);
+}
+
+__global__ void __launch_bounds__(256) jf_column_tail(JfState* st, const double* part3, int nblk, double rtol,
+                                                      int seq, JfStatus* hs, int pub) {
+    jf_column_tail_body(st, part3, nblk, rtol, seq);
+    if (pub) {  // the last kernel of a queued round: the status for the host
+        __syncthreads();
+        jf_publish(st, hs);
+    }
 }
 
 // Arnoldi pass 1: V_k = src / vnorm_k (stored; src = rhs or r at k = 0, the
@@ -695,7 +717,7 @@ __global__ void __launch_bounds__(256) jf_update_kernel(long long N, double* __r
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-__global__ void __launch_bounds__(256) jf_update_tail(JfState* st, const double* part, int nblk) {
+__device__ __forceinline__ void jf_update_tail_body(JfState* st, const double* part, int nblk) {
     __shared__ double tot[1];
     if (st->nd || st->phase != kJfUpdate) return;
     if (st->done) {
@@ -707,6 +729,15 @@ __global__ void __launch_bounds__(256) jf_update_tail(JfState* st, const double*
         st->xnorm = sqrt(tot[0]);
         st->first = 0;
         st->phase = kJfRestartRes;
+    }
+}
+
+__global__ void __launch_bounds__(256) jf_update_tail(JfState* st, const double* part, int nblk, JfStatus* hs,
+                                                      int pub) {
+    jf_update_tail_body(st, part, nblk);
+    if (pub) {  // the last kernel of a queued round: the status for the host
+        __syncthreads();
+        jf_publish(st, hs);
     }
 }
 
@@ -960,8 +991,7 @@ class Jfnk {
         double* const* dV = (double* const*)d_V;
         const double* cpart = d_part;
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 0,
-                           qexp);
-        hipLaunchKernelGGL(jf_init_kernel, one, T, 0, stream, d_st);
+                           qexp, d_st);
         int seq = 1;  // sequence numbers of the queued Arnoldi steps and their round-closing kernels
         ZAMR_HIP(hipGetLastError());
         // `steps` Arnoldi steps, the first of them right after Newton
@@ -970,7 +1000,8 @@ class Jfnk {
         // otherwise applied by the next Newton pass)
         // Every step gets a sequence number; a step ends the one before it
         // (when that one ran), and a one-block kernel ends the round's last.
-        auto arnoldi = [&](int steps, bool with_update, int newton_it) {
+        // last: the round's final kernel also copies the status for the host.
+        auto arnoldi = [&](int steps, bool with_update, int newton_it, bool last) {
             for (int k = 0; k < steps; ++k) {
                 const int u = seq++;
                 hipLaunchKernelGGL((jf_arnoldi1_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
@@ -983,10 +1014,11 @@ class Jfnk {
                 hipLaunchKernelGGL(jf_arnoldi23_kernel<1>, Ge, T, 0, stream, len, w, dV, d_st,
                                    (const double*)d_part2, nblk_e, d_part3, u);
             }
-            hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, (const double*)d_part3, nblk_e, rtol, seq++);
+            hipLaunchKernelGGL(jf_column_tail, one, T, 0, stream, d_st, (const double*)d_part3, nblk_e, rtol, seq++, hs,
+                               (int)(last && !with_update));
             if (with_update) {
                 hipLaunchKernelGGL((jf_update_kernel<M>), G, T, 0, stream, N, x, q, dV, (const JfState*)d_st, d_part);
-                hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, cpart, nblk);
+                hipLaunchKernelGGL(jf_update_tail, one, T, 0, stream, d_st, cpart, nblk, hs, (int)last);
             }
             ZAMR_HIP(hipGetLastError());
         };
@@ -994,27 +1026,26 @@ class Jfnk {
         // steps (the first applies the pass's end); queued ahead as far as
         // the last solve went (its converging residual test included, with
         // the pass's end on its own), then read back once
-        auto iteration = [&](int it, int steps) {
+        auto iteration = [&](int it, int steps, bool last) {
             hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
                                (const double*)x, dV, S, rhs, dt, p, time, d_st, d_npart, it);
-            arnoldi(steps, false, it);
+            arnoldi(steps, false, it, last);
         };
         if ((int)steps_pred.size() == newton_pred && newton_pred > 0 && newton_pred < kJfNewtonMax) {
             // as the last solve went: its iterations with their own Arnoldi
             // step counts, then the residual pass that found it converged
             // (a solve that needs more is finished by the rounds below)
-            for (int it = 0; it < newton_pred; ++it) iteration(it, std::max(steps_pred[it], 1));
+            for (int it = 0; it < newton_pred; ++it) iteration(it, std::max(steps_pred[it], 1), false);
             hipLaunchKernelGGL((jf_newton_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, q, (const double*)qexp,
                                (const double*)x, dV, S, rhs, dt, p, time, d_st, d_npart, newton_pred);
-            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, (const double*)d_npart, nblk, newton_pred);
+            hipLaunchKernelGGL(jf_newton_tail, one, T, 0, stream, d_st, (const double*)d_npart, nblk, newton_pred, hs, 1);
             ZAMR_HIP(hipGetLastError());
         } else {
             const int ahead = std::min(kJfNewtonMax, std::max(newton_pred, 1) + 1);
-            for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2));
+            for (int it = 0; it < ahead; ++it) iteration(it, std::max(chunk, 2), it == ahead - 1);
         }
-        // the round's status for the host, then the one wait
+        // the one wait of a round (its final kernel has copied the status)
         auto wait = [&]() {
-            hipLaunchKernelGGL(jf_publish_kernel, one, dim3(64), 0, stream, (const JfState*)d_st, hs);
             ZAMR_HIP(hipGetLastError());
             ZAMR_HIP(hipStreamSynchronize(stream));
         };
@@ -1025,7 +1056,7 @@ class Jfnk {
             if (++rounds > kJfMaxIter || h_status->its >= kJfMaxIter)
                 throw Error(4, "internal: the global source solve's GMRES did not settle");
             if (h_status->phase == kJfFinished) {  // the next iteration was not queued, or ran early
-                iteration((int)h_status->newton_its, std::max(chunk, 2));
+                iteration((int)h_status->newton_its, std::max(chunk, 2), true);
             } else {  // the running cycle needs more steps, a restart, or its update
                 hipLaunchKernelGGL((jf_restart_kernel<M>), G, T, 0, stream, boxes, d_bstart, nb, N, (const double*)q,
                                    (const double*)S, (const double*)rhs, (const double*)x, r, dt, p, time,
@@ -1034,7 +1065,7 @@ class Jfnk {
                 hipLaunchKernelGGL((jf_qupdate_kernel<M>), G, T, 0, stream, N, q, (const double*)x,
                                    (const JfState*)d_st);
                 hipLaunchKernelGGL(jf_qupdate_tail, one, dim3(64), 0, stream, d_st);
-                arnoldi(4, true, -1);
+                arnoldi(4, true, -1, true);
             }
             wait();
         }
@@ -1045,7 +1076,7 @@ class Jfnk {
         newton_pred = it;
         steps_pred.assign(h_status->its_hist, h_status->its_hist + std::min(it, 16));
         hipLaunchKernelGGL((jf_move_kernel<M>), dim3(g1(N)), dim3(256), 0, stream, boxes, d_bstart, nb, N, buf, q, 1,
-                           (double*)nullptr);
+                           (double*)nullptr, (JfState*)nullptr);
         ZAMR_HIP(hipGetLastError());
         newton_its = h_status->newton_its;
         gmres_its = (long)h_status->gmres_total;
